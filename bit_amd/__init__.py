@@ -1,0 +1,342 @@
+"""bit_amd -- MI355X particle-cluster treecode (host mirror over the C-ABI).
+
+The native library ``bit_amd/libbit_amd.so`` (include/bit_amd.h) holds the
+whole evaluation path: the host tree build (bit-exact with the reference
+``treemake``, tree3d.cpp:1345) and the gfx950 HIP kernels (moments, batch
+interaction lists, far field, near field).  This module loads it with ctypes
+and mirrors the reference's operations on numpy / torch arrays:
+
+    ===========================  ==========================================
+    reference (src/tree3d.h)     here
+    ===========================  ==========================================
+    treemake                     System(ctx, x, y, z, q, maxmemb)
+    treemoments                  System.moments(nterms) / moments_cube()
+    shielded_treepot loop        System.potential(nterms, acc, kappa, ...)
+    treepotential loop           System.potential(kappa=0, delta2=DEL, ...)
+    treeforce loop               System.field(nterms, acc, delta2, ...)
+    shielded_pot_particle_ds     System.direct(kappa, delta2, tidx)
+    ===========================  ==========================================
+
+There is no CPU fallback: if the library is missing, importing this module
+raises ImportError (build it with ``python -m bit_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbit_amd.so")
+
+BIT_OK = 0
+BIT_E_INVALID = -1
+BIT_E_OOM = -2
+BIT_E_HIP = -3
+BIT_E_TREE = -4
+BIT_E_STATE = -5
+BIT_FP64 = 0
+BIT_FP32 = 1
+MAX_NTERMS = 13
+
+# Reference compile-time constants (problem.h:11-13, constants.h:10).
+ACC = 0.5
+DEL = 0.01
+MAXTERM = 12
+
+
+class BitError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bit_amd error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("nterms", C.c_int), ("acc", C.c_double), ("kappa", C.c_double),
+                ("delta2", C.c_double), ("mac_sub", C.c_double), ("precision", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("near_pairs", C.c_int64), ("far_pairs", C.c_int64), ("far_items", C.c_int64),
+                ("near_items", C.c_int64), ("lane_far", C.c_int64),
+                ("lane_near_pairs", C.c_int64), ("ms_tree", C.c_double),
+                ("ms_upload", C.c_double), ("ms_moments", C.c_double), ("ms_lists", C.c_double),
+                ("ms_far", C.c_double), ("ms_near", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# Every symbol include/bit_amd.h declares.
+EXPORTS = [
+    "bit_params_default", "bit_last_error", "bit_ctx_create", "bit_ctx_destroy",
+    "bit_ctx_set_stream", "bit_ctx_sync", "bit_ctx_bytes_in_use",
+    "bit_ctx_inject_alloc_failures", "bit_system_create", "bit_system_create_device",
+    "bit_system_destroy", "bit_system_numtree", "bit_system_maxdepth",
+    "bit_system_export_tree", "bit_moments", "bit_moments_download", "bit_eval_potential",
+    "bit_eval_field", "bit_system_stats", "bit_direct_potential", "bit_tree_build",
+    "bit_tree_destroy", "bit_tree_numtree", "bit_tree_maxdepth", "bit_tree_export",
+    "bit_system_export_lists",
+]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `python -m bit_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.bit_last_error.restype = C.c_char_p
+    L.bit_params_default.argtypes = [C.POINTER(Params)]
+    L.bit_ctx_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]
+    L.bit_ctx_destroy.argtypes = [vp]
+    L.bit_ctx_set_stream.argtypes = [vp, vp]
+    L.bit_ctx_sync.argtypes = [vp]
+    L.bit_ctx_bytes_in_use.argtypes = [vp]
+    L.bit_ctx_bytes_in_use.restype = C.c_size_t
+    L.bit_ctx_inject_alloc_failures.argtypes = [vp, C.c_int]
+    for f in ("bit_system_create", "bit_system_create_device"):
+        getattr(L, f).argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.POINTER(vp)]
+    L.bit_system_destroy.argtypes = [vp]
+    L.bit_system_numtree.argtypes = [vp]
+    L.bit_system_maxdepth.argtypes = [vp]
+    L.bit_system_export_tree.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.bit_moments.argtypes = [vp, C.c_int]
+    L.bit_moments_download.argtypes = [vp, vp]
+    L.bit_eval_potential.argtypes = [vp, C.POINTER(Params), vp, C.c_int]
+    L.bit_eval_field.argtypes = [vp, C.POINTER(Params), vp, C.c_int]
+    L.bit_system_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.bit_direct_potential.argtypes = [vp, C.c_double, C.c_double, C.c_int, vp, vp]
+    L.bit_system_export_lists.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp,
+                                          vp, vp]
+    L.bit_tree_build.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.POINTER(vp)]
+    L.bit_tree_destroy.argtypes = [vp]
+    L.bit_tree_numtree.argtypes = [vp]
+    L.bit_tree_maxdepth.argtypes = [vp]
+    L.bit_tree_export.argtypes = [vp, vp, vp, vp, vp, vp]
+    for name in EXPORTS:
+        if name not in ("bit_last_error", "bit_params_default", "bit_ctx_bytes_in_use"):
+            getattr(L, name).restype = C.c_int
+    return L
+
+
+class _Lib:
+    """The native library, loaded on first use (ImportError if it is not built;
+    there is no fallback).  Loading lazily lets `python -m bit_amd.build`
+    import this package before the library exists."""
+
+    _L = None
+
+    def __getattr__(self, name):
+        if _Lib._L is None:
+            _Lib._L = _load()
+        return getattr(_Lib._L, name)
+
+
+lib = _Lib()
+
+
+def _check(rc: int):
+    if rc != BIT_OK:
+        raise BitError(rc, lib.bit_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def make_params(nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None, precision="fp64"):
+    """Evaluation parameters; mac_sub defaults to delta2 (the MAC on the true
+    squared distance), the reference's treepotential/treeforce convention."""
+    p = Params()
+    p.nterms = int(nterms)
+    p.acc = float(acc)
+    p.kappa = float(kappa)
+    p.delta2 = float(delta2)
+    p.mac_sub = float(delta2 if mac_sub is None else mac_sub)
+    p.precision = BIT_FP32 if precision in ("fp32", BIT_FP32) else BIT_FP64
+    return p
+
+
+class Context:
+    """One device + one stream + an HBM budget (bytes, 0 = unlimited)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        _check(lib.bit_ctx_create(int(device), int(hbm_budget), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.bit_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr: int | None):
+        _check(lib.bit_ctx_set_stream(self.h, C.c_void_p(stream_ptr or 0)))
+
+    def sync(self):
+        _check(lib.bit_ctx_sync(self.h))
+
+    @property
+    def bytes_in_use(self) -> int:
+        return int(lib.bit_ctx_bytes_in_use(self.h))
+
+    def inject_alloc_failures(self, k: int):
+        _check(lib.bit_ctx_inject_alloc_failures(self.h, int(k)))
+
+
+@dataclass
+class TreeExport:
+    numtree: int
+    boxes: np.ndarray     # [numtree, 13] as TREE (tree3d.h:51-74)
+    children: np.ndarray  # [numtree, 8]
+    begin: np.ndarray
+    end: np.ndarray
+    perm: np.ndarray      # sorted slot -> original index
+
+    def members(self, i: int) -> np.ndarray:
+        """Slots of node i in sorted order (a leaf's are ascending indices)."""
+        return self.perm[self.begin[i]:self.end[i]]
+
+    def member_list(self, i: int) -> np.ndarray:
+        """TREE::members of node i: ascending original indices."""
+        return np.sort(self.members(i))
+
+
+def build_tree(x, y, z, maxmemb: int) -> TreeExport:
+    """Host-only tree build (treemake, tree3d.cpp:1345); needs no device."""
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    n = len(x)
+    h = C.c_void_p()
+    _check(lib.bit_tree_build(n, _ptr(x), _ptr(y), _ptr(z), int(maxmemb), C.byref(h)))
+    try:
+        nt = lib.bit_tree_numtree(h)
+        boxes = np.zeros(nt * 13)
+        ch = np.zeros(nt * 8, dtype=np.int32)
+        b = np.zeros(nt, dtype=np.int32)
+        e = np.zeros(nt, dtype=np.int32)
+        p = np.zeros(n, dtype=np.int32)
+        _check(lib.bit_tree_export(h, _ptr(boxes), _ptr(ch), _ptr(b), _ptr(e), _ptr(p)))
+        t = TreeExport(nt, boxes.reshape(nt, 13), ch.reshape(nt, 8), b, e, p)
+        t.maxdepth = lib.bit_tree_maxdepth(h)
+        return t
+    finally:
+        lib.bit_tree_destroy(h)
+
+
+class System:
+    """One particle system resident on the device (tree + sorted sources)."""
+
+    def __init__(self, ctx: Context, x, y, z, q, maxmemb: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        if hasattr(x, "is_cuda") and x.is_cuda:  # torch device tensors (float64)
+            n = x.numel()
+            _check(lib.bit_system_create_device(ctx.h, n, C.c_void_p(x.data_ptr()),
+                                                C.c_void_p(y.data_ptr()),
+                                                C.c_void_p(z.data_ptr()),
+                                                C.c_void_p(q.data_ptr()), int(maxmemb),
+                                                C.byref(h)))
+        else:
+            x, y, z, q = _f64(x), _f64(y), _f64(z), _f64(q)
+            n = len(x)
+            _check(lib.bit_system_create(ctx.h, n, _ptr(x), _ptr(y), _ptr(z), _ptr(q),
+                                         int(maxmemb), C.byref(h)))
+        self.h = h
+        self.n = n
+        self.numtree = lib.bit_system_numtree(h)
+        self.maxdepth = lib.bit_system_maxdepth(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.bit_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export_tree(self) -> TreeExport:
+        nt, n = self.numtree, self.n
+        boxes = np.zeros(nt * 13)
+        ch = np.zeros(nt * 8, dtype=np.int32)
+        b = np.zeros(nt, dtype=np.int32)
+        e = np.zeros(nt, dtype=np.int32)
+        p = np.zeros(n, dtype=np.int32)
+        _check(lib.bit_system_export_tree(self.h, _ptr(boxes), _ptr(ch), _ptr(b), _ptr(e),
+                                          _ptr(p)))
+        return TreeExport(nt, boxes.reshape(nt, 13), ch.reshape(nt, 8), b, e, p)
+
+    def moments(self, nterms: int):
+        _check(lib.bit_moments(self.h, int(nterms)))
+        self._mom_nterms = int(nterms)
+
+    def moments_cube(self) -> np.ndarray:
+        p = self._mom_nterms
+        out = np.zeros(self.numtree * p ** 3)
+        _check(lib.bit_moments_download(self.h, _ptr(out)))
+        return out.reshape(self.numtree, p, p, p)
+
+    def potential(self, nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None,
+                  precision="fp64", out=None):
+        """+phi at every particle (self excluded), original order.  `out` may be
+        a float64 torch device tensor of n elements (result stays in HBM)."""
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+        if out is not None and hasattr(out, "is_cuda") and out.is_cuda:
+            _check(lib.bit_eval_potential(self.h, C.byref(prm), C.c_void_p(out.data_ptr()), 1))
+            return out
+        res = np.zeros(self.n)
+        _check(lib.bit_eval_potential(self.h, C.byref(prm), _ptr(res), 0))
+        return res
+
+    def field(self, nterms=6, acc=ACC, delta2=DEL, mac_sub=None):
+        """E = sum q (x_i - x_j)/R^3 (regularized Coulomb), [n, 3], original order."""
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64")
+        res = np.zeros(3 * self.n)
+        _check(lib.bit_eval_field(self.h, C.byref(prm), _ptr(res), 0))
+        return res.reshape(self.n, 3)
+
+    def direct(self, kappa, delta2, tidx):
+        """FP64 GPU direct sum at particle targets tidx (original indices)."""
+        t = np.ascontiguousarray(tidx, dtype=np.int32)
+        out = np.zeros(len(t))
+        _check(lib.bit_direct_potential(self.h, float(kappa), float(delta2), len(t), _ptr(t),
+                                        _ptr(out)))
+        return out
+
+    def lists(self, nterms=6, acc=ACC, delta2=0.0, mac_sub=None):
+        """Batch interaction lists for the MAC parameters: dict of far/near
+        offsets [nbatch+1], masks (uint64) and nodes."""
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64")
+        nb, nf, nn = C.c_int(), C.c_int(), C.c_int()
+        _check(lib.bit_system_export_lists(self.h, C.byref(prm), C.byref(nb), C.byref(nf),
+                                           C.byref(nn), None, None, None, None, None, None))
+        fo = np.zeros(nb.value + 1, dtype=np.int32)
+        no = np.zeros(nb.value + 1, dtype=np.int32)
+        fm = np.zeros(max(nf.value, 1), dtype=np.uint64)
+        fn = np.zeros(max(nf.value, 1), dtype=np.int32)
+        nm = np.zeros(max(nn.value, 1), dtype=np.uint64)
+        nd = np.zeros(max(nn.value, 1), dtype=np.int32)
+        _check(lib.bit_system_export_lists(self.h, C.byref(prm), C.byref(nb), C.byref(nf),
+                                           C.byref(nn), _ptr(fo), _ptr(fm), _ptr(fn), _ptr(no),
+                                           _ptr(nm), _ptr(nd)))
+        return dict(far_off=fo, far_mask=fm[:nf.value], far_node=fn[:nf.value], near_off=no,
+                    near_mask=nm[:nn.value], near_node=nd[:nn.value])
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib.bit_system_stats(self.h, C.byref(s)))
+        return s.as_dict()

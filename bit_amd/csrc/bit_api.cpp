@@ -1,0 +1,731 @@
+// bit_api.cpp -- implementation of the C-ABI in include/bit_amd.h.
+//
+// Host side of one system: the tree is built on the host (bit-exact with the
+// reference treemake), the sources are sorted into the tree's slot order and
+// uploaded once, and every evaluation runs on the context's stream:
+//   moments (cached per nterms) -> batch lists (cached per MAC parameters)
+//   -> far field -> near field + scatter to the caller's order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bit_amd.h"
+#include "kernels.h"
+#include "tree_build.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(BIT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+#define BIT_TRY(expr)         \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != BIT_OK) return r_; \
+  } while (0)
+
+constexpr int kMomChunk = 4096;
+
+int ncoef(int nt) { return nt * (nt + 1) * (nt + 2) / 6; }
+
+}  // namespace
+
+struct bit_tree {
+  bit::HostTree tree;
+};
+
+struct bit_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t in_use = 0;
+  int inject = 0;
+};
+
+namespace {
+
+// A device allocation accounted against the context's HBM budget.
+struct DevBuf {
+  bit_ctx* ctx = nullptr;
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      ctx->in_use -= bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grows (never shrinks) to at least `need` bytes; contents are not kept.
+  int ensure(bit_ctx* c, size_t need) {
+    ctx = c;
+    if (need <= bytes && p) return BIT_OK;
+    release();
+    if (need == 0) need = 8;
+    if (c->inject > 0) {
+      c->inject--;
+      return fail(BIT_E_OOM, "injected device allocation failure");
+    }
+    if (c->budget && c->in_use + need > c->budget)
+      return fail(BIT_E_OOM, "HBM budget exceeded: need " + std::to_string(need) +
+                                 " bytes, in use " + std::to_string(c->in_use) + " of " +
+                                 std::to_string(c->budget));
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(BIT_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    bytes = need;
+    c->in_use += need;
+    return BIT_OK;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~Timer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  void start(hipStream_t s) {
+    if (!a) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+    }
+    (void)hipEventRecord(a, s);
+  }
+  void stop(hipStream_t s) { (void)hipEventRecord(b, s); }
+  double ms() {
+    float v = 0.f;
+    if (a && b && hipEventElapsedTime(&v, a, b) == hipSuccess) return v;
+    return 0.0;
+  }
+};
+
+}  // namespace
+
+struct bit_system {
+  bit_ctx* ctx = nullptr;
+  bit::HostTree tree;
+  int stack_cap = 0;
+  int nleaves = 0;
+  // tree + sources
+  DevBuf center, child, range, scale, perm, leaves, sx, sy, sz, sq, sloc;
+  // moments
+  int mom_nterms = 0;
+  int nchunks = 0;
+  DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, mom_hat32;
+  // lists
+  bool lists_valid = false;
+  double l_acc = 0, l_delta2 = 0, l_sub = 0;
+  int n_far_items = 0, n_near_items = 0;
+  DevBuf far_cnt, near_cnt, far_off, near_off, far_items, near_items, scan_tmp, lstats, err_flag;
+  bit::dev::ListStats hstats{};
+  // outputs
+  DevBuf phi_far, out;
+  // direct sum
+  DevBuf tslot, dpart, dout;
+  bit_stats stats{};
+  Timer t_mom, t_lists, t_far, t_near;
+
+  bit::dev::TreeView view() const {
+    bit::dev::TreeView v;
+    v.n = tree.n;
+    v.numtree = tree.numtree;
+    v.center = center.as<double4>();
+    v.child = child.as<int>();
+    v.range = range.as<int2>();
+    v.scale = scale.as<double>();
+    v.sx = sx.as<double>();
+    v.sy = sy.as<double>();
+    v.sz = sz.as<double>();
+    v.sq = sq.as<double>();
+    v.sloc = sloc.as<float4>();
+    v.perm = perm.as<int>();
+    return v;
+  }
+};
+
+namespace {
+
+int upload(bit_ctx* c, DevBuf& b, const void* host, size_t bytes) {
+  BIT_TRY(b.ensure(c, bytes));
+  HIP_TRY(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, c->stream));
+  return BIT_OK;
+}
+
+bool finite_all(int n, const double* a) {
+  for (int i = 0; i < n; i++)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
+
+// Upload the node table and create the system; sorted sources come from
+// `sorted` (host, 4 arrays) or are gathered on the device from d_src.
+int finish_system(bit_ctx* ctx, bit_system* s, const double* const* sorted_host,
+                  const double* const* d_src) {
+  const bit::HostTree& T = s->tree;
+  const int nt = T.numtree, n = T.n;
+  std::vector<double> cen(4 * (size_t)nt), scl(nt);
+  std::vector<int> rng(2 * (size_t)nt), lv;
+  for (int i = 0; i < nt; i++) {
+    const double* f = &T.box[(size_t)i * bit::BOX_FIELDS];
+    cen[4 * i + 0] = f[bit::BXMID];
+    cen[4 * i + 1] = f[bit::BYMID];
+    cen[4 * i + 2] = f[bit::BZMID];
+    cen[4 * i + 3] = f[bit::BSQR];
+    scl[i] = f[bit::BSQR] > 0 ? std::sqrt(f[bit::BSQR]) : 1.0;
+    rng[2 * i] = T.begin[i];
+    rng[2 * i + 1] = T.end[i];
+    if (T.children[(size_t)i * 8] < 0) lv.push_back(i);
+  }
+  s->nleaves = (int)lv.size();
+  s->stack_cap = 8 * (T.maxdepth + 2) + 8;
+  BIT_TRY(upload(ctx, s->center, cen.data(), cen.size() * sizeof(double)));
+  BIT_TRY(upload(ctx, s->scale, scl.data(), scl.size() * sizeof(double)));
+  BIT_TRY(upload(ctx, s->range, rng.data(), rng.size() * sizeof(int)));
+  BIT_TRY(upload(ctx, s->child, T.children.data(), T.children.size() * sizeof(int)));
+  BIT_TRY(upload(ctx, s->perm, T.perm.data(), (size_t)n * sizeof(int)));
+  BIT_TRY(upload(ctx, s->leaves, lv.data(), std::max<size_t>(1, lv.size()) * sizeof(int)));
+  const size_t nb = (size_t)n * sizeof(double);
+  if (sorted_host) {
+    BIT_TRY(upload(ctx, s->sx, sorted_host[0], nb));
+    BIT_TRY(upload(ctx, s->sy, sorted_host[1], nb));
+    BIT_TRY(upload(ctx, s->sz, sorted_host[2], nb));
+    BIT_TRY(upload(ctx, s->sq, sorted_host[3], nb));
+  } else {
+    BIT_TRY(s->sx.ensure(ctx, nb));
+    BIT_TRY(s->sy.ensure(ctx, nb));
+    BIT_TRY(s->sz.ensure(ctx, nb));
+    BIT_TRY(s->sq.ensure(ctx, nb));
+    HIP_TRY(bit::dev::gather_sources(s->view(), d_src[0], d_src[1], d_src[2], d_src[3],
+                                     s->sx.as<double>(), s->sy.as<double>(),
+                                     s->sz.as<double>(), s->sq.as<double>(), ctx->stream));
+  }
+  BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
+  HIP_TRY(bit::dev::leaf_local(s->view(), s->leaves.as<int>(), s->nleaves, s->sloc.as<float4>(),
+                               ctx->stream));
+  return BIT_OK;
+}
+
+int check_params(const bit_system* s, const bit_params* p) {
+  if (!s || !p) return fail(BIT_E_INVALID, "null system or params");
+  if (p->nterms < 1 || p->nterms > BIT_MAX_NTERMS)
+    return fail(BIT_E_INVALID, "nterms must be in [1, " + std::to_string(BIT_MAX_NTERMS) + "]");
+  if (!std::isfinite(p->acc) || !std::isfinite(p->kappa) || !std::isfinite(p->delta2) ||
+      !std::isfinite(p->mac_sub) || p->delta2 < 0 || p->kappa < 0)
+    return fail(BIT_E_INVALID, "acc/kappa/delta2/mac_sub must be finite, kappa,delta2 >= 0");
+  if (p->precision != BIT_FP64 && p->precision != BIT_FP32)
+    return fail(BIT_E_INVALID, "precision must be BIT_FP64 or BIT_FP32");
+  return BIT_OK;
+}
+
+int ensure_lists(bit_system* s, const bit_params* p) {
+  if (s->lists_valid && s->l_acc == p->acc && s->l_delta2 == p->delta2 && s->l_sub == p->mac_sub)
+    return BIT_OK;
+  bit_ctx* c = s->ctx;
+  const int n = s->tree.n;
+  const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
+  bit::dev::MacParams mp{p->acc, p->delta2, p->mac_sub};
+  BIT_TRY(s->far_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(s->near_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(s->far_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(s->near_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(s->lstats.ensure(c, sizeof(bit::dev::ListStats)));
+  BIT_TRY(s->err_flag.ensure(c, sizeof(int)));
+  HIP_TRY(hipMemsetAsync(s->far_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
+  HIP_TRY(hipMemsetAsync(s->near_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
+  HIP_TRY(hipMemsetAsync(s->lstats.p, 0, sizeof(bit::dev::ListStats), c->stream));
+  HIP_TRY(hipMemsetAsync(s->err_flag.p, 0, sizeof(int), c->stream));
+  s->t_lists.start(c->stream);
+  HIP_TRY(bit::dev::lists_count(s->view(), mp, s->stack_cap, s->far_cnt.as<int>(),
+                                s->near_cnt.as<int>(), s->lstats.as<bit::dev::ListStats>(),
+                                s->err_flag.as<int>(), c->stream));
+  size_t tmp = 0;
+  HIP_TRY(bit::dev::exclusive_scan(s->far_cnt.as<int>(), s->far_off.as<int>(), nb + 1, nullptr,
+                                   tmp, c->stream));
+  BIT_TRY(s->scan_tmp.ensure(c, tmp));
+  tmp = s->scan_tmp.bytes;
+  HIP_TRY(bit::dev::exclusive_scan(s->far_cnt.as<int>(), s->far_off.as<int>(), nb + 1,
+                                   s->scan_tmp.p, tmp, c->stream));
+  tmp = s->scan_tmp.bytes;
+  HIP_TRY(bit::dev::exclusive_scan(s->near_cnt.as<int>(), s->near_off.as<int>(), nb + 1,
+                                   s->scan_tmp.p, tmp, c->stream));
+  int tot[2] = {0, 0};
+  int eflag = 0;
+  HIP_TRY(hipMemcpyAsync(&tot[0], s->far_off.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(&tot[1], s->near_off.as<int>() + nb, sizeof(int),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&s->hstats, s->lstats.p, sizeof(bit::dev::ListStats),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&eflag, s->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (eflag) return fail(BIT_E_STATE, "traversal stack overflow");
+  s->n_far_items = tot[0];
+  s->n_near_items = tot[1];
+  BIT_TRY(s->far_items.ensure(c, (size_t)std::max(1, tot[0]) * sizeof(bit::dev::Item)));
+  BIT_TRY(s->near_items.ensure(c, (size_t)std::max(1, tot[1]) * sizeof(bit::dev::Item)));
+  HIP_TRY(bit::dev::lists_fill(s->view(), mp, s->stack_cap, s->far_off.as<int>(),
+                               s->near_off.as<int>(), s->far_items.as<bit::dev::Item>(),
+                               s->near_items.as<bit::dev::Item>(), s->err_flag.as<int>(),
+                               c->stream));
+  s->t_lists.stop(c->stream);
+  s->lists_valid = true;
+  s->l_acc = p->acc;
+  s->l_delta2 = p->delta2;
+  s->l_sub = p->mac_sub;
+  s->stats.far_items = tot[0];
+  s->stats.near_items = tot[1];
+  s->stats.far_pairs = (int64_t)s->hstats.far_pairs;
+  s->stats.near_pairs = (int64_t)s->hstats.near_pairs;
+  s->stats.lane_far = 64 * (int64_t)tot[0];
+  s->stats.lane_near_pairs = 64 * (int64_t)s->hstats.near_slots;
+  s->stats.ms_lists = -1.0;  // filled after the next synchronisation
+  return BIT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void bit_params_default(bit_params* p) {
+  p->nterms = 6;
+  p->acc = 0.5;
+  p->kappa = 0.0;
+  p->delta2 = 0.0;
+  p->mac_sub = 0.0;
+  p->precision = BIT_FP64;
+}
+
+const char* bit_last_error(void) { return g_err.c_str(); }
+
+int bit_ctx_create(int device, size_t hbm_budget_bytes, bit_ctx** out) {
+  if (!out) return fail(BIT_E_INVALID, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return fail(BIT_E_INVALID, "device " + std::to_string(device) + " not present (" +
+                                   std::to_string(ndev) + " visible)");
+  HIP_TRY(hipSetDevice(device));
+  bit_ctx* c = new bit_ctx();
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(BIT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  c->stream = c->own;
+  *out = c;
+  return BIT_OK;
+}
+
+int bit_ctx_destroy(bit_ctx* c) {
+  if (!c) return BIT_OK;
+  (void)hipStreamSynchronize(c->stream);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+  return BIT_OK;
+}
+
+int bit_ctx_set_stream(bit_ctx* c, void* stream) {
+  if (!c) return fail(BIT_E_INVALID, "null context");
+  c->stream = stream ? static_cast<hipStream_t>(stream) : c->own;
+  return BIT_OK;
+}
+
+int bit_ctx_sync(bit_ctx* c) {
+  if (!c) return fail(BIT_E_INVALID, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
+size_t bit_ctx_bytes_in_use(const bit_ctx* c) { return c ? c->in_use : 0; }
+
+int bit_ctx_inject_alloc_failures(bit_ctx* c, int k) {
+  if (!c) return fail(BIT_E_INVALID, "null context");
+  c->inject = k < 0 ? 0 : k;
+  return BIT_OK;
+}
+
+static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, const double* z,
+                         const double* q, int maxmemb, bit_system** out, bool device_src) {
+  if (!ctx || !out) return fail(BIT_E_INVALID, "null context or out");
+  *out = nullptr;
+  if (n < 1) return fail(BIT_E_INVALID, "n must be >= 1");
+  if (maxmemb < 1) return fail(BIT_E_INVALID, "maxmemb must be >= 1");
+  if (!x || !y || !z || !q) return fail(BIT_E_INVALID, "null coordinate or charge array");
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::vector<double> hx, hy, hz, hq;
+  const double *px = x, *py = y, *pz = z, *pq = q;
+  if (device_src) {
+    hx.resize(n);
+    hy.resize(n);
+    hz.resize(n);
+    hq.resize(n);
+    HIP_TRY(hipMemcpyAsync(hx.data(), x, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hy.data(), y, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hz.data(), z, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hq.data(), q, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    px = hx.data();
+    py = hy.data();
+    pz = hz.data();
+    pq = hq.data();
+  }
+  if (!finite_all(n, px) || !finite_all(n, py) || !finite_all(n, pz) || !finite_all(n, pq))
+    return fail(BIT_E_INVALID, "non-finite particle coordinate or charge");
+  bit_system* s = new bit_system();
+  s->ctx = ctx;
+  auto t0 = std::chrono::steady_clock::now();
+  std::string err;
+  if (!bit::build_tree(n, px, py, pz, maxmemb, s->tree, err)) {
+    delete s;
+    return fail(BIT_E_TREE, err);
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  s->stats.ms_tree = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  int rc;
+  if (device_src) {
+    const double* src[4] = {x, y, z, q};
+    rc = finish_system(ctx, s, nullptr, src);
+  } else {
+    std::vector<double> a(n), b(n), c(n), d(n);
+    const std::vector<int>& P = s->tree.perm;
+    for (int i = 0; i < n; i++) {
+      a[i] = px[P[i]];
+      b[i] = py[P[i]];
+      c[i] = pz[P[i]];
+      d[i] = pq[P[i]];
+    }
+    const double* srt[4] = {a.data(), b.data(), c.data(), d.data()};
+    rc = finish_system(ctx, s, srt, nullptr);
+    if (rc == BIT_OK) {
+      hipError_t e = hipStreamSynchronize(ctx->stream);  // host staging dies here
+      if (e != hipSuccess) rc = fail(BIT_E_HIP, hipGetErrorString(e));
+    }
+  }
+  if (rc != BIT_OK) {
+    delete s;
+    return rc;
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  s->stats.ms_upload = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  *out = s;
+  return BIT_OK;
+}
+
+int bit_system_create(bit_ctx* ctx, int n, const double* x, const double* y, const double* z,
+                      const double* q, int maxmemb, bit_system** out) {
+  return create_common(ctx, n, x, y, z, q, maxmemb, out, false);
+}
+
+int bit_system_create_device(bit_ctx* ctx, int n, const double* x, const double* y,
+                             const double* z, const double* q, int maxmemb, bit_system** out) {
+  return create_common(ctx, n, x, y, z, q, maxmemb, out, true);
+}
+
+int bit_system_destroy(bit_system* s) {
+  if (!s) return BIT_OK;
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return BIT_OK;
+}
+
+int bit_system_numtree(const bit_system* s) { return s ? s->tree.numtree : 0; }
+int bit_system_maxdepth(const bit_system* s) { return s ? s->tree.maxdepth : 0; }
+
+static void export_host_tree(const bit::HostTree& T, double* boxes, int* children, int* begin,
+                             int* end, int* perm) {
+  if (boxes) std::memcpy(boxes, T.box.data(), T.box.size() * sizeof(double));
+  if (children) std::memcpy(children, T.children.data(), T.children.size() * sizeof(int));
+  if (begin) std::memcpy(begin, T.begin.data(), T.begin.size() * sizeof(int));
+  if (end) std::memcpy(end, T.end.data(), T.end.size() * sizeof(int));
+  if (perm) std::memcpy(perm, T.perm.data(), T.perm.size() * sizeof(int));
+}
+
+int bit_tree_build(int n, const double* x, const double* y, const double* z, int maxmemb,
+                   bit_tree** out) {
+  if (!out) return fail(BIT_E_INVALID, "null out");
+  *out = nullptr;
+  if (n < 1 || !x || !y || !z) return fail(BIT_E_INVALID, "need n >= 1 and coordinates");
+  if (!finite_all(n, x) || !finite_all(n, y) || !finite_all(n, z))
+    return fail(BIT_E_INVALID, "non-finite particle coordinate");
+  bit_tree* t = new bit_tree();
+  std::string err;
+  if (!bit::build_tree(n, x, y, z, maxmemb, t->tree, err)) {
+    delete t;
+    return fail(BIT_E_TREE, err);
+  }
+  *out = t;
+  return BIT_OK;
+}
+
+int bit_tree_destroy(bit_tree* t) {
+  delete t;
+  return BIT_OK;
+}
+
+int bit_tree_numtree(const bit_tree* t) { return t ? t->tree.numtree : 0; }
+int bit_tree_maxdepth(const bit_tree* t) { return t ? t->tree.maxdepth : 0; }
+
+int bit_tree_export(const bit_tree* t, double* boxes, int* children, int* begin, int* end,
+                    int* perm) {
+  if (!t) return fail(BIT_E_INVALID, "null tree");
+  export_host_tree(t->tree, boxes, children, begin, end, perm);
+  return BIT_OK;
+}
+
+int bit_system_export_tree(const bit_system* s, double* boxes, int* children, int* begin,
+                           int* end, int* perm) {
+  if (!s) return fail(BIT_E_INVALID, "null system");
+  export_host_tree(s->tree, boxes, children, begin, end, perm);
+  return BIT_OK;
+}
+
+int bit_moments(bit_system* s, int nterms) {
+  if (!s) return fail(BIT_E_INVALID, "null system");
+  if (nterms < 1 || nterms > BIT_MAX_NTERMS)
+    return fail(BIT_E_INVALID, "nterms must be in [1, " + std::to_string(BIT_MAX_NTERMS) + "]");
+  if (s->mom_nterms == nterms) return BIT_OK;
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const bit::HostTree& T = s->tree;
+  const int nt = T.numtree, nc = ncoef(nterms);
+  if (s->nchunks == 0) {
+    std::vector<int> cn, c0(nt), ck(nt), cr;
+    for (int i = 0; i < nt; i++) {
+      c0[i] = (int)cn.size();
+      int b = T.begin[i], e = T.end[i];
+      int k = 0;
+      for (int x = b; x < e; x += kMomChunk, k++) {
+        cn.push_back(i);
+        cr.push_back(x);
+        cr.push_back(std::min(e, x + kMomChunk));
+      }
+      ck[i] = k;
+    }
+    s->nchunks = (int)cn.size();
+    BIT_TRY(upload(c, s->chunk_node, cn.data(), cn.size() * sizeof(int)));
+    BIT_TRY(upload(c, s->chunk_rng, cr.data(), cr.size() * sizeof(int)));
+    BIT_TRY(upload(c, s->node_chunk0, c0.data(), c0.size() * sizeof(int)));
+    BIT_TRY(upload(c, s->node_nchunk, ck.data(), ck.size() * sizeof(int)));
+  }
+  BIT_TRY(s->partial.ensure(c, (size_t)s->nchunks * nc * sizeof(double)));
+  BIT_TRY(s->mom.ensure(c, (size_t)nt * nc * sizeof(double)));
+  BIT_TRY(s->mom_hat64.ensure(c, (size_t)nt * nc * sizeof(double)));
+  BIT_TRY(s->mom_hat32.ensure(c, (size_t)nt * nc * sizeof(float)));
+  s->t_mom.start(c->stream);
+  HIP_TRY(bit::dev::moments_partial(s->view(), nterms, s->nchunks, s->chunk_node.as<int>(),
+                                    s->chunk_rng.as<int2>(), s->partial.as<double>(),
+                                    c->stream));
+  HIP_TRY(bit::dev::moments_reduce(s->view(), nterms, s->node_chunk0.as<int>(),
+                                   s->node_nchunk.as<int>(), s->partial.as<double>(),
+                                   s->mom.as<double>(), s->mom_hat64.as<double>(),
+                                   s->mom_hat32.as<float>(), c->stream));
+  s->t_mom.stop(c->stream);
+  s->mom_nterms = nterms;
+  s->stats.ms_moments = -1.0;
+  return BIT_OK;
+}
+
+int bit_moments_download(const bit_system* s, double* out) {
+  if (!s || !out) return fail(BIT_E_INVALID, "null system or output");
+  if (s->mom_nterms == 0) return fail(BIT_E_STATE, "moments not computed");
+  const int nt = s->tree.numtree, p = s->mom_nterms, nc = ncoef(p);
+  std::vector<double> packed((size_t)nt * nc);
+  HIP_TRY(hipMemcpyAsync(packed.data(), s->mom.p, packed.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  const size_t cube = (size_t)p * p * p;
+  for (int t = 0; t < nt; t++) {
+    double* M = out + cube * t;
+    std::fill(M, M + cube, 0.0);
+    for (int n = 0; n < p; n++)
+      for (int m = 0; m <= n; m++)
+        for (int k = 0; k <= m; k++) {
+          int i = n - m, j = m - k;
+          int c = n * (n + 1) * (n + 2) / 6 + m * (m + 1) / 2 + k;
+          M[((size_t)i * p + j) * p + k] = packed[(size_t)t * nc + c];
+        }
+  }
+  return BIT_OK;
+}
+
+static void collect_times(bit_system* s) {
+  if (s->stats.ms_moments < 0) s->stats.ms_moments = s->t_mom.ms();
+  if (s->stats.ms_lists < 0) s->stats.ms_lists = s->t_lists.ms();
+  s->stats.ms_far = s->t_far.ms();
+  s->stats.ms_near = s->t_near.ms();
+}
+
+int bit_eval_potential(bit_system* s, const bit_params* p, double* out, int out_on_device) {
+  BIT_TRY(check_params(s, p));
+  if (!out) return fail(BIT_E_INVALID, "null output");
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = s->tree.n;
+  BIT_TRY(bit_moments(s, p->nterms));
+  BIT_TRY(ensure_lists(s, p));
+  BIT_TRY(s->phi_far.ensure(c, (size_t)n * sizeof(double)));
+  double* dst = out;
+  if (!out_on_device) {
+    BIT_TRY(s->out.ensure(c, (size_t)n * sizeof(double)));
+    dst = s->out.as<double>();
+  }
+  const bool fp32 = p->precision == BIT_FP32;
+  s->t_far.start(c->stream);
+  HIP_TRY(bit::dev::far_field(s->view(), p->nterms, p->kappa, p->delta2, fp32,
+                              s->mom_hat64.as<double>(), s->mom_hat32.as<float>(),
+                              s->far_off.as<int>(), s->far_items.as<bit::dev::Item>(),
+                              s->phi_far.as<double>(), c->stream));
+  s->t_far.stop(c->stream);
+  s->t_near.start(c->stream);
+  HIP_TRY(bit::dev::near_field(s->view(), p->kappa, p->delta2, fp32, s->near_off.as<int>(),
+                               s->near_items.as<bit::dev::Item>(), s->phi_far.as<double>(), dst,
+                               c->stream));
+  s->t_near.stop(c->stream);
+  if (!out_on_device)
+    HIP_TRY(hipMemcpyAsync(out, dst, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  collect_times(s);
+  return BIT_OK;
+}
+
+int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_device) {
+  BIT_TRY(check_params(s, p));
+  if (!out) return fail(BIT_E_INVALID, "null output");
+  if (p->kappa != 0.0) return fail(BIT_E_INVALID, "field evaluation supports kappa = 0 only");
+  if (p->nterms > BIT_MAX_NTERMS - 1)
+    return fail(BIT_E_INVALID, "field needs nterms <= " + std::to_string(BIT_MAX_NTERMS - 1));
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = s->tree.n;
+  BIT_TRY(bit_moments(s, p->nterms));
+  BIT_TRY(ensure_lists(s, p));
+  BIT_TRY(s->phi_far.ensure(c, (size_t)3 * n * sizeof(double)));
+  double* dst = out;
+  if (!out_on_device) {
+    BIT_TRY(s->out.ensure(c, (size_t)3 * n * sizeof(double)));
+    dst = s->out.as<double>();
+  }
+  s->t_far.start(c->stream);
+  HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, s->mom_hat64.as<double>(),
+                                   s->far_off.as<int>(), s->far_items.as<bit::dev::Item>(),
+                                   s->phi_far.as<double>(), c->stream));
+  s->t_far.stop(c->stream);
+  s->t_near.start(c->stream);
+  HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, s->near_off.as<int>(),
+                                    s->near_items.as<bit::dev::Item>(), s->phi_far.as<double>(),
+                                    dst, c->stream));
+  s->t_near.stop(c->stream);
+  if (!out_on_device)
+    HIP_TRY(hipMemcpyAsync(out, dst, (size_t)3 * n * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  collect_times(s);
+  return BIT_OK;
+}
+
+int bit_system_stats(const bit_system* s, bit_stats* out) {
+  if (!s || !out) return fail(BIT_E_INVALID, "null system or output");
+  *out = s->stats;
+  return BIT_OK;
+}
+
+int bit_system_export_lists(bit_system* s, const bit_params* p, int* nbatch, int* nfar,
+                            int* nnear, int* far_off, uint64_t* far_mask, int* far_node,
+                            int* near_off, uint64_t* near_mask, int* near_node) {
+  BIT_TRY(check_params(s, p));
+  if (!nbatch || !nfar || !nnear) return fail(BIT_E_INVALID, "null size outputs");
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  BIT_TRY(ensure_lists(s, p));
+  const int nb = (s->tree.n + bit::dev::kBatch - 1) / bit::dev::kBatch;
+  *nbatch = nb;
+  *nfar = s->n_far_items;
+  *nnear = s->n_near_items;
+  if (!far_off) return BIT_OK;
+  std::vector<bit::dev::Item> f(std::max(1, s->n_far_items)), n(std::max(1, s->n_near_items));
+  HIP_TRY(hipMemcpyAsync(far_off, s->far_off.p, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(near_off, s->near_off.p, (nb + 1) * sizeof(int),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(f.data(), s->far_items.p, s->n_far_items * sizeof(bit::dev::Item),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(n.data(), s->near_items.p, s->n_near_items * sizeof(bit::dev::Item),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < s->n_far_items; i++) {
+    far_mask[i] = f[i].mask;
+    far_node[i] = f[i].node;
+  }
+  for (int i = 0; i < s->n_near_items; i++) {
+    near_mask[i] = n[i].mask;
+    near_node[i] = n[i].node;
+  }
+  return BIT_OK;
+}
+
+int bit_direct_potential(bit_system* s, double kappa, double delta2, int ntargets,
+                         const int* tidx, double* out) {
+  if (!s || (!tidx && ntargets > 0) || (!out && ntargets > 0))
+    return fail(BIT_E_INVALID, "null argument");
+  if (ntargets < 0) return fail(BIT_E_INVALID, "ntargets < 0");
+  if (ntargets == 0) return BIT_OK;
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = s->tree.n;
+  std::vector<int> inv(n), slot(ntargets);
+  for (int i = 0; i < n; i++) inv[s->tree.perm[i]] = i;
+  for (int t = 0; t < ntargets; t++) {
+    if (tidx[t] < 0 || tidx[t] >= n) return fail(BIT_E_INVALID, "target index out of range");
+    slot[t] = inv[tidx[t]];
+  }
+  int tblocks = (ntargets + 255) / 256;
+  int nsplit = std::max(1, std::min(1024, 4096 / tblocks));
+  nsplit = std::min(nsplit, std::max(1, n / 1024));
+  BIT_TRY(upload(c, s->tslot, slot.data(), slot.size() * sizeof(int)));
+  BIT_TRY(s->dpart.ensure(c, (size_t)nsplit * ntargets * sizeof(double)));
+  BIT_TRY(s->dout.ensure(c, (size_t)ntargets * sizeof(double)));
+  HIP_TRY(bit::dev::direct_potential(s->view(), kappa, delta2, ntargets, s->tslot.as<int>(),
+                                     nsplit, s->dpart.as<double>(), s->dout.as<double>(),
+                                     c->stream));
+  HIP_TRY(hipMemcpyAsync(out, s->dout.p, (size_t)ntargets * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
+}  // extern "C"

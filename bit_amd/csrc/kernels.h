@@ -1,0 +1,94 @@
+// kernels.h -- launchers of the CDNA4 kernels of the treecode (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bit {
+namespace dev {
+
+constexpr int kBatch = 64;     // targets per batch = one wavefront
+constexpr int kMaxNterms = 13; // BIT_MAX_NTERMS
+
+// One entry of a batch interaction list: the cluster/leaf `node` applies to
+// the batch's lanes set in `mask` (exactly those targets for which the
+// reference traversal reaches that node and takes that branch).
+struct Item {
+  unsigned long long mask;
+  int node;
+  int pad;
+};
+
+// Per-system device view of the tree and the sorted sources.
+struct TreeView {
+  int n;        // particles
+  int numtree;  // nodes
+  const double4* center;  // (xmid, ymid, zmid, sqradius) per node
+  const int* child;       // 8 per node, -1 = none
+  const int2* range;      // [begin, end) into the sorted arrays
+  const double* scale;    // s = sqrt(sqradius) (1 when 0) per node
+  const double* sx;       // sorted sources, FP64
+  const double* sy;
+  const double* sz;
+  const double* sq;
+  const float4* sloc;     // sorted sources in leaf-centre coordinates, FP32 (x,y,z,q)
+  const int* perm;        // sorted slot -> original index
+};
+
+struct MacParams {
+  double acc;
+  double delta2;
+  double mac_sub;
+};
+
+// counters accumulated by the list-count pass
+struct ListStats {
+  unsigned long long far_items, near_items, far_pairs, near_pairs, near_slots;
+};
+
+hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,
+                          const double* z, const double* q, double* sx, double* sy,
+                          double* sz, double* sq, hipStream_t s);
+hipError_t leaf_local(const TreeView& tv, const int* leaves, int nleaves, float4* sloc,
+                      hipStream_t s);
+
+hipError_t moments_partial(const TreeView& tv, int nterms, int nchunks, const int* chunk_node,
+                           const int2* chunk_rng, double* partial, hipStream_t s);
+hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0,
+                          const int* node_nchunk, const double* partial, double* mom,
+                          double* mom_hat64, float* mom_hat32, hipStream_t s);
+
+hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
+                       int* near_cnt, ListStats* stats, int* err_flag, hipStream_t s);
+hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,
+                      const int* near_off, Item* far_items, Item* near_items, int* err_flag,
+                      hipStream_t s);
+hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp_bytes,
+                          hipStream_t s);
+
+// far field: phi_far[sorted t]; fp32 selects the FP32 + Kahan kernel
+hipError_t far_field(const TreeView& tv, int nterms, double kappa, double delta2, bool fp32,
+                     const double* mom_hat64, const float* mom_hat32, const int* far_off,
+                     const Item* far_items, double* phi_far, hipStream_t s);
+// near field + combine: out[perm[t]] = phi_far[t] + near(t)
+hipError_t near_field(const TreeView& tv, double kappa, double delta2, bool fp32,
+                      const int* near_off, const Item* near_items, const double* phi_far,
+                      double* out, hipStream_t s);
+
+// field (Coulomb, regularized): E_far sorted [3n], then near + scatter [3n]
+hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, const double* mom_hat64,
+                          const int* far_off, const Item* far_items, double* e_far,
+                          hipStream_t s);
+hipError_t near_field_grad(const TreeView& tv, double delta2, const int* near_off,
+                           const Item* near_items, const double* e_far, double* out,
+                           hipStream_t s);
+
+// FP64 direct sum at ntargets sorted target slots; partial sums over
+// nsplit source chunks, then reduced in order.
+hipError_t direct_potential(const TreeView& tv, double kappa, double delta2, int ntargets,
+                            const int* tslot, int nsplit, double* partial, double* out,
+                            hipStream_t s);
+
+}  // namespace dev
+}  // namespace bit

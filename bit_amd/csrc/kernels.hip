@@ -1,0 +1,937 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the particle-cluster treecode.
+//
+// Layout in HBM (built by bit_api.cpp):
+//   * sources sorted so every tree node is a contiguous slot range; FP64 SoA
+//     (sx, sy, sz, sq) and an FP32 float4 copy in leaf-centre coordinates;
+//   * node table SoA: double4 centre+sqradius, int[8] children, int2 range,
+//     double scale s = sqrt(sqradius);
+//   * moments packed by total-degree shell, T(p) = C(p+3,3) per node, FP64,
+//     plus copies scaled by s^-|k| in FP64 and FP32;
+//   * interaction lists per batch of 64 consecutive sorted targets: (mask,
+//     node) items, far and near, exactly the reference traversal's choices.
+// Work mapping: one wavefront per batch, one lane per target.  Every lane owns
+// its target's sums, so a target's result does not depend on what else runs.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <utility>
+
+#include "kernels.h"
+
+namespace bit {
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// Coefficient tables.  Index of multi-index (i,j,k), n = i+j+k:
+//   n(n+1)(n+2)/6 + (j+k)(j+k+1)/2 + k
+// i.e. shells of increasing degree, inside a shell i descending, then k
+// ascending.  For every coefficient the table holds the direction d of the
+// one-direction recurrence (first index equal to 1, else first nonzero index:
+// the choice of the reference's index families, tree3d.cpp:3075-3214) and the
+// positions of k-e_e and k-2e_e.
+// ---------------------------------------------------------------------------
+constexpr int cidx(int i, int j, int k) {
+  int n = i + j + k, m = j + k;
+  return n * (n + 1) * (n + 2) / 6 + m * (m + 1) / 2 + k;
+}
+constexpr int ncoef(int nt) { return nt * (nt + 1) * (nt + 2) / 6; }
+
+template <int NT>
+struct Tab {
+  static constexpr int NC = ncoef(NT);
+  int k[NC][3];
+  int d[NC];
+  int m1[NC][3];
+  int m2[NC][3];
+  constexpr Tab() : k(), d(), m1(), m2() {
+    for (int n = 0; n < NT; n++)
+      for (int m = 0; m <= n; m++)
+        for (int kk = 0; kk <= m; kk++) {
+          int i = n - m, j = m - kk;
+          int c = cidx(i, j, kk);
+          int kv[3] = {i, j, kk};
+          for (int e = 0; e < 3; e++) k[c][e] = kv[e];
+          int dd = -1;
+          for (int e = 0; e < 3 && dd < 0; e++)
+            if (kv[e] == 1) dd = e;
+          for (int e = 0; e < 3 && dd < 0; e++)
+            if (kv[e] > 0) dd = e;
+          d[c] = dd < 0 ? 0 : dd;
+          for (int e = 0; e < 3; e++) {
+            int a1[3] = {i, j, kk}, a2[3] = {i, j, kk};
+            a1[e] -= 1;
+            a2[e] -= 2;
+            m1[c][e] = kv[e] >= 1 ? cidx(a1[0], a1[1], a1[2]) : -1;
+            m2[c][e] = kv[e] >= 2 ? cidx(a2[0], a2[1], a2[2]) : -1;
+          }
+        }
+  }
+};
+template <int NT>
+inline constexpr Tab<NT> kTab{};
+
+// One coefficient of the recurrence (scaled variables, see far_eval):
+//   V   = x_d a_{k-e_d} - a_{k-2e_d}
+//   b_k = (kappa/k_d) V
+//   a_k = fac [ sum_e (2 x_e a_{k-e} - a_{k-2e}) - V/k_d
+//               + (kappa/k_d)(x_d b_{k-e_d} - b_{k-2e_d}) ]
+// b holds the coefficients of exp(-kappa R); a those of exp(-kappa R)/R.
+// Each coefficient is folded into the result as soon as it exists, so only
+// the last two degree shells stay live in registers:
+//   GRAD = false: acc0 += M_k a_k                      (potential)
+//   GRAD = true:  acc_e += k_e M_{k-e} a_k, e = x,y,z   (field, M of degree < NT-1)
+template <typename T, int NT, bool SCR, bool GRAD, int C>
+__device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (&x2)[3],
+                                          T fac, const T* kinv, const T* __restrict__ M,
+                                          T (&acc)[3]) {
+  constexpr const Tab<NT>& tb = kTab<NT>;
+  constexpr int d = tb.d[C];
+  constexpr int kd = tb.k[C][d];
+  constexpr int e0 = tb.m1[C][0], e1 = tb.m1[C][1], e2 = tb.m1[C][2];
+  constexpr int f0 = tb.m2[C][0], f1 = tb.m2[C][1], f2 = tb.m2[C][2];
+  constexpr int md1 = tb.m1[C][d], md2 = tb.m2[C][d];
+  T S;
+  if constexpr (e0 >= 0) {
+    S = x2[0] * a[e0];
+    if constexpr (e1 >= 0) S = fma(x2[1], a[e1], S);
+    if constexpr (e2 >= 0) S = fma(x2[2], a[e2], S);
+  } else if constexpr (e1 >= 0) {
+    S = x2[1] * a[e1];
+    if constexpr (e2 >= 0) S = fma(x2[2], a[e2], S);
+  } else {
+    S = x2[2] * a[e2];
+  }
+  if constexpr (f0 >= 0) S -= a[f0];
+  if constexpr (f1 >= 0) S -= a[f1];
+  if constexpr (f2 >= 0) S -= a[f2];
+  T V;
+  if constexpr (md2 >= 0) V = fma(x[d], a[md1], -a[md2]);
+  else V = x[d] * a[md1];
+  constexpr double invk = 1.0 / kd;
+  T t = fma(T(-invk), V, S);
+  if constexpr (SCR) {
+    T W;
+    if constexpr (md2 >= 0) W = fma(x[d], b[md1], -b[md2]);
+    else W = x[d] * b[md1];
+    t = fma(kinv[kd], W, t);
+    b[C] = kinv[kd] * V;
+  }
+  a[C] = fac * t;
+  if constexpr (!GRAD) {
+    acc[0] = fma(M[C], a[C], acc[0]);
+  } else {
+    if constexpr (e0 >= 0) acc[0] = fma(T(tb.k[C][0]) * M[e0], a[C], acc[0]);
+    if constexpr (e1 >= 0) acc[1] = fma(T(tb.k[C][1]) * M[e1], a[C], acc[1]);
+    if constexpr (e2 >= 0) acc[2] = fma(T(tb.k[C][2]) * M[e2], a[C], acc[2]);
+  }
+}
+
+template <typename T, int NT, bool SCR, bool GRAD, int... C>
+__device__ __forceinline__ void coef_all(std::integer_sequence<int, C...>, T* a, T* b,
+                                         const T (&x)[3], const T (&x2)[3], T fac,
+                                         const T* kinv, const T* __restrict__ M,
+                                         T (&acc)[3]) {
+  (coef_step<T, NT, SCR, GRAD, C + 1>(a, b, x, x2, fac, kinv, M, acc), ...);
+}
+
+// Far field of one cluster at one target, in units of the cluster scale s:
+// x = (target - centre)/s, fac = s^2/R^2, a0 = exp(-kh Rh)/Rh, b0 = exp(-kh Rh),
+// kh = kappa s.  Returns sum_k Mhat_k ahat_k (times 1/s = the potential).
+template <typename T, int NT, bool SCR>
+__device__ __forceinline__ T far_eval(T x0, T x1, T x2_, T fac, T a0, T b0, T kh,
+                                      const T* __restrict__ M) {
+  constexpr int NC = ncoef(NT);
+  T a[NC];
+  T b[SCR ? NC : 1];
+  a[0] = a0;
+  if constexpr (SCR) b[0] = b0;
+  T kinv[NT > 1 ? NT : 2];
+  if constexpr (SCR) {
+#pragma unroll
+    for (int m = 1; m < NT; m++) kinv[m] = kh * T(1.0 / m);
+  }
+  const T x[3] = {x0, x1, x2_};
+  const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
+  T acc[3] = {M[0] * a0, T(0), T(0)};
+  if constexpr (NC > 1)
+    coef_all<T, NT, SCR, false>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, fac,
+                                kinv, M, acc);
+  return acc[0];
+}
+
+// Field: E_e = sum_k (k_e+1) M_k a_{k+e}, coefficients to degree NT (Coulomb).
+template <typename T, int NT>
+__device__ __forceinline__ void far_eval_grad(T x0, T x1, T x2_, T fac, T a0,
+                                              const T* __restrict__ M, T& ex, T& ey, T& ez) {
+  constexpr int NC = ncoef(NT + 1);
+  T a[NC];
+  T b[1];
+  a[0] = a0;
+  const T x[3] = {x0, x1, x2_};
+  const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
+  T kinv[2] = {T(0), T(0)};
+  T acc[3] = {T(0), T(0), T(0)};
+  coef_all<T, NT + 1, false, true>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, fac,
+                                   kinv, M, acc);
+  ex = acc[0];
+  ey = acc[1];
+  ez = acc[2];
+}
+
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long v) {
+  unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// ---------------------------------------------------------------------------
+// Sorting the sources
+// ---------------------------------------------------------------------------
+__global__ void k_gather(int n, const int* __restrict__ perm, const double* __restrict__ x,
+                         const double* __restrict__ y, const double* __restrict__ z,
+                         const double* __restrict__ q, double* sx, double* sy, double* sz,
+                         double* sq) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int p = perm[t];
+  sx[t] = x[p];
+  sy[t] = y[p];
+  sz[t] = z[p];
+  sq[t] = q[p];
+}
+
+hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,
+                          const double* z, const double* q, double* sx, double* sy,
+                          double* sz, double* sq, hipStream_t s) {
+  int blocks = (tv.n + 255) / 256;
+  hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, tv.n, tv.perm, x, y, z, q, sx,
+                     sy, sz, sq);
+  return hipGetLastError();
+}
+
+// One block per leaf: FP32 source copy relative to the leaf's box midpoint.
+__global__ void k_leaf_local(const int* __restrict__ leaves, const double4* __restrict__ center,
+                             const int2* __restrict__ range, const double* __restrict__ sx,
+                             const double* __restrict__ sy, const double* __restrict__ sz,
+                             const double* __restrict__ sq, float4* sloc) {
+  int leaf = leaves[blockIdx.x];
+  double4 c = center[leaf];
+  int2 r = range[leaf];
+  for (int t = r.x + threadIdx.x; t < r.y; t += blockDim.x) {
+    sloc[t] = make_float4((float)(sx[t] - c.x), (float)(sy[t] - c.y), (float)(sz[t] - c.z),
+                          (float)sq[t]);
+  }
+}
+
+hipError_t leaf_local(const TreeView& tv, const int* leaves, int nleaves, float4* sloc,
+                      hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_leaf_local, dim3(nleaves), dim3(256), 0, s, leaves, tv.center, tv.range,
+                     tv.sx, tv.sy, tv.sz, tv.sq, sloc);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Moments (treemoments, tree3d.cpp:2631-2711), FP64.  A chunk is a slice of one
+// node's slot range; 256 threads, one coefficient per thread (two above 256).
+// Powers q*dx^i, dy^j, dz^k of 64 particles at a time are staged in LDS.
+// Partials are summed per node in chunk order (deterministic).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_moments_partial(
+    int nterms, const int* __restrict__ chunk_node, const int2* __restrict__ chunk_rng,
+    const double4* __restrict__ center, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ sz,
+    const double* __restrict__ sq, double* __restrict__ partial) {
+  __shared__ double PX[64][kMaxNterms + 1];
+  __shared__ double PY[64][kMaxNterms + 1];
+  __shared__ double PZ[64][kMaxNterms + 1];
+  const int ch = blockIdx.x;
+  const int node = chunk_node[ch];
+  const int2 r = chunk_rng[ch];
+  const double4 c = center[node];
+  const int nc = ncoef(nterms);
+  const int tid = threadIdx.x;
+  // coefficient(s) of this thread
+  int ci[2][3];
+  int cnt = 0;
+  for (int cc = tid; cc < nc && cnt < 2; cc += 256, cnt++) {
+    // invert the packed index
+    int n = 0;
+    while (ncoef(n + 1) <= cc) n++;
+    int rem = cc - ncoef(n);
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= rem) m++;
+    int k = rem - m * (m + 1) / 2;
+    ci[cnt][0] = n - m;
+    ci[cnt][1] = m - k;
+    ci[cnt][2] = k;
+  }
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int base = r.x; base < r.y; base += 64) {
+    if (tid < 64) {
+      int j = base + tid;
+      double px = 0.0, py = 0.0, pz = 0.0;
+      if (j < r.y) {
+        px = sq[j];
+        py = 1.0;
+        pz = 1.0;
+      }
+      double dx = j < r.y ? sx[j] - c.x : 0.0;
+      double dy = j < r.y ? sy[j] - c.y : 0.0;
+      double dz = j < r.y ? sz[j] - c.z : 0.0;
+      for (int k = 0; k < nterms; k++) {
+        PX[tid][k] = px;
+        PY[tid][k] = py;
+        PZ[tid][k] = pz;
+        px *= dx;
+        py *= dy;
+        pz *= dz;
+      }
+    }
+    __syncthreads();
+    if (cnt > 0) {
+      const int i0 = ci[0][0], j0 = ci[0][1], k0 = ci[0][2];
+#pragma unroll 8
+      for (int m = 0; m < 64; m++) acc0 = fma(PX[m][i0] * PY[m][j0], PZ[m][k0], acc0);
+    }
+    if (cnt > 1) {
+      const int i1 = ci[1][0], j1 = ci[1][1], k1 = ci[1][2];
+#pragma unroll 8
+      for (int m = 0; m < 64; m++) acc1 = fma(PX[m][i1] * PY[m][j1], PZ[m][k1], acc1);
+    }
+    __syncthreads();
+  }
+  if (cnt > 0) partial[(size_t)ch * nc + tid] = acc0;
+  if (cnt > 1) partial[(size_t)ch * nc + tid + 256] = acc1;
+}
+
+hipError_t moments_partial(const TreeView& tv, int nterms, int nchunks, const int* chunk_node,
+                           const int2* chunk_rng, double* partial, hipStream_t s) {
+  if (nchunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_moments_partial, dim3(nchunks), dim3(256), 0, s, nterms, chunk_node,
+                     chunk_rng, tv.center, tv.sx, tv.sy, tv.sz, tv.sq, partial);
+  return hipGetLastError();
+}
+
+__global__ void k_moments_reduce(int numtree, int nterms, const int* __restrict__ node_chunk0,
+                                 const int* __restrict__ node_nchunk,
+                                 const double* __restrict__ scale,
+                                 const double* __restrict__ partial, double* mom,
+                                 double* mom_hat64, float* mom_hat32) {
+  const int nc = ncoef(nterms);
+  long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)numtree * nc) return;
+  int node = (int)(g / nc), c = (int)(g % nc);
+  int c0 = node_chunk0[node], k = node_nchunk[node];
+  double s = 0.0;
+  for (int i = 0; i < k; i++) s += partial[(size_t)(c0 + i) * nc + c];
+  int n = 0;
+  while (ncoef(n + 1) <= c) n++;
+  double inv = 1.0 / scale[node], f = 1.0;
+  for (int i = 0; i < n; i++) f *= inv;
+  mom[g] = s;
+  mom_hat64[g] = s * f;
+  mom_hat32[g] = (float)(s * f);
+}
+
+hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0,
+                          const int* node_nchunk, const double* partial, double* mom,
+                          double* mom_hat64, float* mom_hat32, hipStream_t s) {
+  long long total = (long long)tv.numtree * ncoef(nterms);
+  int blocks = (int)((total + 255) / 256);
+  hipLaunchKernelGGL(k_moments_reduce, dim3(blocks), dim3(256), 0, s, tv.numtree, nterms,
+                     node_chunk0, node_nchunk, tv.scale, partial, mom, mom_hat64, mom_hat32);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batch interaction lists.  One wavefront walks the tree for its 64 targets
+// with a shared depth-first stack of (node, mask) entries, children pushed in
+// reverse so they are visited in children[0..7] order.  At each node every
+// lane evaluates the reference MAC (tree3d.cpp:3469-3473 / 4687-4691) in FP64
+// without contraction, so the decisions are bit-identical to the reference:
+//   rs = dx*dx + dy*dy + dz*dz + delta2;  accept = sqradius < acc*(rs - mac_sub)
+// accepted lanes -> far item; rejected lanes at a childless node -> near item.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool mac_accept(double px, double py, double pz, double4 c,
+                                        MacParams mp) {
+#pragma clang fp contract(off)
+  double dx = px - c.x;
+  double dy = py - c.y;
+  double dz = pz - c.z;
+  double rs = dx * dx + dy * dy + dz * dz + mp.delta2;
+  return c.w < mp.acc * (rs - mp.mac_sub);
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int stack_cap,
+                                              int* __restrict__ far_cnt,
+                                              int* __restrict__ near_cnt,
+                                              ListStats* __restrict__ stats,
+                                              const int* __restrict__ far_off,
+                                              const int* __restrict__ near_off,
+                                              Item* __restrict__ far_items,
+                                              Item* __restrict__ near_items, int* err_flag) {
+  extern __shared__ unsigned char smem[];
+  unsigned long long* st_mask = reinterpret_cast<unsigned long long*>(smem);
+  int* st_node = reinterpret_cast<int*>(st_mask + stack_cap);
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.n;
+  const int tt = valid ? t : tv.n - 1;
+  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  const unsigned long long active = ballot(valid);
+  if (lane == 0) {
+    st_node[0] = 0;
+    st_mask[0] = active;
+  }
+  int sp = 1;
+  int nf = 0, nn = 0;
+  unsigned long long far_pairs = 0, near_pairs = 0, near_slots = 0;
+  const int fbase = FILL ? far_off[b] : 0;
+  const int nbase = FILL ? near_off[b] : 0;
+  __syncthreads();
+  while (sp > 0) {
+    sp--;
+    const int node = uniform(st_node[sp]);
+    const unsigned long long m = uniform64(st_mask[sp]);
+    __syncthreads();
+    const double4 c = tv.center[node];
+    const bool acc = mac_accept(px, py, pz, c, mp);
+    const unsigned long long am = ballot(acc) & m;
+    if (am) {
+      if (FILL && lane == 0) far_items[fbase + nf] = Item{am, node, 0};
+      nf++;
+      far_pairs += __popcll(am);
+    }
+    const unsigned long long rm = m & ~am;
+    if (rm) {
+      const int* ch = tv.child + (size_t)node * 8;
+      if (ch[0] < 0) {
+        if (FILL && lane == 0) near_items[nbase + nn] = Item{rm, node, 0};
+        nn++;
+        const int2 r = tv.range[node];
+        near_pairs += (unsigned long long)__popcll(rm) * (unsigned long long)(r.y - r.x);
+        near_slots += (unsigned long long)(r.y - r.x);
+      } else {
+        int nch = 0;
+        while (nch < 8 && ch[nch] >= 0) nch++;
+        if (sp + nch > stack_cap) {
+          if (lane == 0) atomicOr(err_flag, 1);
+          break;
+        }
+        if (lane == 0)
+          for (int k = nch - 1; k >= 0; k--) {
+            st_node[sp + (nch - 1 - k)] = ch[k];
+            st_mask[sp + (nch - 1 - k)] = rm;
+          }
+        sp += nch;
+      }
+    }
+    __syncthreads();
+  }
+  if (!FILL && lane == 0) {
+    far_cnt[b] = nf;
+    near_cnt[b] = nn;
+    atomicAdd(&stats->far_items, (unsigned long long)nf);
+    atomicAdd(&stats->near_items, (unsigned long long)nn);
+    atomicAdd(&stats->far_pairs, far_pairs);
+    atomicAdd(&stats->near_pairs, near_pairs);
+    atomicAdd(&stats->near_slots, near_slots);
+  }
+}
+
+hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
+                       int* near_cnt, ListStats* stats, int* err_flag, hipStream_t s) {
+  int nb = (tv.n + kBatch - 1) / kBatch;
+  size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
+  hipLaunchKernelGGL(k_lists<false>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap, far_cnt,
+                     near_cnt, stats, (const int*)nullptr, (const int*)nullptr,
+                     (Item*)nullptr, (Item*)nullptr, err_flag);
+  return hipGetLastError();
+}
+
+hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,
+                      const int* near_off, Item* far_items, Item* near_items, int* err_flag,
+                      hipStream_t s) {
+  int nb = (tv.n + kBatch - 1) / kBatch;
+  size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
+  hipLaunchKernelGGL(k_lists<true>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
+                     (int*)nullptr, (int*)nullptr, (ListStats*)nullptr, far_off, near_off,
+                     far_items, near_items, err_flag);
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp_bytes,
+                          hipStream_t s) {
+  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, n, s);
+}
+
+// ---------------------------------------------------------------------------
+// Far field.  Per item the wave's lanes in the mask evaluate the cluster's
+// Taylor expansion; the recurrence is fully unrolled for the compile-time order
+// so all coefficients live in registers, and the moments are read with
+// wave-uniform addresses (scalar loads).  FP32: the per-cluster value is added
+// with Kahan compensation; FP64: plain sum.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+  double s = 0.0;
+  __device__ void add(double v) { s += v; }
+  __device__ double value() const { return s; }
+};
+template <>
+struct Acc<float> {
+  float s = 0.0f, c = 0.0f;
+  __device__ void add(float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ double value() const { return (double)s - (double)c; }
+};
+
+template <typename T, int NT, bool SCR>
+__global__ __launch_bounds__(64) void k_far(TreeView tv, double kappa, double delta2,
+                                            const T* __restrict__ mom_hat,
+                                            const int* __restrict__ far_off,
+                                            const Item* __restrict__ items,
+                                            double* __restrict__ phi_far) {
+  constexpr int NC = ncoef(NT);
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.n;
+  const int tt = valid ? t : tv.n - 1;
+  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  Acc<T> acc;
+  const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+    if ((mask >> lane) & 1ull) {
+      const double4 c = tv.center[node];
+      const double s = tv.scale[node];
+      const double inv = 1.0 / s;
+      const double dx = px - c.x, dy = py - c.y, dz = pz - c.z;
+      const double R2 = dx * dx + dy * dy + dz * dz + delta2;
+      const double R2h = R2 * inv * inv;
+      const double Rh = sqrt(R2h);
+      const double kh = kappa * s;
+      const double b0 = SCR ? exp(-kh * Rh) : 1.0;
+      const double a0 = b0 / Rh;
+      T v = far_eval<T, NT, SCR>((T)(dx * inv), (T)(dy * inv), (T)(dz * inv), (T)(1.0 / R2h),
+                                 (T)a0, (T)b0, (T)kh, mom_hat + (size_t)node * NC);
+      acc.add(v * (T)inv);
+    }
+  }
+  if (valid) phi_far[t] = acc.value();
+}
+
+template <typename T, bool SCR, int NT>
+static void launch_far(const TreeView& tv, double kappa, double delta2, const T* mom,
+                       const int* far_off, const Item* items, double* phi_far, hipStream_t s) {
+  int nb = (tv.n + kBatch - 1) / kBatch;
+  hipLaunchKernelGGL((k_far<T, NT, SCR>), dim3(nb), dim3(kBatch), 0, s, tv, kappa, delta2, mom,
+                     far_off, items, phi_far);
+}
+
+template <typename T, bool SCR, int... NTs>
+static bool dispatch_far(int nterms, std::integer_sequence<int, NTs...>, const TreeView& tv,
+                         double kappa, double delta2, const T* mom, const int* far_off,
+                         const Item* items, double* phi_far, hipStream_t s) {
+  bool done = false;
+  ((nterms == NTs + 1 ? (launch_far<T, SCR, NTs + 1>(tv, kappa, delta2, mom, far_off, items,
+                                                     phi_far, s),
+                         done = true)
+                      : false),
+   ...);
+  return done;
+}
+
+hipError_t far_field(const TreeView& tv, int nterms, double kappa, double delta2, bool fp32,
+                     const double* mom_hat64, const float* mom_hat32, const int* far_off,
+                     const Item* far_items, double* phi_far, hipStream_t s) {
+  using Seq = std::make_integer_sequence<int, kMaxNterms>;
+  bool scr = kappa != 0.0;
+  bool ok;
+  if (fp32)
+    ok = scr ? dispatch_far<float, true>(nterms, Seq{}, tv, kappa, delta2, mom_hat32, far_off,
+                                         far_items, phi_far, s)
+             : dispatch_far<float, false>(nterms, Seq{}, tv, kappa, delta2, mom_hat32, far_off,
+                                          far_items, phi_far, s);
+  else
+    ok = scr ? dispatch_far<double, true>(nterms, Seq{}, tv, kappa, delta2, mom_hat64, far_off,
+                                          far_items, phi_far, s)
+             : dispatch_far<double, false>(nterms, Seq{}, tv, kappa, delta2, mom_hat64, far_off,
+                                           far_items, phi_far, s);
+  if (!ok) return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Near field (leaf direct sum, tree3d.cpp:3573-3621): sources of each listed
+// leaf are staged 64 at a time in LDS and read by all lanes as broadcasts.
+// FP32: positions in leaf-centre coordinates (the target's offset from the
+// leaf centre is formed once in FP64), hardware rsqrt/exp2, groups of 16
+// terms summed plainly and added with Kahan compensation.  The self pair is
+// excluded only in leaves that hold some of the batch's targets.
+// ---------------------------------------------------------------------------
+template <bool SCR>
+__device__ __forceinline__ float pair32(float ox, float oy, float oz, float4 s, float d2,
+                                        float kneg2) {
+  float dx = ox - s.x, dy = oy - s.y, dz = oz - s.z;
+  float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
+  float ri = __builtin_amdgcn_rsqf(r2);
+  if constexpr (SCR) {
+    float e = __builtin_amdgcn_exp2f(kneg2 * (r2 * ri));
+    return s.w * e * ri;
+  } else {
+    return s.w * ri;
+  }
+}
+
+template <bool SCR>
+__device__ __forceinline__ double pair64(double px, double py, double pz, double4 s, double d2,
+                                         double kappa) {
+  double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
+  double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, d2)));
+  double R = sqrt(r2);
+  if constexpr (SCR) return s.w * exp(-kappa * R) / R;
+  else return s.w / R;
+}
+
+template <bool SCR>
+__global__ __launch_bounds__(64) void k_near32(TreeView tv, float d2, float kneg2,
+                                               const int* __restrict__ near_off,
+                                               const Item* __restrict__ items,
+                                               const double* __restrict__ phi_far,
+                                               double* __restrict__ out) {
+  __shared__ float4 tile[64];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.n;
+  const int tt = valid ? t : tv.n - 1;
+  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
+  Acc<float> acc;
+  const int i0 = uniform(near_off[b]), i1 = uniform(near_off[b + 1]);
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+    const int2 r = tv.range[node];
+    const double4 c = tv.center[node];
+    const bool act = (mask >> lane) & 1ull;
+    const float ox = (float)(px - c.x), oy = (float)(py - c.y), oz = (float)(pz - c.z);
+    const bool selfp = r.x < t1 && r.y > t0;
+    for (int base = r.x; base < r.y; base += 64) {
+      const int cnt = min(64, r.y - base);
+      if (lane < cnt) tile[lane] = tv.sloc[base + lane];
+      __syncthreads();
+      if (act) {
+        if (!selfp) {
+          int j = 0;
+          for (; j + 16 <= cnt; j += 16) {
+            float g = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 16; u++) g += pair32<SCR>(ox, oy, oz, tile[j + u], d2, kneg2);
+            acc.add(g);
+          }
+          float g = 0.0f;
+          for (; j < cnt; j++) g += pair32<SCR>(ox, oy, oz, tile[j], d2, kneg2);
+          acc.add(g);
+        } else {
+          const int me = t - base;
+          int j = 0;
+          for (; j + 16 <= cnt; j += 16) {
+            float g = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+              float v = pair32<SCR>(ox, oy, oz, tile[j + u], d2, kneg2);
+              g += (j + u == me) ? 0.0f : v;
+            }
+            acc.add(g);
+          }
+          float g = 0.0f;
+          for (; j < cnt; j++) {
+            float v = pair32<SCR>(ox, oy, oz, tile[j], d2, kneg2);
+            g += (j == me) ? 0.0f : v;
+          }
+          acc.add(g);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (valid) out[tv.perm[t]] = phi_far[t] + acc.value();
+}
+
+template <bool SCR>
+__global__ __launch_bounds__(64) void k_near64(TreeView tv, double d2, double kappa,
+                                               const int* __restrict__ near_off,
+                                               const Item* __restrict__ items,
+                                               const double* __restrict__ phi_far,
+                                               double* __restrict__ out) {
+  __shared__ double4 tile[64];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.n;
+  const int tt = valid ? t : tv.n - 1;
+  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
+  double acc = 0.0;
+  const int i0 = uniform(near_off[b]), i1 = uniform(near_off[b + 1]);
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+    const int2 r = tv.range[node];
+    const bool act = (mask >> lane) & 1ull;
+    const bool selfp = r.x < t1 && r.y > t0;
+    for (int base = r.x; base < r.y; base += 64) {
+      const int cnt = min(64, r.y - base);
+      if (lane < cnt) {
+        int j = base + lane;
+        tile[lane] = make_double4(tv.sx[j], tv.sy[j], tv.sz[j], tv.sq[j]);
+      }
+      __syncthreads();
+      if (act) {
+        const int me = selfp ? t - base : -1;
+        for (int j = 0; j < cnt; j++) {
+          if (j == me) continue;
+          acc += pair64<SCR>(px, py, pz, tile[j], d2, kappa);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (valid) out[tv.perm[t]] = phi_far[t] + acc;
+}
+
+hipError_t near_field(const TreeView& tv, double kappa, double delta2, bool fp32,
+                      const int* near_off, const Item* near_items, const double* phi_far,
+                      double* out, hipStream_t s) {
+  int nb = (tv.n + kBatch - 1) / kBatch;
+  bool scr = kappa != 0.0;
+  if (fp32) {
+    float kneg2 = (float)(-kappa * 1.4426950408889634);
+    if (scr)
+      hipLaunchKernelGGL(k_near32<true>, dim3(nb), dim3(kBatch), 0, s, tv, (float)delta2, kneg2,
+                         near_off, near_items, phi_far, out);
+    else
+      hipLaunchKernelGGL(k_near32<false>, dim3(nb), dim3(kBatch), 0, s, tv, (float)delta2,
+                         kneg2, near_off, near_items, phi_far, out);
+  } else {
+    if (scr)
+      hipLaunchKernelGGL(k_near64<true>, dim3(nb), dim3(kBatch), 0, s, tv, delta2, kappa,
+                         near_off, near_items, phi_far, out);
+    else
+      hipLaunchKernelGGL(k_near64<false>, dim3(nb), dim3(kBatch), 0, s, tv, delta2, kappa,
+                         near_off, near_items, phi_far, out);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Field of the regularized Coulomb kernel (treeforce, tree3d.cpp:851-971), FP64.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2,
+                                                 const double* __restrict__ mom_hat,
+                                                 const int* __restrict__ far_off,
+                                                 const Item* __restrict__ items,
+                                                 double* __restrict__ e_far) {
+  constexpr int NC = ncoef(NT);
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.n;
+  const int tt = valid ? t : tv.n - 1;
+  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  double ex = 0, ey = 0, ez = 0;
+  const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+    if ((mask >> lane) & 1ull) {
+      const double4 c = tv.center[node];
+      const double s = tv.scale[node];
+      const double inv = 1.0 / s;
+      const double dx = px - c.x, dy = py - c.y, dz = pz - c.z;
+      const double R2h = (dx * dx + dy * dy + dz * dz + delta2) * inv * inv;
+      const double a0 = 1.0 / sqrt(R2h);
+      double gx, gy, gz;
+      far_eval_grad<double, NT>(dx * inv, dy * inv, dz * inv, 1.0 / R2h, a0,
+                                mom_hat + (size_t)node * NC, gx, gy, gz);
+      // d/dx of (1/s) f(x/s) -> 1/s^2
+      ex += gx * inv * inv;
+      ey += gy * inv * inv;
+      ez += gz * inv * inv;
+    }
+  }
+  if (valid) {
+    e_far[3 * (size_t)t + 0] = ex;
+    e_far[3 * (size_t)t + 1] = ey;
+    e_far[3 * (size_t)t + 2] = ez;
+  }
+}
+
+template <int NT>
+static void launch_far_grad(const TreeView& tv, double delta2, const double* mom,
+                            const int* far_off, const Item* items, double* e_far, hipStream_t s) {
+  int nb = (tv.n + kBatch - 1) / kBatch;
+  hipLaunchKernelGGL((k_far_grad<NT>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, mom, far_off,
+                     items, e_far);
+}
+
+template <int... NTs>
+static bool dispatch_far_grad(int nterms, std::integer_sequence<int, NTs...>,
+                              const TreeView& tv, double delta2, const double* mom,
+                              const int* far_off, const Item* items, double* e_far,
+                              hipStream_t s) {
+  bool done = false;
+  ((nterms == NTs + 1
+        ? (launch_far_grad<NTs + 1>(tv, delta2, mom, far_off, items, e_far, s), done = true)
+        : false),
+   ...);
+  return done;
+}
+
+hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, const double* mom_hat64,
+                          const int* far_off, const Item* far_items, double* e_far,
+                          hipStream_t s) {
+  if (!dispatch_far_grad(nterms, std::make_integer_sequence<int, kMaxNterms - 1>{}, tv, delta2,
+                         mom_hat64, far_off, far_items, e_far, s))
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
+                                                  const int* __restrict__ near_off,
+                                                  const Item* __restrict__ items,
+                                                  const double* __restrict__ e_far,
+                                                  double* __restrict__ out) {
+  __shared__ double4 tile[64];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.n;
+  const int tt = valid ? t : tv.n - 1;
+  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
+  double ex = 0, ey = 0, ez = 0;
+  const int i0 = uniform(near_off[b]), i1 = uniform(near_off[b + 1]);
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+    const int2 r = tv.range[node];
+    const bool act = (mask >> lane) & 1ull;
+    const bool selfp = r.x < t1 && r.y > t0;
+    for (int base = r.x; base < r.y; base += 64) {
+      const int cnt = min(64, r.y - base);
+      if (lane < cnt) {
+        int j = base + lane;
+        tile[lane] = make_double4(tv.sx[j], tv.sy[j], tv.sz[j], tv.sq[j]);
+      }
+      __syncthreads();
+      if (act) {
+        const int me = selfp ? t - base : -1;
+        for (int j = 0; j < cnt; j++) {
+          if (j == me) continue;
+          double4 s = tile[j];
+          double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
+          double r2 = dx * dx + dy * dy + dz * dz + d2;
+          double w = s.w / (r2 * sqrt(r2));
+          ex += dx * w;
+          ey += dy * w;
+          ez += dz * w;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (valid) {
+    size_t o = 3 * (size_t)tv.perm[t];
+    out[o + 0] = e_far[3 * (size_t)t + 0] + ex;
+    out[o + 1] = e_far[3 * (size_t)t + 1] + ey;
+    out[o + 2] = e_far[3 * (size_t)t + 2] + ez;
+  }
+}
+
+hipError_t near_field_grad(const TreeView& tv, double delta2, const int* near_off,
+                           const Item* near_items, const double* e_far, double* out,
+                           hipStream_t s) {
+  int nb = (tv.n + kBatch - 1) / kBatch;
+  hipLaunchKernelGGL(k_near_grad, dim3(nb), dim3(kBatch), 0, s, tv, delta2, near_off,
+                     near_items, e_far, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum (shielded_pot_particle_ds, tree3d.cpp:1966-2016): 256 targets
+// per block, sources split into nsplit slices whose partial sums are added in
+// slice order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_direct(TreeView tv, double kappa, double d2,
+                                                int ntargets, const int* __restrict__ tslot,
+                                                int nsplit, double* __restrict__ partial) {
+  __shared__ double4 tile[256];
+  const int tb = blockIdx.x, sl = blockIdx.y;
+  const int ti = tb * 256 + threadIdx.x;
+  const bool valid = ti < ntargets;
+  const int me = valid ? tslot[ti] : 0;
+  const double px = tv.sx[me], py = tv.sy[me], pz = tv.sz[me];
+  const long long per = ((long long)tv.n + nsplit - 1) / nsplit;
+  const int s0 = (int)min((long long)tv.n, per * sl), s1 = (int)min((long long)tv.n, per * (sl + 1));
+  double acc = 0.0;
+  for (int base = s0; base < s1; base += 256) {
+    const int cnt = min(256, s1 - base);
+    if ((int)threadIdx.x < cnt) {
+      int j = base + threadIdx.x;
+      tile[threadIdx.x] = make_double4(tv.sx[j], tv.sy[j], tv.sz[j], tv.sq[j]);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; j++) {
+      if (base + j == me) continue;
+      double4 s = tile[j];
+      double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
+      double R = sqrt(dx * dx + dy * dy + dz * dz + d2);
+      acc += s.w * exp(-kappa * R) / R;
+    }
+    __syncthreads();
+  }
+  if (valid) partial[(size_t)sl * ntargets + ti] = acc;
+}
+
+__global__ void k_direct_reduce(int ntargets, int nsplit, const double* __restrict__ partial,
+                                double* out) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= ntargets) return;
+  double s = 0.0;
+  for (int k = 0; k < nsplit; k++) s += partial[(size_t)k * ntargets + ti];
+  out[ti] = s;
+}
+
+hipError_t direct_potential(const TreeView& tv, double kappa, double delta2, int ntargets,
+                            const int* tslot, int nsplit, double* partial, double* out,
+                            hipStream_t s) {
+  if (ntargets == 0) return hipSuccess;
+  dim3 grid((ntargets + 255) / 256, nsplit);
+  hipLaunchKernelGGL(k_direct, grid, dim3(256), 0, s, tv, kappa, delta2, ntargets, tslot,
+                     nsplit, partial);
+  hipLaunchKernelGGL(k_direct_reduce, dim3((ntargets + 255) / 256), dim3(256), 0, s, ntargets,
+                     nsplit, partial, out);
+  return hipGetLastError();
+}
+
+}  // namespace dev
+}  // namespace bit

@@ -1,0 +1,34 @@
+// tree_build.h -- host construction of the bisection tree, bit-exact with the
+// reference's treemake (tree3d.cpp:1345-1551).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bit {
+
+// Box fields of one node, in the order of TREE (tree3d.h:51-74).
+enum BoxField {
+  BX0 = 0, BX1, BY0, BY1, BZ0, BZ1, BXMID, BYMID, BZMID, BAXY, BAXZ, BAYZ, BSQR, BOX_FIELDS
+};
+
+struct HostTree {
+  int n = 0;
+  int numtree = 0;
+  int maxdepth = 0;
+  std::vector<double> box;    // BOX_FIELDS per node
+  std::vector<int> children;  // 8 per node, -1 = none, reference order
+  std::vector<int> begin;     // node i's members are perm[begin[i] .. end[i]-1]
+  std::vector<int> end;
+  std::vector<int> depth;
+  std::vector<int> perm;      // sorted slot -> original particle index
+};
+
+// Builds the tree over n particles (coordinates in original order).  Returns
+// false with a message when the particles cannot be separated into leaves of
+// at most maxmemb (coincident points) or the input is invalid.
+bool build_tree(int n, const double* x, const double* y, const double* z, int maxmemb,
+                HostTree& out, std::string& err);
+
+}  // namespace bit

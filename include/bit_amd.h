@@ -1,0 +1,164 @@
+/*
+ * bit_amd.h -- C-ABI of the MI355X particle-cluster treecode.
+ *
+ * This is the drop-in boundary under the reference's C++ library API
+ * (src/tree3d.h of ongbw/bit).  Every entry point takes plain pointers and
+ * sizes, returns an int status (BIT_OK = 0, negative BIT_E_* on failure, the
+ * message in bit_last_error()), never throws, and issues its device work on
+ * the context's stream.  One host thread owns a context at a time.
+ *
+ * Kernel evaluated (potential at particle targets, self excluded):
+ *     phi_i = sum_{j != i} q_j exp(-kappa R_ij) / R_ij,   R_ij = sqrt(|x_i-x_j|^2 + delta2)
+ * Multipole acceptance (reference convention, tree3d.cpp:3473 / 4691):
+ *     accept(cluster, target) = sqradius < acc * ((r^2 + delta2) - mac_sub),
+ * r = distance from the target to the cluster's box midpoint.
+ * The reference's functions are parameter choices of this one evaluation:
+ *     shielded_treepot (tree3d.cpp:3448):   delta2 = 0, mac_sub = 0.01 (DEL), result -phi
+ *     treepotential p=-1 (tree3d.cpp:4679): kappa = 0, delta2 = mac_sub = 0.01
+ */
+#ifndef BIT_AMD_H
+#define BIT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BIT_OK 0
+#define BIT_E_INVALID (-1)   /* bad argument (size, NaN/Inf input, unsupported nterms) */
+#define BIT_E_OOM (-2)       /* device allocation failed or HBM budget exceeded */
+#define BIT_E_HIP (-3)       /* HIP runtime error */
+#define BIT_E_TREE (-4)      /* tree could not be built (inseparable particles) */
+#define BIT_E_STATE (-5)     /* call out of order (e.g. evaluate before moments) */
+
+/* precision modes */
+#define BIT_FP64 0 /* moments, far and near field in FP64 */
+#define BIT_FP32 1 /* FP64 moments and lists; far/near in FP32, Kahan-compensated */
+
+/* largest supported Taylor order: nterms <= BIT_MAX_NTERMS (p = nterms-1) */
+#define BIT_MAX_NTERMS 13
+
+typedef struct bit_ctx bit_ctx;
+typedef struct bit_system bit_system;
+typedef struct bit_tree bit_tree;
+
+/* Evaluation parameters.  Defaults (bit_params_default) follow problem.h:11-13:
+ * acc = ACC = 0.5, delta2 = 0, kappa = 0, mac_sub = delta2. */
+typedef struct bit_params {
+  int nterms;       /* Taylor terms: degrees 0..nterms-1 (reference `nterms`) */
+  double acc;       /* MAC parameter (reference `acc`, ACC in problem.h:11) */
+  double kappa;     /* screening (reference `alpha` of shielded_treepot) */
+  double delta2;    /* regularization delta^2 (reference DEL, problem.h:12) */
+  double mac_sub;   /* subtracted in the MAC; reference: DEL */
+  int precision;    /* BIT_FP64 or BIT_FP32 */
+} bit_params;
+
+/* Interaction statistics of the last evaluation of a system (exact counts). */
+typedef struct bit_stats {
+  int64_t near_pairs;  /* source-target pairs summed directly */
+  int64_t far_pairs;   /* cluster-target Taylor evaluations */
+  int64_t far_items;   /* batch list entries (far), incl. masked lanes */
+  int64_t near_items;  /* batch list entries (near) */
+  int64_t lane_far;    /* 64 x far_items: lane slots the far kernel ran */
+  int64_t lane_near_pairs; /* 64 x sum over near items of leaf size */
+  double ms_tree, ms_upload, ms_moments, ms_lists, ms_far, ms_near; /* last timings */
+} bit_stats;
+
+void bit_params_default(bit_params *p);
+
+/* Thread-local message of the last failing call on this thread. */
+const char *bit_last_error(void);
+
+/* Context: one device, one stream (default priority), an HBM budget for
+ * everything the context allocates (0 = no limit). */
+int bit_ctx_create(int device, size_t hbm_budget_bytes, bit_ctx **out);
+int bit_ctx_destroy(bit_ctx *ctx);
+/* Use an existing hipStream_t (e.g. torch's current stream) instead of the
+ * context's own; NULL restores the context stream. */
+int bit_ctx_set_stream(bit_ctx *ctx, void *hip_stream);
+int bit_ctx_sync(bit_ctx *ctx);
+/* Bytes the context currently holds on the device. */
+size_t bit_ctx_bytes_in_use(const bit_ctx *ctx);
+/* Test hook: make the next k device allocations of this context fail with
+ * BIT_E_OOM (k = 0 disables). */
+int bit_ctx_inject_alloc_failures(bit_ctx *ctx, int k);
+
+/* Builds the tree and uploads the sorted sources.  Replaces treemake
+ * (tree3d.cpp:1345-1551; tree3d.h:115): same bisection rules, node numbering,
+ * children order and member lists (leaf size maxmemb = reference `maxmemb`).
+ * x,y,z,q are HOST arrays of n doubles (q = PARTICLE::tot_charge). */
+int bit_system_create(bit_ctx *ctx, int n, const double *x, const double *y,
+                      const double *z, const double *q, int maxmemb, bit_system **out);
+/* Same, from DEVICE arrays of n doubles already resident in HBM. */
+int bit_system_create_device(bit_ctx *ctx, int n, const double *d_x, const double *d_y,
+                             const double *d_z, const double *d_q, int maxmemb,
+                             bit_system **out);
+int bit_system_destroy(bit_system *sys);
+int bit_system_numtree(const bit_system *sys);
+int bit_system_maxdepth(const bit_system *sys);
+/* Host copy of the tree: boxes[13*numtree] = x0,x1,y0,y1,z0,z1,xmid,ymid,zmid,
+ * aspectxy,aspectxz,aspectyz,sqradius (TREE, tree3d.h:51-74); children[8*numtree];
+ * begin/end[numtree] index perm[n]; node i's member list (TREE::members, ascending
+ * original indices) is the sorted perm[begin[i] .. end[i]-1]; for a leaf that
+ * range is already ascending.  Any pointer may be NULL. */
+int bit_system_export_tree(const bit_system *sys, double *boxes, int *children, int *begin,
+                           int *end, int *perm);
+
+/* Host-only tree build, no device needed: the same tree bit_system_create
+ * builds (treemake, tree3d.cpp:1345-1551).  Export layout as
+ * bit_system_export_tree. */
+int bit_tree_build(int n, const double *x, const double *y, const double *z, int maxmemb,
+                   bit_tree **out);
+int bit_tree_destroy(bit_tree *t);
+int bit_tree_numtree(const bit_tree *t);
+int bit_tree_maxdepth(const bit_tree *t);
+int bit_tree_export(const bit_tree *t, double *boxes, int *children, int *begin, int *end,
+                    int *perm);
+
+/* Cluster moments about each box midpoint, FP64, degrees < nterms.  Replaces
+ * treemoments (tree3d.cpp:2631-2711; tree3d.h:126). */
+int bit_moments(bit_system *sys, int nterms);
+/* Host copy of the moments as the reference's cube: out[numtree][nterms]^3,
+ * M[i][j][k] = sum q dx^i dy^j dz^k (entries with i+j+k >= nterms are 0). */
+int bit_moments_download(const bit_system *sys, double *out);
+
+/* Potential at every particle, self excluded.  Replaces the per-target loops
+ * over shielded_treepot (tree3d.cpp:3448-3627) and treepotential
+ * (tree3d.cpp:4679-4778), batched as in shielded_freespace_pot_ds
+ * (tree3d.cpp:2218-2236).  out[n] receives +phi in the caller's ORIGINAL
+ * particle order; out_on_device selects a device (1) or host (0) pointer.
+ * Computes moments first if bit_moments was not called for params->nterms. */
+int bit_eval_potential(bit_system *sys, const bit_params *params, double *out,
+                       int out_on_device);
+/* Field E_i = sum_{j != i} q_j (x_i - x_j)/R^3 for the regularized Coulomb
+ * kernel (kappa must be 0); replaces treeforce (tree3d.cpp:851-971) as used by
+ * freespace_particles_tree (tree3d.cpp:2268-2335).  out[3n] = Ex,Ey,Ez
+ * interleaved per particle, original order. */
+int bit_eval_field(bit_system *sys, const bit_params *params, double *out,
+                   int out_on_device);
+int bit_system_stats(const bit_system *sys, bit_stats *out);
+
+/* The batch interaction lists the evaluation uses for params' MAC (acc,
+ * delta2, mac_sub), i.e. the reference traversal's per-target decisions
+ * (tree3d.cpp:3473 accept -> far, childless -> near) grouped by batches of 64
+ * consecutive sorted targets.  First call with NULL arrays to get the sizes
+ * (nbatch, nfar, nnear), then with arrays of nbatch+1 / nfar / nnear entries.
+ * Item t of batch b covers sorted slot 64*b + lane for every set lane bit. */
+int bit_system_export_lists(bit_system *sys, const bit_params *params, int *nbatch, int *nfar,
+                            int *nnear, int *far_off, uint64_t *far_mask, int *far_node,
+                            int *near_off, uint64_t *near_mask, int *near_node);
+
+/* FP64 direct sum on the GPU at ntargets particle targets (original indices
+ * tidx, host array): phi_t = sum_{j != tidx[t]} q_j exp(-kappa R)/R.
+ * Replaces shielded_pot_particle_ds (tree3d.cpp:1966-2016) with +phi sign.
+ * out is a host array of ntargets doubles. */
+int bit_direct_potential(bit_system *sys, double kappa, double delta2, int ntargets,
+                         const int *tidx, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

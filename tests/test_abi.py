@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every symbol include/bit_amd.h declares;
+no compute call is made here (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import bit_amd
+from conftest import ROOT
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "bit_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bit_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_exported():
+    lib = ctypes.CDLL(bit_amd.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(bit_amd.EXPORTS) == names
+
+
+def test_library_has_gfx950_code():
+    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", bit_amd.LIB_PATH], capture_output=True,
+                         text=True)
+    if out.returncode != 0:
+        return  # tool unavailable
+    assert "gfx950" in out.stdout
+
+
+def test_context_without_device_fails_cleanly():
+    if os.environ.get("HIP_VISIBLE_DEVICES", None) is None:
+        try:
+            c = bit_amd.Context(0)
+        except bit_amd.BitError as e:
+            assert e.code in (bit_amd.BIT_E_HIP, bit_amd.BIT_E_INVALID)
+            assert bit_amd.lib.bit_last_error()
+            return
+        c.close()  # a device is present: nothing to check here
+
+
+def test_params_default_follow_problem_h():
+    p = bit_amd.Params()
+    bit_amd.lib.bit_params_default(ctypes.byref(p))
+    assert p.acc == 0.5 and p.kappa == 0.0 and p.precision == bit_amd.BIT_FP64

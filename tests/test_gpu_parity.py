@@ -1,0 +1,210 @@
+"""HIP path (through the C-ABI) against the reference fixtures and the oracle.
+
+Bars: trees and interaction lists bit-exact; FP64 potentials and fields within
+1e-10 relative (max norm) of the reference; FP32 (Kahan) within the stated
+tolerance of the FP64 result.
+"""
+import numpy as np
+import pytest
+
+import bit_amd
+from bit_amd import workloads
+from conftest import FIXTURES, fixture_members, load_fixture, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+# Reference entry point of each fixture's phi -> (kappa, delta2, mac_sub).
+PHI_KEYS = {
+    "phi_coulomb": lambda fx: (0.0, 0.0, 0.01),       # shielded_treepot, alpha 0
+    "phi_regcoulomb": lambda fx: (0.0, 0.01, 0.01),   # treepotential, p=-1
+    "phi_screened": lambda fx: (float(fx["kappa"]), 0.0, 0.01),  # shielded_treepot
+}
+FP32_TOL = 2e-6  # relative L2 of the FP32 form against the FP64 form
+
+
+def _system(ctx, fx):
+    return bit_amd.System(ctx, fx["x"], fx["y"], fx["z"], fx["q"], int(fx["maxmemb"]))
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_system_tree_bit_exact(ctx, name):
+    fx = load_fixture(name)
+    s = _system(ctx, fx)
+    t = s.export_tree()
+    assert np.array_equal(t.children, fx["children"])
+    assert np.array_equal(t.boxes.view(np.uint64), fx["boxes"].view(np.uint64))
+    mem = fixture_members(fx)
+    for i in range(t.numtree):
+        assert np.array_equal(t.member_list(i), mem[i])
+
+
+@pytest.mark.parametrize("name", ["box16k", "uniform10k"])
+def test_moments_match_reference(ctx, name):
+    fx = load_fixture(name)
+    s = _system(ctx, fx)
+    nt = int(fx["nterms"])
+    s.moments(nt)
+    m = s.moments_cube()[fx["mom_nodes"]]
+    ref = fx["moments"]
+    for a, r in zip(m, ref):
+        assert np.max(np.abs(a - r)) <= 1e-12 * max(np.max(np.abs(r)), 1e-300)
+
+
+def _per_target_lists(lst, n):
+    far = [[] for _ in range(n)]
+    near = [[] for _ in range(n)]
+    for key, out in (("far", far), ("near", near)):
+        off, masks, nodes = lst[key + "_off"], lst[key + "_mask"], lst[key + "_node"]
+        for b in range(len(off) - 1):
+            for it in range(off[b], off[b + 1]):
+                m = int(masks[it])
+                for lane in range(64):
+                    if (m >> lane) & 1:
+                        out[b * 64 + lane].append(int(nodes[it]))
+    return far, near
+
+
+@pytest.mark.parametrize("name,acc,d2,sub", [("uniform10k", 0.5, 0.0, 0.01),
+                                             ("blobs12k_screened", 0.7, 0.0, 0.01),
+                                             ("uniform12k_reg", 0.5, 0.01, 0.01),
+                                             ("aniso6k", 0.5, 0.0, 0.01),
+                                             ("blobs12k_screened", 0.3, 2.5e-5, 2.5e-5),
+                                             ("tiny300", 0.5, 0.0, 0.01)])
+def test_lists_bit_exact(ctx, oracle, name, acc, d2, sub):
+    """Every target's far clusters and near leaves, in traversal order, equal
+    the oracle's per-target recursion (whose tree equals the reference's)."""
+    fx = load_fixture(name)
+    s = _system(ctx, fx)
+    lst = s.lists(4, acc, d2, sub)
+    far, near = _per_target_lists(lst, s.n)
+    perm = s.export_tree().perm
+    o = oracle.tree(fx["x"], fx["y"], fx["z"], int(fx["maxmemb"]))
+    for slot in range(s.n):
+        f, nr = o.lists(int(perm[slot]), 4, acc, d2, sub)
+        assert far[slot] == list(f), slot
+        assert near[slot] == list(nr), slot
+
+
+@pytest.mark.parametrize("name", ["uniform10k", "uniform12k_reg", "blobs12k_screened", "aniso6k",
+                                  "tiny1", "tiny5", "tiny300"])
+def test_potential_fp64_matches_reference(ctx, name):
+    fx = load_fixture(name)
+    s = _system(ctx, fx)
+    key = next(k for k in PHI_KEYS if k in fx)
+    kappa, d2, sub = PHI_KEYS[key](fx)
+    phi = s.potential(int(fx["nterms"]), float(fx["acc"]), kappa, d2, sub, "fp64")
+    ref = fx[key]
+    got = phi[fx["tidx"]]
+    if fx["x"].size == 1:
+        assert got[0] == 0.0 and ref[0] == 0.0
+        return
+    assert rel_max(got, ref) < 1e-10
+    assert rel_l2(got, ref) < 1e-11
+
+
+@pytest.mark.parametrize("name", ["uniform10k", "uniform12k_reg", "blobs12k_screened", "aniso6k",
+                                  "tiny300"])
+def test_potential_fp32_within_tolerance(ctx, name):
+    fx = load_fixture(name)
+    s = _system(ctx, fx)
+    key = next(k for k in PHI_KEYS if k in fx)
+    kappa, d2, sub = PHI_KEYS[key](fx)
+    args = (int(fx["nterms"]), float(fx["acc"]), kappa, d2, sub)
+    p64 = s.potential(*args, precision="fp64")
+    p32 = s.potential(*args, precision="fp32")
+    assert rel_l2(p32, p64) < FP32_TOL
+
+
+def test_field_fp64_matches_treeforce(ctx):
+    fx = load_fixture("box16k")
+    s = _system(ctx, fx)
+    E = s.field(5, 0.5, 0.01, 0.01)[fx["tidx"]]
+    assert rel_max(E, fx["field_tree"]) < 1e-10
+
+
+def test_gpu_direct_sum_matches_oracle(ctx, oracle):
+    fx = load_fixture("blobs12k_screened")
+    s = _system(ctx, fx)
+    got = s.direct(0.5, 0.0, fx["tidx"])
+    assert rel_max(got, fx["phi_screened_ds"]) < 1e-12
+    o = oracle.direct(fx["x"], fx["y"], fx["z"], fx["q"], 1.3, 1e-4, fx["tidx"])
+    assert rel_max(s.direct(1.3, 1e-4, fx["tidx"]), o) < 1e-12
+
+
+@pytest.mark.parametrize("kappa,d2,nterms,acc", [(0.5, 2.5e-5, 9, 0.7), (0.0, 1e-4, 12, 0.4),
+                                                 (2.0, 4e-4, 5, 0.5), (1.0, 0.0, 13, 0.3)])
+def test_clean_mac_matches_oracle(ctx, oracle, kappa, d2, nterms, acc):
+    """The default MAC (mac_sub = delta2, the reference's treepotential rule)
+    with regularized screened kernels the reference cannot be compiled for."""
+    P = workloads.gaussian_blobs(20000, seed=21)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 150)
+    tidx = workloads.sample_targets(P.n, 2000, seed=3)
+    o = oracle.tree(P.x, P.y, P.z, 150)
+    m = o.moments(P.q, nterms)
+    ref = o.potential(P.q, m, nterms, acc, kappa, d2, d2, tidx)
+    got = s.potential(nterms, acc, kappa, d2, None, "fp64")[tidx]
+    assert rel_max(got, ref) < 1e-10
+    got32 = s.potential(nterms, acc, kappa, d2, None, "fp32")[tidx]
+    assert rel_l2(got32, ref) < FP32_TOL
+
+
+def test_device_resident_inputs(ctx):
+    torch = pytest.importorskip("torch")
+    P = workloads.gaussian_blobs(50000, seed=22)
+    dev = [torch.tensor(a, dtype=torch.float64, device="cuda") for a in (P.x, P.y, P.z, P.q)]
+    s1 = bit_amd.System(ctx, *dev, 300)
+    s2 = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 300)
+    out = torch.empty(P.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    s1.potential(8, 0.5, 0.5, 1e-4, None, "fp32", out=out)
+    a = out.cpu().numpy()
+    b = s2.potential(8, 0.5, 0.5, 1e-4, None, "fp32")
+    assert np.array_equal(a, b)
+
+
+def test_repeat_is_deterministic(ctx):
+    P = workloads.gaussian_blobs(40000, seed=23)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 200)
+    a = s.potential(9, 0.6, 0.5, 1e-4, None, "fp32")
+    b = s.potential(9, 0.6, 0.5, 1e-4, None, "fp32")
+    assert np.array_equal(a, b)
+
+
+def test_stats_count_interactions(ctx, oracle):
+    fx = load_fixture("blobs12k_screened")
+    s = _system(ctx, fx)
+    s.potential(9, 0.7, 0.5, 0.0, 0.01, "fp64")
+    st = s.stats()
+    o = oracle.tree(fx["x"], fx["y"], fx["z"], int(fx["maxmemb"]))
+    m = o.moments(fx["q"], 2)
+    _, nn, nf = o.potential(fx["q"], m, 2, 0.7, 0.5, 0.0, 0.01, None, counts=True)
+    # the oracle's near count excludes the self pair
+    assert st["near_pairs"] == int(nn.sum()) + s.n
+    assert st["far_pairs"] == int(nf.sum())
+
+
+def test_oom_and_budget_errors_leave_context_usable(ctx):
+    P = workloads.uniform_cube(5000, seed=24)
+    small = bit_amd.Context(0, hbm_budget=1 << 16)
+    with pytest.raises(bit_amd.BitError) as e:
+        bit_amd.System(small, P.x, P.y, P.z, P.q, 100)
+    assert e.value.code == bit_amd.BIT_E_OOM
+    small.close()
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 100)
+    ctx.inject_alloc_failures(1)
+    with pytest.raises(bit_amd.BitError) as e:
+        s.potential(6, 0.5, 0.0, 0.0, None, "fp64")
+    assert e.value.code == bit_amd.BIT_E_OOM
+    ok = s.potential(6, 0.5, 0.0, 0.0, None, "fp64")
+    assert np.isfinite(ok).all()
+
+
+def test_invalid_params_rejected(ctx):
+    P = workloads.uniform_cube(1000, seed=25)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 100)
+    for bad in (dict(nterms=0), dict(nterms=14), dict(kappa=float("nan")), dict(delta2=-1.0)):
+        kw = dict(nterms=6, acc=0.5, kappa=0.0, delta2=0.0)
+        kw.update(bad)
+        with pytest.raises(bit_amd.BitError) as e:
+            s.potential(**kw)
+        assert e.value.code == bit_amd.BIT_E_INVALID

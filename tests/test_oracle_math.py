@@ -1,0 +1,122 @@
+"""The oracle against what the mathematics fixes: an independent power-series
+expansion of the kernel, closed-form direct sums in extended precision,
+brute-force moments, and convergence of the treecode to the direct sum."""
+import numpy as np
+import pytest
+
+from bit_amd import workloads
+from conftest import rel_l2, rel_max
+from series import taylor_coeffs
+
+
+@pytest.mark.parametrize("kappa,delta2", [(0.0, 0.0), (0.7, 0.0), (2.0, 0.03), (0.0, 0.01)])
+@pytest.mark.parametrize("x", [(0.9, -0.4, 0.3), (-0.2, 1.3, 0.05), (0.5, 0.5, -0.8)])
+def test_taylor_coeffs_vs_power_series(oracle, kappa, delta2, x):
+    P = 8
+    a = oracle.taylor_coeffs(P + 1, x[0], x[1], x[2], x[0] ** 2 + x[1] ** 2 + x[2] ** 2 + delta2,
+                             kappa)
+    ref = taylor_coeffs(P, x, kappa, delta2)
+    i, j, k = np.indices((P + 1,) * 3)
+    m = (i + j + k) <= P
+    err = np.abs(a[:P + 1, :P + 1, :P + 1][m] - ref[m])
+    scale = np.abs(ref[m]).max()
+    assert err.max() <= 1e-12 * scale
+
+
+def test_taylor_series_converges_to_kernel(oracle):
+    """sum_k a_k y^k -> G(x - y) for a source offset y inside the radius."""
+    x = np.array([0.8, -0.5, 0.4])
+    y = np.array([0.05, 0.08, -0.06])
+    kappa, d2 = 1.1, 0.002
+    R = np.sqrt(np.sum((x - y) ** 2) + d2)
+    G = np.exp(-kappa * R) / R
+    errs = []
+    for nt in (3, 6, 9, 12):
+        a = oracle.taylor_coeffs(nt, *x, float(np.sum(x ** 2) + d2), kappa)
+        i, j, k = np.indices(a.shape)
+        m = (i + j + k) < nt
+        errs.append(abs(np.sum(a[m] * (y[0] ** i * y[1] ** j * y[2] ** k)[m]) - G) / G)
+    assert all(e2 < e1 for e1, e2 in zip(errs, errs[1:]))
+    assert errs[-1] < 1e-10
+
+
+def test_direct_sum_closed_form(oracle):
+    P = workloads.uniform_cube(60, seed=7)
+    kappa, d2 = 0.8, 0.0004
+    got = oracle.direct(P.x, P.y, P.z, P.q, kappa, d2)
+    pos = np.stack([P.x, P.y, P.z], 1).astype(np.longdouble)
+    ref = np.zeros(P.n, dtype=np.longdouble)
+    for i in range(P.n):
+        r = np.sqrt(np.sum((pos - pos[i]) ** 2, axis=1) + np.longdouble(d2))
+        v = P.q.astype(np.longdouble) * np.exp(-np.longdouble(kappa) * r) / r
+        v[i] = 0  # self excluded
+        ref[i] = v.sum()
+    assert rel_max(got, ref.astype(np.float64)) < 1e-13
+
+
+def test_moments_brute_force(oracle):
+    P = workloads.gaussian_blobs(3000, seed=8)
+    t = oracle.tree(P.x, P.y, P.z, 50)
+    e = t.export()
+    nt = 6
+    m = t.moments(P.q, nt)
+    for node in (0, 3, e.numtree - 1):
+        mem = e.node_members(node)
+        b = e.boxes[node]
+        dx = (P.x[mem] - b[6]).astype(np.longdouble)
+        dy = (P.y[mem] - b[7]).astype(np.longdouble)
+        dz = (P.z[mem] - b[8]).astype(np.longdouble)
+        q = P.q[mem].astype(np.longdouble)
+        for i in range(nt):
+            for j in range(nt - i):
+                for k in range(nt - i - j):
+                    ref = np.sum(q * dx ** i * dy ** j * dz ** k)
+                    tol = 1e-13 * float(np.sum(np.abs(q) * (np.abs(dx) + np.abs(dy) + np.abs(dz) + 1e-300) ** (i + j + k)))
+                    assert abs(m[node, i, j, k] - float(ref)) <= tol
+
+
+@pytest.mark.parametrize("kappa,delta2", [(0.0, 0.0), (0.5, 0.0001)])
+def test_treecode_converges_to_direct_sum(oracle, kappa, delta2):
+    P = workloads.gaussian_blobs(6000, seed=9)
+    t = oracle.tree(P.x, P.y, P.z, 100)
+    tidx = workloads.sample_targets(P.n, 300, seed=1)
+    ds = oracle.direct(P.x, P.y, P.z, P.q, kappa, delta2, tidx)
+    errs = []
+    for nt in (2, 4, 6, 8, 10, 12):
+        m = t.moments(P.q, nt)
+        phi = t.potential(P.q, m, nt, 0.25, kappa, delta2, delta2, tidx)
+        errs.append(rel_l2(phi, ds))
+    assert all(e2 < e1 for e1, e2 in zip(errs, errs[1:]))
+    assert errs[-1] < 1e-6
+
+
+def test_field_is_minus_gradient_of_potential(oracle):
+    """E from the field traversal equals -grad phi of the regularized Coulomb
+    direct sum (central differences), and the tree field converges to it."""
+    P = workloads.uniform_cube(2000, seed=10)
+    d2 = 0.01
+    tidx = np.arange(0, 2000, 97, dtype=np.int32)
+    Ed = oracle.direct_force(P.x, P.y, P.z, P.q, d2, tidx)
+    h = 1e-5
+    pos = np.stack([P.x, P.y, P.z], 1)
+    for r, i in enumerate(tidx[:5]):
+        g = []
+        for e in range(3):
+            vals = []
+            for sgn in (1, -1):
+                p = pos[i].copy()
+                p[e] += sgn * h
+                d = pos - p
+                R = np.sqrt(np.sum(d ** 2, 1) + d2)
+                v = P.q / R
+                v[i] = 0
+                vals.append(v.sum())
+            g.append(-(vals[0] - vals[1]) / (2 * h))
+        assert np.allclose(Ed[r], g, rtol=1e-6, atol=1e-8 * np.abs(g).max())
+    t = oracle.tree(P.x, P.y, P.z, 60)
+    errs = []
+    for nt in (4, 8, 12):
+        m = t.moments(P.q, nt)
+        errs.append(rel_l2(t.treeforce(P.q, m, nt, 0.25, d2, d2, tidx), Ed))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[-1] < 3e-5
