@@ -1,0 +1,207 @@
+"""Benchmark: treecode potential evaluation at N=4M (16 Gaussian blobs, screened).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64]
+                    [--nterms P1] [--acc THETA] [--maxmemb N0] [--no-cpu-baseline]
+
+One step = one whole evaluation of the potential at all N particles of one
+system whose coordinates and charges are already resident in HBM: tree build
+(host, bit-exact with the reference treemake, after copying the coordinates
+down), upload of the node table and sorted sources, FP64 moments, batch
+interaction lists, far field and near field, result left in HBM.
+
+Prints ONE JSON line.  Extra keys: `accuracy` (relative L2 error of the last
+step against an FP64 GPU direct sum on 10,000 seeded targets), `breakdown`
+(ms per stage of the last step), `fp64_ms_per_step` (one FP64 step, untimed
+for the metric), `roofline` (dominant kernel) and `cpu_baseline` (the
+oracle's OpenMP treecode on the host cores, extrapolated from a bounded
+target sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3e12  # MI355X vector FP32, FLOP/s
+PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
+
+
+def ncoef(nterms: int) -> int:
+    return nterms * (nterms + 1) * (nterms + 2) // 6
+
+
+def flops(stats: dict, nterms: int, screened: bool) -> tuple[float, float]:
+    """Algorithmic FLOPs of the near and far kernels: 16 (screened) / 12 per
+    near pair; 24 T(p) (screened) / 12 T(p) per far (cluster, target) pair."""
+    fn = 16.0 if screened else 12.0
+    ff = (24.0 if screened else 12.0) * ncoef(nterms)
+    return stats["near_pairs"] * fn, stats["far_pairs"] * ff
+
+
+def log(msg: str):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def cpu_baseline(P, nterms, acc, kappa, d2, maxmemb, sample: int, seed: int = 5):
+    """The oracle (FP64 C restatement of the reference treecode, OpenMP over
+    targets) on the host: tree + moments timed in full, the per-target
+    traversal timed on `sample` seeded targets and extrapolated to N."""
+    from bit_amd import workloads
+    from oracle.pyoracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    O = Oracle()
+    t0 = time.perf_counter()
+    t = O.tree(P.x, P.y, P.z, maxmemb)
+    t1 = time.perf_counter()
+    m = t.moments(P.q, nterms)
+    t2 = time.perf_counter()
+    tidx = workloads.sample_targets(P.n, sample, seed=seed)
+    t.potential(P.q, m, nterms, acc, kappa, d2, d2, tidx, nthreads=threads)
+    t3 = time.perf_counter()
+    per_target = (t3 - t2) / len(tidx)
+    total = (t1 - t0) + (t2 - t1) + P.n * per_target
+    return {
+        "value": P.n / total, "unit": "targets/s", "cores": threads, "kind": "port",
+        "sample": f"tree+moments of all {P.n} particles timed in full ({t1 - t0:.2f}s + "
+                  f"{t2 - t1:.2f}s); traversal timed on {len(tidx)} seeded targets "
+                  f"({t3 - t2:.2f}s on {threads} threads) and extrapolated to N",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4_000_000)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--nterms", type=int, default=12)
+    ap.add_argument("--acc", type=float, default=0.35)
+    ap.add_argument("--maxmemb", type=int, default=2000)
+    ap.add_argument("--kappa", type=float, default=0.5)
+    ap.add_argument("--delta", type=float, default=0.005)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--accuracy-targets", type=int, default=10_000)
+    ap.add_argument("--cpu-sample", type=int, default=3000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the treecode's multi-GPU path (target shards, "
+              "one broadcast of sources and moments) is a design, not part of this build; "
+              "run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import bit_amd
+    from bit_amd import workloads
+
+    d2 = args.delta ** 2
+    screened = args.kappa != 0.0
+    log(f"[bench] generating {args.n} blob particles (seed {args.seed})")
+    P = workloads.gaussian_blobs(args.n, seed=args.seed)
+    ctx = bit_amd.Context(0)
+    dev = [bit_amd.DeviceArray.from_numpy(ctx, a) for a in (P.x, P.y, P.z, P.q)]
+    out = bit_amd.DeviceArray(ctx, args.n)
+
+    def step(precision):
+        s = bit_amd.System(ctx, *dev, args.maxmemb)
+        s.potential(args.nterms, args.acc, args.kappa, d2, None, precision, out=out)
+        st = s.stats()
+        s.close()
+        return st
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        step(args.precision)
+        log(f"[bench] warmup {w}: {1e3 * (time.perf_counter() - t0):.1f} ms")
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = step(args.precision)
+    ctx.sync()  # every step already ends synchronised (results resident in HBM)
+    t1 = time.perf_counter()
+    ms = 1e3 * (t1 - t0) / args.steps
+    log(f"[bench] {args.steps} steps: {ms:.1f} ms/step")
+
+    # accuracy of the last step's result against the FP64 direct sum
+    tidx = workloads.sample_targets(args.n, args.accuracy_targets, seed=7)
+    phi = out.numpy()
+    s = bit_amd.System(ctx, *dev, args.maxmemb)
+    ref = s.direct(args.kappa, d2, tidx)
+    s.close()
+    err = float(np.linalg.norm(phi[tidx] - ref) / np.linalg.norm(ref))
+    log(f"[bench] rel L2 error vs FP64 direct sum on {len(tidx)} targets: {err:.3e}")
+
+    fp64_ms = None
+    if not args.no_fp64_compare and args.precision == "fp32":
+        ctx.sync()
+        a = time.perf_counter()
+        st64 = step("fp64")
+        ctx.sync()
+        fp64_ms = 1e3 * (time.perf_counter() - a)
+        log(f"[bench] one fp64 step: {fp64_ms:.1f} ms (far {st64['ms_far']:.1f}, near "
+            f"{st64['ms_near']:.1f})")
+
+    f_near, f_far = flops(st, args.nterms, screened)
+    kern = "far" if st["ms_far"] >= st["ms_near"] else "near"
+    kms = st["ms_far"] if kern == "far" else st["ms_near"]
+    kfl = f_far if kern == "far" else f_near
+    peak = PEAK_FP32 if args.precision == "fp32" else PEAK_FP64
+    achieved = kfl / (kms * 1e-3)
+    roof = {"bound": "valu", "kernel": f"k_far<{'float' if args.precision == 'fp32' else 'double'},"
+            f"{args.nterms},{str(screened).lower()}>" if kern == "far" else
+            f"k_near{'32' if args.precision == 'fp32' else '64'}<{str(screened).lower()}>",
+            "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None,
+            "flops_per_launch": kfl, "kernel_ms": kms,
+            "other_kernel": {"name": "near" if kern == "far" else "far",
+                             "tflops": (f_near / (st["ms_near"] * 1e-3) if kern == "far" else
+                                        f_far / (st["ms_far"] * 1e-3)) / 1e12}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"[bench] CPU baseline (oracle port), sample {args.cpu_sample} targets")
+        cpu = cpu_baseline(P, args.nterms, args.acc, args.kappa, d2, args.maxmemb,
+                           args.cpu_sample)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} targets/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "targets/sec at N=4M, rel L2 err<=1e-6 vs FP64; p50/p99 request latency",
+        "value": args.n / (ms * 1e-3),
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (seeded 16-blob Gaussian mixture, charges U[-1,1])",
+        "config": {"workload": "blobs4m_screened", "n": args.n, "kappa": args.kappa,
+                   "delta": args.delta, "nterms": args.nterms, "p": args.nterms - 1,
+                   "acc": args.acc, "maxmemb": args.maxmemb, "precision": args.precision,
+                   "parallelism": "single GPU"},
+        "accuracy": {"rel_l2_vs_fp64_direct": err, "targets": len(tidx)},
+        "breakdown": {k: st[k] for k in ("ms_tree", "ms_upload", "ms_moments", "ms_lists",
+                                         "ms_far", "ms_near")},
+        "interactions": {k: st[k] for k in ("near_pairs", "far_pairs", "far_items",
+                                            "near_items", "lane_far", "lane_near_pairs")},
+        "fp64_ms_per_step": fp64_ms,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -78,13 +78,19 @@ EXPORTS = [
     "bit_system_export_tree", "bit_moments", "bit_moments_download", "bit_eval_potential",
     "bit_eval_field", "bit_system_stats", "bit_direct_potential", "bit_tree_build",
     "bit_tree_destroy", "bit_tree_numtree", "bit_tree_maxdepth", "bit_tree_export",
-    "bit_system_export_lists",
+    "bit_system_export_lists", "bit_device_alloc", "bit_device_free", "bit_copy_to_device",
+    "bit_copy_to_host",
 ]
 
 
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `python -m bit_amd.build`")
+    # torch (ROCm build) ships its own HIP runtime under the same soname.  If
+    # torch is imported first, this library binds to that runtime (one HIP
+    # runtime per process).  Importing torch after this library has loaded the
+    # image's runtime gives the process two runtimes and torch then finds no
+    # device: import torch first when both are used.
     L = C.CDLL(LIB_PATH)
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     L.bit_last_error.restype = C.c_char_p
@@ -110,6 +116,10 @@ def _load():
     L.bit_direct_potential.argtypes = [vp, C.c_double, C.c_double, C.c_int, vp, vp]
     L.bit_system_export_lists.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp,
                                           vp, vp]
+    L.bit_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    L.bit_device_free.argtypes = [vp, vp]
+    L.bit_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
+    L.bit_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
     L.bit_tree_build.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.POINTER(vp)]
     L.bit_tree_destroy.argtypes = [vp]
     L.bit_tree_numtree.argtypes = [vp]
@@ -196,6 +206,45 @@ class Context:
         _check(lib.bit_ctx_inject_alloc_failures(self.h, int(k)))
 
 
+class DeviceArray:
+    """A float64 (or int32) array in HBM owned by a Context (bit_device_alloc)."""
+
+    def __init__(self, ctx: Context, n: int, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        h = C.c_void_p()
+        _check(lib.bit_device_alloc(ctx.h, self.n * self.dtype.itemsize, C.byref(h)))
+        self.ptr = h.value
+
+    @classmethod
+    def from_numpy(cls, ctx: Context, a) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        _check(lib.bit_copy_to_device(ctx.h, C.c_void_p(d.ptr), _ptr(a), a.nbytes))
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(lib.bit_copy_to_host(self.ctx.h, _ptr(out), C.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib.bit_device_free(self.ctx.h, C.c_void_p(self.ptr))
+        self.ptr = None
+
+
+def _dev_ptr(a):
+    """Device address of a DeviceArray or a torch device tensor, else None."""
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if hasattr(a, "is_cuda") and a.is_cuda:
+        return a.data_ptr()
+    return None
+
+
 @dataclass
 class TreeExport:
     numtree: int
@@ -241,13 +290,10 @@ class System:
     def __init__(self, ctx: Context, x, y, z, q, maxmemb: int):
         self.ctx = ctx
         h = C.c_void_p()
-        if hasattr(x, "is_cuda") and x.is_cuda:  # torch device tensors (float64)
-            n = x.numel()
-            _check(lib.bit_system_create_device(ctx.h, n, C.c_void_p(x.data_ptr()),
-                                                C.c_void_p(y.data_ptr()),
-                                                C.c_void_p(z.data_ptr()),
-                                                C.c_void_p(q.data_ptr()), int(maxmemb),
-                                                C.byref(h)))
+        if _dev_ptr(x) is not None:  # device-resident float64 arrays
+            n = x.n if isinstance(x, DeviceArray) else x.numel()
+            ptrs = [C.c_void_p(_dev_ptr(a)) for a in (x, y, z, q)]
+            _check(lib.bit_system_create_device(ctx.h, n, *ptrs, int(maxmemb), C.byref(h)))
         else:
             x, y, z, q = _f64(x), _f64(y), _f64(z), _f64(q)
             n = len(x)
@@ -293,10 +339,11 @@ class System:
     def potential(self, nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None,
                   precision="fp64", out=None):
         """+phi at every particle (self excluded), original order.  `out` may be
-        a float64 torch device tensor of n elements (result stays in HBM)."""
+        a float64 DeviceArray / torch device tensor of n elements (the result
+        then stays in HBM)."""
         prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
-        if out is not None and hasattr(out, "is_cuda") and out.is_cuda:
-            _check(lib.bit_eval_potential(self.h, C.byref(prm), C.c_void_p(out.data_ptr()), 1))
+        if out is not None and _dev_ptr(out) is not None:
+            _check(lib.bit_eval_potential(self.h, C.byref(prm), C.c_void_p(_dev_ptr(out)), 1))
             return out
         res = np.zeros(self.n)
         _check(lib.bit_eval_potential(self.h, C.byref(prm), _ptr(res), 0))

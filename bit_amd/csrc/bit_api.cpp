@@ -58,6 +58,7 @@ struct bit_ctx {
   size_t budget = 0;
   size_t in_use = 0;
   int inject = 0;
+  std::vector<std::pair<void*, size_t>> user_bufs;  // bit_device_alloc
 };
 
 namespace {
@@ -233,6 +234,8 @@ int finish_system(bit_ctx* ctx, bit_system* s, const double* const* sorted_host,
   BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
   HIP_TRY(bit::dev::leaf_local(s->view(), s->leaves.as<int>(), s->nleaves, s->sloc.as<float4>(),
                                ctx->stream));
+  // the node tables above are host temporaries of this function
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   return BIT_OK;
 }
 
@@ -251,6 +254,7 @@ int check_params(const bit_system* s, const bit_params* p) {
 int ensure_lists(bit_system* s, const bit_params* p) {
   if (s->lists_valid && s->l_acc == p->acc && s->l_delta2 == p->delta2 && s->l_sub == p->mac_sub)
     return BIT_OK;
+  s->lists_valid = false;  // set again only when both passes were issued
   bit_ctx* c = s->ctx;
   const int n = s->tree.n;
   const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
@@ -353,6 +357,7 @@ int bit_ctx_create(int device, size_t hbm_budget_bytes, bit_ctx** out) {
 int bit_ctx_destroy(bit_ctx* c) {
   if (!c) return BIT_OK;
   (void)hipStreamSynchronize(c->stream);
+  for (auto& b : c->user_bufs) (void)hipFree(b.first);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
   return BIT_OK;
@@ -375,6 +380,54 @@ size_t bit_ctx_bytes_in_use(const bit_ctx* c) { return c ? c->in_use : 0; }
 int bit_ctx_inject_alloc_failures(bit_ctx* c, int k) {
   if (!c) return fail(BIT_E_INVALID, "null context");
   c->inject = k < 0 ? 0 : k;
+  return BIT_OK;
+}
+
+int bit_device_alloc(bit_ctx* c, size_t bytes, void** out) {
+  if (!c || !out) return fail(BIT_E_INVALID, "null context or out");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(c->device));
+  if (bytes == 0) bytes = 8;
+  if (c->budget && c->in_use + bytes > c->budget)
+    return fail(BIT_E_OOM, "HBM budget exceeded by a device buffer of " + std::to_string(bytes));
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(BIT_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  c->in_use += bytes;
+  c->user_bufs.emplace_back(p, bytes);
+  *out = p;
+  return BIT_OK;
+}
+
+int bit_device_free(bit_ctx* c, void* p) {
+  if (!c) return fail(BIT_E_INVALID, "null context");
+  if (!p) return BIT_OK;
+  for (size_t i = 0; i < c->user_bufs.size(); i++) {
+    if (c->user_bufs[i].first == p) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipFree(p));
+      c->in_use -= c->user_bufs[i].second;
+      c->user_bufs.erase(c->user_bufs.begin() + i);
+      return BIT_OK;
+    }
+  }
+  return fail(BIT_E_INVALID, "pointer was not allocated by bit_device_alloc on this context");
+}
+
+int bit_copy_to_device(bit_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return fail(BIT_E_INVALID, "null argument");
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
+int bit_copy_to_host(bit_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return fail(BIT_E_INVALID, "null argument");
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return BIT_OK;
 }
 
@@ -430,12 +483,9 @@ static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, 
     }
     const double* srt[4] = {a.data(), b.data(), c.data(), d.data()};
     rc = finish_system(ctx, s, srt, nullptr);
-    if (rc == BIT_OK) {
-      hipError_t e = hipStreamSynchronize(ctx->stream);  // host staging dies here
-      if (e != hipSuccess) rc = fail(BIT_E_HIP, hipGetErrorString(e));
-    }
   }
   if (rc != BIT_OK) {
+    (void)hipStreamSynchronize(ctx->stream);  // no copy may still read host staging
     delete s;
     return rc;
   }
@@ -522,6 +572,9 @@ int bit_moments(bit_system* s, int nterms) {
   HIP_TRY(hipSetDevice(c->device));
   const bit::HostTree& T = s->tree;
   const int nt = T.numtree, nc = ncoef(nterms);
+  // State changes only after every allocation and upload succeeded: a failed
+  // call (e.g. BIT_E_OOM) leaves the system as it was, never half-initialised.
+  s->mom_nterms = 0;
   if (s->nchunks == 0) {
     std::vector<int> cn, c0(nt), ck(nt), cr;
     for (int i = 0; i < nt; i++) {
@@ -535,11 +588,12 @@ int bit_moments(bit_system* s, int nterms) {
       }
       ck[i] = k;
     }
-    s->nchunks = (int)cn.size();
     BIT_TRY(upload(c, s->chunk_node, cn.data(), cn.size() * sizeof(int)));
     BIT_TRY(upload(c, s->chunk_rng, cr.data(), cr.size() * sizeof(int)));
     BIT_TRY(upload(c, s->node_chunk0, c0.data(), c0.size() * sizeof(int)));
     BIT_TRY(upload(c, s->node_nchunk, ck.data(), ck.size() * sizeof(int)));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // host tables die at scope end
+    s->nchunks = (int)cn.size();
   }
   BIT_TRY(s->partial.ensure(c, (size_t)s->nchunks * nc * sizeof(double)));
   BIT_TRY(s->mom.ensure(c, (size_t)nt * nc * sizeof(double)));
@@ -717,6 +771,7 @@ int bit_direct_potential(bit_system* s, double kappa, double delta2, int ntarget
   int nsplit = std::max(1, std::min(1024, 4096 / tblocks));
   nsplit = std::min(nsplit, std::max(1, n / 1024));
   BIT_TRY(upload(c, s->tslot, slot.data(), slot.size() * sizeof(int)));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // `slot` is a host temporary
   BIT_TRY(s->dpart.ensure(c, (size_t)nsplit * ntargets * sizeof(double)));
   BIT_TRY(s->dout.ensure(c, (size_t)ntargets * sizeof(double)));
   HIP_TRY(bit::dev::direct_potential(s->view(), kappa, delta2, ntargets, s->tslot.as<int>(),

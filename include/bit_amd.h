@@ -86,6 +86,14 @@ size_t bit_ctx_bytes_in_use(const bit_ctx *ctx);
  * BIT_E_OOM (k = 0 disables). */
 int bit_ctx_inject_alloc_failures(bit_ctx *ctx, int k);
 
+/* Device buffers for callers without a GPU framework (no counterpart in the
+ * reference: plumbing so that inputs and outputs can stay resident in HBM).
+ * Allocations count against the context's HBM budget. */
+int bit_device_alloc(bit_ctx *ctx, size_t bytes, void **out);
+int bit_device_free(bit_ctx *ctx, void *ptr);
+int bit_copy_to_device(bit_ctx *ctx, void *dst, const void *src, size_t bytes);
+int bit_copy_to_host(bit_ctx *ctx, void *dst, const void *src, size_t bytes);
+
 /* Builds the tree and uploads the sorted sources.  Replaces treemake
  * (tree3d.cpp:1345-1551; tree3d.h:115): same bisection rules, node numbering,
  * children order and member lists (leaf size maxmemb = reference `maxmemb`).

@@ -149,17 +149,49 @@ def test_clean_mac_matches_oracle(ctx, oracle, kappa, d2, nterms, acc):
 
 
 def test_device_resident_inputs(ctx):
-    torch = pytest.importorskip("torch")
     P = workloads.gaussian_blobs(50000, seed=22)
-    dev = [torch.tensor(a, dtype=torch.float64, device="cuda") for a in (P.x, P.y, P.z, P.q)]
+    dev = [bit_amd.DeviceArray.from_numpy(ctx, a) for a in (P.x, P.y, P.z, P.q)]
+    out = bit_amd.DeviceArray(ctx, P.n)
     s1 = bit_amd.System(ctx, *dev, 300)
     s2 = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 300)
-    out = torch.empty(P.n, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    assert np.array_equal(s1.export_tree().perm, s2.export_tree().perm)
     s1.potential(8, 0.5, 0.5, 1e-4, None, "fp32", out=out)
-    a = out.cpu().numpy()
+    a = out.numpy()
     b = s2.potential(8, 0.5, 0.5, 1e-4, None, "fp32")
     assert np.array_equal(a, b)
+    for d in dev + [out]:
+        d.free()
+
+
+TORCH_INTEROP = """
+import numpy as np, torch
+import bit_amd
+from bit_amd import workloads
+P = workloads.uniform_cube(20000, seed=26)
+dev = [torch.tensor(a, dtype=torch.float64, device="cuda") for a in (P.x, P.y, P.z, P.q)]
+out = torch.empty(P.n, dtype=torch.float64, device="cuda")
+ctx = bit_amd.Context(0)
+ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+s = bit_amd.System(ctx, *dev, 200)
+s.potential(7, 0.5, 0.3, 0.0, None, "fp64", out=out)
+torch.cuda.synchronize()
+a = out.cpu().numpy()
+b = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 200).potential(7, 0.5, 0.3, 0.0, None, "fp64")
+assert np.array_equal(a, b), float(np.abs(a - b).max())
+print("torch interop ok")
+"""
+
+
+def test_torch_interop_subprocess():
+    """torch tensors in and out, torch's stream; torch imported first so both
+    share one HIP runtime (own process: this session already loaded one)."""
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    r = subprocess.run([sys.executable, "-c", TORCH_INTEROP], capture_output=True, text=True,
+                       timeout=300, cwd=__import__("conftest").ROOT)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "torch interop ok" in r.stdout
 
 
 def test_repeat_is_deterministic(ctx):
