@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -177,13 +178,21 @@ class Context:
     """One device + one stream + an HBM budget (bytes, 0 = unlimited)."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.h = None
+        self._systems = weakref.WeakSet()
+        self._arrays = weakref.WeakSet()
         h = C.c_void_p()
         _check(lib.bit_ctx_create(int(device), int(hbm_budget), C.byref(h)))
         self.h = h
 
     def close(self):
+        """Closes the context's systems and device arrays, then the context."""
         if self.h:
-            lib.bit_ctx_destroy(self.h)
+            for s in list(self._systems):
+                s.close()
+            for a in list(self._arrays):
+                a.ptr = None  # freed with the context
+            _check(lib.bit_ctx_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -214,6 +223,7 @@ class DeviceArray:
         h = C.c_void_p()
         _check(lib.bit_device_alloc(ctx.h, self.n * self.dtype.itemsize, C.byref(h)))
         self.ptr = h.value
+        ctx._arrays.add(self)
 
     @classmethod
     def from_numpy(cls, ctx: Context, a) -> "DeviceArray":
@@ -232,8 +242,14 @@ class DeviceArray:
 
     def free(self):
         if self.ptr and self.ctx.h:
-            lib.bit_device_free(self.ctx.h, C.c_void_p(self.ptr))
+            _check(lib.bit_device_free(self.ctx.h, C.c_void_p(self.ptr)))
         self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def _dev_ptr(a):
@@ -255,7 +271,7 @@ class TreeExport:
     perm: np.ndarray      # sorted slot -> original index
 
     def members(self, i: int) -> np.ndarray:
-        """Slots of node i in sorted order (a leaf's are ascending indices)."""
+        """Original indices of node i in slot order (inside a leaf: Morton order)."""
         return self.perm[self.begin[i]:self.end[i]]
 
     def member_list(self, i: int) -> np.ndarray:
@@ -300,13 +316,15 @@ class System:
             _check(lib.bit_system_create(ctx.h, n, _ptr(x), _ptr(y), _ptr(z), _ptr(q),
                                          int(maxmemb), C.byref(h)))
         self.h = h
+        ctx._systems.add(self)
         self.n = n
         self.numtree = lib.bit_system_numtree(h)
         self.maxdepth = lib.bit_system_maxdepth(h)
 
     def close(self):
         if getattr(self, "h", None):
-            lib.bit_system_destroy(self.h)
+            if self.ctx.h:  # a closed context already destroyed its systems
+                lib.bit_system_destroy(self.h)
             self.h = None
 
     def __del__(self):

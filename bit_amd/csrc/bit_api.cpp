@@ -59,6 +59,7 @@ struct bit_ctx {
   size_t in_use = 0;
   int inject = 0;
   std::vector<std::pair<void*, size_t>> user_bufs;  // bit_device_alloc
+  int live_systems = 0;
 };
 
 namespace {
@@ -356,6 +357,9 @@ int bit_ctx_create(int device, size_t hbm_budget_bytes, bit_ctx** out) {
 
 int bit_ctx_destroy(bit_ctx* c) {
   if (!c) return BIT_OK;
+  if (c->live_systems > 0)
+    return fail(BIT_E_STATE, std::to_string(c->live_systems) +
+                                 " system(s) still alive: destroy them before their context");
   (void)hipStreamSynchronize(c->stream);
   for (auto& b : c->user_bufs) (void)hipFree(b.first);
   if (c->own) (void)hipStreamDestroy(c->own);
@@ -491,6 +495,7 @@ static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, 
   }
   auto t2 = std::chrono::steady_clock::now();
   s->stats.ms_upload = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  ctx->live_systems++;
   *out = s;
   return BIT_OK;
 }
@@ -508,6 +513,7 @@ int bit_system_create_device(bit_ctx* ctx, int n, const double* x, const double*
 int bit_system_destroy(bit_system* s) {
   if (!s) return BIT_OK;
   (void)hipStreamSynchronize(s->ctx->stream);
+  s->ctx->live_systems--;
   delete s;
   return BIT_OK;
 }

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 
 namespace bit {
@@ -272,6 +273,66 @@ struct Splitter {
   }
 };
 
+// Spreads the low 10 bits of v so that two zero bits separate each bit.
+inline uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+inline uint32_t quant(double v, double lo, double hi) {
+  if (!(hi > lo)) return 0;
+  double t = (v - lo) / (hi - lo) * 1023.0;
+  if (t < 0) t = 0;
+  if (t > 1023) t = 1023;
+  return (uint32_t)t;
+}
+
+// Orders the slots of every leaf along a Morton curve over the leaf's box, so
+// that 64 consecutive slots (one target batch) are close in space and take
+// the same traversal decisions.  Leaf ranges, and therefore every node's
+// member set, are unchanged.
+void order_leaves(HostTree& t, Arrays& a, int threads) {
+  std::vector<int> leaves;
+  for (int i = 0; i < t.numtree; i++)
+    if (t.children[(size_t)i * 8] < 0 && t.end[i] - t.begin[i] > 1) leaves.push_back(i);
+#pragma omp parallel num_threads(threads)
+  {
+    std::vector<std::pair<uint32_t, int>> key;
+#pragma omp for schedule(dynamic, 4)
+    for (size_t li = 0; li < leaves.size(); li++) {
+      const int lf = leaves[li];
+      const int b = t.begin[lf], e = t.end[lf];
+      const double* f = &t.box[(size_t)lf * BOX_FIELDS];
+      key.resize(e - b);
+      for (int s = b; s < e; s++) {
+        uint32_t k = spread3(quant(a.X[s], f[BX0], f[BX1])) |
+                     (spread3(quant(a.Y[s], f[BY0], f[BY1])) << 1) |
+                     (spread3(quant(a.Z[s], f[BZ0], f[BZ1])) << 2);
+        key[s - b] = {k, s};
+      }
+      std::stable_sort(key.begin(), key.end(),
+                       [](const std::pair<uint32_t, int>& u, const std::pair<uint32_t, int>& v) {
+                         return u.first < v.first;
+                       });
+      for (int s = b; s < e; s++) {
+        int src = key[s - b].second;
+        a.tP[s] = a.P[src];
+        a.tX[s] = a.X[src];
+        a.tY[s] = a.Y[src];
+        a.tZ[s] = a.Z[src];
+      }
+      std::memcpy(a.P + b, a.tP + b, sizeof(int) * (e - b));
+      std::memcpy(a.X + b, a.tX + b, sizeof(double) * (e - b));
+      std::memcpy(a.Y + b, a.tY + b, sizeof(double) * (e - b));
+      std::memcpy(a.Z + b, a.tZ + b, sizeof(double) * (e - b));
+    }
+  }
+}
+
 }  // namespace
 
 bool build_tree(int n, const double* x, const double* y, const double* z, int maxmemb,
@@ -371,6 +432,7 @@ bool build_tree(int n, const double* x, const double* y, const double* z, int ma
     gen_hi = next;
   }
   out.numtree = gen_hi;
+  order_leaves(out, a, threads);
   out.perm = std::move(P);
   return true;
 }

@@ -19,7 +19,8 @@ struct HostTree {
   int maxdepth = 0;
   std::vector<double> box;    // BOX_FIELDS per node
   std::vector<int> children;  // 8 per node, -1 = none, reference order
-  std::vector<int> begin;     // node i's members are perm[begin[i] .. end[i]-1]
+  std::vector<int> begin;     // node i's members are the set perm[begin[i] .. end[i]-1];
+                              // inside a leaf, slots follow a Morton curve over the box
   std::vector<int> end;
   std::vector<int> depth;
   std::vector<int> perm;      // sorted slot -> original particle index

@@ -75,6 +75,8 @@ const char *bit_last_error(void);
 /* Context: one device, one stream (default priority), an HBM budget for
  * everything the context allocates (0 = no limit). */
 int bit_ctx_create(int device, size_t hbm_budget_bytes, bit_ctx **out);
+/* Fails with BIT_E_STATE while systems of the context are alive; frees the
+ * context's bit_device_alloc buffers. */
 int bit_ctx_destroy(bit_ctx *ctx);
 /* Use an existing hipStream_t (e.g. torch's current stream) instead of the
  * context's own; NULL restores the context stream. */
@@ -110,8 +112,9 @@ int bit_system_maxdepth(const bit_system *sys);
 /* Host copy of the tree: boxes[13*numtree] = x0,x1,y0,y1,z0,z1,xmid,ymid,zmid,
  * aspectxy,aspectxz,aspectyz,sqradius (TREE, tree3d.h:51-74); children[8*numtree];
  * begin/end[numtree] index perm[n]; node i's member list (TREE::members, ascending
- * original indices) is the sorted perm[begin[i] .. end[i]-1]; for a leaf that
- * range is already ascending.  Any pointer may be NULL. */
+ * original indices) is the sorted perm[begin[i] .. end[i]-1].  Slots inside a
+ * leaf follow a Morton curve over the leaf box (the order targets are batched
+ * in).  Any pointer may be NULL. */
 int bit_system_export_tree(const bit_system *sys, double *boxes, int *children, int *begin,
                            int *end, int *perm);
 

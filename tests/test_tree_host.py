@@ -14,10 +14,7 @@ def _same_tree(t, children, boxes, members):
     assert np.array_equal(t.children, children)
     assert np.array_equal(t.boxes.view(np.uint64), boxes.view(np.uint64))
     for i in range(t.numtree):
-        got = t.members(i)
-        if children[i, 0] >= 0:  # internal node: its range is its children's, in order
-            got = np.sort(got)
-        assert np.array_equal(got, members[i]), i
+        assert np.array_equal(t.member_list(i), members[i]), i
 
 
 @pytest.mark.parametrize("name", FIXTURES)
