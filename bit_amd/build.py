@@ -75,6 +75,13 @@ def build(incremental: bool = False) -> str:
               "-Wall", "-I" + SRC, *inc, os.path.join(SRC, "tree3d.cpp"), "-o", tmp,
               "-L" + PKG, "-lbit_amd", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, LIBTREE)
+        # caller of the reference interface used by tests/test_dropin.py
+        drv = os.path.join(ROOT, "tests", "cpp", "dropin_treecode.cpp")
+        if os.path.exists(drv):
+            os.makedirs(os.path.join(ROOT, "tests", "cpp", "bin"), exist_ok=True)
+            _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I" + SRC, "-I" + os.path.join(ROOT, "include"),
+                  drv, "-o", os.path.join(ROOT, "tests", "cpp", "bin", "dropin_treecode"),
+                  "-L" + PKG, "-ltree3d", "-lbit_amd", "-Wl,-rpath," + PKG])
     return LIB
 
 

@@ -1,0 +1,435 @@
+// tree3d.cpp -- the reference library interface (tree3d.h) on top of the
+// MI355X C-ABI (include/bit_amd.h).  Built as bit_amd/libtree3d.so.
+//
+// A tree made by treemake is bound to its PARTICLE array; the first evaluation
+// that needs the device creates a bit_system for it (same tree: both are built
+// by the same bit-exact host code) and every batched result is cached per
+// (tree, quantity, nterms, acc, alpha) until treeempty releases the tree.
+#include "tree3d.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <set>
+#include <tuple>
+#include <vector>
+
+#include "bit_amd.h"
+#include "problem.h"
+
+namespace {
+
+std::mutex g_mu;
+bit_ctx* g_ctx = nullptr;
+
+// Self term shielded_treepot adds for the target particle itself, per unit
+// tot_charge/weight (tree3d.cpp:3575-3596 of the reference).
+const double kSelfIntG = -0.0000483811229725;
+
+enum Quantity { kField = 0, kPot = 1, kShielded = 2 };
+
+struct Bound {
+  PARTICLE* part = nullptr;
+  int npart = 0;
+  int numtree = 0;
+  int maxmemb = 0;
+  bit_system* sys = nullptr;
+  std::map<std::tuple<int, int, double, double>, std::vector<double>> cache;
+};
+std::map<TREE*, Bound> g_bound;
+std::set<double***> g_own_moments;  // moment blocks allocated by treemoments
+
+bit_ctx* context() {
+  if (!g_ctx && bit_ctx_create(0, 0, &g_ctx) != BIT_OK) {
+    cout << "[tree3d] no GPU context: " << bit_last_error() << endl;
+    g_ctx = nullptr;
+  }
+  return g_ctx;
+}
+
+void coords(const PARTICLE* part, int n, std::vector<double>& x, std::vector<double>& y,
+            std::vector<double>& z, std::vector<double>& q) {
+  x.resize(n);
+  y.resize(n);
+  z.resize(n);
+  q.resize(n);
+  for (int i = 0; i < n; i++) {
+    x[i] = part[i].x;
+    y[i] = part[i].y;
+    z[i] = part[i].z;
+    q[i] = part[i].tot_charge;
+  }
+}
+
+bit_system* make_system(PARTICLE* part, int n, int maxmemb) {
+  bit_ctx* c = context();
+  if (!c) return nullptr;
+  std::vector<double> x, y, z, q;
+  coords(part, n, x, y, z, q);
+  bit_system* s = nullptr;
+  if (bit_system_create(c, n, x.data(), y.data(), z.data(), q.data(), maxmemb, &s) != BIT_OK) {
+    cout << "[tree3d] " << bit_last_error() << endl;
+    return nullptr;
+  }
+  return s;
+}
+
+void release(Bound& b) {
+  if (b.sys) bit_system_destroy(b.sys);
+  b.sys = nullptr;
+  b.cache.clear();
+}
+
+bit_params params(int nterms, double acc, double kappa, double delta2, double mac_sub) {
+  bit_params p;
+  bit_params_default(&p);
+  p.nterms = nterms;
+  p.acc = acc;
+  p.kappa = kappa;
+  p.delta2 = delta2;
+  p.mac_sub = mac_sub;
+  p.precision = BIT_FP64;
+  return p;
+}
+
+// Field (3 per particle) or potential of every particle, in the reference
+// semantics of the per-target function the quantity stands for.
+bool evaluate(bit_system* s, int n, const PARTICLE* part, int what, int nterms, double acc,
+              double alpha, std::vector<double>& out) {
+  if (what == kField) {
+    out.resize(3 * (size_t)n);
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL);  // treeforce: rs = r^2 + DEL
+    if (bit_eval_field(s, &p, out.data(), 0) != BIT_OK) return false;
+    return true;
+  }
+  out.resize(n);
+  if (what == kPot) {
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL);  // treepotential: rs = r^2 + DEL
+    return bit_eval_potential(s, &p, out.data(), 0) == BIT_OK;
+  }
+  // shielded_treepot: unregularized kernel, MAC shifted by DEL, result -phi plus
+  // the target's self term.
+  bit_params p = params(nterms, acc, alpha, 0.0, DEL);
+  if (bit_eval_potential(s, &p, out.data(), 0) != BIT_OK) return false;
+  for (int i = 0; i < n; i++)
+    out[i] = -out[i] + part[i].tot_charge * kSelfIntG / part[i].weight;
+  return true;
+}
+
+// The cached batched result for a per-target call, or nullptr (message printed).
+const double* lookup(const char* fn, TREE* tree, PARTICLE* part, int ind, int p, double x,
+                     double y, double z, int what, int nterms, double acc, double alpha) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_bound.find(tree);
+  if (it == g_bound.end() || it->second.part != part) {
+    cout << "[" << fn << "] tree was not built by treemake for this particle array" << endl;
+    return nullptr;
+  }
+  Bound& b = it->second;
+  if (ind != 0 || p < 0 || p >= b.npart || part[p].x != x || part[p].y != y || part[p].z != z) {
+    cout << "[" << fn << "] served for whole-tree calls (ind == 0) at particle targets "
+         << "(0 <= p < npart, x,y,z = particle p) only" << endl;
+    return nullptr;
+  }
+  auto key = std::make_tuple(what, nterms, acc, alpha);
+  auto c = b.cache.find(key);
+  if (c == b.cache.end()) {
+    if (!b.sys) b.sys = make_system(part, b.npart, b.maxmemb);
+    if (!b.sys) return nullptr;
+    std::vector<double> v;
+    if (!evaluate(b.sys, b.npart, part, what, nterms, acc, alpha, v)) {
+      cout << "[" << fn << "] " << bit_last_error() << endl;
+      return nullptr;
+    }
+    c = b.cache.emplace(key, std::move(v)).first;
+  }
+  return c->second.data();
+}
+
+void fill_tree_node(TREE& t, const double* f) {
+  t.x0 = f[0]; t.x1 = f[1]; t.y0 = f[2]; t.y1 = f[3]; t.z0 = f[4]; t.z1 = f[5];
+  t.xmid = f[6]; t.ymid = f[7]; t.zmid = f[8];
+  t.aspectxy = f[9]; t.aspectxz = f[10]; t.aspectyz = f[11];
+  t.sqradius = f[12];
+}
+
+// Whole-system driver: part[i].{x,y,z}force from the batched result.
+void drive(const char* fn, int npart, PARTICLE* part, int what, int nterms, double acc,
+           double alpha, int maxmemb, bool print_numtree) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  bit_system* s = make_system(part, npart, maxmemb);
+  if (!s) return;
+  if (print_numtree) cout << "numtree = " << bit_system_numtree(s) << endl;
+  std::vector<double> v;
+  if (!evaluate(s, npart, part, what, nterms, acc, alpha, v)) {
+    cout << "[" << fn << "] " << bit_last_error() << endl;
+    bit_system_destroy(s);
+    return;
+  }
+  for (int i = 0; i < npart; i++) {
+    double f = charge_constant * part[i].tot_charge;
+    if (what == kField) {
+      part[i].xforce = v[3 * (size_t)i] * f;
+      part[i].yforce = v[3 * (size_t)i + 1] * f;
+      part[i].zforce = v[3 * (size_t)i + 2] * f;
+    } else {
+      part[i].xforce = v[i] * f;
+    }
+  }
+  bit_system_destroy(s);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// tree
+// ---------------------------------------------------------------------------
+void treeinit(TREE* t, double x0, double x1, double y0, double y1, double z0, double z1) {
+  double x = 0.5 * (x1 + x0), y = 0.5 * (y1 + y0), z = 0.5 * (z1 + z0);
+  double f[13] = {x0, x1, y0, y1, z0, z1, x, y, z, fabs((x1 - x0) / (y1 - y0)),
+                  fabs((x1 - x0) / (z1 - z0)), fabs((y1 - y0) / (z1 - z0)), 0.0};
+  double hx = x1 - x, hy = y1 - y, hz = z1 - z;
+  f[12] = hx * hx + hy * hy + hz * hz;
+  fill_tree_node(*t, f);
+  for (int i = 0; i < NUMCHILD; i++) t->children[i] = -1;
+}
+
+void treeshrink(TREE* t, PARTICLE* part) {
+  int j = t->members[0];
+  double mx = part[j].x, Mx = mx, my = part[j].y, My = my, mz = part[j].z, Mz = mz;
+  for (int i = 0; i < t->nummemb; i++) {
+    j = t->members[i];
+    Mx = std::max(Mx, part[j].x);
+    mx = std::min(mx, part[j].x);
+    My = std::max(My, part[j].y);
+    my = std::min(my, part[j].y);
+    Mz = std::max(Mz, part[j].z);
+    mz = std::min(mz, part[j].z);
+  }
+  treeinit(t, mx, Mx, my, My, mz, Mz);
+}
+
+int treedivide(int dir, TREE* split, TREE* c1, TREE* c2, PARTICLE* part) {
+  double mid = dir == 0 ? split->xmid : (dir == 1 ? split->ymid : split->zmid);
+  std::vector<int> side[2];
+  for (int j = 0; j < split->nummemb; j++) {
+    int i = split->members[j];
+    double v = dir == 0 ? part[i].x : (dir == 1 ? part[i].y : part[i].z);
+    side[v <= mid ? 0 : 1].push_back(i);
+  }
+  TREE* dst = c1;
+  int made = 0;
+  for (int s = 0; s < 2; s++) {
+    if (side[s].empty()) continue;
+    delete[] dst->members;
+    dst->nummemb = (int)side[s].size();
+    dst->members = new int[side[s].size()];
+    std::copy(side[s].begin(), side[s].end(), dst->members);
+    treeshrink(dst, part);
+    made++;
+    dst = c2;
+  }
+  return made;
+}
+
+int treemake(int npart, int nterms, int maxmemb, PARTICLE* part, TREE* tree) {
+  (void)nterms;
+  std::vector<double> x, y, z, q;
+  coords(part, npart, x, y, z, q);
+  bit_tree* t = nullptr;
+  if (bit_tree_build(npart, x.data(), y.data(), z.data(), maxmemb, &t) != BIT_OK) {
+    cout << "[treemake] " << bit_last_error() << endl;
+    return 0;
+  }
+  const int nt = bit_tree_numtree(t);
+  std::vector<double> boxes(13 * (size_t)nt);
+  std::vector<int> ch(8 * (size_t)nt), b(nt), e(nt), perm(npart);
+  bit_tree_export(t, boxes.data(), ch.data(), b.data(), e.data(), perm.data());
+  bit_tree_destroy(t);
+  for (int i = 0; i < nt; i++) {
+    TREE& d = tree[i];
+    fill_tree_node(d, &boxes[13 * (size_t)i]);
+    for (int c = 0; c < NUMCHILD; c++) d.children[c] = ch[8 * (size_t)i + c];
+    d.nummemb = e[i] - b[i];
+    delete[] d.members;
+    d.members = new int[d.nummemb];
+    std::copy(perm.begin() + b[i], perm.begin() + e[i], d.members);
+    std::sort(d.members, d.members + d.nummemb);  // reference order: ascending indices
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  Bound& bd = g_bound[tree];
+  release(bd);
+  bd.part = part;
+  bd.npart = npart;
+  bd.numtree = nt;
+  bd.maxmemb = maxmemb;
+  return nt;
+}
+
+void treeempty(int numtree, TREE* tree) {
+  if (!tree) return;
+  for (int i = 0; i < numtree; i++) {
+    delete[] tree[i].members;
+    tree[i].members = nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int i = 0; i < numtree; i++) {
+    if (!tree[i].moments) continue;
+    if (g_own_moments.erase(tree[i].moments)) {  // one block made by treemoments
+      delete[] tree[i].moments[0][0];
+      delete[] tree[i].moments[0];
+    }
+    delete[] tree[i].moments;  // as the reference: the outer array only otherwise
+    tree[i].moments = nullptr;
+  }
+  auto it = g_bound.find(tree);
+  if (it != g_bound.end()) {
+    release(it->second);
+    g_bound.erase(it);
+  }
+}
+
+void treemoments(int numtree, int nterms, TREE* tree, PARTICLE* part) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_bound.find(tree);
+  if (it == g_bound.end() || it->second.part != part) {
+    cout << "[treemoments] tree was not built by treemake for this particle array" << endl;
+    return;
+  }
+  Bound& b = it->second;
+  if (!b.sys) b.sys = make_system(part, b.npart, b.maxmemb);
+  if (!b.sys) return;
+  if (bit_moments(b.sys, nterms) != BIT_OK) {
+    cout << "[treemoments] " << bit_last_error() << endl;
+    return;
+  }
+  const size_t cube = (size_t)nterms * nterms * nterms;
+  std::vector<double> m(cube * numtree);
+  if (bit_moments_download(b.sys, m.data()) != BIT_OK) {
+    cout << "[treemoments] " << bit_last_error() << endl;
+    return;
+  }
+  for (int t = 0; t < numtree; t++) {
+    if (!tree[t].moments) {
+      // one contiguous block, indexable as moments[i][j][k]
+      double* data = new double[cube];
+      double** rows = new double*[(size_t)nterms * nterms];
+      tree[t].moments = new double**[nterms];
+      g_own_moments.insert(tree[t].moments);
+      for (int i = 0; i < nterms; i++) {
+        tree[t].moments[i] = rows + (size_t)i * nterms;
+        for (int j = 0; j < nterms; j++) rows[(size_t)i * nterms + j] = data + ((size_t)i * nterms + j) * nterms;
+      }
+    }
+    for (int i = 0; i < nterms; i++)
+      for (int j = 0; j < nterms; j++)
+        for (int k = 0; k < nterms; k++)
+          tree[t].moments[i][j][k] = m[cube * t + ((size_t)i * nterms + j) * nterms + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-target evaluation
+// ---------------------------------------------------------------------------
+double treeforce(int ind, int p, int nterms, double x, double y, double z, double acc,
+                 double* xfor, double* yfor, double* zfor, TREE* tree, PARTICLE* part) {
+  const double* E = lookup("treeforce", tree, part, ind, p, x, y, z, kField, nterms, acc, 0.0);
+  if (E) {
+    *xfor += E[3 * (size_t)p];
+    *yfor += E[3 * (size_t)p + 1];
+    *zfor += E[3 * (size_t)p + 2];
+  }
+  return 0;
+}
+
+double treepotential(int ind, int p, int nterms, double x, double y, double z, double acc,
+                     double* pot, TREE* tree, PARTICLE* part) {
+  const double* v = lookup("treepotential", tree, part, ind, p, x, y, z, kPot, nterms, acc, 0.0);
+  if (v) *pot += v[p];
+  return 0;
+}
+
+double shielded_treepot(int ind, int p, int nterms, double x, double y, double z, double alpha,
+                        double acc, double* pot, TREE* tree, PARTICLE* part, int, int, int,
+                        double, double, double) {
+  const double* v =
+      lookup("shielded_treepot", tree, part, ind, p, x, y, z, kShielded, nterms, acc, alpha);
+  if (v) *pot += v[p];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// whole-system drivers
+// ---------------------------------------------------------------------------
+void freespace_particles_tree(int npart, PARTICLE* part, int nterms, double acc, TREE*, int,
+                              int maxmemb) {
+  drive("freespace_particles_tree", npart, part, kField, nterms, acc, 0.0, maxmemb, true);
+}
+
+void freespace_pot_tree(int npart, PARTICLE* part, int nterms, double acc, TREE*, int,
+                        int maxmemb) {
+  drive("freespace_pot_tree", npart, part, kPot, nterms, acc, 0.0, maxmemb, false);
+}
+
+void shielded_freespace_pot_tree(int npart, PARTICLE* part, double alpha, int nterms,
+                                 double acc, TREE*, int, int maxmemb) {
+  drive("shielded_freespace_pot_tree", npart, part, kShielded, nterms, acc, alpha, maxmemb,
+        false);
+}
+
+void shielded_freespace_pot_ds(int npart, PARTICLE* part, double alpha) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  bit_system* s = make_system(part, npart, std::max(1, npart));
+  if (!s) return;
+  std::vector<int> idx(npart);
+  for (int i = 0; i < npart; i++) idx[i] = i;
+  std::vector<double> phi(npart);
+  if (bit_direct_potential(s, alpha, 0.0, npart, idx.data(), phi.data()) != BIT_OK) {
+    cout << "[shielded_freespace_pot_ds] " << bit_last_error() << endl;
+  } else {
+    for (int i = 0; i < npart; i++)  // shielded_pot_particle_ds returns -phi
+      part[i].xforce = -phi[i] * charge_constant * part[i].tot_charge;
+  }
+  bit_system_destroy(s);
+}
+
+// ---------------------------------------------------------------------------
+// particle set-up (the reference's test-data generators)
+// ---------------------------------------------------------------------------
+double RandL(void) { return rand() / ((double)RAND_MAX); }
+
+void boxparticles(int npart, double dlength, double dheight, double ddepth, PARTICLE* part,
+                  double density) {
+  const double weight = density * dlength * dheight * ddepth / npart;
+  srand(100);
+  const double hx = dlength / 2, hy = dheight / 2, hz = ddepth / 2;
+  for (int i = 0; i < npart; i++) {
+    part[i].charge = -1.0;
+    part[i].weight = weight;
+    part[i].x = hx / 2 + RandL() * hx;
+    part[i].y = hy / 2 + RandL() * hy;
+    part[i].z = hz / 2 + RandL() * hz;
+    part[i].u = (RandL() - 0.5) * 1e6;
+    part[i].v = (RandL() - 0.5) * 1e6;
+    part[i].w = (RandL() - 0.5) * 1e6;
+    part[i].tot_charge = part[i].charge * part[i].weight * elec;
+  }
+}
+
+int bounding_box(int npart, PARTICLE* part, double& Mx, double& mx, double& My, double& my,
+                 double& Mz, double& mz) {
+  mx = Mx = part[0].x;
+  my = My = part[0].y;
+  mz = Mz = part[0].z;
+  for (int i = 0; i < npart; i++) {
+    mx = std::min(mx, part[i].x);
+    Mx = std::max(Mx, part[i].x);
+    my = std::min(my, part[i].y);
+    My = std::max(My, part[i].y);
+    mz = std::min(mz, part[i].z);
+    Mz = std::max(Mz, part[i].z);
+  }
+  return 0;
+}
+
+double sq(double x) { return x * x; }

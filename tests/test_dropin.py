@@ -1,0 +1,57 @@
+"""The drop-in reference interface (bit_amd/libtree3d.so, bit_amd/src/tree3d.h)
+driven by a C++ caller written against the reference headers
+(tests/cpp/dropin_treecode.cpp), compared with the reference's own outputs."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_fixture, rel_max
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "tests", "cpp", "bin", "dropin_treecode")
+INTG = -0.0000483811229725
+
+
+def _run(*args):
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} not built: run python -m bit_amd.build")
+    r = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "Test PASSED" in r.stdout, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_treecode_mp_field(tmp_path):
+    """tests/treecode_mp.cpp's recipe: boxparticles(16000), maxmemb 100,
+    nterms 5, serial == OpenMP treeforce; values equal the reference's."""
+    out = tmp_path / "f.bin"
+    _run("field", str(out))
+    E = np.fromfile(out).reshape(-1, 3)
+    fx = load_fixture("box16k")
+    assert rel_max(E[fx["tidx"]], fx["field_tree"]) < 1e-10
+
+
+def test_treecode_convergence(tmp_path):
+    """tests/treecode.cpp's criterion: the max-abs field error against the
+    direct sum does not grow with nterms (1..4)."""
+    out = tmp_path / "c.bin"
+    _run("conv", str(out))
+    E = np.fromfile(out).reshape(4, -1, 3)
+    fx = load_fixture("box16k")
+    ds = fx["field_ds"]
+    errs = [float(np.abs(E[k][fx["tidx"]] - ds).max()) for k in range(4)]
+    assert all(b <= a for a, b in zip(errs, errs[1:])), errs
+
+
+def test_shielded_treepot_and_batch_driver(tmp_path):
+    fx = load_fixture("blobs12k_screened")
+    n = fx["x"].size
+    inp = tmp_path / "in.bin"
+    hdr = np.array([n, 200, 9, 0.7, 0.5], dtype=np.float64)
+    np.concatenate([hdr, fx["x"], fx["y"], fx["z"], fx["q"]]).tofile(inp)
+    out = tmp_path / "p.bin"
+    _run("screened", str(inp), str(out))
+    raw = np.fromfile(out)
+    want = -fx["phi_screened"] + fx["q"][fx["tidx"]] * INTG  # the reference's raw value
+    assert rel_max(raw[fx["tidx"]], want) < 1e-10
