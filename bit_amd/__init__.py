@@ -80,7 +80,7 @@ EXPORTS = [
     "bit_eval_field", "bit_system_stats", "bit_direct_potential", "bit_tree_build",
     "bit_tree_destroy", "bit_tree_numtree", "bit_tree_maxdepth", "bit_tree_export",
     "bit_system_export_lists", "bit_device_alloc", "bit_device_free", "bit_copy_to_device",
-    "bit_copy_to_host",
+    "bit_copy_to_host", "bit_system_set_charges", "bit_solve_second_kind", "bit_gmres_dense",
 ]
 
 
@@ -121,6 +121,10 @@ def _load():
     L.bit_device_free.argtypes = [vp, vp]
     L.bit_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
     L.bit_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
+    L.bit_system_set_charges.argtypes = [vp, vp, C.c_int]
+    L.bit_solve_second_kind.argtypes = [vp, C.POINTER(Params), C.c_double, C.c_double, vp, vp,
+                                        vp, C.c_int, C.c_double, C.c_int, vp, vp]
+    L.bit_gmres_dense.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_double, C.c_int, vp, vp]
     L.bit_tree_build.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.POINTER(vp)]
     L.bit_tree_destroy.argtypes = [vp]
     L.bit_tree_numtree.argtypes = [vp]
@@ -300,6 +304,17 @@ def build_tree(x, y, z, maxmemb: int) -> TreeExport:
         lib.bit_tree_destroy(h)
 
 
+def gmres_dense(A, b, x0=None, restart=50, tol=1e-10, maxit=1000):
+    """Host-only FP64 restarted GMRES on a dense matrix (the solver core)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = _f64(b)
+    x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
+    it, rr = C.c_int(), C.c_double()
+    _check(lib.bit_gmres_dense(len(b), _ptr(A), _ptr(b), _ptr(x), int(restart), float(tol),
+                               int(maxit), C.byref(it), C.byref(rr)))
+    return x, it.value, rr.value
+
+
 class System:
     """One particle system resident on the device (tree + sorted sources)."""
 
@@ -400,6 +415,28 @@ class System:
                                            _ptr(nm), _ptr(nd)))
         return dict(far_off=fo, far_mask=fm[:nf.value], far_node=fn[:nf.value], near_off=no,
                     near_mask=nm[:nn.value], near_node=nd[:nn.value])
+
+    def set_charges(self, q):
+        """New charges for the same positions (tree and lists are kept)."""
+        if _dev_ptr(q) is not None:
+            _check(lib.bit_system_set_charges(self.h, C.c_void_p(_dev_ptr(q)), 1))
+        else:
+            q = _f64(q)
+            _check(lib.bit_system_set_charges(self.h, _ptr(q), 0))
+
+    def solve_second_kind(self, w, f, diag, self_coef=0.0, sigma0=None, restart=50, tol=1e-6,
+                          maxit=500, nterms=7, acc=0.6, kappa=0.0, delta2=0.0, mac_sub=None,
+                          precision="fp64"):
+        """GMRES solve of diag*s + S_w s (+ self*w*s) = f with treecode
+        matvecs; returns (sigma, iterations, relative residual)."""
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+        w, f = _f64(w), _f64(f)
+        sig = np.zeros(self.n) if sigma0 is None else _f64(sigma0).copy()
+        it, rr = C.c_int(), C.c_double()
+        _check(lib.bit_solve_second_kind(self.h, C.byref(prm), float(diag), float(self_coef),
+                                         _ptr(w), _ptr(f), _ptr(sig), int(restart), float(tol),
+                                         int(maxit), C.byref(it), C.byref(rr)))
+        return sig, it.value, rr.value
 
     def stats(self) -> dict:
         s = Stats()

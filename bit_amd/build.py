@@ -82,6 +82,11 @@ def build(incremental: bool = False) -> str:
             _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I" + SRC, "-I" + os.path.join(ROOT, "include"),
                   drv, "-o", os.path.join(ROOT, "tests", "cpp", "bin", "dropin_treecode"),
                   "-L" + PKG, "-ltree3d", "-lbit_amd", "-Wl,-rpath," + PKG])
+    drv = os.path.join(ROOT, "tests", "cpp", "solver_dense.cpp")
+    if os.path.exists(drv):
+        os.makedirs(os.path.join(ROOT, "tests", "cpp", "bin"), exist_ok=True)
+        _run(["g++", "-O2", "-std=c++17", "-I" + SRC, drv, "-o",
+              os.path.join(ROOT, "tests", "cpp", "bin", "solver_dense")])
     return LIB
 
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/bit_amd.h"
+#include "gmres.h"
 #include "kernels.h"
 #include "tree_build.h"
 
@@ -155,6 +156,8 @@ struct bit_system {
   DevBuf phi_far, out;
   // direct sum
   DevBuf tslot, dpart, dout;
+  // charge updates
+  DevBuf qin;
   bit_stats stats{};
   Timer t_mom, t_lists, t_far, t_near;
 
@@ -755,6 +758,70 @@ int bit_system_export_lists(bit_system* s, const bit_params* p, int* nbatch, int
     near_mask[i] = n[i].mask;
     near_node[i] = n[i].node;
   }
+  return BIT_OK;
+}
+
+int bit_system_set_charges(bit_system* s, const double* q, int on_device) {
+  if (!s || !q) return fail(BIT_E_INVALID, "null system or charges");
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = s->tree.n;
+  const double* src = q;
+  if (!on_device) {
+    if (!finite_all(n, q)) return fail(BIT_E_INVALID, "non-finite charge");
+    BIT_TRY(upload(c, s->qin, q, (size_t)n * sizeof(double)));
+    src = s->qin.as<double>();
+  }
+  s->mom_nterms = 0;  // moments depend on the charges
+  HIP_TRY(bit::dev::set_charges(s->view(), src, s->sq.as<double>(), s->sloc.as<float4>(),
+                                c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // the host array may be reused at once
+  return BIT_OK;
+}
+
+int bit_solve_second_kind(bit_system* s, const bit_params* p, double diag, double self,
+                          const double* w, const double* f, double* sigma, int restart,
+                          double tol, int maxit, int* iters, double* relres) {
+  BIT_TRY(check_params(s, p));
+  if (!w || !f || !sigma) return fail(BIT_E_INVALID, "null w, f or sigma");
+  if (restart < 1 || maxit < 1 || !(tol > 0)) return fail(BIT_E_INVALID, "bad GMRES settings");
+  const int n = s->tree.n;
+  std::vector<double> q(n), phi(n);
+  int rc = BIT_OK;
+  auto matvec = [&](const double* in, double* out) {
+    if (rc != BIT_OK) {
+      for (int i = 0; i < n; i++) out[i] = 0.0;
+      return;
+    }
+    for (int i = 0; i < n; i++) q[i] = w[i] * in[i];
+    rc = bit_system_set_charges(s, q.data(), 0);
+    if (rc == BIT_OK) rc = bit_eval_potential(s, p, phi.data(), 0);
+    if (rc != BIT_OK) {
+      for (int i = 0; i < n; i++) out[i] = 0.0;
+      return;
+    }
+    for (int i = 0; i < n; i++) out[i] = (diag + self * w[i]) * in[i] + phi[i];
+  };
+  bit::GmresResult r = bit::restarted_gmres(matvec, f, sigma, n, restart, tol, maxit);
+  if (rc != BIT_OK) return rc;
+  if (iters) *iters = r.iters;
+  if (relres) *relres = r.relres;
+  return BIT_OK;
+}
+
+int bit_gmres_dense(int n, const double* A, const double* b, double* x, int restart, double tol,
+                    int maxit, int* iters, double* relres) {
+  if (n < 1 || !A || !b || !x) return fail(BIT_E_INVALID, "bad dense GMRES arguments");
+  auto matvec = [&](const double* in, double* out) {
+    for (int i = 0; i < n; i++) {
+      double acc = 0.0;
+      for (int j = 0; j < n; j++) acc += A[(size_t)i * n + j] * in[j];
+      out[i] = acc;
+    }
+  };
+  bit::GmresResult r = bit::restarted_gmres(matvec, b, x, n, restart, tol, maxit);
+  if (iters) *iters = r.iters;
+  if (relres) *relres = r.relres;
   return BIT_OK;
 }
 

@@ -50,6 +50,8 @@ struct ListStats {
 hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,
                           const double* z, const double* q, double* sx, double* sy,
                           double* sz, double* sq, hipStream_t s);
+hipError_t set_charges(const TreeView& tv, const double* q, double* sq, float4* sloc,
+                       hipStream_t s);
 hipError_t leaf_local(const TreeView& tv, const int* leaves, int nleaves, float4* sloc,
                       hipStream_t s);
 

@@ -213,6 +213,23 @@ hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,
   return hipGetLastError();
 }
 
+// New charges for the same positions (original order in q).
+__global__ void k_set_charges(int n, const int* __restrict__ perm, const double* __restrict__ q,
+                              double* sq, float4* sloc) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double v = q[perm[t]];
+  sq[t] = v;
+  sloc[t].w = (float)v;
+}
+
+hipError_t set_charges(const TreeView& tv, const double* q, double* sq, float4* sloc,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_set_charges, dim3((tv.n + 255) / 256), dim3(256), 0, s, tv.n, tv.perm, q,
+                     sq, sloc);
+  return hipGetLastError();
+}
+
 // One block per leaf: FP32 source copy relative to the leaf's box midpoint.
 __global__ void k_leaf_local(const int* __restrict__ leaves, const double4* __restrict__ center,
                              const int2* __restrict__ range, const double* __restrict__ sx,

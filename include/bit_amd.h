@@ -162,6 +162,29 @@ int bit_system_export_lists(bit_system *sys, const bit_params *params, int *nbat
                             int *nnear, int *far_off, uint64_t *far_mask, int *far_node,
                             int *near_off, uint64_t *near_mask, int *near_node);
 
+/* Replaces the charges of a system (same positions, same tree and lists):
+ * q[n] in the caller's original order, host (on_device 0) or device (1)
+ * array.  Moments are recomputed by the next evaluation.  Used by iterative
+ * solvers whose every matrix-vector product is a treecode evaluation. */
+int bit_system_set_charges(bit_system *sys, const double *q, int on_device);
+
+/* Second-kind integral equation on the particles of a system, solved by
+ * restarted GMRES with FP64 Arnoldi (modified Gram-Schmidt), Givens rotations
+ * and residual norms, every matrix-vector product a treecode evaluation:
+ *     diag * s_i + sum_{j != i} w_j s_j G(x_i - x_j) + self * w_i s_i = f_i
+ * (G as bit_eval_potential with params; w = quadrature weights).  The
+ * reference's solver::gmres (solver.h:157-260) is dense and unrestarted; this
+ * is its matrix-free counterpart.  sigma holds the initial guess on entry.
+ * iters/relres (may be NULL) receive the Arnoldi steps and ||r||/||f||. */
+int bit_solve_second_kind(bit_system *sys, const bit_params *params, double diag, double self,
+                          const double *w, const double *f, double *sigma, int restart,
+                          double tol, int maxit, int *iters, double *relres);
+
+/* Host-only restarted GMRES on a dense row-major n x n matrix (same FP64
+ * core as bit_solve_second_kind; no device needed). */
+int bit_gmres_dense(int n, const double *A, const double *b, double *x, int restart, double tol,
+                    int maxit, int *iters, double *relres);
+
 /* FP64 direct sum on the GPU at ntargets particle targets (original indices
  * tidx, host array): phi_t = sum_{j != tidx[t]} q_j exp(-kappa R)/R.
  * Replaces shielded_pot_particle_ds (tree3d.cpp:1966-2016) with +phi sign.
