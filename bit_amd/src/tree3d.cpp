@@ -166,14 +166,13 @@ void drive(const char* fn, int npart, PARTICLE* part, int what, int nterms, doub
     bit_system_destroy(s);
     return;
   }
-  for (int i = 0; i < npart; i++) {
-    double f = charge_constant * part[i].tot_charge;
+  for (int i = 0; i < npart; i++) {  // the reference's product order
     if (what == kField) {
-      part[i].xforce = v[3 * (size_t)i] * f;
-      part[i].yforce = v[3 * (size_t)i + 1] * f;
-      part[i].zforce = v[3 * (size_t)i + 2] * f;
+      part[i].xforce = v[3 * (size_t)i] * charge_constant * part[i].tot_charge;
+      part[i].yforce = v[3 * (size_t)i + 1] * charge_constant * part[i].tot_charge;
+      part[i].zforce = v[3 * (size_t)i + 2] * charge_constant * part[i].tot_charge;
     } else {
-      part[i].xforce = v[i] * f;
+      part[i].xforce = v[i] * charge_constant * part[i].tot_charge;
     }
   }
   bit_system_destroy(s);
