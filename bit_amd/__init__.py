@@ -81,7 +81,12 @@ EXPORTS = [
     "bit_tree_destroy", "bit_tree_numtree", "bit_tree_maxdepth", "bit_tree_export",
     "bit_system_export_lists", "bit_device_alloc", "bit_device_free", "bit_copy_to_device",
     "bit_copy_to_host", "bit_system_set_charges", "bit_solve_second_kind", "bit_gmres_dense",
+    "bit_system_prepare", "bit_eval_potential_multi", "bit_gmres", "bit_ctx_set_host_threads",
 ]
+
+
+# void (*)(void *user, const double *in, double *out)
+MATVEC_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
 def _load():
@@ -125,6 +130,11 @@ def _load():
     L.bit_solve_second_kind.argtypes = [vp, C.POINTER(Params), C.c_double, C.c_double, vp, vp,
                                         vp, C.c_int, C.c_double, C.c_int, vp, vp]
     L.bit_gmres_dense.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_double, C.c_int, vp, vp]
+    L.bit_system_prepare.argtypes = [vp, C.POINTER(Params)]
+    L.bit_eval_potential_multi.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp]
+    L.bit_gmres.argtypes = [C.c_int, MATVEC_FN, vp, vp, vp, C.c_int, C.c_double, C.c_int, vp,
+                            vp]
+    L.bit_ctx_set_host_threads.argtypes = [vp, C.c_int]
     L.bit_tree_build.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.POINTER(vp)]
     L.bit_tree_destroy.argtypes = [vp]
     L.bit_tree_numtree.argtypes = [vp]
@@ -217,6 +227,9 @@ class Context:
 
     def inject_alloc_failures(self, k: int):
         _check(lib.bit_ctx_inject_alloc_failures(self.h, int(k)))
+
+    def set_host_threads(self, k: int):
+        _check(lib.bit_ctx_set_host_threads(self.h, int(k)))
 
 
 class DeviceArray:
@@ -313,6 +326,42 @@ def gmres_dense(A, b, x0=None, restart=50, tol=1e-10, maxit=1000):
     _check(lib.bit_gmres_dense(len(b), _ptr(A), _ptr(b), _ptr(x), int(restart), float(tol),
                                int(maxit), C.byref(it), C.byref(rr)))
     return x, it.value, rr.value
+
+
+def gmres(n, matvec, b, x0=None, restart=50, tol=1e-6, maxit=500):
+    """FP64 restarted GMRES with a Python operator matvec(x) -> A x (numpy)."""
+    b = _f64(b)
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    err = []
+
+    def cb(_user, pin, pout):
+        try:
+            v = np.ctypeslib.as_array(pin, shape=(n,)).copy()
+            np.ctypeslib.as_array(pout, shape=(n,))[:] = matvec(v)
+        except Exception as e:  # keep the C loop alive; report after it returns
+            err.append(e)
+            np.ctypeslib.as_array(pout, shape=(n,))[:] = 0.0
+
+    fn = MATVEC_FN(cb)
+    it, rr = C.c_int(), C.c_double()
+    _check(lib.bit_gmres(int(n), fn, None, _ptr(b), _ptr(x), int(restart), float(tol),
+                         int(maxit), C.byref(it), C.byref(rr)))
+    if err:
+        raise err[0]
+    return x, it.value, rr.value
+
+
+def eval_multi(ctx: "Context", systems, params, precision_out=None):
+    """Potentials of several systems in mixed launches (one per parameter
+    group); returns (list of numpy arrays, GPU ms of the launches)."""
+    k = len(systems)
+    outs = [np.zeros(s.n) for s in systems]
+    sys_arr = (C.c_void_p * k)(*[s.h.value for s in systems])
+    prm_arr = (Params * k)(*params)
+    out_arr = (C.c_void_p * k)(*[o.ctypes.data for o in outs])
+    ms = C.c_double()
+    _check(lib.bit_eval_potential_multi(ctx.h, k, sys_arr, prm_arr, out_arr, 0, C.byref(ms)))
+    return outs, ms.value
 
 
 class System:
@@ -437,6 +486,12 @@ class System:
                                          _ptr(w), _ptr(f), _ptr(sig), int(restart), float(tol),
                                          int(maxit), C.byref(it), C.byref(rr)))
         return sig, it.value, rr.value
+
+    def prepare(self, nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None,
+                precision="fp64"):
+        """Moments + interaction lists for these parameters (no evaluation)."""
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+        _check(lib.bit_system_prepare(self.h, C.byref(prm)))
 
     def stats(self) -> dict:
         s = Stats()

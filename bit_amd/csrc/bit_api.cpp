@@ -8,7 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -57,10 +59,11 @@ struct bit_ctx {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   size_t budget = 0;
-  size_t in_use = 0;
-  int inject = 0;
+  std::atomic<size_t> in_use{0};  // systems of one context may be used from several threads
+  std::atomic<int> inject{0};
+  int host_threads = 0;           // tree-build threads, 0 = all
   std::vector<std::pair<void*, size_t>> user_bufs;  // bit_device_alloc
-  int live_systems = 0;
+  std::atomic<int> live_systems{0};
 };
 
 namespace {
@@ -88,10 +91,8 @@ struct DevBuf {
     if (need <= bytes && p) return BIT_OK;
     release();
     if (need == 0) need = 8;
-    if (c->inject > 0) {
-      c->inject--;
+    if (c->inject.load() > 0 && c->inject.fetch_sub(1) > 0)
       return fail(BIT_E_OOM, "injected device allocation failure");
-    }
     if (c->budget && c->in_use + need > c->budget)
       return fail(BIT_E_OOM, "HBM budget exceeded: need " + std::to_string(need) +
                                  " bytes, in use " + std::to_string(c->in_use) + " of " +
@@ -158,6 +159,9 @@ struct bit_system {
   DevBuf tslot, dpart, dout;
   // charge updates
   DevBuf qin;
+  // evaluation descriptor (host copy kept alive for the async upload)
+  bit::dev::EvalDesc hdesc{};
+  DevBuf desc;
   bit_stats stats{};
   Timer t_mom, t_lists, t_far, t_near;
 
@@ -382,11 +386,17 @@ int bit_ctx_sync(bit_ctx* c) {
   return BIT_OK;
 }
 
-size_t bit_ctx_bytes_in_use(const bit_ctx* c) { return c ? c->in_use : 0; }
+size_t bit_ctx_bytes_in_use(const bit_ctx* c) { return c ? c->in_use.load() : 0; }
+
+int bit_ctx_set_host_threads(bit_ctx* c, int threads) {
+  if (!c || threads < 0) return fail(BIT_E_INVALID, "bad context or thread count");
+  c->host_threads = threads;
+  return BIT_OK;
+}
 
 int bit_ctx_inject_alloc_failures(bit_ctx* c, int k) {
   if (!c) return fail(BIT_E_INVALID, "null context");
-  c->inject = k < 0 ? 0 : k;
+  c->inject.store(k < 0 ? 0 : k);
   return BIT_OK;
 }
 
@@ -469,7 +479,7 @@ static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, 
   s->ctx = ctx;
   auto t0 = std::chrono::steady_clock::now();
   std::string err;
-  if (!bit::build_tree(n, px, py, pz, maxmemb, s->tree, err)) {
+  if (!bit::build_tree(n, px, py, pz, maxmemb, s->tree, err, ctx->host_threads)) {
     delete s;
     return fail(BIT_E_TREE, err);
   }
@@ -652,37 +662,129 @@ static void collect_times(bit_system* s) {
   s->stats.ms_near = s->t_near.ms();
 }
 
-int bit_eval_potential(bit_system* s, const bit_params* p, double* out, int out_on_device) {
+// Moments and lists for params, the far-field buffer, and the evaluation
+// descriptor (result to `dst`, a device pointer); nothing is launched for the
+// far/near kernels.
+static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
   BIT_TRY(check_params(s, p));
-  if (!out) return fail(BIT_E_INVALID, "null output");
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   const int n = s->tree.n;
   BIT_TRY(bit_moments(s, p->nterms));
   BIT_TRY(ensure_lists(s, p));
   BIT_TRY(s->phi_far.ensure(c, (size_t)n * sizeof(double)));
-  double* dst = out;
-  if (!out_on_device) {
+  if (!dst) {
     BIT_TRY(s->out.ensure(c, (size_t)n * sizeof(double)));
     dst = s->out.as<double>();
   }
-  const bool fp32 = p->precision == BIT_FP32;
+  bit::dev::EvalDesc& d = s->hdesc;
+  d.tv = s->view();
+  d.kappa = p->kappa;
+  d.delta2 = p->delta2;
+  d.mom64 = s->mom_hat64.as<double>();
+  d.mom32 = s->mom_hat32.as<float>();
+  d.far_off = s->far_off.as<int>();
+  d.far_items = s->far_items.as<bit::dev::Item>();
+  d.near_off = s->near_off.as<int>();
+  d.near_items = s->near_items.as<bit::dev::Item>();
+  d.phi_far = s->phi_far.as<double>();
+  d.out = dst;
+  return BIT_OK;
+}
+
+int bit_eval_potential(bit_system* s, const bit_params* p, double* out, int out_on_device) {
+  if (!out) return fail(BIT_E_INVALID, "null output");
+  BIT_TRY(prepare_eval(s, p, out_on_device ? out : nullptr));
+  bit_ctx* c = s->ctx;
+  const int n = s->tree.n;
+  const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
+  BIT_TRY(upload(c, s->desc, &s->hdesc, sizeof(bit::dev::EvalDesc)));
+  const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
+  const bit::dev::EvalDesc* dd = s->desc.as<bit::dev::EvalDesc>();
   s->t_far.start(c->stream);
-  HIP_TRY(bit::dev::far_field(s->view(), p->nterms, p->kappa, p->delta2, fp32,
-                              s->mom_hat64.as<double>(), s->mom_hat32.as<float>(),
-                              s->far_off.as<int>(), s->far_items.as<bit::dev::Item>(),
-                              s->phi_far.as<double>(), c->stream));
+  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
   s->t_far.stop(c->stream);
   s->t_near.start(c->stream);
-  HIP_TRY(bit::dev::near_field(s->view(), p->kappa, p->delta2, fp32, s->near_off.as<int>(),
-                               s->near_items.as<bit::dev::Item>(), s->phi_far.as<double>(), dst,
-                               c->stream));
+  HIP_TRY(bit::dev::near_field(dd, nullptr, nb, fp32, scr, c->stream));
   s->t_near.stop(c->stream);
   if (!out_on_device)
-    HIP_TRY(hipMemcpyAsync(out, dst, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(out, s->hdesc.out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   collect_times(s);
+  return BIT_OK;
+}
+
+int bit_system_prepare(bit_system* s, const bit_params* p) {
+  BIT_TRY(prepare_eval(s, p, nullptr));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return BIT_OK;
+}
+
+int bit_eval_potential_multi(bit_ctx* ctx, int count, bit_system* const* systems,
+                             const bit_params* params, double* const* outs, int out_on_device,
+                             double* kernel_ms) {
+  if (!ctx || count < 0 || (count > 0 && (!systems || !params || !outs)))
+    return fail(BIT_E_INVALID, "bad multi-evaluation arguments");
+  if (count == 0) return BIT_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  for (int i = 0; i < count; i++) {
+    if (!systems[i] || !outs[i]) return fail(BIT_E_INVALID, "null system or output");
+    if (systems[i]->ctx->device != ctx->device)
+      return fail(BIT_E_INVALID, "systems of a mixed launch must share the device");
+    BIT_TRY(prepare_eval(systems[i], &params[i], out_on_device ? outs[i] : nullptr));
+    HIP_TRY(hipStreamSynchronize(systems[i]->ctx->stream));  // lists/moments ready
+  }
+  // group by the kernel instantiation: (nterms, precision, screened)
+  std::vector<int> order(count);
+  for (int i = 0; i < count; i++) order[i] = i;
+  auto key = [&](int i) {
+    return std::make_tuple(params[i].nterms, params[i].precision, params[i].kappa != 0.0);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  std::vector<bit::dev::EvalDesc> descs(count);
+  std::vector<int2> map;
+  std::vector<std::tuple<int, int, int, int>> groups;  // (first map entry, blocks, i0 in order)
+  for (int k = 0; k < count;) {
+    int e = k;
+    while (e < count && key(order[e]) == key(order[k])) e++;
+    int first = (int)map.size();
+    for (int j = k; j < e; j++) {
+      bit_system* s = systems[order[j]];
+      descs[j] = s->hdesc;
+      int nb = (s->tree.n + bit::dev::kBatch - 1) / bit::dev::kBatch;
+      for (int b = 0; b < nb; b++) map.push_back(make_int2(j, b));
+    }
+    groups.emplace_back(first, (int)map.size() - first, order[k], 0);
+    k = e;
+  }
+  DevBuf d_desc, d_map;
+  BIT_TRY(upload(ctx, d_desc, descs.data(), descs.size() * sizeof(bit::dev::EvalDesc)));
+  BIT_TRY(upload(ctx, d_map, map.data(), map.size() * sizeof(int2)));
+  Timer tm;
+  tm.start(ctx->stream);
+  for (auto& g : groups) {
+    const bit_params& p = params[std::get<2>(g)];
+    const int2* m = d_map.as<int2>() + std::get<0>(g);
+    const bool fp32 = p.precision == BIT_FP32, scr = p.kappa != 0.0;
+    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), p.nterms,
+                                fp32, scr, ctx->stream));
+  }
+  for (auto& g : groups) {
+    const bit_params& p = params[std::get<2>(g)];
+    const int2* m = d_map.as<int2>() + std::get<0>(g);
+    const bool fp32 = p.precision == BIT_FP32, scr = p.kappa != 0.0;
+    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), fp32, scr,
+                                 ctx->stream));
+  }
+  tm.stop(ctx->stream);
+  if (!out_on_device)
+    for (int i = 0; i < count; i++)
+      HIP_TRY(hipMemcpyAsync(outs[i], systems[i]->hdesc.out,
+                             (size_t)systems[i]->tree.n * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (kernel_ms) *kernel_ms = tm.ms();
   return BIT_OK;
 }
 
@@ -820,6 +922,16 @@ int bit_gmres_dense(int n, const double* A, const double* b, double* x, int rest
     }
   };
   bit::GmresResult r = bit::restarted_gmres(matvec, b, x, n, restart, tol, maxit);
+  if (iters) *iters = r.iters;
+  if (relres) *relres = r.relres;
+  return BIT_OK;
+}
+
+int bit_gmres(int n, bit_matvec_fn matvec, void* user, const double* b, double* x, int restart,
+              double tol, int maxit, int* iters, double* relres) {
+  if (n < 1 || !matvec || !b || !x) return fail(BIT_E_INVALID, "bad GMRES arguments");
+  auto mv = [&](const double* in, double* out) { matvec(user, in, out); };
+  bit::GmresResult r = bit::restarted_gmres(mv, b, x, n, restart, tol, maxit);
   if (iters) *iters = r.iters;
   if (relres) *relres = r.relres;
   return BIT_OK;

@@ -69,14 +69,30 @@ hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int
 hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp_bytes,
                           hipStream_t s);
 
-// far field: phi_far[sorted t]; fp32 selects the FP32 + Kahan kernel
-hipError_t far_field(const TreeView& tv, int nterms, double kappa, double delta2, bool fp32,
-                     const double* mom_hat64, const float* mom_hat32, const int* far_off,
-                     const Item* far_items, double* phi_far, hipStream_t s);
-// near field + combine: out[perm[t]] = phi_far[t] + near(t)
-hipError_t near_field(const TreeView& tv, double kappa, double delta2, bool fp32,
-                      const int* near_off, const Item* near_items, const double* phi_far,
-                      double* out, hipStream_t s);
+// One system of a (mixed) evaluation launch, read from HBM by every
+// work-group that serves one of its batches.
+struct EvalDesc {
+  TreeView tv;
+  double kappa;
+  double delta2;
+  const double* mom64;   // moments scaled by s^-|k|, FP64
+  const float* mom32;    // same, FP32
+  const int* far_off;    // [nbatch+1]
+  const Item* far_items;
+  const int* near_off;
+  const Item* near_items;
+  double* phi_far;       // far-field result, sorted order
+  double* out;           // potential, caller's original order
+};
+
+// far field -> d.phi_far; map == nullptr: one system, work-group b = batch b;
+// else map[g] = (system, batch).  One launch serves one (nterms, precision,
+// screened) group.
+hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nterms, bool fp32,
+                     bool screened, hipStream_t s);
+// near field + combine: d.out[perm[t]] = d.phi_far[t] + near(t)
+hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool fp32,
+                      bool screened, hipStream_t s);
 
 // field (Coulomb, regularized): E_far sorted [3n], then near + scatter [3n]
 hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, const double* mom_hat64,

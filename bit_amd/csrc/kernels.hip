@@ -516,24 +516,50 @@ struct Acc<float> {
   __device__ double value() const { return (double)s - (double)c; }
 };
 
+// Work-group -> (system, batch): identity for a single system (map == nullptr),
+// else the mixed launch's table.
+__device__ __forceinline__ void which(const int2* __restrict__ map, int& sys, int& b) {
+  if (map) {
+    int2 m = map[blockIdx.x];
+    sys = uniform(m.x);
+    b = uniform(m.y);
+  } else {
+    sys = 0;
+    b = blockIdx.x;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ const T* mom_of(const EvalDesc& d);
+template <>
+__device__ __forceinline__ const float* mom_of<float>(const EvalDesc& d) {
+  return d.mom32;
+}
+template <>
+__device__ __forceinline__ const double* mom_of<double>(const EvalDesc& d) {
+  return d.mom64;
+}
+
 template <typename T, int NT, bool SCR>
-__global__ __launch_bounds__(64) void k_far(TreeView tv, double kappa, double delta2,
-                                            const T* __restrict__ mom_hat,
-                                            const int* __restrict__ far_off,
-                                            const Item* __restrict__ items,
-                                            double* __restrict__ phi_far) {
+__global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
+                                            const int2* __restrict__ map) {
   constexpr int NC = ncoef(NT);
-  const int b = blockIdx.x;
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const TreeView& tv = d.tv;
+  const double kappa = d.kappa, delta2 = d.delta2;
+  const T* __restrict__ mom_hat = mom_of<T>(d);
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
   const bool valid = t < tv.n;
   const int tt = valid ? t : tv.n - 1;
   const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
   Acc<T> acc;
-  const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
+  const int i0 = uniform(d.far_off[b]), i1 = uniform(d.far_off[b + 1]);
   for (int it = i0; it < i1; it++) {
-    const unsigned long long mask = uniform64(items[it].mask);
-    const int node = uniform(items[it].node);
+    const unsigned long long mask = uniform64(d.far_items[it].mask);
+    const int node = uniform(d.far_items[it].node);
     if ((mask >> lane) & 1ull) {
       const double4 c = tv.center[node];
       const double s = tv.scale[node];
@@ -550,46 +576,35 @@ __global__ __launch_bounds__(64) void k_far(TreeView tv, double kappa, double de
       acc.add(v * (T)inv);
     }
   }
-  if (valid) phi_far[t] = acc.value();
+  if (valid) d.phi_far[t] = acc.value();
 }
 
 template <typename T, bool SCR, int NT>
-static void launch_far(const TreeView& tv, double kappa, double delta2, const T* mom,
-                       const int* far_off, const Item* items, double* phi_far, hipStream_t s) {
-  int nb = (tv.n + kBatch - 1) / kBatch;
-  hipLaunchKernelGGL((k_far<T, NT, SCR>), dim3(nb), dim3(kBatch), 0, s, tv, kappa, delta2, mom,
-                     far_off, items, phi_far);
+static void launch_far(const EvalDesc* descs, const int2* map, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((k_far<T, NT, SCR>), dim3(nblocks), dim3(kBatch), 0, s, descs, map);
 }
 
 template <typename T, bool SCR, int... NTs>
-static bool dispatch_far(int nterms, std::integer_sequence<int, NTs...>, const TreeView& tv,
-                         double kappa, double delta2, const T* mom, const int* far_off,
-                         const Item* items, double* phi_far, hipStream_t s) {
+static bool dispatch_far(int nterms, std::integer_sequence<int, NTs...>, const EvalDesc* descs,
+                         const int2* map, int nblocks, hipStream_t s) {
   bool done = false;
-  ((nterms == NTs + 1 ? (launch_far<T, SCR, NTs + 1>(tv, kappa, delta2, mom, far_off, items,
-                                                     phi_far, s),
-                         done = true)
+  ((nterms == NTs + 1 ? (launch_far<T, SCR, NTs + 1>(descs, map, nblocks, s), done = true)
                       : false),
    ...);
   return done;
 }
 
-hipError_t far_field(const TreeView& tv, int nterms, double kappa, double delta2, bool fp32,
-                     const double* mom_hat64, const float* mom_hat32, const int* far_off,
-                     const Item* far_items, double* phi_far, hipStream_t s) {
+hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nterms, bool fp32,
+                     bool screened, hipStream_t s) {
+  if (nblocks == 0) return hipSuccess;
   using Seq = std::make_integer_sequence<int, kMaxNterms>;
-  bool scr = kappa != 0.0;
   bool ok;
   if (fp32)
-    ok = scr ? dispatch_far<float, true>(nterms, Seq{}, tv, kappa, delta2, mom_hat32, far_off,
-                                         far_items, phi_far, s)
-             : dispatch_far<float, false>(nterms, Seq{}, tv, kappa, delta2, mom_hat32, far_off,
-                                          far_items, phi_far, s);
+    ok = screened ? dispatch_far<float, true>(nterms, Seq{}, descs, map, nblocks, s)
+                  : dispatch_far<float, false>(nterms, Seq{}, descs, map, nblocks, s);
   else
-    ok = scr ? dispatch_far<double, true>(nterms, Seq{}, tv, kappa, delta2, mom_hat64, far_off,
-                                          far_items, phi_far, s)
-             : dispatch_far<double, false>(nterms, Seq{}, tv, kappa, delta2, mom_hat64, far_off,
-                                           far_items, phi_far, s);
+    ok = screened ? dispatch_far<double, true>(nterms, Seq{}, descs, map, nblocks, s)
+                  : dispatch_far<double, false>(nterms, Seq{}, descs, map, nblocks, s);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -627,13 +642,15 @@ __device__ __forceinline__ double pair64(double px, double py, double pz, double
 }
 
 template <bool SCR>
-__global__ __launch_bounds__(64) void k_near32(TreeView tv, float d2, float kneg2,
-                                               const int* __restrict__ near_off,
-                                               const Item* __restrict__ items,
-                                               const double* __restrict__ phi_far,
-                                               double* __restrict__ out) {
+__global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ descs,
+                                               const int2* __restrict__ map) {
   __shared__ float4 tile[64];
-  const int b = blockIdx.x;
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const TreeView& tv = d.tv;
+  const float d2 = (float)d.delta2;
+  const float kneg2 = (float)(-d.kappa * 1.4426950408889634);
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
   const bool valid = t < tv.n;
@@ -641,10 +658,10 @@ __global__ __launch_bounds__(64) void k_near32(TreeView tv, float d2, float kneg
   const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
   const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
   Acc<float> acc;
-  const int i0 = uniform(near_off[b]), i1 = uniform(near_off[b + 1]);
+  const int i0 = uniform(d.near_off[b]), i1 = uniform(d.near_off[b + 1]);
   for (int it = i0; it < i1; it++) {
-    const unsigned long long mask = uniform64(items[it].mask);
-    const int node = uniform(items[it].node);
+    const unsigned long long mask = uniform64(d.near_items[it].mask);
+    const int node = uniform(d.near_items[it].node);
     const int2 r = tv.range[node];
     const double4 c = tv.center[node];
     const bool act = (mask >> lane) & 1ull;
@@ -689,17 +706,18 @@ __global__ __launch_bounds__(64) void k_near32(TreeView tv, float d2, float kneg
       __syncthreads();
     }
   }
-  if (valid) out[tv.perm[t]] = phi_far[t] + acc.value();
+  if (valid) d.out[tv.perm[t]] = d.phi_far[t] + acc.value();
 }
 
 template <bool SCR>
-__global__ __launch_bounds__(64) void k_near64(TreeView tv, double d2, double kappa,
-                                               const int* __restrict__ near_off,
-                                               const Item* __restrict__ items,
-                                               const double* __restrict__ phi_far,
-                                               double* __restrict__ out) {
+__global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ descs,
+                                               const int2* __restrict__ map) {
   __shared__ double4 tile[64];
-  const int b = blockIdx.x;
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const TreeView& tv = d.tv;
+  const double d2 = d.delta2, kappa = d.kappa;
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
   const bool valid = t < tv.n;
@@ -707,10 +725,10 @@ __global__ __launch_bounds__(64) void k_near64(TreeView tv, double d2, double ka
   const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
   const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
   double acc = 0.0;
-  const int i0 = uniform(near_off[b]), i1 = uniform(near_off[b + 1]);
+  const int i0 = uniform(d.near_off[b]), i1 = uniform(d.near_off[b + 1]);
   for (int it = i0; it < i1; it++) {
-    const unsigned long long mask = uniform64(items[it].mask);
-    const int node = uniform(items[it].node);
+    const unsigned long long mask = uniform64(d.near_items[it].mask);
+    const int node = uniform(d.near_items[it].node);
     const int2 r = tv.range[node];
     const bool act = (mask >> lane) & 1ull;
     const bool selfp = r.x < t1 && r.y > t0;
@@ -731,29 +749,22 @@ __global__ __launch_bounds__(64) void k_near64(TreeView tv, double d2, double ka
       __syncthreads();
     }
   }
-  if (valid) out[tv.perm[t]] = phi_far[t] + acc;
+  if (valid) d.out[tv.perm[t]] = d.phi_far[t] + acc;
 }
 
-hipError_t near_field(const TreeView& tv, double kappa, double delta2, bool fp32,
-                      const int* near_off, const Item* near_items, const double* phi_far,
-                      double* out, hipStream_t s) {
-  int nb = (tv.n + kBatch - 1) / kBatch;
-  bool scr = kappa != 0.0;
+hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool fp32,
+                      bool screened, hipStream_t s) {
+  if (nblocks == 0) return hipSuccess;
   if (fp32) {
-    float kneg2 = (float)(-kappa * 1.4426950408889634);
-    if (scr)
-      hipLaunchKernelGGL(k_near32<true>, dim3(nb), dim3(kBatch), 0, s, tv, (float)delta2, kneg2,
-                         near_off, near_items, phi_far, out);
+    if (screened)
+      hipLaunchKernelGGL(k_near32<true>, dim3(nblocks), dim3(kBatch), 0, s, descs, map);
     else
-      hipLaunchKernelGGL(k_near32<false>, dim3(nb), dim3(kBatch), 0, s, tv, (float)delta2,
-                         kneg2, near_off, near_items, phi_far, out);
+      hipLaunchKernelGGL(k_near32<false>, dim3(nblocks), dim3(kBatch), 0, s, descs, map);
   } else {
-    if (scr)
-      hipLaunchKernelGGL(k_near64<true>, dim3(nb), dim3(kBatch), 0, s, tv, delta2, kappa,
-                         near_off, near_items, phi_far, out);
+    if (screened)
+      hipLaunchKernelGGL(k_near64<true>, dim3(nblocks), dim3(kBatch), 0, s, descs, map);
     else
-      hipLaunchKernelGGL(k_near64<false>, dim3(nb), dim3(kBatch), 0, s, tv, delta2, kappa,
-                         near_off, near_items, phi_far, out);
+      hipLaunchKernelGGL(k_near64<false>, dim3(nblocks), dim3(kBatch), 0, s, descs, map);
   }
   return hipGetLastError();
 }

@@ -336,7 +336,7 @@ void order_leaves(HostTree& t, Arrays& a, int threads) {
 }  // namespace
 
 bool build_tree(int n, const double* x, const double* y, const double* z, int maxmemb,
-                HostTree& out, std::string& err) {
+                HostTree& out, std::string& err, int max_threads) {
   if (n <= 0) {
     err = "tree needs at least one particle";
     return false;
@@ -346,7 +346,9 @@ bool build_tree(int n, const double* x, const double* y, const double* z, int ma
     return false;
   }
   const double lim = 1.0 / std::sqrt(2.0);  // aspectlimit, tree3d.cpp:1371
-  const int threads = std::max(1, omp_get_max_threads());
+  const int threads =
+      std::max(1, max_threads > 0 ? std::min(max_threads, omp_get_max_threads())
+                                  : omp_get_max_threads());
   const int node_cap = 4 * n + 64;
 
   std::vector<int> P(n), tP(n);

@@ -29,7 +29,8 @@ struct HostTree {
 // Builds the tree over n particles (coordinates in original order).  Returns
 // false with a message when the particles cannot be separated into leaves of
 // at most maxmemb (coincident points) or the input is invalid.
+// max_threads > 0 caps the OpenMP threads used (0 = all available).
 bool build_tree(int n, const double* x, const double* y, const double* z, int maxmemb,
-                HostTree& out, std::string& err);
+                HostTree& out, std::string& err, int max_threads = 0);
 
 }  // namespace bit

@@ -82,6 +82,9 @@ int bit_ctx_destroy(bit_ctx *ctx);
  * context's own; NULL restores the context stream. */
 int bit_ctx_set_stream(bit_ctx *ctx, void *hip_stream);
 int bit_ctx_sync(bit_ctx *ctx);
+/* Host threads the tree build of this context's systems may use (0 = all);
+ * a request server building many trees at once gives each builder a few. */
+int bit_ctx_set_host_threads(bit_ctx *ctx, int threads);
 /* Bytes the context currently holds on the device. */
 size_t bit_ctx_bytes_in_use(const bit_ctx *ctx);
 /* Test hook: make the next k device allocations of this context fail with
@@ -144,6 +147,23 @@ int bit_moments_download(const bit_system *sys, double *out);
  * Computes moments first if bit_moments was not called for params->nterms. */
 int bit_eval_potential(bit_system *sys, const bit_params *params, double *out,
                        int out_on_device);
+/* Moments and interaction lists for params, without evaluating (so a later
+ * evaluation, alone or mixed, only runs the far and near kernels). */
+int bit_system_prepare(bit_system *sys, const bit_params *params);
+
+/* Potentials of `count` independent systems (any sizes, per-system params) in
+ * mixed launches on ctx's stream: one far-field and one near-field launch per
+ * (nterms, precision, screened) group, each work-group serving one 64-target
+ * batch of one system.  outs[i] receives system i's +phi (original order);
+ * every target's sums are formed exactly as in bit_eval_potential, so results
+ * are bit-identical alone and in any mix.  Systems may belong to other
+ * contexts on the same device.  kernel_ms (may be NULL) gets the GPU time of
+ * the launches.  No reference counterpart: batches the reference's per-system
+ * drivers (tree3d.cpp:2368-2436) across requests. */
+int bit_eval_potential_multi(bit_ctx *ctx, int count, bit_system *const *systems,
+                             const bit_params *params, double *const *outs, int out_on_device,
+                             double *kernel_ms);
+
 /* Field E_i = sum_{j != i} q_j (x_i - x_j)/R^3 for the regularized Coulomb
  * kernel (kappa must be 0); replaces treeforce (tree3d.cpp:851-971) as used by
  * freespace_particles_tree (tree3d.cpp:2268-2335).  out[3n] = Ex,Ey,Ez
@@ -184,6 +204,13 @@ int bit_solve_second_kind(bit_system *sys, const bit_params *params, double diag
  * core as bit_solve_second_kind; no device needed). */
 int bit_gmres_dense(int n, const double *A, const double *b, double *x, int restart, double tol,
                     int maxit, int *iters, double *relres);
+
+/* The same FP64 GMRES(restart) with the caller's operator: matvec(user, in,
+ * out) computes out = A in (n doubles, host).  For solvers whose products are
+ * scheduled elsewhere (e.g. batched with other requests). */
+typedef void (*bit_matvec_fn)(void *user, const double *in, double *out);
+int bit_gmres(int n, bit_matvec_fn matvec, void *user, const double *b, double *x, int restart,
+              double tol, int maxit, int *iters, double *relres);
 
 /* FP64 direct sum on the GPU at ntargets particle targets (original indices
  * tidx, host array): phi_t = sum_{j != tidx[t]} q_j exp(-kappa R)/R.
