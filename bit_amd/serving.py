@@ -221,7 +221,12 @@ class Server:
                 job.done.wait()
                 return (0.5 + selfc * w) * v + job.out
 
-            sig, it, rr = bit_amd.gmres(s.n, matvec, f, None, 50, r.tol, 500)
+            try:
+                sig, it, rr = bit_amd.gmres(s.n, matvec, f, None, 50, r.tol, 500)
+            except Exception as e:
+                s.close()
+                fail(r, e)
+                return
             r.result, r.iters, r.relres = sig, it, rr
             r.t_done = now()
             release(r, s)
