@@ -197,12 +197,158 @@ bool finite_all(int n, const double* a) {
   return true;
 }
 
+// Split rounds of the tree build executed on the slot arrays in HBM.
+class DeviceEngine : public bit::PartitionEngine {
+ public:
+  DeviceEngine(bit_ctx* c, const bit::dev::SlotArrays& a) : c_(c), a_(a) {}
+  int status = BIT_OK;
+
+  bool run(const std::vector<bit::PartOp>& ops, std::vector<bit::PartResult>& res,
+           std::string& err) override {
+    const int nops = (int)ops.size();
+    hops_.resize(nops);
+    c0_.resize(nops);
+    nc_.resize(nops);
+    chunks_.clear();
+    for (int o = 0; o < nops; o++) {
+      hops_[o] = bit::dev::PartOpDev{ops[o].b, ops[o].e, ops[o].dir, 0, ops[o].mid};
+      c0_[o] = (int)chunks_.size();
+      for (int s0 = ops[o].b; s0 < ops[o].e; s0 += bit::dev::kPartChunk)
+        chunks_.push_back(bit::dev::PartChunk{o, s0, std::min(ops[o].e, s0 + bit::dev::kPartChunk), 0});
+      nc_[o] = (int)chunks_.size() - c0_[o];
+    }
+    const int nch = (int)chunks_.size();
+    hres_.resize(nops);
+    status = step(nops, nch);
+    if (status != BIT_OK) {
+      err = bit_last_error();
+      return false;
+    }
+    res.resize(nops);
+    for (int o = 0; o < nops; o++) {
+      const bit::dev::ChunkResult& r = hres_[o];
+      res[o].nl = r.nl;
+      for (int sd = 0; sd < 2; sd++)
+        for (int d = 0; d < 3; d++) {
+          res[o].mn[sd][d] = r.v[3 * sd + d];
+          res[o].mx[sd][d] = r.v[6 + 3 * sd + d];
+        }
+    }
+    return true;
+  }
+
+ private:
+  int step(int nops, int nch) {
+    hipStream_t st = c_->stream;
+    BIT_TRY(upload(c_, d_ops_, hops_.data(), nops * sizeof(bit::dev::PartOpDev)));
+    BIT_TRY(upload(c_, d_chunks_, chunks_.data(), nch * sizeof(bit::dev::PartChunk)));
+    BIT_TRY(upload(c_, d_c0_, c0_.data(), nops * sizeof(int)));
+    BIT_TRY(upload(c_, d_nc_, nc_.data(), nops * sizeof(int)));
+    BIT_TRY(d_cres_.ensure(c_, (size_t)nch * sizeof(bit::dev::ChunkResult)));
+    BIT_TRY(d_cbase_.ensure(c_, (size_t)nch * sizeof(int2)));
+    BIT_TRY(d_ores_.ensure(c_, (size_t)nops * sizeof(bit::dev::ChunkResult)));
+    HIP_TRY(bit::dev::partition_round(a_, nops, d_ops_.as<bit::dev::PartOpDev>(), nch,
+                                      d_chunks_.as<bit::dev::PartChunk>(), d_c0_.as<int>(),
+                                      d_nc_.as<int>(), d_cres_.as<bit::dev::ChunkResult>(),
+                                      d_cbase_.as<int2>(), d_ores_.as<bit::dev::ChunkResult>(), st));
+    HIP_TRY(hipMemcpyAsync(hres_.data(), d_ores_.p, nops * sizeof(bit::dev::ChunkResult),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return BIT_OK;
+  }
+
+  bit_ctx* c_;
+  bit::dev::SlotArrays a_;
+  std::vector<bit::dev::PartOpDev> hops_;
+  std::vector<bit::dev::PartChunk> chunks_;
+  std::vector<int> c0_, nc_;
+  std::vector<bit::dev::ChunkResult> hres_;
+  DevBuf d_ops_, d_chunks_, d_c0_, d_nc_, d_cres_, d_cbase_, d_ores_;
+};
+
+// Builds the tree of n particles whose coordinates are in HBM (original order)
+// and leaves the system's sorted sources, permutation and node table on the
+// device.  The split logic runs on the host (bit-exact program of treemake);
+// the data never leaves the device.
+int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const double* dy,
+                    const double* dz, const double* dq, int maxmemb) {
+  hipStream_t st = ctx->stream;
+  const size_t nb = (size_t)n * sizeof(double);
+  DevBuf tP, tX, tY, tZ;
+  BIT_TRY(s->perm.ensure(ctx, (size_t)n * sizeof(int)));
+  BIT_TRY(s->sx.ensure(ctx, nb));
+  BIT_TRY(s->sy.ensure(ctx, nb));
+  BIT_TRY(s->sz.ensure(ctx, nb));
+  BIT_TRY(tP.ensure(ctx, (size_t)n * sizeof(int)));
+  BIT_TRY(tX.ensure(ctx, nb));
+  BIT_TRY(tY.ensure(ctx, nb));
+  BIT_TRY(tZ.ensure(ctx, nb));
+  HIP_TRY(hipMemcpyAsync(s->sx.p, dx, nb, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->sy.p, dy, nb, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->sz.p, dz, nb, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(bit::dev::iota(n, s->perm.as<int>(), st));
+  bit::dev::SlotArrays a{s->perm.as<int>(), s->sx.as<double>(), s->sy.as<double>(),
+                         s->sz.as<double>(), tP.as<int>(), tX.as<double>(), tY.as<double>(),
+                         tZ.as<double>()};
+  // root bounding box (tree3d.cpp:1385-1403)
+  const int mblocks = std::max(1, std::min(256, (n + 255) / 256));
+  DevBuf part;
+  BIT_TRY(part.ensure(ctx, (size_t)mblocks * 6 * sizeof(double)));
+  HIP_TRY(bit::dev::minmax(n, s->sx.as<double>(), s->sy.as<double>(), s->sz.as<double>(),
+                           mblocks, part.as<double>(), st));
+  std::vector<double> hp((size_t)mblocks * 6);
+  HIP_TRY(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  double mn[3] = {hp[0], hp[1], hp[2]}, mx[3] = {hp[3], hp[4], hp[5]};
+  for (int b = 1; b < mblocks; b++)
+    for (int d = 0; d < 3; d++) {
+      mn[d] = std::min(mn[d], hp[6 * b + d]);
+      mx[d] = std::max(mx[d], hp[6 * b + 3 + d]);
+    }
+  DeviceEngine eng(ctx, a);
+  std::string err;
+  if (!bit::build_tree_generic(eng, n, mn, mx, maxmemb, s->tree, err))
+    return eng.status != BIT_OK ? eng.status : fail(BIT_E_TREE, err);
+  // Morton order inside leaves (same keys as the host build), via t* arrays
+  std::vector<bit::dev::LeafBox> lb;
+  const bit::HostTree& T = s->tree;
+  for (int i = 0; i < T.numtree; i++) {
+    if (T.children[(size_t)i * 8] >= 0) continue;
+    const double* f = &T.box[(size_t)i * bit::BOX_FIELDS];
+    lb.push_back(bit::dev::LeafBox{T.begin[i], T.end[i], f[bit::BX0], f[bit::BX1], f[bit::BY0],
+                                   f[bit::BY1], f[bit::BZ0], f[bit::BZ1]});
+  }
+  DevBuf d_lb;
+  BIT_TRY(upload(ctx, d_lb, lb.data(), lb.size() * sizeof(bit::dev::LeafBox)));
+  HIP_TRY(bit::dev::leaf_morton(a, (int)lb.size(), d_lb.as<bit::dev::LeafBox>(), st));
+  HIP_TRY(hipMemcpyAsync(s->perm.p, tP.p, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->sx.p, tX.p, nb, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->sy.p, tY.p, nb, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->sz.p, tZ.p, nb, hipMemcpyDeviceToDevice, st));
+  // charges in slot order (the FP32 copy is completed by leaf_local)
+  BIT_TRY(s->sq.ensure(ctx, nb));
+  BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
+  HIP_TRY(bit::dev::set_charges(s->view(), dq, s->sq.as<double>(), s->sloc.as<float4>(), st));
+  HIP_TRY(hipStreamSynchronize(st));  // leaf table and scratch die here
+  return BIT_OK;
+}
+
+// Host copy of the permutation (downloaded on first use).
+int ensure_host_perm(const bit_system* cs) {
+  bit_system* s = const_cast<bit_system*>(cs);
+  if (!s->tree.perm.empty()) return BIT_OK;
+  s->tree.perm.resize(s->tree.n);
+  HIP_TRY(hipMemcpyAsync(s->tree.perm.data(), s->perm.p, (size_t)s->tree.n * sizeof(int),
+                         hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return BIT_OK;
+}
+
 // Upload the node table and create the system; sorted sources come from
 // `sorted` (host, 4 arrays) or are gathered on the device from d_src.
-int finish_system(bit_ctx* ctx, bit_system* s, const double* const* sorted_host,
-                  const double* const* d_src) {
+int finish_system(bit_ctx* ctx, bit_system* s) {
   const bit::HostTree& T = s->tree;
-  const int nt = T.numtree, n = T.n;
+  const int nt = T.numtree;
   std::vector<double> cen(4 * (size_t)nt), scl(nt);
   std::vector<int> rng(2 * (size_t)nt), lv;
   for (int i = 0; i < nt; i++) {
@@ -222,24 +368,7 @@ int finish_system(bit_ctx* ctx, bit_system* s, const double* const* sorted_host,
   BIT_TRY(upload(ctx, s->scale, scl.data(), scl.size() * sizeof(double)));
   BIT_TRY(upload(ctx, s->range, rng.data(), rng.size() * sizeof(int)));
   BIT_TRY(upload(ctx, s->child, T.children.data(), T.children.size() * sizeof(int)));
-  BIT_TRY(upload(ctx, s->perm, T.perm.data(), (size_t)n * sizeof(int)));
   BIT_TRY(upload(ctx, s->leaves, lv.data(), std::max<size_t>(1, lv.size()) * sizeof(int)));
-  const size_t nb = (size_t)n * sizeof(double);
-  if (sorted_host) {
-    BIT_TRY(upload(ctx, s->sx, sorted_host[0], nb));
-    BIT_TRY(upload(ctx, s->sy, sorted_host[1], nb));
-    BIT_TRY(upload(ctx, s->sz, sorted_host[2], nb));
-    BIT_TRY(upload(ctx, s->sq, sorted_host[3], nb));
-  } else {
-    BIT_TRY(s->sx.ensure(ctx, nb));
-    BIT_TRY(s->sy.ensure(ctx, nb));
-    BIT_TRY(s->sz.ensure(ctx, nb));
-    BIT_TRY(s->sq.ensure(ctx, nb));
-    HIP_TRY(bit::dev::gather_sources(s->view(), d_src[0], d_src[1], d_src[2], d_src[3],
-                                     s->sx.as<double>(), s->sy.as<double>(),
-                                     s->sz.as<double>(), s->sq.as<double>(), ctx->stream));
-  }
-  BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
   HIP_TRY(bit::dev::leaf_local(s->view(), s->leaves.as<int>(), s->nleaves, s->sloc.as<float4>(),
                                ctx->stream));
   // the node tables above are host temporaries of this function
@@ -456,51 +585,42 @@ static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, 
   if (maxmemb < 1) return fail(BIT_E_INVALID, "maxmemb must be >= 1");
   if (!x || !y || !z || !q) return fail(BIT_E_INVALID, "null coordinate or charge array");
   HIP_TRY(hipSetDevice(ctx->device));
-  std::vector<double> hx, hy, hz, hq;
-  const double *px = x, *py = y, *pz = z, *pq = q;
-  if (device_src) {
-    hx.resize(n);
-    hy.resize(n);
-    hz.resize(n);
-    hq.resize(n);
-    HIP_TRY(hipMemcpyAsync(hx.data(), x, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(hy.data(), y, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(hz.data(), z, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(hq.data(), q, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    px = hx.data();
-    py = hy.data();
-    pz = hz.data();
-    pq = hq.data();
-  }
-  if (!finite_all(n, px) || !finite_all(n, py) || !finite_all(n, pz) || !finite_all(n, pq))
-    return fail(BIT_E_INVALID, "non-finite particle coordinate or charge");
   bit_system* s = new bit_system();
   s->ctx = ctx;
   auto t0 = std::chrono::steady_clock::now();
-  std::string err;
-  if (!bit::build_tree(n, px, py, pz, maxmemb, s->tree, err, ctx->host_threads)) {
-    delete s;
-    return fail(BIT_E_TREE, err);
+  const double *dx = x, *dy = y, *dz = z, *dq = q;
+  DevBuf ux, uy, uz, uq;  // host inputs are staged in HBM first
+  int rc = BIT_OK;
+  if (!device_src) {
+    if (!finite_all(n, x) || !finite_all(n, y) || !finite_all(n, z) || !finite_all(n, q)) {
+      delete s;
+      return fail(BIT_E_INVALID, "non-finite particle coordinate or charge");
+    }
+    const size_t nb = (size_t)n * sizeof(double);
+    if ((rc = upload(ctx, ux, x, nb)) == BIT_OK && (rc = upload(ctx, uy, y, nb)) == BIT_OK &&
+        (rc = upload(ctx, uz, z, nb)) == BIT_OK && (rc = upload(ctx, uq, q, nb)) == BIT_OK) {
+      dx = ux.as<double>();
+      dy = uy.as<double>();
+      dz = uz.as<double>();
+      dq = uq.as<double>();
+    }
   }
+  if (rc == BIT_OK && device_src) {  // inputs from HBM are checked on the device
+    DevBuf flag;
+    int h = 0;
+    if ((rc = flag.ensure(ctx, sizeof(int))) == BIT_OK) {
+      hipError_t e = hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream);
+      if (e == hipSuccess) e = bit::dev::nonfinite(n, dx, dy, dz, dq, flag.as<int>(), ctx->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) rc = fail(BIT_E_HIP, hipGetErrorString(e));
+      else if (h) rc = fail(BIT_E_INVALID, "non-finite particle coordinate or charge");
+    }
+  }
+  if (rc == BIT_OK) rc = build_on_device(ctx, s, n, dx, dy, dz, dq, maxmemb);
   auto t1 = std::chrono::steady_clock::now();
   s->stats.ms_tree = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  int rc;
-  if (device_src) {
-    const double* src[4] = {x, y, z, q};
-    rc = finish_system(ctx, s, nullptr, src);
-  } else {
-    std::vector<double> a(n), b(n), c(n), d(n);
-    const std::vector<int>& P = s->tree.perm;
-    for (int i = 0; i < n; i++) {
-      a[i] = px[P[i]];
-      b[i] = py[P[i]];
-      c[i] = pz[P[i]];
-      d[i] = pq[P[i]];
-    }
-    const double* srt[4] = {a.data(), b.data(), c.data(), d.data()};
-    rc = finish_system(ctx, s, srt, nullptr);
-  }
+  if (rc == BIT_OK) rc = finish_system(ctx, s);
   if (rc != BIT_OK) {
     (void)hipStreamSynchronize(ctx->stream);  // no copy may still read host staging
     delete s;
@@ -578,6 +698,7 @@ int bit_tree_export(const bit_tree* t, double* boxes, int* children, int* begin,
 int bit_system_export_tree(const bit_system* s, double* boxes, int* children, int* begin,
                            int* end, int* perm) {
   if (!s) return fail(BIT_E_INVALID, "null system");
+  if (perm) BIT_TRY(ensure_host_perm(s));
   export_host_tree(s->tree, boxes, children, begin, end, perm);
   return BIT_OK;
 }
@@ -946,6 +1067,7 @@ int bit_direct_potential(bit_system* s, double kappa, double delta2, int ntarget
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   const int n = s->tree.n;
+  BIT_TRY(ensure_host_perm(s));
   std::vector<int> inv(n), slot(ntargets);
   for (int i = 0; i < n; i++) inv[s->tree.perm[i]] = i;
   for (int t = 0; t < ntargets; t++) {

@@ -47,6 +47,49 @@ struct ListStats {
   unsigned long long far_items, near_items, far_pairs, near_pairs, near_slots;
 };
 
+// ---- tree build on the device ----
+struct SlotArrays {  // slot order: P (original index), coordinates; t* = scratch
+  int* P;
+  double* X;
+  double* Y;
+  double* Z;
+  int* tP;
+  double* tX;
+  double* tY;
+  double* tZ;
+};
+struct PartOpDev {
+  int b, e, dir, pad;
+  double mid;
+};
+struct PartChunk {
+  int op, s0, s1, pad;
+};
+struct ChunkResult {
+  int nl, pad;
+  double v[12];  // min side0 xyz, min side1 xyz, max side0 xyz, max side1 xyz
+};
+struct LeafBox {
+  int b, e;
+  double x0, x1, y0, y1, z0, z1;
+};
+constexpr int kPartChunk = 4096;  // slots per work-group of a split
+constexpr int kMortonMax = 4096;  // largest leaf sorted along the Morton curve
+
+hipError_t iota(int n, int* p, hipStream_t s);
+// flag |= 1 if any of x, y, z, q[0..n) is NaN or infinite
+hipError_t nonfinite(int n, const double* x, const double* y, const double* z, const double* q,
+                     int* flag, hipStream_t s);
+hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int blocks,
+                  double* part, hipStream_t s);
+// All split ops of one round; after it, ores[o] holds op o's count and boxes
+// and the slot arrays hold the split ranges.
+hipError_t partition_round(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
+                           const PartChunk* chunks, const int* op_chunk0, const int* op_nchunk,
+                           ChunkResult* cres, int2* cbase, ChunkResult* ores, hipStream_t s);
+// Morton order inside leaves, written to the t* arrays (every slot covered).
+hipError_t leaf_morton(const SlotArrays& a, int nleaves, const LeafBox* leaves, hipStream_t s);
+
 hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,
                           const double* z, const double* q, double* sx, double* sy,
                           double* sz, double* sq, hipStream_t s);

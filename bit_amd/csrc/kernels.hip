@@ -253,6 +253,284 @@ hipError_t leaf_local(const TreeView& tv, const int* leaves, int nleaves, float4
 }
 
 // ---------------------------------------------------------------------------
+// Tree build on the device: stable splits of slot ranges (treedivide,
+// tree3d.cpp:1163-1304) for all nodes of a round, and the Morton order inside
+// leaves.  Only comparisons and min/max: the results are exact, so the tree is
+// the one the host build (and the reference) produces.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double coord_of(const SlotArrays& a, int dir, int i) {
+  return dir == 0 ? a.X[i] : (dir == 1 ? a.Y[i] : a.Z[i]);
+}
+
+__global__ __launch_bounds__(256) void k_minmax(int n, const double* __restrict__ X,
+                                                const double* __restrict__ Y,
+                                                const double* __restrict__ Z, double* part) {
+  __shared__ double red[6][256];
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double c[3] = {X[i], Y[i], Z[i]};
+    for (int d = 0; d < 3; d++) {
+      v[d] = fmin(v[d], c[d]);
+      v[3 + d] = fmax(v[3 + d], c[d]);
+    }
+  }
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 6; k++)
+        red[k][threadIdx.x] = k < 3 ? fmin(red[k][threadIdx.x], red[k][threadIdx.x + w])
+                                    : fmax(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_nonfinite(int n, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, const double* __restrict__ q, int* flag) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && !(isfinite(x[t]) && isfinite(y[t]) && isfinite(z[t]) && isfinite(q[t])))
+    atomicOr(flag, 1);
+}
+
+hipError_t nonfinite(int n, const double* x, const double* y, const double* z, const double* q,
+                     int* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_nonfinite, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, q, flag);
+  return hipGetLastError();
+}
+
+__global__ void k_iota(int n, int* p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = t;
+}
+
+hipError_t iota(int n, int* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, n, p);
+  return hipGetLastError();
+}
+
+hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int blocks,
+                  double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(256), 0, s, n, X, Y, Z, part);
+  return hipGetLastError();
+}
+
+// Per chunk: how many slots go to the `<=` side, and both sides' boxes.
+__global__ __launch_bounds__(256) void k_part_count(SlotArrays a, const PartOpDev* __restrict__ ops,
+                                                    const PartChunk* __restrict__ chunks,
+                                                    ChunkResult* __restrict__ cres) {
+  __shared__ double red[12][256];
+  __shared__ int cnt[256];
+  const PartChunk ch = chunks[blockIdx.x];
+  const PartOpDev op = ops[ch.op];
+  int c = 0;
+  double v[12];
+  for (int k = 0; k < 6; k++) {
+    v[k] = INFINITY;      // min: side 0 x,y,z then side 1 x,y,z
+    v[6 + k] = -INFINITY; // max
+  }
+  for (int i = ch.s0 + threadIdx.x; i < ch.s1; i += 256) {
+    const double p[3] = {a.X[i], a.Y[i], a.Z[i]};
+    const int side = coord_of(a, op.dir, i) <= op.mid ? 0 : 1;
+    c += side == 0;
+    for (int d = 0; d < 3; d++) {
+      v[3 * side + d] = fmin(v[3 * side + d], p[d]);
+      v[6 + 3 * side + d] = fmax(v[6 + 3 * side + d], p[d]);
+    }
+  }
+  cnt[threadIdx.x] = c;
+  for (int k = 0; k < 12; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      cnt[threadIdx.x] += cnt[threadIdx.x + w];
+      for (int k = 0; k < 12; k++)
+        red[k][threadIdx.x] = k < 6 ? fmin(red[k][threadIdx.x], red[k][threadIdx.x + w])
+                                    : fmax(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ChunkResult r;
+    r.nl = cnt[0];
+    for (int k = 0; k < 12; k++) r.v[k] = red[k][0];
+    cres[blockIdx.x] = r;
+  }
+}
+
+// Per op (one thread): chunk bases in chunk order, totals and boxes.
+__global__ void k_part_offsets(int nops, const PartOpDev* __restrict__ ops,
+                               const PartChunk* __restrict__ chunks, const int* __restrict__ op_chunk0,
+                               const int* __restrict__ op_nchunk, const ChunkResult* __restrict__ cres,
+                               int2* __restrict__ cbase, ChunkResult* __restrict__ ores) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nops) return;
+  int c0 = op_chunk0[o], nc = op_nchunk[o];
+  int nl = 0, nr = 0;
+  ChunkResult r;
+  for (int k = 0; k < 6; k++) {
+    r.v[k] = INFINITY;
+    r.v[6 + k] = -INFINITY;
+  }
+  for (int c = c0; c < c0 + nc; c++) {
+    cbase[c] = make_int2(nl, nr);
+    const ChunkResult& x = cres[c];
+    nl += x.nl;
+    nr += (chunks[c].s1 - chunks[c].s0) - x.nl;
+    for (int k = 0; k < 6; k++) {
+      r.v[k] = fmin(r.v[k], x.v[k]);
+      r.v[6 + k] = fmax(r.v[6 + k], x.v[6 + k]);
+    }
+  }
+  r.nl = nl;
+  ores[o] = r;
+}
+
+// Stable scatter of each chunk into the temporary arrays: 256-slot tiles,
+// ranks from a work-group scan of the side flags.
+__global__ __launch_bounds__(256) void k_part_scatter(SlotArrays a, const PartOpDev* __restrict__ ops,
+                                                      const PartChunk* __restrict__ chunks,
+                                                      const int2* __restrict__ cbase,
+                                                      const ChunkResult* __restrict__ ores) {
+  __shared__ int wsum[4];
+  const PartChunk ch = chunks[blockIdx.x];
+  const PartOpDev op = ops[ch.op];
+  const int opnl = ores[ch.op].nl;
+  const int2 base = cbase[blockIdx.x];
+  int lrun = base.x, rrun = base.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t0 = ch.s0; t0 < ch.s1; t0 += 256) {
+    const int i = t0 + threadIdx.x;
+    const bool in = i < ch.s1;
+    const bool left = in && coord_of(a, op.dir, i) <= op.mid;
+    const unsigned long long bl = __ballot(left);
+    const int before = __popcll(bl & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wave] = __popcll(bl);
+    __syncthreads();
+    int woff = 0, tot = 0;
+    for (int w = 0; w < 4; w++) {
+      if (w < wave) woff += wsum[w];
+      tot += wsum[w];
+    }
+    const int ntile = min(256, ch.s1 - t0);
+    if (in) {
+      const int lrank = woff + before;
+      const int rank_in_tile = threadIdx.x;
+      int dst;
+      if (left) dst = op.b + lrun + lrank;
+      else dst = op.b + opnl + rrun + (rank_in_tile - lrank);
+      a.tP[dst] = a.P[i];
+      a.tX[dst] = a.X[i];
+      a.tY[dst] = a.Y[i];
+      a.tZ[dst] = a.Z[i];
+    }
+    lrun += tot;
+    rrun += ntile - tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_part_copy(SlotArrays a, const PartChunk* __restrict__ chunks) {
+  const PartChunk ch = chunks[blockIdx.x];
+  for (int i = ch.s0 + threadIdx.x; i < ch.s1; i += 256) {
+    a.P[i] = a.tP[i];
+    a.X[i] = a.tX[i];
+    a.Y[i] = a.tY[i];
+    a.Z[i] = a.tZ[i];
+  }
+}
+
+hipError_t partition_round(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
+                           const PartChunk* chunks, const int* op_chunk0, const int* op_nchunk,
+                           ChunkResult* cres, int2* cbase, ChunkResult* ores, hipStream_t s) {
+  if (nchunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_part_count, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cres);
+  hipLaunchKernelGGL(k_part_offsets, dim3((nops + 63) / 64), dim3(64), 0, s, nops, ops, chunks,
+                     op_chunk0, op_nchunk, cres, cbase, ores);
+  hipLaunchKernelGGL(k_part_scatter, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cbase, ores);
+  hipLaunchKernelGGL(k_part_copy, dim3(nchunks), dim3(256), 0, s, a, chunks);
+  return hipGetLastError();
+}
+
+// Morton order inside each leaf (same key as the host build): bitonic sort of
+// (key, position) pairs in LDS, then a gather into the temporary arrays.
+// Leaves larger than kMortonMax keep their order.
+__device__ __forceinline__ unsigned spread3_dev(unsigned v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ unsigned quant_dev(double v, double lo, double hi) {
+#pragma clang fp contract(off)
+  if (!(hi > lo)) return 0;
+  double t = (v - lo) / (hi - lo) * 1023.0;
+  if (t < 0) t = 0;
+  if (t > 1023) t = 1023;
+  return (unsigned)t;
+}
+
+__global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays a, const LeafBox* __restrict__ leaves) {
+  __shared__ unsigned long long key[kMortonMax];
+  const LeafBox L = leaves[blockIdx.x];
+  const int n = L.e - L.b;
+  if (n <= 1 || n > kMortonMax) {
+    for (int i = threadIdx.x; i < n; i += 256) {
+      a.tP[L.b + i] = a.P[L.b + i];
+      a.tX[L.b + i] = a.X[L.b + i];
+      a.tY[L.b + i] = a.Y[L.b + i];
+      a.tZ[L.b + i] = a.Z[L.b + i];
+    }
+    return;
+  }
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int i = threadIdx.x; i < m; i += 256) {
+    unsigned long long k = ~0ull;
+    if (i < n) {
+      const int s = L.b + i;
+      unsigned mk = spread3_dev(quant_dev(a.X[s], L.x0, L.x1)) |
+                    (spread3_dev(quant_dev(a.Y[s], L.y0, L.y1)) << 1) |
+                    (spread3_dev(quant_dev(a.Z[s], L.z0, L.z1)) << 2);
+      k = ((unsigned long long)mk << 32) | (unsigned)i;
+    }
+    key[i] = k;
+  }
+  __syncthreads();
+  for (int size = 2; size <= m; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < m; i += 256) {
+        int j = i ^ stride;
+        if (j > i) {
+          bool up = (i & size) == 0;
+          unsigned long long u = key[i], v = key[j];
+          if ((u > v) == up) {
+            key[i] = v;
+            key[j] = u;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int src = L.b + (int)(key[i] & 0xffffffffu);
+    a.tP[L.b + i] = a.P[src];
+    a.tX[L.b + i] = a.X[src];
+    a.tY[L.b + i] = a.Y[src];
+    a.tZ[L.b + i] = a.Z[src];
+  }
+}
+
+hipError_t leaf_morton(const SlotArrays& a, int nleaves, const LeafBox* leaves, hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_leaf_morton, dim3(nleaves), dim3(256), 0, s, a, leaves);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Moments (treemoments, tree3d.cpp:2631-2711), FP64.  A chunk is a slice of one
 // node's slot range; 256 threads, one coefficient per thread (two above 256).
 // Powers q*dx^i, dy^j, dz^k of 64 particles at a time are staged in LDS.

@@ -174,103 +174,237 @@ struct Block {
   int kids[8];
 };
 
-struct Splitter {
-  Arrays& a;
-  int maxmemb;
-  double lim;
-  int threads;
+// One pass of treemake's loop body for a node with too many members
+// (tree3d.cpp:1418-1545) as a resumable program: it requests one stable split
+// (treedivide, tree3d.cpp:1163-1304) at a time and continues when the split's
+// result (side counts and bounding boxes) is applied, so the splits of all
+// nodes of a generation can be executed together by a partition engine.  Slots
+// are numbered from the node's block start; after a y-split the z-split check
+// addresses slot j + k*(nchild2+1) exactly as the reference indexes
+// tree[numtree+j+k*(nchild2+1)] (tree3d.cpp:1458-1477).
+class SplitProgram {
+ public:
+  Block B;
+  bool failed = false;
 
-  // treedivide (tree3d.cpp:1163-1304): the first non-empty side goes to c1,
-  // the second to c2; c1 may be the split node itself.
-  int divide(Block& B, int dir, Slot src, int c1, int c2) {
-    MinMax mm[2];
-    int nl = stable_partition(a, src.b, src.e, dir, src.f[BXMID + dir], mm, threads);
-    int lo[2] = {src.b, src.b + nl}, hi[2] = {src.b + nl, src.e};
-    int target = c1, made = 0;
+  void start(const Slot& parent, int maxmemb, double lim) {
+    parent_ = parent;
+    maxmemb_ = maxmemb;
+    lim_ = lim;
+    pc_ = 0;
+    advance();
+  }
+  bool done() const { return pc_ == kDone; }
+  const PartOp& op() const { return op_; }
+
+  // treedivide's outcome: the first non-empty side goes to c1, the second to c2.
+  void apply(const PartResult& r) {
+    int lo[2] = {src_.b, src_.b + r.nl}, hi[2] = {src_.b + r.nl, src_.e};
+    int target = c1_;
+    made_ = 0;
     for (int sd = 0; sd < 2; sd++) {
       if (hi[sd] > lo[sd]) {
         Slot& t = B.s[target];
         t.used = true;
         t.b = lo[sd];
         t.e = hi[sd];
-        box_init(t.f, mm[sd].mn[0], mm[sd].mx[0], mm[sd].mn[1], mm[sd].mx[1], mm[sd].mn[2],
-                 mm[sd].mx[2]);
-        made++;
-        target = c2;
+        box_init(t.f, r.mn[sd][0], r.mx[sd][0], r.mn[sd][1], r.mx[sd][1], r.mn[sd][2],
+                 r.mx[sd][2]);
+        made_++;
+        target = c2_;
       }
     }
-    return made;
-  }
-  bool want_y(const Slot& c) const {
-    return c.f[BAXY] < lim && c.f[BAYZ] > lim && c.nummemb() > maxmemb;
-  }
-  bool want_z(const Slot& c) const {
-    return c.used && c.f[BAXZ] < lim && c.f[BAYZ] < lim && c.nummemb() > maxmemb;
+    advance();
   }
 
-  // One pass of treemake's loop body for a node with too many members
-  // (tree3d.cpp:1418-1545), on slots numbered from the block start.  After a
-  // y-split, the z-split check addresses slot j + k*(nchild2+1) exactly as the
-  // reference indexes tree[numtree+j+k*(nchild2+1)] (tree3d.cpp:1458-1477).
-  bool split(const Slot& parent, Block& B) {
-    int& nchild = B.nchild;
-    int nchild2 = 0;
-    auto add = [&](int slot) {
-      if (nchild >= 8) return false;
-      B.kids[nchild++] = slot;
-      return true;
+ private:
+  static constexpr int kDone = -1;
+  Slot parent_, src_;
+  int maxmemb_ = 0;
+  double lim_ = 0;
+  int pc_ = 0, xc_ = 0, yc_ = 0, j_ = 0, k_ = 0, ind2_ = 0, nchild2_ = 0, made_ = 0;
+  int c1_ = 0, c2_ = 0;
+  PartOp op_{};
+
+  bool want_y(const Slot& c) const {
+    return c.used && c.f[BAXY] < lim_ && c.f[BAYZ] > lim_ && c.nummemb() > maxmemb_;
+  }
+  bool want_z(const Slot& c) const {
+    return c.used && c.f[BAXZ] < lim_ && c.f[BAYZ] < lim_ && c.nummemb() > maxmemb_;
+  }
+  void request(int dir, const Slot& src, int c1, int c2) {
+    src_ = src;
+    c1_ = c1;
+    c2_ = c2;
+    op_ = PartOp{src.b, src.e, dir, src.f[BXMID + dir]};
+  }
+  bool add(int slot) {
+    if (B.nchild >= 8) {
+      failed = true;
+      pc_ = kDone;
+      return false;
+    }
+    B.kids[B.nchild++] = slot;
+    return true;
+  }
+
+  // Runs the program until it requests a split (returns) or ends (pc_ = kDone).
+  void advance() {
+    for (;;) {
+      switch (pc_) {
+        case 0:  // first split direction from the parent's aspect ratios
+          if (parent_.f[BAXY] > lim_ && parent_.f[BAXZ] > lim_) {
+            request(0, parent_, 0, 1);
+            pc_ = 1;
+          } else if (parent_.f[BAXY] < lim_ && parent_.f[BAYZ] > lim_) {
+            request(1, parent_, 0, 1);
+            pc_ = 20;
+          } else {
+            request(2, parent_, 0, 1);
+            pc_ = 30;
+          }
+          return;
+        case 1:  // after the x-split
+          xc_ = made_;
+          ind2_ = xc_;
+          j_ = 0;
+          pc_ = 2;
+          break;
+        case 2:  // loop over x-children j
+          if (j_ >= xc_) {
+            pc_ = kDone;
+            return;
+          }
+          if (!add(j_)) return;
+          if (want_y(B.s[j_])) {
+            request(1, B.s[j_], j_, ind2_);
+            pc_ = 3;
+            return;
+          }
+          if (want_z(B.s[j_])) {
+            request(2, B.s[j_], j_, ind2_);
+            pc_ = 6;
+            return;
+          }
+          j_++;
+          break;
+        case 3:  // after the y-split of x-child j
+          yc_ = made_;
+          if (yc_ == 2) {
+            if (!add(ind2_)) return;
+            ind2_++;
+          }
+          k_ = 0;
+          pc_ = 4;
+          break;
+        case 4: {  // z-split checks after a y-split
+          if (k_ >= yc_) {
+            j_++;
+            pc_ = 2;
+            break;
+          }
+          int t = j_ + k_ * (nchild2_ + 1);
+          if (t < 16 && want_z(B.s[t])) {
+            request(2, B.s[t], t, ind2_);
+            pc_ = 5;
+            return;
+          }
+          k_++;
+          break;
+        }
+        case 5:  // after a z-split in the k loop
+          if (made_ == 2) {
+            if (!add(ind2_)) return;
+            nchild2_++;
+            ind2_++;
+          }
+          k_++;
+          pc_ = 4;
+          break;
+        case 6:  // after a z-split of an x-child that was not y-split
+          if (made_ == 2) {
+            if (!add(ind2_)) return;
+            ind2_++;
+          }
+          j_++;
+          pc_ = 2;
+          break;
+        case 20:  // after a y-first split
+          yc_ = made_;
+          ind2_ = yc_;
+          j_ = 0;
+          pc_ = 21;
+          break;
+        case 21:
+          if (j_ >= yc_) {
+            pc_ = kDone;
+            return;
+          }
+          if (!add(j_)) return;
+          if (want_z(B.s[j_])) {
+            request(2, B.s[j_], j_, ind2_);
+            pc_ = 22;
+            return;
+          }
+          j_++;
+          break;
+        case 22:
+          if (made_ == 2) {
+            if (!add(ind2_)) return;
+            ind2_++;
+          }
+          j_++;
+          pc_ = 21;
+          break;
+        case 30:  // after a z-only split
+          for (int jj = 0; jj < made_; jj++)
+            if (!add(jj)) return;
+          pc_ = kDone;
+          return;
+        default:
+          return;
+      }
+      if (ind2_ >= 15) {
+        failed = true;
+        pc_ = kDone;
+        return;
+      }
+    }
+  }
+};
+
+// Partitions on the host arrays: ops of one round in parallel, or one big op
+// with a parallel partition.
+class HostEngine : public PartitionEngine {
+ public:
+  HostEngine(Arrays& a, int threads) : a_(a), threads_(threads) {}
+  bool run(const std::vector<PartOp>& ops, std::vector<PartResult>& res,
+           std::string&) override {
+    res.resize(ops.size());
+    auto one = [&](size_t i, int th) {
+      MinMax mm[2];
+      const PartOp& o = ops[i];
+      int nl = stable_partition(a_, o.b, o.e, o.dir, o.mid, mm, th);
+      PartResult& r = res[i];
+      r.nl = nl;
+      for (int sd = 0; sd < 2; sd++)
+        for (int d = 0; d < 3; d++) {
+          r.mn[sd][d] = mm[sd].mn[d];
+          r.mx[sd][d] = mm[sd].mx[d];
+        }
     };
-    if (parent.f[BAXY] > lim && parent.f[BAXZ] > lim) {
-      int xc = divide(B, 0, parent, 0, 1);
-      int ind2 = xc;
-      for (int j = 0; j < xc; j++) {
-        if (!add(j)) return false;
-        if (want_y(B.s[j])) {
-          int yc = divide(B, 1, B.s[j], j, ind2);
-          if (yc == 2) {
-            if (!add(ind2)) return false;
-            ind2++;
-          }
-          for (int k = 0; k < yc; k++) {
-            int t = j + k * (nchild2 + 1);
-            if (t < 16 && want_z(B.s[t])) {
-              int zc = divide(B, 2, B.s[t], t, ind2);
-              if (zc == 2) {
-                if (!add(ind2)) return false;
-                nchild2++;
-                ind2++;
-              }
-            }
-          }
-        } else if (want_z(B.s[j])) {
-          int zc = divide(B, 2, B.s[j], j, ind2);
-          if (zc == 2) {
-            if (!add(ind2)) return false;
-            ind2++;
-          }
-        }
-        if (ind2 >= 15) return false;
-      }
-    } else if (parent.f[BAXY] < lim && parent.f[BAYZ] > lim) {
-      int yc = divide(B, 1, parent, 0, 1);
-      int ind2 = yc;
-      for (int j = 0; j < yc; j++) {
-        if (!add(j)) return false;
-        if (want_z(B.s[j])) {
-          int zc = divide(B, 2, B.s[j], j, ind2);
-          if (zc == 2) {
-            if (!add(ind2)) return false;
-            ind2++;
-          }
-        }
-      }
+    if ((int)ops.size() < 2 * threads_) {
+      for (size_t i = 0; i < ops.size(); i++) one(i, threads_);
     } else {
-      int zc = divide(B, 2, parent, 0, 1);
-      for (int j = 0; j < zc; j++)
-        if (!add(j)) return false;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads_)
+      for (size_t i = 0; i < ops.size(); i++) one(i, 1);
     }
     return true;
   }
+
+ private:
+  Arrays& a_;
+  int threads_;
 };
 
 // Spreads the low 10 bits of v so that two zero bits separate each bit.
@@ -309,9 +443,7 @@ void order_leaves(HostTree& t, Arrays& a, int threads) {
       const double* f = &t.box[(size_t)lf * BOX_FIELDS];
       key.resize(e - b);
       for (int s = b; s < e; s++) {
-        uint32_t k = spread3(quant(a.X[s], f[BX0], f[BX1])) |
-                     (spread3(quant(a.Y[s], f[BY0], f[BY1])) << 1) |
-                     (spread3(quant(a.Z[s], f[BZ0], f[BZ1])) << 2);
+        uint32_t k = leaf_morton_key(a.X[s], a.Y[s], a.Z[s], f);
         key[s - b] = {k, s};
       }
       std::stable_sort(key.begin(), key.end(),
@@ -335,8 +467,13 @@ void order_leaves(HostTree& t, Arrays& a, int threads) {
 
 }  // namespace
 
-bool build_tree(int n, const double* x, const double* y, const double* z, int maxmemb,
-                HostTree& out, std::string& err, int max_threads) {
+uint32_t leaf_morton_key(double x, double y, double z, const double* f) {
+  return spread3(quant(x, f[BX0], f[BX1])) | (spread3(quant(y, f[BY0], f[BY1])) << 1) |
+         (spread3(quant(z, f[BZ0], f[BZ1])) << 2);
+}
+
+bool build_tree_generic(PartitionEngine& eng, int n, const double mn[3], const double mx[3],
+                        int maxmemb, HostTree& out, std::string& err) {
   if (n <= 0) {
     err = "tree needs at least one particle";
     return false;
@@ -346,67 +483,61 @@ bool build_tree(int n, const double* x, const double* y, const double* z, int ma
     return false;
   }
   const double lim = 1.0 / std::sqrt(2.0);  // aspectlimit, tree3d.cpp:1371
-  const int threads =
-      std::max(1, max_threads > 0 ? std::min(max_threads, omp_get_max_threads())
-                                  : omp_get_max_threads());
   const int node_cap = 4 * n + 64;
-
-  std::vector<int> P(n), tP(n);
-  std::vector<double> X(x, x + n), Y(y, y + n), Z(z, z + n), tX(n), tY(n), tZ(n);
-  for (int i = 0; i < n; i++) P[i] = i;
-  Arrays a{P.data(), X.data(), Y.data(), Z.data(), tP.data(), tX.data(), tY.data(), tZ.data()};
-
   out = HostTree();
   out.n = n;
-  // Root box over all particles (tree3d.cpp:1385-1403).
-  double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0], mnz = z[0], mxz = z[0];
-  for (int i = 0; i < n; i++) {
-    if (mnx > x[i]) mnx = x[i];
-    if (mxx < x[i]) mxx = x[i];
-    if (mny > y[i]) mny = y[i];
-    if (mxy < y[i]) mxy = y[i];
-    if (mnz > z[i]) mnz = z[i];
-    if (mxz < z[i]) mxz = z[i];
-  }
   out.box.resize(BOX_FIELDS);
-  box_init(out.box.data(), mnx, mxx, mny, mxy, mnz, mxz);
+  box_init(out.box.data(), mn[0], mx[0], mn[1], mx[1], mn[2], mx[2]);  // tree3d.cpp:1385-1403
   out.children.assign(8, -1);
   out.begin.push_back(0);
   out.end.push_back(n);
   out.depth.push_back(0);
 
   int gen_lo = 0, gen_hi = 1;
+  std::vector<PartOp> ops;
+  std::vector<PartResult> res;
+  std::vector<int> who;
   while (gen_lo < gen_hi) {
     const int cnt = gen_hi - gen_lo;
-    std::vector<Block> blocks(cnt);
-    std::vector<char> ok(cnt, 1);
-    auto run = [&](int idx, int part_threads) {
+    std::vector<SplitProgram> prog(cnt);
+    std::vector<int> active;
+    for (int idx = 0; idx < cnt; idx++) {
       int i = gen_lo + idx;
-      if (out.end[i] - out.begin[i] <= maxmemb) return;
+      if (out.end[i] - out.begin[i] <= maxmemb) continue;
       Slot parent;
       parent.used = true;
       parent.b = out.begin[i];
       parent.e = out.end[i];
       std::memcpy(parent.f, &out.box[(size_t)i * BOX_FIELDS], sizeof(parent.f));
-      Splitter sp{a, maxmemb, lim, part_threads};
-      ok[idx] = sp.split(parent, blocks[idx]) ? 1 : 0;
-    };
-    if (cnt < 2 * threads) {
-      for (int idx = 0; idx < cnt; idx++) run(idx, threads);
-    } else {
-#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
-      for (int idx = 0; idx < cnt; idx++) run(idx, 1);
+      prog[idx].start(parent, maxmemb, lim);
+      active.push_back(idx);
     }
-    for (int idx = 0; idx < cnt; idx++)
-      if (!ok[idx]) {
-        err = "tree split overflow (more than 8 children)";
-        return false;
-      }
+    // rounds: every node with a pending split advances by one split
+    while (!active.empty()) {
+      ops.clear();
+      who.clear();
+      for (int idx : active)
+        if (!prog[idx].done()) {
+          ops.push_back(prog[idx].op());
+          who.push_back(idx);
+        }
+      if (ops.empty()) break;
+      if (!eng.run(ops, res, err)) return false;
+      for (size_t k = 0; k < ops.size(); k++) prog[who[k]].apply(res[k]);
+      std::vector<int> still;
+      for (int idx : active)
+        if (!prog[idx].done()) still.push_back(idx);
+      active.swap(still);
+    }
     std::vector<int> start(cnt);
     int next = gen_hi;
     for (int idx = 0; idx < cnt; idx++) {
+      if (prog[idx].failed) {
+        err = "tree split overflow (more than 8 children)";
+        return false;
+      }
       start[idx] = next;
-      next += blocks[idx].nchild;
+      next += prog[idx].B.nchild;
     }
     if (next > node_cap) {
       err = "tree exceeds 4n+64 nodes: particles cannot be separated into leaves of maxmemb";
@@ -418,22 +549,51 @@ bool build_tree(int n, const double* x, const double* y, const double* z, int ma
     out.end.resize(next);
     out.depth.resize(next);
     for (int idx = 0; idx < cnt; idx++) {
-      const Block& B = blocks[idx];
+      const Block& B = prog[idx].B;
       int parent = gen_lo + idx;
-      for (int s = 0; s < B.nchild; s++) {
-        int g = start[idx] + s;
-        std::memcpy(&out.box[(size_t)g * BOX_FIELDS], B.s[s].f, sizeof(B.s[s].f));
-        out.begin[g] = B.s[s].b;
-        out.end[g] = B.s[s].e;
+      for (int sl = 0; sl < B.nchild; sl++) {
+        int g = start[idx] + sl;
+        std::memcpy(&out.box[(size_t)g * BOX_FIELDS], B.s[sl].f, sizeof(B.s[sl].f));
+        out.begin[g] = B.s[sl].b;
+        out.end[g] = B.s[sl].e;
         out.depth[g] = out.depth[parent] + 1;
         out.maxdepth = std::max(out.maxdepth, out.depth[g]);
       }
-      for (int c = 0; c < B.nchild; c++) out.children[(size_t)parent * 8 + c] = start[idx] + B.kids[c];
+      for (int c = 0; c < B.nchild; c++)
+        out.children[(size_t)parent * 8 + c] = start[idx] + B.kids[c];
     }
     gen_lo = gen_hi;
     gen_hi = next;
   }
   out.numtree = gen_hi;
+  return true;
+}
+
+bool build_tree(int n, const double* x, const double* y, const double* z, int maxmemb,
+                HostTree& out, std::string& err, int max_threads) {
+  if (n <= 0) {
+    err = "tree needs at least one particle";
+    return false;
+  }
+  const int threads =
+      std::max(1, max_threads > 0 ? std::min(max_threads, omp_get_max_threads())
+                                  : omp_get_max_threads());
+  std::vector<int> P(n), tP(n);
+  std::vector<double> X(x, x + n), Y(y, y + n), Z(z, z + n), tX(n), tY(n), tZ(n);
+  for (int i = 0; i < n; i++) P[i] = i;
+  Arrays a{P.data(), X.data(), Y.data(), Z.data(), tP.data(), tX.data(), tY.data(), tZ.data()};
+  // root bounding box over all particles (tree3d.cpp:1385-1403)
+  double mn[3] = {x[0], y[0], z[0]}, mx[3] = {x[0], y[0], z[0]};
+  for (int i = 0; i < n; i++) {
+    if (mn[0] > x[i]) mn[0] = x[i];
+    if (mx[0] < x[i]) mx[0] = x[i];
+    if (mn[1] > y[i]) mn[1] = y[i];
+    if (mx[1] < y[i]) mx[1] = y[i];
+    if (mn[2] > z[i]) mn[2] = z[i];
+    if (mx[2] < z[i]) mx[2] = z[i];
+  }
+  HostEngine eng(a, threads);
+  if (!build_tree_generic(eng, n, mn, mx, maxmemb, out, err)) return false;
   order_leaves(out, a, threads);
   out.perm = std::move(P);
   return true;

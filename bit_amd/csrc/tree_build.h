@@ -29,6 +29,37 @@ struct HostTree {
 // Builds the tree over n particles (coordinates in original order).  Returns
 // false with a message when the particles cannot be separated into leaves of
 // at most maxmemb (coincident points) or the input is invalid.
+// One stable split request: slots [b, e) by coordinate `dir` <= mid.
+struct PartOp {
+  int b, e, dir;
+  double mid;
+};
+// Its outcome: nl slots on the `<=` side, then each side's bounding box
+// (mn/mx[side][axis], meaningful for a non-empty side).
+struct PartResult {
+  int nl;
+  double mn[2][3];
+  double mx[2][3];
+};
+
+// Executes the split requests of one round (disjoint ranges) on whatever
+// holds the slot arrays (host memory or HBM).
+class PartitionEngine {
+ public:
+  virtual ~PartitionEngine() {}
+  virtual bool run(const std::vector<PartOp>& ops, std::vector<PartResult>& res,
+                   std::string& err) = 0;
+};
+
+// The node structure of treemake for n particles whose bounding box is
+// [mn, mx]: generations of nodes, each node's split program advanced one
+// split per round, splits executed by `eng`.  Fills everything but perm.
+bool build_tree_generic(PartitionEngine& eng, int n, const double mn[3], const double mx[3],
+                        int maxmemb, HostTree& out, std::string& err);
+
+// Morton key of a position inside a leaf box (10 bits per axis, interleaved).
+uint32_t leaf_morton_key(double x, double y, double z, const double* box);
+
 // max_threads > 0 caps the OpenMP threads used (0 = all available).
 bool build_tree(int n, const double* x, const double* y, const double* z, int maxmemb,
                 HostTree& out, std::string& err, int max_threads = 0);

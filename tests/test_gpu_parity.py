@@ -240,3 +240,35 @@ def test_invalid_params_rejected(ctx):
         with pytest.raises(bit_amd.BitError) as e:
             s.potential(**kw)
         assert e.value.code == bit_amd.BIT_E_INVALID
+
+
+@pytest.mark.parametrize("kind,n,seed,mm", [("blobs", 300_000, 31, 2000), ("aniso", 60_000, 32, 25),
+                                            ("uniform", 100_000, 33, 7)])
+def test_device_tree_equals_host_and_oracle(ctx, oracle, kind, n, seed, mm):
+    """The device tree build (split rounds on HBM-resident slots) gives the host
+    build's tree and slot order, and the oracle's (= reference's) tree."""
+    P = workloads.make(kind, n, seed)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, mm)
+    d = s.export_tree()
+    h = bit_amd.build_tree(P.x, P.y, P.z, mm)
+    assert d.numtree == h.numtree
+    assert np.array_equal(d.children, h.children)
+    assert np.array_equal(d.boxes.view(np.uint64), h.boxes.view(np.uint64))
+    assert np.array_equal(d.begin, h.begin) and np.array_equal(d.end, h.end)
+    assert np.array_equal(d.perm, h.perm)
+    e = oracle.tree(P.x, P.y, P.z, mm).export()
+    assert np.array_equal(d.children, e.children)
+    assert np.array_equal(d.boxes.view(np.uint64), e.boxes.view(np.uint64))
+    for i in range(0, d.numtree, max(1, d.numtree // 50)):
+        assert np.array_equal(d.member_list(i), e.node_members(i))
+
+
+def test_device_inputs_rejected_when_nonfinite(ctx):
+    x = np.random.default_rng(1).uniform(size=1000)
+    x[17] = np.nan
+    dev = [bit_amd.DeviceArray.from_numpy(ctx, a) for a in (x, x, x, x)]
+    with pytest.raises(bit_amd.BitError) as e:
+        bit_amd.System(ctx, *dev, 10)
+    assert e.value.code == bit_amd.BIT_E_INVALID
+    for d in dev:
+        d.free()
