@@ -12,9 +12,10 @@ interaction lists, far field and near field, result left in HBM.
 Prints ONE JSON line.  Extra keys: `accuracy` (relative L2 error of the last
 step against an FP64 GPU direct sum on 10,000 seeded targets), `breakdown`
 (ms per stage of the last step), `fp64_ms_per_step` (one FP64 step, untimed
-for the metric), `roofline` (dominant kernel) and `cpu_baseline` (the
-oracle's OpenMP treecode on the host cores, extrapolated from a bounded
-target sample).
+for the metric), `roofline` (dominant kernel), `serving` (p50/p99 request
+latency and throughput of a seeded Poisson request stream served with
+continuous batching) and `cpu_baseline` (the oracle's OpenMP treecode on the
+host cores, extrapolated from a bounded target sample).
 """
 from __future__ import annotations
 
@@ -93,6 +94,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=800,
+                    help="requests of the serving leg (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=400.0)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -167,6 +171,23 @@ def main():
                              "tflops": (f_near / (st["ms_near"] * 1e-3) if kern == "far" else
                                         f_far / (st["ms_far"] * 1e-3)) / 1e12}}
 
+    serving = None
+    if args.serving_requests > 0:
+        from bit_amd import serving as srv_mod
+        log(f"[bench] serving leg: {args.serving_requests} requests at {args.serving_rate}/s")
+        for d in dev + [out]:
+            d.free()
+        reqs = srv_mod.make_requests(args.serving_requests, args.serving_rate, seed=2024)
+        srv = srv_mod.Server(hbm_budget=16 << 30, builders=8)
+        serving = srv.run(reqs)
+        srv.close()
+        serving.update(rate_per_s=args.serving_rate, seed=2024,
+                       errors=sum("error" in r.extra for r in reqs),
+                       workload="Poisson arrivals; N log-uniform [1e3,2e5]; uniform/blobs; "
+                                "delta U[0,0.02], kappa U[0,2], p in {4,6,8}; 20% sphere BIE "
+                                "GMRES solves (levels 3-5)")
+        log(f"[bench] serving p50 {serving['p50_ms']:.1f} ms p99 {serving['p99_ms']:.1f} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline (oracle port), sample {args.cpu_sample} targets")
@@ -198,6 +219,7 @@ def main():
                                             "near_items", "lane_far", "lane_near_pairs")},
         "fp64_ms_per_step": fp64_ms,
         "roofline": roof,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
