@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=800,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=400.0)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also copy this many MB device-to-device (a plain copy kernel of "
+                         "known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -115,6 +118,15 @@ def main():
     ctx = bit_amd.Context(0)
     dev = [bit_amd.DeviceArray.from_numpy(ctx, a) for a in (P.x, P.y, P.z, P.q)]
     out = bit_amd.DeviceArray(ctx, args.n)
+
+    if args.copy_calibration_mb > 0:
+        m = args.copy_calibration_mb * (1 << 20) // 8
+        a_ = bit_amd.DeviceArray(ctx, m)
+        b_ = bit_amd.DeviceArray(ctx, m)
+        b_.copy_from(a_)
+        a_.free()
+        b_.free()
+        log(f"[bench] copied {args.copy_calibration_mb} MB device-to-device (calibration)")
 
     def step(precision):
         s = bit_amd.System(ctx, *dev, args.maxmemb)

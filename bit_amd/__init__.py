@@ -82,6 +82,7 @@ EXPORTS = [
     "bit_system_export_lists", "bit_device_alloc", "bit_device_free", "bit_copy_to_device",
     "bit_copy_to_host", "bit_system_set_charges", "bit_solve_second_kind", "bit_gmres_dense",
     "bit_system_prepare", "bit_eval_potential_multi", "bit_gmres", "bit_ctx_set_host_threads",
+    "bit_copy_on_device",
 ]
 
 
@@ -126,6 +127,7 @@ def _load():
     L.bit_device_free.argtypes = [vp, vp]
     L.bit_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
     L.bit_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
+    L.bit_copy_on_device.argtypes = [vp, vp, vp, C.c_size_t]
     L.bit_system_set_charges.argtypes = [vp, vp, C.c_int]
     L.bit_solve_second_kind.argtypes = [vp, C.POINTER(Params), C.c_double, C.c_double, vp, vp,
                                         vp, C.c_int, C.c_double, C.c_int, vp, vp]
@@ -256,6 +258,11 @@ class DeviceArray:
 
     def data_ptr(self) -> int:
         return self.ptr
+
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of another array of the same size."""
+        _check(lib.bit_copy_on_device(self.ctx.h, C.c_void_p(self.ptr), C.c_void_p(other.ptr),
+                                      self.n * self.dtype.itemsize))
 
     def free(self):
         if self.ptr and self.ctx.h:

@@ -577,6 +577,13 @@ int bit_copy_to_host(bit_ctx* c, void* dst, const void* src, size_t bytes) {
   return BIT_OK;
 }
 
+int bit_copy_on_device(bit_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return fail(BIT_E_INVALID, "null argument");
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
 static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, const double* z,
                          const double* q, int maxmemb, bit_system** out, bool device_src) {
   if (!ctx || !out) return fail(BIT_E_INVALID, "null context or out");

@@ -98,6 +98,7 @@ int bit_device_alloc(bit_ctx *ctx, size_t bytes, void **out);
 int bit_device_free(bit_ctx *ctx, void *ptr);
 int bit_copy_to_device(bit_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bit_copy_to_host(bit_ctx *ctx, void *dst, const void *src, size_t bytes);
+int bit_copy_on_device(bit_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* Builds the tree and uploads the sorted sources.  Replaces treemake
  * (tree3d.cpp:1345-1551; tree3d.h:115): same bisection rules, node numbering,
