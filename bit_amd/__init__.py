@@ -82,7 +82,7 @@ EXPORTS = [
     "bit_system_export_lists", "bit_device_alloc", "bit_device_free", "bit_copy_to_device",
     "bit_copy_to_host", "bit_system_set_charges", "bit_solve_second_kind", "bit_gmres_dense",
     "bit_system_prepare", "bit_eval_potential_multi", "bit_gmres", "bit_ctx_set_host_threads",
-    "bit_copy_on_device",
+    "bit_copy_on_device", "bit_eval_potential_points", "bit_eval_field_points",
 ]
 
 
@@ -137,6 +137,8 @@ def _load():
     L.bit_gmres.argtypes = [C.c_int, MATVEC_FN, vp, vp, vp, C.c_int, C.c_double, C.c_int, vp,
                             vp]
     L.bit_ctx_set_host_threads.argtypes = [vp, C.c_int]
+    for f in ("bit_eval_potential_points", "bit_eval_field_points"):
+        getattr(L, f).argtypes = [vp, C.POINTER(Params), C.c_int, vp, vp, vp, vp]
     L.bit_tree_build.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.POINTER(vp)]
     L.bit_tree_destroy.argtypes = [vp]
     L.bit_tree_numtree.argtypes = [vp]
@@ -444,6 +446,25 @@ class System:
         res = np.zeros(3 * self.n)
         _check(lib.bit_eval_field(self.h, C.byref(prm), _ptr(res), 0))
         return res.reshape(self.n, 3)
+
+    def potential_at(self, px, py, pz, nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None,
+                     precision="fp64"):
+        """+phi at arbitrary points (all sources contribute)."""
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        out = np.zeros(len(px))
+        _check(lib.bit_eval_potential_points(self.h, C.byref(prm), len(px), _ptr(px), _ptr(py),
+                                             _ptr(pz), _ptr(out)))
+        return out
+
+    def field_at(self, px, py, pz, nterms=6, acc=ACC, delta2=DEL, mac_sub=None):
+        """Regularized Coulomb field at arbitrary points, [m, 3]."""
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64")
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        out = np.zeros(3 * len(px))
+        _check(lib.bit_eval_field_points(self.h, C.byref(prm), len(px), _ptr(px), _ptr(py),
+                                         _ptr(pz), _ptr(out)))
+        return out.reshape(-1, 3)
 
     def direct(self, kappa, delta2, tidx):
         """FP64 GPU direct sum at particle targets tidx (original indices)."""

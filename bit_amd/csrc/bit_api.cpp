@@ -136,6 +136,14 @@ struct Timer {
 
 }  // namespace
 
+// Interaction lists of one target set.
+struct ListSet {
+  bool valid = false;
+  double acc = 0, delta2 = 0, sub = 0;
+  int nfar = 0, nnear = 0;
+  DevBuf far_cnt, near_cnt, far_off, near_off, far_items, near_items;
+};
+
 struct bit_system {
   bit_ctx* ctx = nullptr;
   bit::HostTree tree;
@@ -147,12 +155,14 @@ struct bit_system {
   int mom_nterms = 0;
   int nchunks = 0;
   DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, mom_hat32;
-  // lists
-  bool lists_valid = false;
-  double l_acc = 0, l_delta2 = 0, l_sub = 0;
-  int n_far_items = 0, n_near_items = 0;
-  DevBuf far_cnt, near_cnt, far_off, near_off, far_items, near_items, scan_tmp, lstats, err_flag;
+  // lists: pl for the particles as targets (cached per MAC parameters), ql for
+  // the last external point set
+  ListSet pl, ql;
+  DevBuf scan_tmp, lstats, err_flag;
   bit::dev::ListStats hstats{};
+  // external point targets (batch order) and their results
+  DevBuf qx, qy, qz, qperm, qphi, qout, qdesc;
+  bit::dev::EvalDesc qhdesc{};
   // outputs
   DevBuf phi_far, out;
   // direct sum
@@ -179,6 +189,12 @@ struct bit_system {
     v.sq = sq.as<double>();
     v.sloc = sloc.as<float4>();
     v.perm = perm.as<int>();
+    v.nt = tree.n;
+    v.self = 1;
+    v.tx = v.sx;
+    v.ty = v.sy;
+    v.tz = v.sz;
+    v.tperm = v.perm;
     return v;
   }
 };
@@ -388,62 +404,59 @@ int check_params(const bit_system* s, const bit_params* p) {
   return BIT_OK;
 }
 
-int ensure_lists(bit_system* s, const bit_params* p) {
-  if (s->lists_valid && s->l_acc == p->acc && s->l_delta2 == p->delta2 && s->l_sub == p->mac_sub)
-    return BIT_OK;
-  s->lists_valid = false;  // set again only when both passes were issued
+// Two passes over the tree per target batch (count, exclusive scan, fill).
+int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const bit_params* p) {
+  L.valid = false;  // set again only when both passes were issued
   bit_ctx* c = s->ctx;
-  const int n = s->tree.n;
-  const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
+  const int nb = (tv.nt + bit::dev::kBatch - 1) / bit::dev::kBatch;
   bit::dev::MacParams mp{p->acc, p->delta2, p->mac_sub};
-  BIT_TRY(s->far_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
-  BIT_TRY(s->near_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
-  BIT_TRY(s->far_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
-  BIT_TRY(s->near_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(L.far_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(L.near_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(L.far_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
+  BIT_TRY(L.near_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
   BIT_TRY(s->lstats.ensure(c, sizeof(bit::dev::ListStats)));
   BIT_TRY(s->err_flag.ensure(c, sizeof(int)));
-  HIP_TRY(hipMemsetAsync(s->far_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
-  HIP_TRY(hipMemsetAsync(s->near_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
+  HIP_TRY(hipMemsetAsync(L.far_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
+  HIP_TRY(hipMemsetAsync(L.near_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
   HIP_TRY(hipMemsetAsync(s->lstats.p, 0, sizeof(bit::dev::ListStats), c->stream));
   HIP_TRY(hipMemsetAsync(s->err_flag.p, 0, sizeof(int), c->stream));
   s->t_lists.start(c->stream);
-  HIP_TRY(bit::dev::lists_count(s->view(), mp, s->stack_cap, s->far_cnt.as<int>(),
-                                s->near_cnt.as<int>(), s->lstats.as<bit::dev::ListStats>(),
-                                s->err_flag.as<int>(), c->stream));
+  HIP_TRY(bit::dev::lists_count(tv, mp, s->stack_cap, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
+                                s->lstats.as<bit::dev::ListStats>(), s->err_flag.as<int>(),
+                                c->stream));
   size_t tmp = 0;
-  HIP_TRY(bit::dev::exclusive_scan(s->far_cnt.as<int>(), s->far_off.as<int>(), nb + 1, nullptr,
-                                   tmp, c->stream));
+  HIP_TRY(bit::dev::exclusive_scan(L.far_cnt.as<int>(), L.far_off.as<int>(), nb + 1, nullptr, tmp,
+                                   c->stream));
   BIT_TRY(s->scan_tmp.ensure(c, tmp));
   tmp = s->scan_tmp.bytes;
-  HIP_TRY(bit::dev::exclusive_scan(s->far_cnt.as<int>(), s->far_off.as<int>(), nb + 1,
-                                   s->scan_tmp.p, tmp, c->stream));
+  HIP_TRY(bit::dev::exclusive_scan(L.far_cnt.as<int>(), L.far_off.as<int>(), nb + 1, s->scan_tmp.p,
+                                   tmp, c->stream));
   tmp = s->scan_tmp.bytes;
-  HIP_TRY(bit::dev::exclusive_scan(s->near_cnt.as<int>(), s->near_off.as<int>(), nb + 1,
+  HIP_TRY(bit::dev::exclusive_scan(L.near_cnt.as<int>(), L.near_off.as<int>(), nb + 1,
                                    s->scan_tmp.p, tmp, c->stream));
   int tot[2] = {0, 0};
   int eflag = 0;
-  HIP_TRY(hipMemcpyAsync(&tot[0], s->far_off.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost,
+  HIP_TRY(hipMemcpyAsync(&tot[0], L.far_off.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost,
                          c->stream));
-  HIP_TRY(hipMemcpyAsync(&tot[1], s->near_off.as<int>() + nb, sizeof(int),
-                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&tot[1], L.near_off.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
   HIP_TRY(hipMemcpyAsync(&s->hstats, s->lstats.p, sizeof(bit::dev::ListStats),
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(&eflag, s->err_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (eflag) return fail(BIT_E_STATE, "traversal stack overflow");
-  s->n_far_items = tot[0];
-  s->n_near_items = tot[1];
-  BIT_TRY(s->far_items.ensure(c, (size_t)std::max(1, tot[0]) * sizeof(bit::dev::Item)));
-  BIT_TRY(s->near_items.ensure(c, (size_t)std::max(1, tot[1]) * sizeof(bit::dev::Item)));
-  HIP_TRY(bit::dev::lists_fill(s->view(), mp, s->stack_cap, s->far_off.as<int>(),
-                               s->near_off.as<int>(), s->far_items.as<bit::dev::Item>(),
-                               s->near_items.as<bit::dev::Item>(), s->err_flag.as<int>(),
-                               c->stream));
+  L.nfar = tot[0];
+  L.nnear = tot[1];
+  BIT_TRY(L.far_items.ensure(c, (size_t)std::max(1, tot[0]) * sizeof(bit::dev::Item)));
+  BIT_TRY(L.near_items.ensure(c, (size_t)std::max(1, tot[1]) * sizeof(bit::dev::Item)));
+  HIP_TRY(bit::dev::lists_fill(tv, mp, s->stack_cap, L.far_off.as<int>(), L.near_off.as<int>(),
+                               L.far_items.as<bit::dev::Item>(), L.near_items.as<bit::dev::Item>(),
+                               s->err_flag.as<int>(), c->stream));
   s->t_lists.stop(c->stream);
-  s->lists_valid = true;
-  s->l_acc = p->acc;
-  s->l_delta2 = p->delta2;
-  s->l_sub = p->mac_sub;
+  L.valid = true;
+  L.acc = p->acc;
+  L.delta2 = p->delta2;
+  L.sub = p->mac_sub;
   s->stats.far_items = tot[0];
   s->stats.near_items = tot[1];
   s->stats.far_pairs = (int64_t)s->hstats.far_pairs;
@@ -452,6 +465,13 @@ int ensure_lists(bit_system* s, const bit_params* p) {
   s->stats.lane_near_pairs = 64 * (int64_t)s->hstats.near_slots;
   s->stats.ms_lists = -1.0;  // filled after the next synchronisation
   return BIT_OK;
+}
+
+// Lists of the particles as targets, cached per MAC parameters.
+int ensure_lists(bit_system* s, const bit_params* p) {
+  ListSet& L = s->pl;
+  if (L.valid && L.acc == p->acc && L.delta2 == p->delta2 && L.sub == p->mac_sub) return BIT_OK;
+  return build_lists(s, L, s->view(), p);
 }
 
 }  // namespace
@@ -811,10 +831,10 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
   d.delta2 = p->delta2;
   d.mom64 = s->mom_hat64.as<double>();
   d.mom32 = s->mom_hat32.as<float>();
-  d.far_off = s->far_off.as<int>();
-  d.far_items = s->far_items.as<bit::dev::Item>();
-  d.near_off = s->near_off.as<int>();
-  d.near_items = s->near_items.as<bit::dev::Item>();
+  d.far_off = s->pl.far_off.as<int>();
+  d.far_items = s->pl.far_items.as<bit::dev::Item>();
+  d.near_off = s->pl.near_off.as<int>();
+  d.near_items = s->pl.near_items.as<bit::dev::Item>();
   d.phi_far = s->phi_far.as<double>();
   d.out = dst;
   return BIT_OK;
@@ -935,12 +955,12 @@ int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_d
   }
   s->t_far.start(c->stream);
   HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, s->mom_hat64.as<double>(),
-                                   s->far_off.as<int>(), s->far_items.as<bit::dev::Item>(),
+                                   s->pl.far_off.as<int>(), s->pl.far_items.as<bit::dev::Item>(),
                                    s->phi_far.as<double>(), c->stream));
   s->t_far.stop(c->stream);
   s->t_near.start(c->stream);
-  HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, s->near_off.as<int>(),
-                                    s->near_items.as<bit::dev::Item>(), s->phi_far.as<double>(),
+  HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, s->pl.near_off.as<int>(),
+                                    s->pl.near_items.as<bit::dev::Item>(), s->phi_far.as<double>(),
                                     dst, c->stream));
   s->t_near.stop(c->stream);
   if (!out_on_device)
@@ -948,6 +968,112 @@ int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_d
                            c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   collect_times(s);
+  return BIT_OK;
+}
+
+// External targets: Morton order over the root box so that each batch of 64
+// points is compact, uploaded with the permutation back to the caller's order.
+static int setup_points(bit_system* s, int m, const double* x, const double* y, const double* z,
+                        bit::dev::TreeView& tv) {
+  if (m < 1 || !x || !y || !z) return fail(BIT_E_INVALID, "need m >= 1 points and coordinates");
+  if (!finite_all(m, x) || !finite_all(m, y) || !finite_all(m, z))
+    return fail(BIT_E_INVALID, "non-finite point coordinate");
+  bit_ctx* c = s->ctx;
+  const double* root = &s->tree.box[0];
+  std::vector<std::pair<uint32_t, int>> key(m);
+  for (int i = 0; i < m; i++) key[i] = {bit::leaf_morton_key(x[i], y[i], z[i], root), i};
+  std::stable_sort(key.begin(), key.end(),
+                   [](const std::pair<uint32_t, int>& u, const std::pair<uint32_t, int>& v) {
+                     return u.first < v.first;
+                   });
+  std::vector<double> hx(m), hy(m), hz(m);
+  std::vector<int> hp(m);
+  for (int i = 0; i < m; i++) {
+    int k = key[i].second;
+    hx[i] = x[k];
+    hy[i] = y[k];
+    hz[i] = z[k];
+    hp[i] = k;
+  }
+  BIT_TRY(upload(c, s->qx, hx.data(), (size_t)m * sizeof(double)));
+  BIT_TRY(upload(c, s->qy, hy.data(), (size_t)m * sizeof(double)));
+  BIT_TRY(upload(c, s->qz, hz.data(), (size_t)m * sizeof(double)));
+  BIT_TRY(upload(c, s->qperm, hp.data(), (size_t)m * sizeof(int)));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // host staging dies here
+  tv = s->view();
+  tv.nt = m;
+  tv.self = 0;
+  tv.tx = s->qx.as<double>();
+  tv.ty = s->qy.as<double>();
+  tv.tz = s->qz.as<double>();
+  tv.tperm = s->qperm.as<int>();
+  return BIT_OK;
+}
+
+int bit_eval_potential_points(bit_system* s, const bit_params* p, int m, const double* x,
+                              const double* y, const double* z, double* out) {
+  BIT_TRY(check_params(s, p));
+  if (m < 0) return fail(BIT_E_INVALID, "negative point count");
+  if (m == 0) return BIT_OK;  // no points, nothing to evaluate
+  if (!out) return fail(BIT_E_INVALID, "null output");
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  BIT_TRY(bit_moments(s, p->nterms));
+  bit::dev::TreeView tv;
+  BIT_TRY(setup_points(s, m, x, y, z, tv));
+  BIT_TRY(build_lists(s, s->ql, tv, p));
+  BIT_TRY(s->qphi.ensure(c, (size_t)m * sizeof(double)));
+  BIT_TRY(s->qout.ensure(c, (size_t)m * sizeof(double)));
+  bit::dev::EvalDesc& d = s->qhdesc;
+  d.tv = tv;
+  d.kappa = p->kappa;
+  d.delta2 = p->delta2;
+  d.mom64 = s->mom_hat64.as<double>();
+  d.mom32 = s->mom_hat32.as<float>();
+  d.far_off = s->ql.far_off.as<int>();
+  d.far_items = s->ql.far_items.as<bit::dev::Item>();
+  d.near_off = s->ql.near_off.as<int>();
+  d.near_items = s->ql.near_items.as<bit::dev::Item>();
+  d.phi_far = s->qphi.as<double>();
+  d.out = s->qout.as<double>();
+  BIT_TRY(upload(c, s->qdesc, &d, sizeof(bit::dev::EvalDesc)));
+  const int nb = (m + bit::dev::kBatch - 1) / bit::dev::kBatch;
+  const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
+  const bit::dev::EvalDesc* dd = s->qdesc.as<bit::dev::EvalDesc>();
+  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
+  HIP_TRY(bit::dev::near_field(dd, nullptr, nb, fp32, scr, c->stream));
+  HIP_TRY(hipMemcpyAsync(out, s->qout.p, (size_t)m * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
+int bit_eval_field_points(bit_system* s, const bit_params* p, int m, const double* x,
+                          const double* y, const double* z, double* out) {
+  BIT_TRY(check_params(s, p));
+  if (m < 0) return fail(BIT_E_INVALID, "negative point count");
+  if (m == 0) return BIT_OK;  // no points, nothing to evaluate
+  if (!out) return fail(BIT_E_INVALID, "null output");
+  if (p->kappa != 0.0) return fail(BIT_E_INVALID, "field evaluation supports kappa = 0 only");
+  if (p->nterms > BIT_MAX_NTERMS - 1)
+    return fail(BIT_E_INVALID, "field needs nterms <= " + std::to_string(BIT_MAX_NTERMS - 1));
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  BIT_TRY(bit_moments(s, p->nterms));
+  bit::dev::TreeView tv;
+  BIT_TRY(setup_points(s, m, x, y, z, tv));
+  BIT_TRY(build_lists(s, s->ql, tv, p));
+  BIT_TRY(s->qphi.ensure(c, (size_t)3 * m * sizeof(double)));
+  BIT_TRY(s->qout.ensure(c, (size_t)3 * m * sizeof(double)));
+  HIP_TRY(bit::dev::far_field_grad(tv, p->nterms, p->delta2, s->mom_hat64.as<double>(),
+                                   s->ql.far_off.as<int>(), s->ql.far_items.as<bit::dev::Item>(),
+                                   s->qphi.as<double>(), c->stream));
+  HIP_TRY(bit::dev::near_field_grad(tv, p->delta2, s->ql.near_off.as<int>(),
+                                    s->ql.near_items.as<bit::dev::Item>(), s->qphi.as<double>(),
+                                    s->qout.as<double>(), c->stream));
+  HIP_TRY(hipMemcpyAsync(out, s->qout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return BIT_OK;
 }
 
@@ -967,24 +1093,24 @@ int bit_system_export_lists(bit_system* s, const bit_params* p, int* nbatch, int
   BIT_TRY(ensure_lists(s, p));
   const int nb = (s->tree.n + bit::dev::kBatch - 1) / bit::dev::kBatch;
   *nbatch = nb;
-  *nfar = s->n_far_items;
-  *nnear = s->n_near_items;
+  *nfar = s->pl.nfar;
+  *nnear = s->pl.nnear;
   if (!far_off) return BIT_OK;
-  std::vector<bit::dev::Item> f(std::max(1, s->n_far_items)), n(std::max(1, s->n_near_items));
-  HIP_TRY(hipMemcpyAsync(far_off, s->far_off.p, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost,
+  std::vector<bit::dev::Item> f(std::max(1, s->pl.nfar)), n(std::max(1, s->pl.nnear));
+  HIP_TRY(hipMemcpyAsync(far_off, s->pl.far_off.p, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost,
                          c->stream));
-  HIP_TRY(hipMemcpyAsync(near_off, s->near_off.p, (nb + 1) * sizeof(int),
+  HIP_TRY(hipMemcpyAsync(near_off, s->pl.near_off.p, (nb + 1) * sizeof(int),
                          hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(f.data(), s->far_items.p, s->n_far_items * sizeof(bit::dev::Item),
+  HIP_TRY(hipMemcpyAsync(f.data(), s->pl.far_items.p, s->pl.nfar * sizeof(bit::dev::Item),
                          hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(n.data(), s->near_items.p, s->n_near_items * sizeof(bit::dev::Item),
+  HIP_TRY(hipMemcpyAsync(n.data(), s->pl.near_items.p, s->pl.nnear * sizeof(bit::dev::Item),
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < s->n_far_items; i++) {
+  for (int i = 0; i < s->pl.nfar; i++) {
     far_mask[i] = f[i].mask;
     far_node[i] = f[i].node;
   }
-  for (int i = 0; i < s->n_near_items; i++) {
+  for (int i = 0; i < s->pl.nnear; i++) {
     near_mask[i] = n[i].mask;
     near_node[i] = n[i].node;
   }

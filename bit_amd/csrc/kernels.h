@@ -34,6 +34,13 @@ struct TreeView {
   const double* sq;
   const float4* sloc;     // sorted sources in leaf-centre coordinates, FP32 (x,y,z,q)
   const int* perm;        // sorted slot -> original index
+  // targets, in batch order (the sources themselves, or external points)
+  int nt;
+  int self;               // 1: targets are the sources (self pair excluded)
+  const double* tx;
+  const double* ty;
+  const double* tz;
+  const int* tperm;       // target slot -> caller's index
 };
 
 struct MacParams {

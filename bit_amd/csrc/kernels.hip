@@ -81,10 +81,9 @@ inline constexpr Tab<NT> kTab{};
 // the last two degree shells stay live in registers:
 //   GRAD = false: acc0 += M_k a_k                      (potential)
 //   GRAD = true:  acc_e += k_e M_{k-e} a_k, e = x,y,z   (field, M of degree < NT-1)
-template <typename T, int NT, bool SCR, bool GRAD, int C>
+template <typename T, int NT, bool SCR, bool GRAD, int C, typename MP>
 __device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (&x2)[3],
-                                          T fac, const T* kinv, const T* __restrict__ M,
-                                          T (&acc)[3]) {
+                                          T fac, const T* kinv, MP M, T (&acc)[3]) {
   constexpr const Tab<NT>& tb = kTab<NT>;
   constexpr int d = tb.d[C];
   constexpr int kd = tb.k[C][d];
@@ -121,26 +120,36 @@ __device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (
   if constexpr (!GRAD) {
     acc[0] = fma(M[C], a[C], acc[0]);
   } else {
-    if constexpr (e0 >= 0) acc[0] = fma(T(tb.k[C][0]) * M[e0], a[C], acc[0]);
-    if constexpr (e1 >= 0) acc[1] = fma(T(tb.k[C][1]) * M[e1], a[C], acc[1]);
-    if constexpr (e2 >= 0) acc[2] = fma(T(tb.k[C][2]) * M[e2], a[C], acc[2]);
+    // the integer factor multiplies the per-lane coefficient, so the moment
+    // stays a scalar operand of the FMA (no per-lane copies of M)
+    if constexpr (e0 >= 0) acc[0] = fma(M[e0], tb.k[C][0] == 1 ? a[C] : T(tb.k[C][0]) * a[C], acc[0]);
+    if constexpr (e1 >= 0) acc[1] = fma(M[e1], tb.k[C][1] == 1 ? a[C] : T(tb.k[C][1]) * a[C], acc[1]);
+    if constexpr (e2 >= 0) acc[2] = fma(M[e2], tb.k[C][2] == 1 ? a[C] : T(tb.k[C][2]) * a[C], acc[2]);
   }
 }
 
-template <typename T, int NT, bool SCR, bool GRAD, int... C>
+template <typename T, int NT, bool SCR, bool GRAD, typename MP, int... C>
 __device__ __forceinline__ void coef_all(std::integer_sequence<int, C...>, T* a, T* b,
                                          const T (&x)[3], const T (&x2)[3], T fac,
-                                         const T* kinv, const T* __restrict__ M,
-                                         T (&acc)[3]) {
-  (coef_step<T, NT, SCR, GRAD, C + 1>(a, b, x, x2, fac, kinv, M, acc), ...);
+                                         const T* kinv, MP M, T (&acc)[3]) {
+  (coef_step<T, NT, SCR, GRAD, C + 1, MP>(a, b, x, x2, fac, kinv, M, acc), ...);
+}
+
+// Pointers read from a descriptor are generic (flat) to the compiler; loads
+// through a global-address-space pointer with a wave-uniform address can be
+// scalar loads (the values then live in SGPRs, not VGPRs).
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(const T* p) {
+  return (gptr<T>)(p);
 }
 
 // Far field of one cluster at one target, in units of the cluster scale s:
 // x = (target - centre)/s, fac = s^2/R^2, a0 = exp(-kh Rh)/Rh, b0 = exp(-kh Rh),
 // kh = kappa s.  Returns sum_k Mhat_k ahat_k (times 1/s = the potential).
-template <typename T, int NT, bool SCR>
-__device__ __forceinline__ T far_eval(T x0, T x1, T x2_, T fac, T a0, T b0, T kh,
-                                      const T* __restrict__ M) {
+template <typename T, int NT, bool SCR, typename MP>
+__device__ __forceinline__ T far_eval(T x0, T x1, T x2_, T fac, T a0, T b0, T kh, MP M) {
   constexpr int NC = ncoef(NT);
   T a[NC];
   T b[SCR ? NC : 1];
@@ -155,15 +164,15 @@ __device__ __forceinline__ T far_eval(T x0, T x1, T x2_, T fac, T a0, T b0, T kh
   const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
   T acc[3] = {M[0] * a0, T(0), T(0)};
   if constexpr (NC > 1)
-    coef_all<T, NT, SCR, false>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, fac,
-                                kinv, M, acc);
+    coef_all<T, NT, SCR, false, MP>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, fac,
+                                    kinv, M, acc);
   return acc[0];
 }
 
 // Field: E_e = sum_k (k_e+1) M_k a_{k+e}, coefficients to degree NT (Coulomb).
-template <typename T, int NT>
-__device__ __forceinline__ void far_eval_grad(T x0, T x1, T x2_, T fac, T a0,
-                                              const T* __restrict__ M, T& ex, T& ey, T& ez) {
+template <typename T, int NT, typename MP>
+__device__ __forceinline__ void far_eval_grad(T x0, T x1, T x2_, T fac, T a0, MP M, T& ex, T& ey,
+                                              T& ez) {
   constexpr int NC = ncoef(NT + 1);
   T a[NC];
   T b[1];
@@ -172,8 +181,8 @@ __device__ __forceinline__ void far_eval_grad(T x0, T x1, T x2_, T fac, T a0,
   const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
   T kinv[2] = {T(0), T(0)};
   T acc[3] = {T(0), T(0), T(0)};
-  coef_all<T, NT + 1, false, true>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, fac,
-                                   kinv, M, acc);
+  coef_all<T, NT + 1, false, true, MP>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx,
+                                       fac, kinv, M, acc);
   ex = acc[0];
   ey = acc[1];
   ez = acc[2];
@@ -677,9 +686,9 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
-  const bool valid = t < tv.n;
-  const int tt = valid ? t : tv.n - 1;
-  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   const unsigned long long active = ballot(valid);
   if (lane == 0) {
     st_node[0] = 0;
@@ -743,7 +752,7 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
                        int* near_cnt, ListStats* stats, int* err_flag, hipStream_t s) {
-  int nb = (tv.n + kBatch - 1) / kBatch;
+  int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
   hipLaunchKernelGGL(k_lists<false>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap, far_cnt,
                      near_cnt, stats, (const int*)nullptr, (const int*)nullptr,
@@ -754,7 +763,7 @@ hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far
 hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,
                       const int* near_off, Item* far_items, Item* near_items, int* err_flag,
                       hipStream_t s) {
-  int nb = (tv.n + kBatch - 1) / kBatch;
+  int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
   hipLaunchKernelGGL(k_lists<true>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
                      (int*)nullptr, (int*)nullptr, (ListStats*)nullptr, far_off, near_off,
@@ -827,30 +836,35 @@ __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
   const EvalDesc& d = descs[sys];
   const TreeView& tv = d.tv;
   const double kappa = d.kappa, delta2 = d.delta2;
-  const T* __restrict__ mom_hat = mom_of<T>(d);
+  const gptr<T> mom_hat = as_global(mom_of<T>(d));
+  const gptr<Item> items = as_global(d.far_items);
+  const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
+  const gptr<double> scale = as_global(tv.scale);
+  const gptr<int> far_off = as_global(d.far_off);
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
-  const bool valid = t < tv.n;
-  const int tt = valid ? t : tv.n - 1;
-  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   Acc<T> acc;
-  const int i0 = uniform(d.far_off[b]), i1 = uniform(d.far_off[b + 1]);
+  const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
   for (int it = i0; it < i1; it++) {
-    const unsigned long long mask = uniform64(d.far_items[it].mask);
-    const int node = uniform(d.far_items[it].node);
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
     if ((mask >> lane) & 1ull) {
-      const double4 c = tv.center[node];
-      const double s = tv.scale[node];
+      const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
+      const double s = scale[node];
       const double inv = 1.0 / s;
-      const double dx = px - c.x, dy = py - c.y, dz = pz - c.z;
+      const double dx = px - cx, dy = py - cy, dz = pz - cz;
       const double R2 = dx * dx + dy * dy + dz * dz + delta2;
       const double R2h = R2 * inv * inv;
       const double Rh = sqrt(R2h);
       const double kh = kappa * s;
       const double b0 = SCR ? exp(-kh * Rh) : 1.0;
       const double a0 = b0 / Rh;
-      T v = far_eval<T, NT, SCR>((T)(dx * inv), (T)(dy * inv), (T)(dz * inv), (T)(1.0 / R2h),
-                                 (T)a0, (T)b0, (T)kh, mom_hat + (size_t)node * NC);
+      T v = far_eval<T, NT, SCR, gptr<T>>((T)(dx * inv), (T)(dy * inv), (T)(dz * inv),
+                                          (T)(1.0 / R2h), (T)a0, (T)b0, (T)kh,
+                                          mom_hat + (size_t)node * NC);
       acc.add(v * (T)inv);
     }
   }
@@ -931,10 +945,10 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
   const float kneg2 = (float)(-d.kappa * 1.4426950408889634);
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
-  const bool valid = t < tv.n;
-  const int tt = valid ? t : tv.n - 1;
-  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
-  const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
+  const int t0 = b * kBatch, t1 = min(tv.nt, t0 + kBatch);
   Acc<float> acc;
   const int i0 = uniform(d.near_off[b]), i1 = uniform(d.near_off[b + 1]);
   for (int it = i0; it < i1; it++) {
@@ -944,7 +958,9 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
     const double4 c = tv.center[node];
     const bool act = (mask >> lane) & 1ull;
     const float ox = (float)(px - c.x), oy = (float)(py - c.y), oz = (float)(pz - c.z);
-    const bool selfp = r.x < t1 && r.y > t0;
+    // particle targets: the self pair lies in leaves holding some of the
+    // batch's targets; point targets: pairs closer than eps are skipped instead
+    const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
     for (int base = r.x; base < r.y; base += 64) {
       const int cnt = min(64, r.y - base);
       if (lane < cnt) tile[lane] = tv.sloc[base + lane];
@@ -961,7 +977,7 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
           float g = 0.0f;
           for (; j < cnt; j++) g += pair32<SCR>(ox, oy, oz, tile[j], d2, kneg2);
           acc.add(g);
-        } else {
+        } else if (tv.self) {
           const int me = t - base;
           int j = 0;
           for (; j + 16 <= cnt; j += 16) {
@@ -979,12 +995,22 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
             g += (j == me) ? 0.0f : v;
           }
           acc.add(g);
+        } else {  // point targets: skip coincident sources (R^2 <= eps)
+          float g = 0.0f;
+          for (int j = 0; j < cnt; j++) {
+            float4 q = tile[j];
+            float dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
+            float v = pair32<SCR>(ox, oy, oz, q, d2, kneg2);
+            g += r2 > 1e-14f ? v : 0.0f;
+          }
+          acc.add(g);
         }
       }
       __syncthreads();
     }
   }
-  if (valid) d.out[tv.perm[t]] = d.phi_far[t] + acc.value();
+  if (valid) d.out[tv.tperm[t]] = d.phi_far[t] + acc.value();
 }
 
 template <bool SCR>
@@ -998,10 +1024,10 @@ __global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ desc
   const double d2 = d.delta2, kappa = d.kappa;
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
-  const bool valid = t < tv.n;
-  const int tt = valid ? t : tv.n - 1;
-  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
-  const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
+  const int t0 = b * kBatch, t1 = min(tv.nt, t0 + kBatch);
   double acc = 0.0;
   const int i0 = uniform(d.near_off[b]), i1 = uniform(d.near_off[b + 1]);
   for (int it = i0; it < i1; it++) {
@@ -1009,7 +1035,9 @@ __global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ desc
     const int node = uniform(d.near_items[it].node);
     const int2 r = tv.range[node];
     const bool act = (mask >> lane) & 1ull;
-    const bool selfp = r.x < t1 && r.y > t0;
+    // particle targets: the self pair lies in leaves holding some of the
+    // batch's targets; point targets: pairs closer than eps are skipped instead
+    const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
     for (int base = r.x; base < r.y; base += 64) {
       const int cnt = min(64, r.y - base);
       if (lane < cnt) {
@@ -1018,16 +1046,21 @@ __global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ desc
       }
       __syncthreads();
       if (act) {
-        const int me = selfp ? t - base : -1;
+        const int me = tv.self && selfp ? t - base : -1;
         for (int j = 0; j < cnt; j++) {
           if (j == me) continue;
+          if (!tv.self) {  // point targets: skip coincident sources (R^2 <= eps)
+            double4 q = tile[j];
+            double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+            if (dx * dx + dy * dy + dz * dz + d2 <= 1e-14) continue;
+          }
           acc += pair64<SCR>(px, py, pz, tile[j], d2, kappa);
         }
       }
       __syncthreads();
     }
   }
-  if (valid) d.out[tv.perm[t]] = d.phi_far[t] + acc;
+  if (valid) d.out[tv.tperm[t]] = d.phi_far[t] + acc;
 }
 
 hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool fp32,
@@ -1060,9 +1093,9 @@ __global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2,
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
-  const bool valid = t < tv.n;
-  const int tt = valid ? t : tv.n - 1;
-  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   double ex = 0, ey = 0, ez = 0;
   const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
   for (int it = i0; it < i1; it++) {
@@ -1076,8 +1109,9 @@ __global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2,
       const double R2h = (dx * dx + dy * dy + dz * dz + delta2) * inv * inv;
       const double a0 = 1.0 / sqrt(R2h);
       double gx, gy, gz;
-      far_eval_grad<double, NT>(dx * inv, dy * inv, dz * inv, 1.0 / R2h, a0,
-                                mom_hat + (size_t)node * NC, gx, gy, gz);
+      far_eval_grad<double, NT, gptr<double>>(dx * inv, dy * inv, dz * inv, 1.0 / R2h, a0,
+                                               as_global(mom_hat) + (size_t)node * NC, gx, gy,
+                                               gz);
       // d/dx of (1/s) f(x/s) -> 1/s^2
       ex += gx * inv * inv;
       ey += gy * inv * inv;
@@ -1094,7 +1128,7 @@ __global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2,
 template <int NT>
 static void launch_far_grad(const TreeView& tv, double delta2, const double* mom,
                             const int* far_off, const Item* items, double* e_far, hipStream_t s) {
-  int nb = (tv.n + kBatch - 1) / kBatch;
+  int nb = (tv.nt + kBatch - 1) / kBatch;
   hipLaunchKernelGGL((k_far_grad<NT>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, mom, far_off,
                      items, e_far);
 }
@@ -1130,10 +1164,10 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
-  const bool valid = t < tv.n;
-  const int tt = valid ? t : tv.n - 1;
-  const double px = tv.sx[tt], py = tv.sy[tt], pz = tv.sz[tt];
-  const int t0 = b * kBatch, t1 = min(tv.n, t0 + kBatch);
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
+  const int t0 = b * kBatch, t1 = min(tv.nt, t0 + kBatch);
   double ex = 0, ey = 0, ez = 0;
   const int i0 = uniform(near_off[b]), i1 = uniform(near_off[b + 1]);
   for (int it = i0; it < i1; it++) {
@@ -1141,7 +1175,9 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
     const int node = uniform(items[it].node);
     const int2 r = tv.range[node];
     const bool act = (mask >> lane) & 1ull;
-    const bool selfp = r.x < t1 && r.y > t0;
+    // particle targets: the self pair lies in leaves holding some of the
+    // batch's targets; point targets: pairs closer than eps are skipped instead
+    const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
     for (int base = r.x; base < r.y; base += 64) {
       const int cnt = min(64, r.y - base);
       if (lane < cnt) {
@@ -1150,7 +1186,7 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
       }
       __syncthreads();
       if (act) {
-        const int me = selfp ? t - base : -1;
+        const int me = tv.self && selfp ? t - base : -1;
         for (int j = 0; j < cnt; j++) {
           if (j == me) continue;
           double4 s = tile[j];
@@ -1166,7 +1202,7 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
     }
   }
   if (valid) {
-    size_t o = 3 * (size_t)tv.perm[t];
+    size_t o = 3 * (size_t)tv.tperm[t];
     out[o + 0] = e_far[3 * (size_t)t + 0] + ex;
     out[o + 1] = e_far[3 * (size_t)t + 1] + ey;
     out[o + 2] = e_far[3 * (size_t)t + 2] + ez;
@@ -1176,7 +1212,7 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
 hipError_t near_field_grad(const TreeView& tv, double delta2, const int* near_off,
                            const Item* near_items, const double* e_far, double* out,
                            hipStream_t s) {
-  int nb = (tv.n + kBatch - 1) / kBatch;
+  int nb = (tv.nt + kBatch - 1) / kBatch;
   hipLaunchKernelGGL(k_near_grad, dim3(nb), dim3(kBatch), 0, s, tv, delta2, near_off,
                      near_items, e_far, out);
   return hipGetLastError();

@@ -173,6 +173,19 @@ int bit_eval_field(bit_system *sys, const bit_params *params, double *out,
                    int out_on_device);
 int bit_system_stats(const bit_system *sys, bit_stats *out);
 
+/* Potential at m arbitrary points (host arrays, any order; out[m] in that
+ * order): every source contributes, pairs with R^2 <= 1e-14 are skipped.
+ * Replaces the reference's point evaluations treepotential(0, -1, x, y, z, ...)
+ * (tree3d.cpp:4744-4757) and shielded_treepot(0, -1, ...) (3538-3551).
+ * m = 0 returns BIT_OK and touches nothing. */
+int bit_eval_potential_points(bit_system *sys, const bit_params *params, int m, const double *x,
+                              const double *y, const double *z, double *out);
+/* Field at m arbitrary points (kappa 0), out[3m]; treeforce(0, -1, ...)
+ * (tree3d.cpp:937-950): every source contributes (the regularized kernel's
+ * gradient vanishes at a coincident source).  m = 0 returns BIT_OK. */
+int bit_eval_field_points(bit_system *sys, const bit_params *params, int m, const double *x,
+                          const double *y, const double *z, double *out);
+
 /* The batch interaction lists the evaluation uses for params' MAC (acc,
  * delta2, mac_sub), i.e. the reference traversal's per-target decisions
  * (tree3d.cpp:3473 accept -> far, childless -> near) grouped by batches of 64

@@ -342,12 +342,12 @@ typedef struct {
 /* shielded_treepot / treepotential traversal, tree3d.cpp:3448-3627 and
  * 4679-4778: accept -> far field; reject -> recurse over children[0..7];
  * childless -> direct sum over members excluding the target itself. */
-static void pot_rec(const ora_tree *T, int ind, int p, const double *x, const double *y,
-                    const double *z, const double *q, const double *mom, ora_params prm,
-                    double *coef, pot_acc *acc) {
+static void pot_rec(const ora_tree *T, int ind, int p, double px, double py, double pz,
+                    const double *x, const double *y, const double *z, const double *q,
+                    const double *mom, ora_params prm, double *coef, pot_acc *acc) {
   const ora_node *d = &T->nodes[ind];
   double dx, dy, dz, rs;
-  if (mac_accept(d, x[p], y[p], z[p], prm, &dx, &dy, &dz, &rs)) {
+  if (mac_accept(d, px, py, pz, prm, &dx, &dy, &dz, &rs)) {
     const int nt = prm.nterms, S = nt + 1;
     ora_taylor_coeffs(nt, dx, dy, dz, rs, prm.kappa, coef);
     const double *M = mom + (int64_t)nt * nt * nt * ind;
@@ -364,15 +364,19 @@ static void pot_rec(const ora_tree *T, int ind, int p, const double *x, const do
   for (int c = 0; c < 8; c++) {
     if (d->children[c] != -1) {
       go = 0;
-      pot_rec(T, d->children[c], p, x, y, z, q, mom, prm, coef, acc);
+      pot_rec(T, d->children[c], p, px, py, pz, x, y, z, q, mom, prm, coef, acc);
     }
   }
   if (go) {
     for (int m = 0; m < d->nummemb; m++) {
       int j = d->members[m];
       if (j == p) continue;
-      double ex = x[p] - x[j], ey = y[p] - y[j], ez = z[p] - z[j];
-      double R = sqrt(ex * ex + ey * ey + ez * ez + prm.delta2);
+      double ex = px - x[j], ey = py - y[j], ez = pz - z[j];
+      double R2 = ex * ex + ey * ey + ez * ez + prm.delta2;
+      /* point targets (p == -1): coincident sources are skipped as in
+       * tree3d.cpp:3547 / 4753 (rs > eps) */
+      if (p < 0 && !(R2 > 1e-14)) continue;
+      double R = sqrt(R2);
       acc->pot += q[j] * exp(-prm.kappa * R) / R;
       acc->nnear++;
     }
@@ -399,10 +403,29 @@ void ora_potential(const ora_tree *T, const double *x, const double *y, const do
 #pragma omp for schedule(dynamic, 16)
     for (int t = 0; t < nt; t++) {
       pot_acc acc = {0.0, 0, 0};
-      pot_rec(T, 0, tidx[t], x, y, z, q, moments, prm, coef, &acc);
+      int p = tidx[t];
+      pot_rec(T, 0, p, x[p], y[p], z[p], x, y, z, q, moments, prm, coef, &acc);
       pot[t] = acc.pot;
       if (n_near) n_near[t] = acc.nnear;
       if (n_far) n_far[t] = acc.nfar;
+    }
+    free(coef);
+  }
+}
+
+void ora_potential_points(const ora_tree *T, const double *x, const double *y,
+                          const double *z, const double *q, const double *moments,
+                          ora_params prm, int m, const double *px, const double *py,
+                          const double *pz, double *pot, int nthreads) {
+  const int S = prm.nterms + 1;
+#pragma omp parallel num_threads(thread_count(nthreads))
+  {
+    double *coef = (double *)malloc(sizeof(double) * (size_t)S * S * S);
+#pragma omp for schedule(dynamic, 16)
+    for (int t = 0; t < m; t++) {
+      pot_acc acc = {0.0, 0, 0};
+      pot_rec(T, 0, -1, px[t], py[t], pz[t], x, y, z, q, moments, prm, coef, &acc);
+      pot[t] = acc.pot;
     }
     free(coef);
   }
@@ -461,12 +484,12 @@ void ora_direct(int n, const double *x, const double *y, const double *z, const 
  * in the source offset, the field is E_x = sum_k (k_x+1) M_k a_{k+e_x}, which is
  * the reference's signed sum over derivs[i+1][k][m]*(i+1).  The near field is
  * eval_grad_kernel_reg (tree3d.cpp:73-97): q (x_t - x_s)/R^3. */
-static void force_rec(const ora_tree *T, int ind, int p, const double *x, const double *y,
-                      const double *z, const double *q, const double *mom, ora_params prm,
-                      double *coef, double *E) {
+static void force_rec(const ora_tree *T, int ind, int p, double px, double py, double pz,
+                      const double *x, const double *y, const double *z, const double *q,
+                      const double *mom, ora_params prm, double *coef, double *E) {
   const ora_node *d = &T->nodes[ind];
   double dx, dy, dz, rs;
-  if (mac_accept(d, x[p], y[p], z[p], prm, &dx, &dy, &dz, &rs)) {
+  if (mac_accept(d, px, py, pz, prm, &dx, &dy, &dz, &rs)) {
     const int nt = prm.nterms, S = nt + 2;
     ora_taylor_coeffs(nt + 1, dx, dy, dz, rs, 0.0, coef);
     const double *M = mom + (int64_t)nt * nt * nt * ind;
@@ -484,13 +507,13 @@ static void force_rec(const ora_tree *T, int ind, int p, const double *x, const 
   for (int c = 0; c < 8; c++)
     if (d->children[c] != -1) {
       go = 0;
-      force_rec(T, d->children[c], p, x, y, z, q, mom, prm, coef, E);
+      force_rec(T, d->children[c], p, px, py, pz, x, y, z, q, mom, prm, coef, E);
     }
   if (go) {
     for (int m = 0; m < d->nummemb; m++) {
       int j = d->members[m];
       if (j == p) continue;
-      double ex = x[p] - x[j], ey = y[p] - y[j], ez = z[p] - z[j];
+      double ex = px - x[j], ey = py - y[j], ez = pz - z[j];
       double r2 = ex * ex + ey * ey + ez * ez + prm.delta2;
       double w = q[j] / (r2 * sqrt(r2));
       E[0] += ex * w;
@@ -510,7 +533,28 @@ void ora_treeforce(const ora_tree *T, const double *x, const double *y, const do
 #pragma omp for schedule(dynamic, 16)
     for (int t = 0; t < nt; t++) {
       double E[3] = {0, 0, 0};
-      force_rec(T, 0, tidx[t], x, y, z, q, moments, prm, coef, E);
+      int p = tidx[t];
+      force_rec(T, 0, p, x[p], y[p], z[p], x, y, z, q, moments, prm, coef, E);
+      ex[t] = E[0];
+      ey[t] = E[1];
+      ez[t] = E[2];
+    }
+    free(coef);
+  }
+}
+
+void ora_treeforce_points(const ora_tree *T, const double *x, const double *y,
+                          const double *z, const double *q, const double *moments,
+                          ora_params prm, int m, const double *px, const double *py,
+                          const double *pz, double *ex, double *ey, double *ez, int nthreads) {
+  const int S = prm.nterms + 2;
+#pragma omp parallel num_threads(thread_count(nthreads))
+  {
+    double *coef = (double *)malloc(sizeof(double) * (size_t)S * S * S);
+#pragma omp for schedule(dynamic, 16)
+    for (int t = 0; t < m; t++) {
+      double E[3] = {0, 0, 0};
+      force_rec(T, 0, -1, px[t], py[t], pz[t], x, y, z, q, moments, prm, coef, E);
       ex[t] = E[0];
       ey[t] = E[1];
       ez[t] = E[2];

@@ -95,6 +95,19 @@ void ora_potential(const ora_tree *t, const double *x, const double *y, const do
                    const int *tidx, double *pot, int64_t *n_near, int64_t *n_far,
                    int nthreads);
 
+/* Potential at arbitrary points (p = -1 of the reference traversals,
+ * tree3d.cpp:3538-3551, 4744-4757): all sources, pairs with R^2 <= 1e-14
+ * skipped. */
+void ora_potential_points(const ora_tree *t, const double *x, const double *y,
+                          const double *z, const double *q, const double *moments,
+                          ora_params prm, int m, const double *px, const double *py,
+                          const double *pz, double *pot, int nthreads);
+/* Field at arbitrary points (treeforce p = -1, tree3d.cpp:937-950). */
+void ora_treeforce_points(const ora_tree *t, const double *x, const double *y,
+                          const double *z, const double *q, const double *moments,
+                          ora_params prm, int m, const double *px, const double *py,
+                          const double *pz, double *ex, double *ey, double *ez, int nthreads);
+
 /* Per-target interaction lists in reference DFS order: far clusters and near
  * leaves of target tidx; returns counts, writes up to cap entries each. */
 void ora_lists(const ora_tree *t, const double *x, const double *y, const double *z,

@@ -78,6 +78,10 @@ class Oracle:
                                  C.c_int, _ip, _dp, C.c_int]
         L.ora_treeforce.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, OraParams, C.c_int,
                                     _ip, _dp, _dp, _dp, C.c_int]
+        L.ora_potential_points.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, OraParams,
+                                           C.c_int, _dp, _dp, _dp, _dp, C.c_int]
+        L.ora_treeforce_points.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, OraParams,
+                                           C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.ora_direct_force.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _ip,
                                        _dp, _dp, _dp, C.c_int]
         self.L = L
@@ -154,6 +158,25 @@ class OracleTree:
         self.ora.L.ora_potential(self.h, self.x, self.y, self.z, q, _f64(moments).ravel(),
                                  prm, nt, tidx, pot, nn, nf, nthreads)
         return (pot, nn, nf) if counts else pot
+
+    def potential_points(self, q, moments, nterms, acc, kappa, delta2, mac_sub, px, py, pz,
+                         nthreads=0):
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        out = np.zeros(len(px))
+        prm = OraParams(nterms, acc, kappa, delta2, mac_sub)
+        self.ora.L.ora_potential_points(self.h, self.x, self.y, self.z, _f64(q),
+                                        _f64(moments).ravel(), prm, len(px), px, py, pz, out,
+                                        nthreads)
+        return out
+
+    def treeforce_points(self, q, moments, nterms, acc, delta2, mac_sub, px, py, pz, nthreads=0):
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        e = [np.zeros(len(px)) for _ in range(3)]
+        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub)
+        self.ora.L.ora_treeforce_points(self.h, self.x, self.y, self.z, _f64(q),
+                                        _f64(moments).ravel(), prm, len(px), px, py, pz, *e,
+                                        nthreads)
+        return np.stack(e, axis=1)
 
     def lists(self, ti, nterms, acc, delta2, mac_sub, cap=4096):
         far = np.zeros(cap, dtype=np.int32)

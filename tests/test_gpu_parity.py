@@ -272,3 +272,80 @@ def test_device_inputs_rejected_when_nonfinite(ctx):
     assert e.value.code == bit_amd.BIT_E_INVALID
     for d in dev:
         d.free()
+
+
+def test_points_at_particles_match_reference_p_minus_1(ctx):
+    """treepotential(0, -1, x_i, ...) at the particle positions includes the
+    target's own regularized term q_i/sqrt(DEL); the fixture stores it removed."""
+    fx = load_fixture("uniform12k_reg")
+    s = _system(ctx, fx)
+    t = fx["tidx"]
+    got = s.potential_at(fx["x"][t], fx["y"][t], fx["z"][t], int(fx["nterms"]),
+                         float(fx["acc"]), 0.0, 0.01, 0.01, "fp64")
+    ref = fx["phi_regcoulomb"] + fx["q"][t] / np.sqrt(0.01)
+    assert rel_max(got, ref) < 1e-10
+
+
+def test_field_at_particles_matches_treeforce(ctx):
+    """treeforce(0, -1) at a particle: its own term has zero gradient, so the
+    point form equals the per-particle form."""
+    fx = load_fixture("box16k")
+    s = _system(ctx, fx)
+    t = fx["tidx"]
+    E = s.field_at(fx["x"][t], fx["y"][t], fx["z"][t], 5, 0.5, 0.01, 0.01)
+    assert rel_max(E, fx["field_tree"]) < 1e-10
+
+
+def _points(P, m, seed):
+    """Points inside and around the particles' bounding box, plus a few
+    exactly on particles (coincident pairs) and far outside."""
+    rng = np.random.default_rng(seed)
+    lo = np.array([P.x.min(), P.y.min(), P.z.min()])
+    hi = np.array([P.x.max(), P.y.max(), P.z.max()])
+    pts = lo + (hi - lo) * rng.uniform(-0.2, 1.2, size=(m, 3))
+    k = rng.choice(P.n, 37, replace=False)
+    pts[:37] = np.stack([P.x[k], P.y[k], P.z[k]], axis=1)
+    pts[37:40] = hi + 5.0 * (hi - lo)
+    return pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy()
+
+
+@pytest.mark.parametrize("kappa,d2,sub,nterms,acc", [(0.5, 0.0, 0.01, 9, 0.5),
+                                                     (0.5, 2.5e-5, 2.5e-5, 10, 0.3),
+                                                     (0.0, 0.01, 0.01, 7, 0.6)])
+def test_points_match_oracle(ctx, oracle, kappa, d2, sub, nterms, acc):
+    P = workloads.gaussian_blobs(30000, seed=41)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 300)
+    px, py, pz = _points(P, 3001, seed=5)
+    o = oracle.tree(P.x, P.y, P.z, 300)
+    m = o.moments(P.q, nterms)
+    ref = o.potential_points(P.q, m, nterms, acc, kappa, d2, sub, px, py, pz)
+    got = s.potential_at(px, py, pz, nterms, acc, kappa, d2, sub, "fp64")
+    assert rel_max(got, ref) < 1e-10
+    got32 = s.potential_at(px, py, pz, nterms, acc, kappa, d2, sub, "fp32")
+    assert rel_l2(got32, ref) < FP32_TOL
+    # the particle-target evaluation is untouched by the point evaluation
+    p64 = s.potential(nterms, acc, kappa, d2, sub, "fp64")
+    tidx = workloads.sample_targets(P.n, 500, seed=6)
+    assert rel_max(p64[tidx], o.potential(P.q, m, nterms, acc, kappa, d2, sub, tidx)) < 1e-10
+
+
+def test_field_points_match_oracle(ctx, oracle):
+    P = workloads.gaussian_blobs(30000, seed=42)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 300)
+    px, py, pz = _points(P, 2000, seed=7)
+    o = oracle.tree(P.x, P.y, P.z, 300)
+    m = o.moments(P.q, 6)
+    ref = o.treeforce_points(P.q, m, 6, 0.5, 0.01, 0.01, px, py, pz)
+    got = s.field_at(px, py, pz, 6, 0.5, 0.01, 0.01)
+    assert rel_max(got, ref) < 1e-10
+
+
+def test_points_empty_and_invalid(ctx):
+    P = workloads.uniform_cube(2000, seed=3)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 100)
+    e = np.zeros(0)
+    assert s.potential_at(e, e, e, 5, 0.5).shape == (0,)
+    assert s.field_at(e, e, e, 5, 0.5).shape == (0, 3)
+    with pytest.raises(bit_amd.BitError):
+        s.potential_at(np.array([np.nan]), np.zeros(1), np.zeros(1), 5, 0.5)
+    assert np.isfinite(s.potential(5, 0.5)).all()

@@ -120,3 +120,44 @@ def test_field_is_minus_gradient_of_potential(oracle):
         errs.append(rel_l2(t.treeforce(P.q, m, nt, 0.25, d2, d2, tidx), Ed))
     assert errs[0] > errs[1] > errs[2]
     assert errs[-1] < 3e-5
+
+
+def test_point_targets(oracle):
+    """Point evaluation (p = -1): at a particle position it equals the
+    particle evaluation plus the target's own term q_i G(0) (kappa 0,
+    delta^2 > 0); at arbitrary points it converges to a numpy direct sum; a
+    point on a particle with delta 0 skips that particle."""
+    P = workloads.gaussian_blobs(3000, seed=9)
+    t = oracle.tree(P.x, P.y, P.z, 60)
+    d2 = 0.01
+    m = t.moments(P.q, 8)
+    tidx = np.arange(0, P.n, 7)
+    phi = t.potential(P.q, m, 8, 0.5, 0.0, d2, d2, tidx)
+    pts = t.potential_points(P.q, m, 8, 0.5, 0.0, d2, d2, P.x[tidx], P.y[tidx], P.z[tidx])
+    assert rel_max(pts, phi + P.q[tidx] / np.sqrt(d2)) < 1e-13
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-1.5, 1.5, size=(200, 3))
+    X[:5] = np.stack([P.x[:5], P.y[:5], P.z[:5]], axis=1)
+    for kappa, dd in ((0.7, 0.0), (0.0, 1e-4)):
+        r = np.sqrt(((X[:, None, :] - np.stack([P.x, P.y, P.z], 1)[None]) ** 2).sum(-1) ** 1 + dd)
+        G = np.where(r * r > 1e-14, np.exp(-kappa * r) / np.where(r > 0, r, 1.0), 0.0)
+        ref = G @ P.q
+        mm = t.moments(P.q, 12)
+        got = t.potential_points(P.q, mm, 12, 0.3, kappa, dd, max(dd, 0.01), *X.T.copy())
+        assert rel_l2(got, ref) < 1e-6
+    Ed = np.zeros((200, 3))
+    S = np.stack([P.x, P.y, P.z], 1)
+    for i in range(200):
+        d = X[i] - S
+        r2 = (d * d).sum(1) + 0.01
+        Ed[i] = (P.q / r2 ** 1.5) @ d
+    errs = []
+    for nt in (4, 8, 12):
+        mm = t.moments(P.q, nt)
+        errs.append(rel_l2(t.treeforce_points(P.q, mm, nt, 0.25, 0.01, 0.01, *X.T.copy()), Ed))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 3e-5
+    # at a particle position the point form adds the particle's own term,
+    # whose regularized gradient is exactly zero
+    mm = t.moments(P.q, 6)
+    Ep = t.treeforce_points(P.q, mm, 6, 0.5, 0.01, 0.01, P.x[tidx], P.y[tidx], P.z[tidx])
+    assert rel_max(Ep, t.treeforce(P.q, mm, 6, 0.5, 0.01, 0.01, tidx)) < 1e-13
