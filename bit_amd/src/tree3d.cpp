@@ -128,7 +128,7 @@ const double* lookup(const char* fn, TREE* tree, PARTICLE* part, int ind, int p,
   Bound& b = it->second;
   if (ind != 0 || p < 0 || p >= b.npart || part[p].x != x || part[p].y != y || part[p].z != z) {
     cout << "[" << fn << "] served for whole-tree calls (ind == 0) at particle targets "
-         << "(0 <= p < npart, x,y,z = particle p) only" << endl;
+         << "(0 <= p < npart, x,y,z = particle p) or at points (p == -1) only" << endl;
     return nullptr;
   }
   auto key = std::make_tuple(what, nterms, acc, alpha);
@@ -144,6 +144,39 @@ const double* lookup(const char* fn, TREE* tree, PARTICLE* part, int ind, int p,
     c = b.cache.emplace(key, std::move(v)).first;
   }
   return c->second.data();
+}
+
+// One arbitrary point (the reference's p = -1 calls): every source
+// contributes, pairs closer than eps skipped.  Returns false (message printed)
+// when the tree is not bound or the evaluation fails.
+bool point_eval(const char* fn, TREE* tree, PARTICLE* part, int what, int nterms, double acc,
+                double alpha, double x, double y, double z, double* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_bound.find(tree);
+  if (it == g_bound.end() || it->second.part != part) {
+    cout << "[" << fn << "] tree was not built by treemake for this particle array" << endl;
+    return false;
+  }
+  Bound& b = it->second;
+  if (!b.sys) b.sys = make_system(part, b.npart, b.maxmemb);
+  if (!b.sys) return false;
+  int rc;
+  if (what == kField) {
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL);
+    rc = bit_eval_field_points(b.sys, &p, 1, &x, &y, &z, out);
+  } else if (what == kPot) {
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL);
+    rc = bit_eval_potential_points(b.sys, &p, 1, &x, &y, &z, out);
+  } else {
+    bit_params p = params(nterms, acc, alpha, 0.0, DEL);
+    rc = bit_eval_potential_points(b.sys, &p, 1, &x, &y, &z, out);
+    out[0] = -out[0];  // shielded_treepot's sign, near and far alike
+  }
+  if (rc != BIT_OK) {
+    cout << "[" << fn << "] " << bit_last_error() << endl;
+    return false;
+  }
+  return true;
 }
 
 void fill_tree_node(TREE& t, const double* f) {
@@ -332,6 +365,15 @@ void treemoments(int numtree, int nterms, TREE* tree, PARTICLE* part) {
 // ---------------------------------------------------------------------------
 double treeforce(int ind, int p, int nterms, double x, double y, double z, double acc,
                  double* xfor, double* yfor, double* zfor, TREE* tree, PARTICLE* part) {
+  if (ind == 0 && p == -1) {
+    double E[3];
+    if (point_eval("treeforce", tree, part, kField, nterms, acc, 0.0, x, y, z, E)) {
+      *xfor += E[0];
+      *yfor += E[1];
+      *zfor += E[2];
+    }
+    return 0;
+  }
   const double* E = lookup("treeforce", tree, part, ind, p, x, y, z, kField, nterms, acc, 0.0);
   if (E) {
     *xfor += E[3 * (size_t)p];
@@ -343,6 +385,11 @@ double treeforce(int ind, int p, int nterms, double x, double y, double z, doubl
 
 double treepotential(int ind, int p, int nterms, double x, double y, double z, double acc,
                      double* pot, TREE* tree, PARTICLE* part) {
+  if (ind == 0 && p == -1) {
+    double v;
+    if (point_eval("treepotential", tree, part, kPot, nterms, acc, 0.0, x, y, z, &v)) *pot += v;
+    return 0;
+  }
   const double* v = lookup("treepotential", tree, part, ind, p, x, y, z, kPot, nterms, acc, 0.0);
   if (v) *pot += v[p];
   return 0;
@@ -351,6 +398,12 @@ double treepotential(int ind, int p, int nterms, double x, double y, double z, d
 double shielded_treepot(int ind, int p, int nterms, double x, double y, double z, double alpha,
                         double acc, double* pot, TREE* tree, PARTICLE* part, int, int, int,
                         double, double, double) {
+  if (ind == 0 && p == -1) {
+    double v;
+    if (point_eval("shielded_treepot", tree, part, kShielded, nterms, acc, alpha, x, y, z, &v))
+      *pot += v;
+    return 0;
+  }
   const double* v =
       lookup("shielded_treepot", tree, part, ind, p, x, y, z, kShielded, nterms, acc, alpha);
   if (v) *pot += v[p];

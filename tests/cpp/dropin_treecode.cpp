@@ -7,6 +7,8 @@
 //   dropin_treecode field  OUT        boxparticles(16000), maxmemb 100, nterms 5
 //   dropin_treecode conv   OUT        nterms 1..4, fields of each
 //   dropin_treecode screened IN OUT   particles from IN (n, x, y, z, q), params from IN
+//   dropin_treecode points IN OUT     p = -1 calls (treepotential, treeforce,
+//                                     shielded_treepot) at the points given in IN
 // OUT receives raw doubles; "PASSED"/"FAILED" on stdout as the reference tests.
 #include <omp.h>
 
@@ -118,10 +120,53 @@ static int screened_mode(const char* in, const char* out) {
   return ok ? 0 : 1;
 }
 
+static int points_mode(const char* in, const char* out) {
+  FILE* f = fopen(in, "rb");
+  double hdr[6];
+  if (fread(hdr, sizeof(double), 6, f) != 6) return 2;
+  int n = (int)hdr[0], maxmemb = (int)hdr[1], nterms = (int)hdr[2], m = (int)hdr[5];
+  double acc = hdr[3], alpha = hdr[4];
+  std::vector<double> buf(4 * (size_t)n + 3 * (size_t)m);
+  if (fread(buf.data(), sizeof(double), buf.size(), f) != buf.size()) return 2;
+  fclose(f);
+  PARTICLE* part = new PARTICLE[n];
+  for (int i = 0; i < n; i++) {
+    part[i] = PARTICLE();
+    part[i].x = buf[i];
+    part[i].y = buf[n + i];
+    part[i].z = buf[2 * n + i];
+    part[i].charge = buf[3 * n + i];
+    part[i].weight = 1.0;
+    part[i].tot_charge = buf[3 * n + i];
+  }
+  const double* px = &buf[4 * (size_t)n];
+  const double* py = px + m;
+  const double* pz = py + m;
+  TREE* tree = new TREE[4 * n + 64];
+  int numtree = treemake(n, nterms, maxmemb, part, tree);
+  treemoments(numtree, nterms, tree, part);
+  std::vector<double> res(5 * (size_t)m, 0.0);  // pot, Ex, Ey, Ez, shielded
+#pragma omp parallel for
+  for (int i = 0; i < m; i++) {
+    treepotential(0, -1, nterms, px[i], py[i], pz[i], acc, &res[i], tree, part);
+    treeforce(0, -1, nterms, px[i], py[i], pz[i], acc, &res[m + 3 * (size_t)i],
+              &res[m + 3 * (size_t)i + 1], &res[m + 3 * (size_t)i + 2], tree, part);
+    shielded_treepot(0, -1, nterms, px[i], py[i], pz[i], alpha, acc, &res[4 * (size_t)m + i],
+                     tree, part, 100, 100, 100, 1.0, 1.0, 1.0);
+  }
+  treeempty(numtree, tree);
+  write_all(out, res);
+  delete[] tree;
+  delete[] part;
+  printf("Test PASSED\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc >= 3 && std::string(argv[1]) == "field") return field_mode(argv[2]);
   if (argc >= 3 && std::string(argv[1]) == "conv") return conv_mode(argv[2]);
   if (argc >= 4 && std::string(argv[1]) == "screened") return screened_mode(argv[2], argv[3]);
-  fprintf(stderr, "usage: dropin_treecode field|conv OUT | screened IN OUT\n");
+  if (argc >= 4 && std::string(argv[1]) == "points") return points_mode(argv[2], argv[3]);
+  fprintf(stderr, "usage: dropin_treecode field|conv OUT | screened|points IN OUT\n");
   return 2;
 }

@@ -55,3 +55,33 @@ def test_shielded_treepot_and_batch_driver(tmp_path):
     raw = np.fromfile(out)
     want = -fx["phi_screened"] + fx["q"][fx["tidx"]] * INTG  # the reference's raw value
     assert rel_max(raw[fx["tidx"]], want) < 1e-10
+
+
+def test_point_calls_p_minus_1(tmp_path, oracle):
+    """treepotential / treeforce / shielded_treepot with p = -1 from OpenMP
+    threads: at particle positions treepotential equals the reference's own
+    values (which include the target's q_i/sqrt(DEL)); at arbitrary points all
+    three equal the oracle's point traversal (shielded: -phi, unregularized)."""
+    fx = load_fixture("uniform12k_reg")
+    n, t = fx["x"].size, fx["tidx"][:600]
+    rng = np.random.default_rng(8)
+    extra = rng.uniform(-0.7, 0.7, size=(200, 3))
+    px = np.concatenate([fx["x"][t], extra[:, 0]])
+    py = np.concatenate([fx["y"][t], extra[:, 1]])
+    pz = np.concatenate([fx["z"][t], extra[:, 2]])
+    m = px.size
+    inp = tmp_path / "in.bin"
+    hdr = np.array([n, 1000, 9, 0.5, 0.8, m], dtype=np.float64)
+    np.concatenate([hdr, fx["x"], fx["y"], fx["z"], fx["q"], px, py, pz]).tofile(inp)
+    out = tmp_path / "p.bin"
+    _run("points", str(inp), str(out))
+    r = np.fromfile(out)
+    pot, E, sh = r[:m], r[m:4 * m].reshape(m, 3), r[4 * m:]
+    ref = fx["phi_regcoulomb"][:600] + fx["q"][t] / np.sqrt(0.01)
+    assert rel_max(pot[:600], ref) < 1e-10
+    o = oracle.tree(fx["x"], fx["y"], fx["z"], 1000)
+    mom = o.moments(fx["q"], 9)
+    assert rel_max(pot, o.potential_points(fx["q"], mom, 9, 0.5, 0.0, 0.01, 0.01, px, py, pz)) < 1e-10
+    assert rel_max(E, o.treeforce_points(fx["q"], mom, 9, 0.5, 0.01, 0.01, px, py, pz)) < 1e-10
+    want = -o.potential_points(fx["q"], mom, 9, 0.5, 0.8, 0.0, 0.01, px, py, pz)
+    assert rel_max(sh, want) < 1e-10
