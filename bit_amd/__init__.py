@@ -56,7 +56,8 @@ class BitError(RuntimeError):
 
 class Params(C.Structure):
     _fields_ = [("nterms", C.c_int), ("acc", C.c_double), ("kappa", C.c_double),
-                ("delta2", C.c_double), ("mac_sub", C.c_double), ("precision", C.c_int)]
+                ("delta2", C.c_double), ("mac_sub", C.c_double), ("precision", C.c_int),
+                ("kernel_order", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -179,7 +180,8 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def make_params(nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None, precision="fp64"):
+def make_params(nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None, precision="fp64",
+                kernel_order=0):
     """Evaluation parameters; mac_sub defaults to delta2 (the MAC on the true
     squared distance), the reference's treepotential/treeforce convention."""
     p = Params()
@@ -189,6 +191,7 @@ def make_params(nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None, precisio
     p.delta2 = float(delta2)
     p.mac_sub = float(delta2 if mac_sub is None else mac_sub)
     p.precision = BIT_FP32 if precision in ("fp32", BIT_FP32) else BIT_FP64
+    p.kernel_order = int(kernel_order)
     return p
 
 
@@ -440,9 +443,10 @@ class System:
         _check(lib.bit_eval_potential(self.h, C.byref(prm), _ptr(res), 0))
         return res
 
-    def field(self, nterms=6, acc=ACC, delta2=DEL, mac_sub=None):
-        """E = sum q (x_i - x_j)/R^3 (regularized Coulomb), [n, 3], original order."""
-        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64")
+    def field(self, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0):
+        """E = sum q (x_i - x_j)/R^3 (regularized Coulomb), [n, 3], original order;
+        kernel_order >= 1 adds the regularized family's higher terms."""
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64", kernel_order)
         res = np.zeros(3 * self.n)
         _check(lib.bit_eval_field(self.h, C.byref(prm), _ptr(res), 0))
         return res.reshape(self.n, 3)
@@ -457,9 +461,9 @@ class System:
                                              _ptr(pz), _ptr(out)))
         return out
 
-    def field_at(self, px, py, pz, nterms=6, acc=ACC, delta2=DEL, mac_sub=None):
+    def field_at(self, px, py, pz, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0):
         """Regularized Coulomb field at arbitrary points, [m, 3]."""
-        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64")
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64", kernel_order)
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         out = np.zeros(3 * len(px))
         _check(lib.bit_eval_field_points(self.h, C.byref(prm), len(px), _ptr(px), _ptr(py),

@@ -392,8 +392,15 @@ int finish_system(bit_ctx* ctx, bit_system* s) {
   return BIT_OK;
 }
 
-int check_params(const bit_system* s, const bit_params* p) {
+// field: the regularized family (kernel_order >= 1) applies to the field only,
+// as KERNEL_ORDER does in the reference (tree3d.cpp:88, 614).
+int check_params(const bit_system* s, const bit_params* p, bool field = false) {
   if (!s || !p) return fail(BIT_E_INVALID, "null system or params");
+  if (p->kernel_order < 0 || p->kernel_order > BIT_MAX_KERNEL_ORDER)
+    return fail(BIT_E_INVALID,
+                "kernel_order must be in [0, " + std::to_string(BIT_MAX_KERNEL_ORDER) + "]");
+  if (!field && p->kernel_order != 0)
+    return fail(BIT_E_INVALID, "kernel_order >= 1 (regularized family) applies to the field only");
   if (p->nterms < 1 || p->nterms > BIT_MAX_NTERMS)
     return fail(BIT_E_INVALID, "nterms must be in [1, " + std::to_string(BIT_MAX_NTERMS) + "]");
   if (!std::isfinite(p->acc) || !std::isfinite(p->kappa) || !std::isfinite(p->delta2) ||
@@ -485,6 +492,7 @@ void bit_params_default(bit_params* p) {
   p->delta2 = 0.0;
   p->mac_sub = 0.0;
   p->precision = BIT_FP64;
+  p->kernel_order = 0;
 }
 
 const char* bit_last_error(void) { return g_err.c_str(); }
@@ -937,7 +945,7 @@ int bit_eval_potential_multi(bit_ctx* ctx, int count, bit_system* const* systems
 }
 
 int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_device) {
-  BIT_TRY(check_params(s, p));
+  BIT_TRY(check_params(s, p, true));
   if (!out) return fail(BIT_E_INVALID, "null output");
   if (p->kappa != 0.0) return fail(BIT_E_INVALID, "field evaluation supports kappa = 0 only");
   if (p->nterms > BIT_MAX_NTERMS - 1)
@@ -954,12 +962,13 @@ int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_d
     dst = s->out.as<double>();
   }
   s->t_far.start(c->stream);
-  HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, s->mom_hat64.as<double>(),
+  HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, p->kernel_order,
+                                   s->mom_hat64.as<double>(),
                                    s->pl.far_off.as<int>(), s->pl.far_items.as<bit::dev::Item>(),
                                    s->phi_far.as<double>(), c->stream));
   s->t_far.stop(c->stream);
   s->t_near.start(c->stream);
-  HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, s->pl.near_off.as<int>(),
+  HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, p->kernel_order, s->pl.near_off.as<int>(),
                                     s->pl.near_items.as<bit::dev::Item>(), s->phi_far.as<double>(),
                                     dst, c->stream));
   s->t_near.stop(c->stream);
@@ -1050,7 +1059,7 @@ int bit_eval_potential_points(bit_system* s, const bit_params* p, int m, const d
 
 int bit_eval_field_points(bit_system* s, const bit_params* p, int m, const double* x,
                           const double* y, const double* z, double* out) {
-  BIT_TRY(check_params(s, p));
+  BIT_TRY(check_params(s, p, true));
   if (m < 0) return fail(BIT_E_INVALID, "negative point count");
   if (m == 0) return BIT_OK;  // no points, nothing to evaluate
   if (!out) return fail(BIT_E_INVALID, "null output");
@@ -1065,10 +1074,10 @@ int bit_eval_field_points(bit_system* s, const bit_params* p, int m, const doubl
   BIT_TRY(build_lists(s, s->ql, tv, p));
   BIT_TRY(s->qphi.ensure(c, (size_t)3 * m * sizeof(double)));
   BIT_TRY(s->qout.ensure(c, (size_t)3 * m * sizeof(double)));
-  HIP_TRY(bit::dev::far_field_grad(tv, p->nterms, p->delta2, s->mom_hat64.as<double>(),
+  HIP_TRY(bit::dev::far_field_grad(tv, p->nterms, p->delta2, p->kernel_order, s->mom_hat64.as<double>(),
                                    s->ql.far_off.as<int>(), s->ql.far_items.as<bit::dev::Item>(),
                                    s->qphi.as<double>(), c->stream));
-  HIP_TRY(bit::dev::near_field_grad(tv, p->delta2, s->ql.near_off.as<int>(),
+  HIP_TRY(bit::dev::near_field_grad(tv, p->delta2, p->kernel_order, s->ql.near_off.as<int>(),
                                     s->ql.near_items.as<bit::dev::Item>(), s->qphi.as<double>(),
                                     s->qout.as<double>(), c->stream));
   HIP_TRY(hipMemcpyAsync(out, s->qout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost,
@@ -1086,7 +1095,7 @@ int bit_system_stats(const bit_system* s, bit_stats* out) {
 int bit_system_export_lists(bit_system* s, const bit_params* p, int* nbatch, int* nfar,
                             int* nnear, int* far_off, uint64_t* far_mask, int* far_node,
                             int* near_off, uint64_t* near_mask, int* near_node) {
-  BIT_TRY(check_params(s, p));
+  BIT_TRY(check_params(s, p, true));
   if (!nbatch || !nfar || !nnear) return fail(BIT_E_INVALID, "null size outputs");
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));

@@ -144,11 +144,12 @@ hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nt
 hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool fp32,
                       bool screened, hipStream_t s);
 
-// field (Coulomb, regularized): E_far sorted [3n], then near + scatter [3n]
-hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, const double* mom_hat64,
-                          const int* far_off, const Item* far_items, double* e_far,
-                          hipStream_t s);
-hipError_t near_field_grad(const TreeView& tv, double delta2, const int* near_off,
+// field (Coulomb, regularized; korder >= 1 adds the terms l = 1..korder of the
+// regularized family): E_far sorted [3n], then near + scatter [3n]
+hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, int korder,
+                          const double* mom_hat64, const int* far_off, const Item* far_items,
+                          double* e_far, hipStream_t s);
+hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, const int* near_off,
                            const Item* near_items, const double* e_far, double* out,
                            hipStream_t s);
 

@@ -188,6 +188,57 @@ __device__ __forceinline__ void far_eval_grad(T x0, T x1, T x2_, T fac, T a0, MP
   ez = acc[2];
 }
 
+// Term l >= 1 of the regularized field family, R^-(2l+1) (tree3d.cpp:614-791):
+// with S1 = sum_e 2 x_e a_{k-e}, S2 = sum_e a_{k-2e} and n = |k|,
+//   a_k = fac [ (1 + (l - 1/2)/n) S1 - (1 + (2l - 1)/n) S2 ],
+// which is |k| R^2 a_k = (2|k|+2l-1) sum x_e a_{k-e} - (|k|+2l-1) sum a_{k-2e}
+// in scaled variables.  lm = l - 1/2, l2 = 2l - 1; folded into the field
+// sums as far_eval_grad does.
+template <typename T, int NT, int C, typename MP>
+__device__ __forceinline__ void pow_step(T* a, const T (&x2)[3], T fac, T lm, T l2, MP M,
+                                         T (&acc)[3]) {
+  constexpr const Tab<NT>& tb = kTab<NT>;
+  constexpr int e0 = tb.m1[C][0], e1 = tb.m1[C][1], e2 = tb.m1[C][2];
+  constexpr int f0 = tb.m2[C][0], f1 = tb.m2[C][1], f2 = tb.m2[C][2];
+  constexpr int n = tb.k[C][0] + tb.k[C][1] + tb.k[C][2];
+  constexpr double invn = 1.0 / n;
+  T S1 = T(0), S2 = T(0);
+  if constexpr (e0 >= 0) S1 = fma(x2[0], a[e0], S1);
+  if constexpr (e1 >= 0) S1 = fma(x2[1], a[e1], S1);
+  if constexpr (e2 >= 0) S1 = fma(x2[2], a[e2], S1);
+  if constexpr (f0 >= 0) S2 += a[f0];
+  if constexpr (f1 >= 0) S2 += a[f1];
+  if constexpr (f2 >= 0) S2 += a[f2];
+  const T c1 = fma(lm, T(invn), T(1)), c2 = fma(l2, T(invn), T(1));
+  a[C] = fac * fma(c1, S1, -c2 * S2);
+  if constexpr (e0 >= 0) acc[0] = fma(M[e0], T(tb.k[C][0]) * a[C], acc[0]);
+  if constexpr (e1 >= 0) acc[1] = fma(M[e1], T(tb.k[C][1]) * a[C], acc[1]);
+  if constexpr (e2 >= 0) acc[2] = fma(M[e2], T(tb.k[C][2]) * a[C], acc[2]);
+}
+
+template <typename T, int NT, typename MP, int... C>
+__device__ __forceinline__ void pow_all(std::integer_sequence<int, C...>, T* a, const T (&x2)[3],
+                                        T fac, T lm, T l2, MP M, T (&acc)[3]) {
+  (pow_step<T, NT, C + 1, MP>(a, x2, fac, lm, l2, M, acc), ...);
+}
+
+// Field of a0 * R^-(2l+1)/R^-(2l+1)(0) in scaled variables, coefficients to
+// degree NT (a0 carries the term's weight and R^-(2l+1)).
+template <typename T, int NT, typename MP>
+__device__ __forceinline__ void far_eval_grad_pow(T x0, T x1, T x2_, T fac, T a0, int l, MP M,
+                                                  T& ex, T& ey, T& ez) {
+  constexpr int NC = ncoef(NT + 1);
+  T a[NC];
+  a[0] = a0;
+  const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
+  T acc[3] = {T(0), T(0), T(0)};
+  pow_all<T, NT + 1, MP>(std::make_integer_sequence<int, NC - 1>{}, a, xx, fac, T(l - 0.5),
+                         T(2 * l - 1), M, acc);
+  ex = acc[0];
+  ey = acc[1];
+  ez = acc[2];
+}
+
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -1083,8 +1134,10 @@ hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool 
 // ---------------------------------------------------------------------------
 // Field of the regularized Coulomb kernel (treeforce, tree3d.cpp:851-971), FP64.
 // ---------------------------------------------------------------------------
-template <int NT>
-__global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2,
+// FAM: regularized family of order korder >= 1 (terms l = 1..korder added to
+// the Coulomb term, each weighted w_l delta2^l, w_l = prod_{j<=l} (j-1/2)/j).
+template <int NT, bool FAM>
+__global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2, int korder,
                                                  const double* __restrict__ mom_hat,
                                                  const int* __restrict__ far_off,
                                                  const Item* __restrict__ items,
@@ -1112,6 +1165,21 @@ __global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2,
       far_eval_grad<double, NT, gptr<double>>(dx * inv, dy * inv, dz * inv, 1.0 / R2h, a0,
                                                as_global(mom_hat) + (size_t)node * NC, gx, gy,
                                                gz);
+      if constexpr (FAM) {
+        // a0 of term l: w_l (delta2/s^2)^l Rh^-(2l+1), built up from term l-1
+        const double fac = 1.0 / R2h, dh = delta2 * inv * inv;
+        double al = a0;
+        for (int l = 1; l <= korder; l++) {
+          al = al * ((l - 0.5) / l) * dh * fac;
+          double hx, hy, hz;
+          far_eval_grad_pow<double, NT, gptr<double>>(dx * inv, dy * inv, dz * inv, fac, al, l,
+                                                      as_global(mom_hat) + (size_t)node * NC,
+                                                      hx, hy, hz);
+          gx += hx;
+          gy += hy;
+          gz += hz;
+        }
+      }
       // d/dx of (1/s) f(x/s) -> 1/s^2
       ex += gx * inv * inv;
       ey += gy * inv * inv;
@@ -1126,36 +1194,42 @@ __global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2,
 }
 
 template <int NT>
-static void launch_far_grad(const TreeView& tv, double delta2, const double* mom,
+static void launch_far_grad(const TreeView& tv, double delta2, int korder, const double* mom,
                             const int* far_off, const Item* items, double* e_far, hipStream_t s) {
   int nb = (tv.nt + kBatch - 1) / kBatch;
-  hipLaunchKernelGGL((k_far_grad<NT>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, mom, far_off,
-                     items, e_far);
+  if (korder > 0)
+    hipLaunchKernelGGL((k_far_grad<NT, true>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, korder,
+                       mom, far_off, items, e_far);
+  else
+    hipLaunchKernelGGL((k_far_grad<NT, false>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, 0, mom,
+                       far_off, items, e_far);
 }
 
 template <int... NTs>
 static bool dispatch_far_grad(int nterms, std::integer_sequence<int, NTs...>,
-                              const TreeView& tv, double delta2, const double* mom,
+                              const TreeView& tv, double delta2, int korder, const double* mom,
                               const int* far_off, const Item* items, double* e_far,
                               hipStream_t s) {
   bool done = false;
   ((nterms == NTs + 1
-        ? (launch_far_grad<NTs + 1>(tv, delta2, mom, far_off, items, e_far, s), done = true)
+        ? (launch_far_grad<NTs + 1>(tv, delta2, korder, mom, far_off, items, e_far, s),
+           done = true)
         : false),
    ...);
   return done;
 }
 
-hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, const double* mom_hat64,
-                          const int* far_off, const Item* far_items, double* e_far,
-                          hipStream_t s) {
+hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, int korder,
+                          const double* mom_hat64, const int* far_off, const Item* far_items,
+                          double* e_far, hipStream_t s) {
+  if (korder < 0) return hipErrorInvalidValue;
   if (!dispatch_far_grad(nterms, std::make_integer_sequence<int, kMaxNterms - 1>{}, tv, delta2,
-                         mom_hat64, far_off, far_items, e_far, s))
+                         korder, mom_hat64, far_off, far_items, e_far, s))
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
+__global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2, int korder,
                                                   const int* __restrict__ near_off,
                                                   const Item* __restrict__ items,
                                                   const double* __restrict__ e_far,
@@ -1192,7 +1266,18 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
           double4 s = tile[j];
           double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
           double r2 = dx * dx + dy * dy + dz * dz + d2;
-          double w = s.w / (r2 * sqrt(r2));
+          double w;
+          if (korder == 0) {
+            w = s.w / (r2 * sqrt(r2));
+          } else {  // eval_grad_kernel_reg, tree3d.cpp:80-91
+            const double r = sqrt(r2);
+            double temp = 1.0 / (r * r2), f = temp;
+            for (int l = 1; l <= korder; l++) {
+              f = f * (l + 0.5) / l * d2 / r2;
+              temp += f;
+            }
+            w = s.w * temp;
+          }
           ex += dx * w;
           ey += dy * w;
           ez += dz * w;
@@ -1209,11 +1294,11 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2,
   }
 }
 
-hipError_t near_field_grad(const TreeView& tv, double delta2, const int* near_off,
+hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, const int* near_off,
                            const Item* near_items, const double* e_far, double* out,
                            hipStream_t s) {
   int nb = (tv.nt + kBatch - 1) / kBatch;
-  hipLaunchKernelGGL(k_near_grad, dim3(nb), dim3(kBatch), 0, s, tv, delta2, near_off,
+  hipLaunchKernelGGL(k_near_grad, dim3(nb), dim3(kBatch), 0, s, tv, delta2, korder, near_off,
                      near_items, e_far, out);
   return hipGetLastError();
 }

@@ -1,7 +1,7 @@
 // problem.h -- default treecode parameters of the reference interface
 // (problem.h of ongbw/bit): MAC parameter ACC, regularization DEL = delta^2,
-// kernel family KERNEL_ORDER (only order 0 is implemented, as the reference's
-// default).
+// order KERNEL_ORDER of the regularized kernel family the field paths
+// (treeforce, freespace_particles_tree) use; 0 is the reference's default.
 #ifndef problem_h
 #define problem_h
 

@@ -99,6 +99,7 @@ bool evaluate(bit_system* s, int n, const PARTICLE* part, int what, int nterms, 
   if (what == kField) {
     out.resize(3 * (size_t)n);
     bit_params p = params(nterms, acc, 0.0, DEL, DEL);  // treeforce: rs = r^2 + DEL
+    p.kernel_order = KERNEL_ORDER;                      // regularized family (problem.h)
     if (bit_eval_field(s, &p, out.data(), 0) != BIT_OK) return false;
     return true;
   }
@@ -163,6 +164,7 @@ bool point_eval(const char* fn, TREE* tree, PARTICLE* part, int what, int nterms
   int rc;
   if (what == kField) {
     bit_params p = params(nterms, acc, 0.0, DEL, DEL);
+    p.kernel_order = KERNEL_ORDER;
     rc = bit_eval_field_points(b.sys, &p, 1, &x, &y, &z, out);
   } else if (what == kPot) {
     bit_params p = params(nterms, acc, 0.0, DEL, DEL);

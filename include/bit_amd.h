@@ -40,6 +40,8 @@ extern "C" {
 
 /* largest supported Taylor order: nterms <= BIT_MAX_NTERMS (p = nterms-1) */
 #define BIT_MAX_NTERMS 13
+/* largest order of the regularized field family (bit_params.kernel_order) */
+#define BIT_MAX_KERNEL_ORDER 8
 
 typedef struct bit_ctx bit_ctx;
 typedef struct bit_system bit_system;
@@ -54,6 +56,10 @@ typedef struct bit_params {
   double delta2;    /* regularization delta^2 (reference DEL, problem.h:12) */
   double mac_sub;   /* subtracted in the MAC; reference: DEL */
   int precision;    /* BIT_FP64 or BIT_FP32 */
+  int kernel_order; /* field only: regularized family G_L = sum_{l<=L} w_l delta2^l
+                       R^-(2l+1), w_l = prod_{j<=l} (j-1/2)/j (reference KERNEL_ORDER,
+                       problem.h:13; tree3d.cpp:88, 614-791); 0 = 1/R.  Potential
+                       entry points require 0. */
 } bit_params;
 
 /* Interaction statistics of the last evaluation of a system (exact counts). */

@@ -479,6 +479,67 @@ void ora_direct(int n, const double *x, const double *y, const double *z, const 
   }
 }
 
+/* The regularized kernel family of treetaylorforce / eval_grad_kernel_reg
+ * (tree3d.cpp:614-791, 73-97; order L = KERNEL_ORDER of problem.h):
+ *   G_L(R) = sum_{l=0..L} w_l delta2^l R^-(2l+1),  w_l = prod_{j<=l} (j - 1/2)/j
+ * (w_l = choose(-1/2, l) (-1)^l, the reference's binomial/factorial).  Its
+ * field is  q (x_t - x_s) sum_l f_l  with f_0 = 1/(R^2 R) and
+ * f_l = f_{l-1} (l + 1/2)/l delta2/R^2, as eval_grad_kernel_reg sums it.
+ *
+ * Coefficients of g = R^-(2l+1) = u^-s, u = R^2, s = l + 1/2: from
+ * u(y) dg/dy_e = 2 s (x_e - y_e) g, matching powers of y gives
+ *   |k| R^2 a_k = (2|k| + 2l - 1) sum_e x_e a_{k-e} - (|k| + 2l - 1) sum_e a_{k-2e},
+ * the relation the reference applies family by family (its signs differ by
+ * (-1)^|k|: it expands in the target offset). */
+void ora_powinv_coeffs(int nterms, int l, double dx, double dy, double dz, double R2,
+                       double *a) {
+  const int S = nterms + 1;
+  const double xv[3] = {dx, dy, dz};
+  const double fac = 1.0 / R2;
+#define IX(i, j, k) (((i) * S + (j)) * S + (k))
+  for (int c = 0; c < S * S * S; c++) a[c] = 0.0;
+  a[IX(0, 0, 0)] = pow(R2, -(2.0 * l + 1.0) / 2.0);
+  for (int n = 1; n < nterms; n++)
+    for (int i = n; i >= 0; i--)
+      for (int j = n - i; j >= 0; j--) {
+        int k = n - i - j, kv[3] = {i, j, k};
+        double s1 = 0.0, s2 = 0.0;
+        for (int e = 0; e < 3; e++) {
+          int m[3] = {i, j, k};
+          if (kv[e] >= 1) {
+            m[e] -= 1;
+            s1 += xv[e] * a[IX(m[0], m[1], m[2])];
+          }
+          if (kv[e] >= 2) {
+            m[e] -= 1;
+            s2 += a[IX(m[0], m[1], m[2])];
+          }
+        }
+        a[IX(i, j, k)] = fac / n * ((2.0 * n + 2.0 * l - 1.0) * s1 - (n + 2.0 * l - 1.0) * s2);
+      }
+#undef IX
+}
+
+static double family_weight(int l) {
+  double w = 1.0;
+  for (int j = 1; j <= l; j++) w *= (j - 0.5) / j;
+  return w;
+}
+
+/* Near-field weight of one source: q/(R^2 R) for L = 0 (the expression used
+ * since the first field restatement), else eval_grad_kernel_reg's sum
+ * (tree3d.cpp:80-91) times q, in its operation order. */
+static double family_near(double q, double r2, double delta2, int L) {
+  if (L == 0) return q / (r2 * sqrt(r2));
+  double r = sqrt(r2);
+  double temp = 1.0 / (r * r2), factor = temp;
+  for (int l = 1; l <= L; l++) {
+    factor = factor * (l + 0.5) / l * delta2 / r2;
+    temp += factor;
+  }
+  return q * temp;
+}
+
 /* treeforce, tree3d.cpp:851-971, with the far field of treetaylorforce
  * (tree3d.cpp:456-810, KERNEL_ORDER 0): for potential coefficients a_k of 1/R
  * in the source offset, the field is E_x = sum_k (k_x+1) M_k a_{k+e_x}, which is
@@ -491,16 +552,24 @@ static void force_rec(const ora_tree *T, int ind, int p, double px, double py, d
   double dx, dy, dz, rs;
   if (mac_accept(d, px, py, pz, prm, &dx, &dy, &dz, &rs)) {
     const int nt = prm.nterms, S = nt + 2;
-    ora_taylor_coeffs(nt + 1, dx, dy, dz, rs, 0.0, coef);
     const double *M = mom + (int64_t)nt * nt * nt * ind;
-    for (int i = 0; i < nt; i++)
-      for (int j = 0; j < nt - i; j++)
-        for (int k = 0; k < nt - i - j; k++) {
-          double m = M[(i * nt + j) * nt + k];
-          E[0] += m * (i + 1) * coef[((i + 1) * S + j) * S + k];
-          E[1] += m * (j + 1) * coef[(i * S + j + 1) * S + k];
-          E[2] += m * (k + 1) * coef[(i * S + j) * S + k + 1];
-        }
+    for (int l = 0; l <= prm.kernel_order; l++) {
+      double w = 1.0;
+      if (l == 0) {
+        ora_taylor_coeffs(nt + 1, dx, dy, dz, rs, 0.0, coef);
+      } else {
+        ora_powinv_coeffs(nt + 1, l, dx, dy, dz, rs, coef);
+        w = family_weight(l) * pow(prm.delta2, l);
+      }
+      for (int i = 0; i < nt; i++)
+        for (int j = 0; j < nt - i; j++)
+          for (int k = 0; k < nt - i - j; k++) {
+            double m = w * M[(i * nt + j) * nt + k];
+            E[0] += m * (i + 1) * coef[((i + 1) * S + j) * S + k];
+            E[1] += m * (j + 1) * coef[(i * S + j + 1) * S + k];
+            E[2] += m * (k + 1) * coef[(i * S + j) * S + k + 1];
+          }
+    }
     return;
   }
   int go = 1;
@@ -515,7 +584,7 @@ static void force_rec(const ora_tree *T, int ind, int p, double px, double py, d
       if (j == p) continue;
       double ex = px - x[j], ey = py - y[j], ez = pz - z[j];
       double r2 = ex * ex + ey * ey + ez * ez + prm.delta2;
-      double w = q[j] / (r2 * sqrt(r2));
+      double w = family_near(q[j], r2, prm.delta2, prm.kernel_order);
       E[0] += ex * w;
       E[1] += ey * w;
       E[2] += ez * w;
@@ -565,8 +634,8 @@ void ora_treeforce_points(const ora_tree *T, const double *x, const double *y,
 
 /* force_particle_ds_reg, tree3d.cpp:298-330 (field q (x_t - x_s)/R^3). */
 void ora_direct_force(int n, const double *x, const double *y, const double *z,
-                      const double *q, double delta2, int nt, const int *tidx, double *ex,
-                      double *ey, double *ez, int nthreads) {
+                      const double *q, double delta2, int kernel_order, int nt, const int *tidx,
+                      double *ex, double *ey, double *ez, int nthreads) {
 #pragma omp parallel for schedule(static) num_threads(thread_count(nthreads))
   for (int t = 0; t < nt; t++) {
     int p = tidx[t];
@@ -575,7 +644,7 @@ void ora_direct_force(int n, const double *x, const double *y, const double *z,
       if (j == p) continue;
       double dx = x[p] - x[j], dy = y[p] - y[j], dz = z[p] - z[j];
       double r2 = dx * dx + dy * dy + dz * dz + delta2;
-      double w = q[j] / (r2 * sqrt(r2));
+      double w = family_near(q[j], r2, delta2, kernel_order);
       E[0] += dx * w;
       E[1] += dy * w;
       E[2] += dz * w;

@@ -57,6 +57,7 @@ typedef struct ora_params {
   double kappa;   /* screening */
   double delta2;  /* regularization delta^2 */
   double mac_sub; /* value subtracted from (r^2+delta2) in the MAC */
+  int kernel_order; /* field only: regularized family G_L (problem.h KERNEL_ORDER) */
 } ora_params;
 
 /* ---- tree (tree3d.cpp:990-1551) ---- */
@@ -125,8 +126,13 @@ void ora_treeforce(const ora_tree *t, const double *x, const double *y, const do
                    const double *q, const double *moments, ora_params prm, int nt,
                    const int *tidx, double *ex, double *ey, double *ez, int nthreads);
 void ora_direct_force(int n, const double *x, const double *y, const double *z,
-                      const double *q, double delta2, int nt, const int *tidx, double *ex,
-                      double *ey, double *ez, int nthreads);
+                      const double *q, double delta2, int kernel_order, int nt, const int *tidx,
+                      double *ex, double *ey, double *ez, int nthreads);
+/* Taylor coefficients, in the source offset y, of R^-(2l+1) with
+ * R^2 = |x - y|^2 + delta2 (x = (dx,dy,dz), R2 = |x|^2 + delta2), degrees
+ * < nterms, layout [(i*S + j)*S + k], S = nterms + 1.  Term l of the
+ * regularized kernel family (tree3d.cpp:614-791). */
+void ora_powinv_coeffs(int nterms, int l, double dx, double dy, double dz, double R2, double *a);
 
 #ifdef __cplusplus
 }

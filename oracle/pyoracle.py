@@ -26,7 +26,7 @@ _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 class OraParams(C.Structure):
     _fields_ = [("nterms", C.c_int), ("acc", C.c_double), ("kappa", C.c_double),
-                ("delta2", C.c_double), ("mac_sub", C.c_double)]
+                ("delta2", C.c_double), ("mac_sub", C.c_double), ("kernel_order", C.c_int)]
 
 
 @dataclass
@@ -82,8 +82,10 @@ class Oracle:
                                            C.c_int, _dp, _dp, _dp, _dp, C.c_int]
         L.ora_treeforce_points.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, OraParams,
                                            C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
-        L.ora_direct_force.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _ip,
-                                       _dp, _dp, _dp, C.c_int]
+        L.ora_direct_force.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                                       C.c_int, _ip, _dp, _dp, _dp, C.c_int]
+        L.ora_powinv_coeffs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _dp]
         self.L = L
 
     # treemake (tree3d.cpp:1345)
@@ -108,12 +110,19 @@ class Oracle:
         self.L.ora_direct(len(x), x, y, z, q, kappa, delta2, len(tidx), tidx, out, nthreads)
         return out
 
-    def direct_force(self, x, y, z, q, delta2, tidx=None, nthreads=0):
+    def direct_force(self, x, y, z, q, delta2, tidx=None, nthreads=0, kernel_order=0):
         x, y, z, q = _f64(x), _f64(y), _f64(z), _f64(q)
         tidx = _i32(np.arange(len(x)) if tidx is None else tidx)
         e = [np.zeros(len(tidx)) for _ in range(3)]
-        self.L.ora_direct_force(len(x), x, y, z, q, delta2, len(tidx), tidx, *e, nthreads)
+        self.L.ora_direct_force(len(x), x, y, z, q, delta2, kernel_order, len(tidx), tidx, *e,
+                                nthreads)
         return np.stack(e, axis=1)
+
+    def powinv_coeffs(self, nterms, l, dx, dy, dz, R2):
+        S = nterms + 1
+        a = np.zeros(S * S * S)
+        self.L.ora_powinv_coeffs(nterms, l, dx, dy, dz, R2, a)
+        return a.reshape(S, S, S)
 
 
 class OracleTree:
@@ -169,10 +178,11 @@ class OracleTree:
                                         nthreads)
         return out
 
-    def treeforce_points(self, q, moments, nterms, acc, delta2, mac_sub, px, py, pz, nthreads=0):
+    def treeforce_points(self, q, moments, nterms, acc, delta2, mac_sub, px, py, pz, nthreads=0,
+                         kernel_order=0):
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         e = [np.zeros(len(px)) for _ in range(3)]
-        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub)
+        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub, kernel_order)
         self.ora.L.ora_treeforce_points(self.h, self.x, self.y, self.z, _f64(q),
                                         _f64(moments).ravel(), prm, len(px), px, py, pz, *e,
                                         nthreads)
@@ -187,11 +197,12 @@ class OracleTree:
                              near, C.byref(nn), cap)
         return far[:nf.value].copy(), near[:nn.value].copy()
 
-    def treeforce(self, q, moments, nterms, acc, delta2, mac_sub, tidx=None, nthreads=0):
+    def treeforce(self, q, moments, nterms, acc, delta2, mac_sub, tidx=None, nthreads=0,
+                  kernel_order=0):
         q = _f64(q)
         tidx = _i32(np.arange(len(q)) if tidx is None else tidx)
         e = [np.zeros(len(tidx)) for _ in range(3)]
-        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub)
+        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub, kernel_order)
         self.ora.L.ora_treeforce(self.h, self.x, self.y, self.z, q, _f64(moments).ravel(), prm,
                                  len(tidx), tidx, *e, nthreads)
         return np.stack(e, axis=1)

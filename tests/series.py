@@ -69,3 +69,18 @@ def taylor_coeffs(P, x, kappa, delta2):
     g = compose(s.scale(-kappa * R0), [1.0 / factorial(m) for m in range(P + 1)])
     h = compose(w, [_gbinom(-0.5, m) for m in range(P + 1)])             # (1+w)^(-1/2)
     return (h * g).scale(exp(-kappa * R0) / R0).c
+
+
+def powinv_coeffs(P, x, l, delta2):
+    """a[i,j,k] = coefficient of y1^i y2^j y3^k in R^-(2l+1), degree <= P."""
+    R02 = x[0] ** 2 + x[1] ** 2 + x[2] ** 2 + delta2
+    w = Series(P)
+    for e in range(3):
+        idx = [0, 0, 0]
+        idx[e] = 1
+        w.c[tuple(idx)] = -2.0 * x[e] / R02
+        idx[e] = 2
+        w.c[tuple(idx)] = 1.0 / R02
+    s = -(2 * l + 1) / 2.0
+    h = compose(w, [_gbinom(s, m) for m in range(P + 1)])                # (1+w)^s
+    return h.scale(R02 ** s).c

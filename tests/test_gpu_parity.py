@@ -349,3 +349,37 @@ def test_points_empty_and_invalid(ctx):
     with pytest.raises(bit_amd.BitError):
         s.potential_at(np.array([np.nan]), np.zeros(1), np.zeros(1), 5, 0.5)
     assert np.isfinite(s.potential(5, 0.5)).all()
+
+
+@pytest.mark.parametrize("L,nterms", [(1, 6), (2, 5), (3, 8)])
+def test_field_family_matches_oracle(ctx, oracle, L, nterms):
+    """KERNEL_ORDER = L regularized field family (tree3d.cpp:73-97, 614-791),
+    particle targets and arbitrary points, FP64 within 1e-10 of the oracle."""
+    P = workloads.gaussian_blobs(20000, seed=43)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 200)
+    o = oracle.tree(P.x, P.y, P.z, 200)
+    m = o.moments(P.q, nterms)
+    tidx = workloads.sample_targets(P.n, 1500, seed=9)
+    ref = o.treeforce(P.q, m, nterms, 0.5, 0.01, 0.01, tidx, kernel_order=L)
+    got = s.field(nterms, 0.5, 0.01, 0.01, kernel_order=L)[tidx]
+    assert rel_max(got, ref) < 1e-10
+    E0 = s.field(nterms, 0.5, 0.01, 0.01)[tidx]
+    assert rel_l2(got, E0) > 1e-6  # the higher terms are present
+    px, py, pz = _points(P, 700, seed=10)
+    refp = o.treeforce_points(P.q, m, nterms, 0.5, 0.01, 0.01, px, py, pz, kernel_order=L)
+    assert rel_max(s.field_at(px, py, pz, nterms, 0.5, 0.01, 0.01, kernel_order=L), refp) < 1e-10
+
+
+def test_kernel_order_rejected_for_potential(ctx):
+    """The family is a field option (KERNEL_ORDER is read by the field paths
+    only); potentials and out-of-range orders are refused, the system stays usable."""
+    import ctypes as C
+    P = workloads.uniform_cube(3000, seed=4)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 100)
+    out = np.zeros(P.n)
+    prm = bit_amd.make_params(5, 0.5, 0.0, 0.01, None, "fp64", kernel_order=1)
+    rc = bit_amd.lib.bit_eval_potential(s.h, C.byref(prm), out.ctypes.data_as(C.c_void_p), 0)
+    assert rc != 0 and b"field only" in bit_amd.lib.bit_last_error()
+    with pytest.raises(bit_amd.BitError):
+        s.field(5, 0.5, 0.01, kernel_order=99)
+    assert np.isfinite(s.field(5, 0.5, 0.01, kernel_order=2)).all()

@@ -161,3 +161,74 @@ def test_point_targets(oracle):
     mm = t.moments(P.q, 6)
     Ep = t.treeforce_points(P.q, mm, 6, 0.5, 0.01, 0.01, P.x[tidx], P.y[tidx], P.z[tidx])
     assert rel_max(Ep, t.treeforce(P.q, mm, 6, 0.5, 0.01, 0.01, tidx)) < 1e-13
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
+@pytest.mark.parametrize("x,d2", [((0.9, -0.4, 0.3), 0.01), ((-0.2, 1.3, 0.05), 0.2),
+                                  ((0.5, 0.5, -0.8), 0.0)])
+def test_powinv_coeffs_vs_power_series(oracle, l, x, d2):
+    """Coefficients of R^-(2l+1) (terms of the regularized field family)
+    against the binomial series of (1 + w)^-(2l+1)/2; l = 0 also against the
+    kappa = 0 recurrence used for the potential."""
+    from series import powinv_coeffs
+    P = 8
+    R2 = x[0] ** 2 + x[1] ** 2 + x[2] ** 2 + d2
+    a = oracle.powinv_coeffs(P + 1, l, *x, R2)
+    ref = powinv_coeffs(P, x, l, d2)
+    i, j, k = np.indices((P + 1,) * 3)
+    m = (i + j + k) <= P
+    assert np.abs(a[:P + 1, :P + 1, :P + 1][m] - ref[m]).max() <= 1e-12 * np.abs(ref[m]).max()
+    if l == 0:
+        t = oracle.taylor_coeffs(P + 1, *x, R2, 0.0)
+        assert np.abs(a - t).max() <= 1e-13 * np.abs(t).max()
+
+
+def _family_phi(P, X, d2, L):
+    """Potential of the family G_L = sum_l w_l d2^l R^-(2l+1) at points X."""
+    S = np.stack([P.x, P.y, P.z], 1)
+    out = np.zeros(len(X))
+    for i, xi in enumerate(X):
+        R2 = ((xi - S) ** 2).sum(1) + d2
+        g = np.zeros_like(R2)
+        w = 1.0
+        for l in range(L + 1):
+            if l:
+                w *= (l - 0.5) / l
+            g += w * d2 ** l * R2 ** (-(2 * l + 1) / 2.0)
+        out[i] = g @ P.q
+    return out
+
+
+@pytest.mark.parametrize("L", [1, 2, 3])
+def test_field_family(oracle, L):
+    """KERNEL_ORDER >= 1: the direct field is -grad of the family's potential
+    (central differences, the field taken at points off the particles), it
+    grows with L as every added term is positive, and the treecode field
+    converges to it as nterms grows."""
+    P = workloads.gaussian_blobs(1500, seed=13)
+    d2 = 0.01
+    rng = np.random.default_rng(L)
+    X = rng.uniform(-0.6, 0.6, size=(6, 3))
+    # a point evaluation via a one-particle "target" appended with charge 0
+    xs = np.concatenate([P.x, X[:, 0]])
+    ys = np.concatenate([P.y, X[:, 1]])
+    zs = np.concatenate([P.z, X[:, 2]])
+    qs = np.concatenate([P.q, np.zeros(len(X))])
+    tidx = np.arange(P.n, P.n + len(X))
+    E = oracle.direct_force(xs, ys, zs, qs, d2, tidx, kernel_order=L)
+    h = 1e-5
+    for e in range(3):
+        dX = np.zeros_like(X)
+        dX[:, e] = h
+        g = -(_family_phi(P, X + dX, d2, L) - _family_phi(P, X - dX, d2, L)) / (2 * h)
+        assert np.allclose(E[:, e], g, rtol=1e-6, atol=1e-7 * np.abs(g).max())
+    tid = workloads.sample_targets(P.n, 300, seed=2)
+    E0 = oracle.direct_force(P.x, P.y, P.z, P.q, d2, tid, kernel_order=0)
+    EL = oracle.direct_force(P.x, P.y, P.z, P.q, d2, tid, kernel_order=L)
+    assert rel_l2(EL, E0) > 1e-4
+    t = oracle.tree(P.x, P.y, P.z, 40)
+    errs = []
+    for nt in (3, 6, 9):
+        m = t.moments(P.q, nt)
+        errs.append(rel_l2(t.treeforce(P.q, m, nt, 0.3, d2, d2, tid, kernel_order=L), EL))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-4
