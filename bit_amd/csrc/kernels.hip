@@ -960,17 +960,18 @@ hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nt
 // terms summed plainly and added with Kahan compensation.  The self pair is
 // excluded only in leaves that hold some of the batch's targets.
 // ---------------------------------------------------------------------------
+// g + q G(r) for one pair, the sum folded into the last FMA.
 template <bool SCR>
-__device__ __forceinline__ float pair32(float ox, float oy, float oz, float4 s, float d2,
+__device__ __forceinline__ float pair32(float g, float ox, float oy, float oz, float4 s, float d2,
                                         float kneg2) {
   float dx = ox - s.x, dy = oy - s.y, dz = oz - s.z;
   float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
   float ri = __builtin_amdgcn_rsqf(r2);
   if constexpr (SCR) {
     float e = __builtin_amdgcn_exp2f(kneg2 * (r2 * ri));
-    return s.w * e * ri;
+    return fmaf(s.w * e, ri, g);
   } else {
-    return s.w * ri;
+    return fmaf(s.w, ri, g);
   }
 }
 
@@ -1022,11 +1023,11 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
           for (; j + 16 <= cnt; j += 16) {
             float g = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 16; u++) g += pair32<SCR>(ox, oy, oz, tile[j + u], d2, kneg2);
+            for (int u = 0; u < 16; u++) g = pair32<SCR>(g, ox, oy, oz, tile[j + u], d2, kneg2);
             acc.add(g);
           }
           float g = 0.0f;
-          for (; j < cnt; j++) g += pair32<SCR>(ox, oy, oz, tile[j], d2, kneg2);
+          for (; j < cnt; j++) g = pair32<SCR>(g, ox, oy, oz, tile[j], d2, kneg2);
           acc.add(g);
         } else if (tv.self) {
           const int me = t - base;
@@ -1035,14 +1036,14 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
             float g = 0.0f;
 #pragma unroll
             for (int u = 0; u < 16; u++) {
-              float v = pair32<SCR>(ox, oy, oz, tile[j + u], d2, kneg2);
+              float v = pair32<SCR>(0.0f, ox, oy, oz, tile[j + u], d2, kneg2);
               g += (j + u == me) ? 0.0f : v;
             }
             acc.add(g);
           }
           float g = 0.0f;
           for (; j < cnt; j++) {
-            float v = pair32<SCR>(ox, oy, oz, tile[j], d2, kneg2);
+            float v = pair32<SCR>(0.0f, ox, oy, oz, tile[j], d2, kneg2);
             g += (j == me) ? 0.0f : v;
           }
           acc.add(g);
@@ -1052,7 +1053,7 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
             float4 q = tile[j];
             float dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
             float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
-            float v = pair32<SCR>(ox, oy, oz, q, d2, kneg2);
+            float v = pair32<SCR>(0.0f, ox, oy, oz, q, d2, kneg2);
             g += r2 > 1e-14f ? v : 0.0f;
           }
           acc.add(g);
