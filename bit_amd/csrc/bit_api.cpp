@@ -254,24 +254,65 @@ class DeviceEngine : public bit::PartitionEngine {
   }
 
  private:
+  static size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+  // One round: the op and chunk tables go up in one copy from pinned memory,
+  // the per-op results come back into pinned memory.
   int step(int nops, int nch) {
     hipStream_t st = c_->stream;
-    BIT_TRY(upload(c_, d_ops_, hops_.data(), nops * sizeof(bit::dev::PartOpDev)));
-    BIT_TRY(upload(c_, d_chunks_, chunks_.data(), nch * sizeof(bit::dev::PartChunk)));
-    BIT_TRY(upload(c_, d_c0_, c0_.data(), nops * sizeof(int)));
-    BIT_TRY(upload(c_, d_nc_, nc_.data(), nops * sizeof(int)));
+    const size_t o_ch = al(nops * sizeof(bit::dev::PartOpDev));
+    const size_t o_c0 = o_ch + al(nch * sizeof(bit::dev::PartChunk));
+    const size_t o_nc = o_c0 + al(nops * sizeof(int));
+    const size_t total = o_nc + al(nops * sizeof(int));
+    const size_t rbytes = nops * sizeof(bit::dev::ChunkResult);
+    BIT_TRY(up_.ensure(total));
+    BIT_TRY(down_.ensure(rbytes));
+    char* h = static_cast<char*>(up_.p);
+    std::memcpy(h, hops_.data(), nops * sizeof(bit::dev::PartOpDev));
+    std::memcpy(h + o_ch, chunks_.data(), nch * sizeof(bit::dev::PartChunk));
+    std::memcpy(h + o_c0, c0_.data(), nops * sizeof(int));
+    std::memcpy(h + o_nc, nc_.data(), nops * sizeof(int));
+    BIT_TRY(d_blob_.ensure(c_, total));
     BIT_TRY(d_cres_.ensure(c_, (size_t)nch * sizeof(bit::dev::ChunkResult)));
     BIT_TRY(d_cbase_.ensure(c_, (size_t)nch * sizeof(int2)));
-    BIT_TRY(d_ores_.ensure(c_, (size_t)nops * sizeof(bit::dev::ChunkResult)));
-    HIP_TRY(bit::dev::partition_round(a_, nops, d_ops_.as<bit::dev::PartOpDev>(), nch,
-                                      d_chunks_.as<bit::dev::PartChunk>(), d_c0_.as<int>(),
-                                      d_nc_.as<int>(), d_cres_.as<bit::dev::ChunkResult>(),
-                                      d_cbase_.as<int2>(), d_ores_.as<bit::dev::ChunkResult>(), st));
-    HIP_TRY(hipMemcpyAsync(hres_.data(), d_ores_.p, nops * sizeof(bit::dev::ChunkResult),
-                           hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    BIT_TRY(d_ores_.ensure(c_, rbytes));
+    HIP_TRY(hipMemcpyAsync(d_blob_.p, h, total, hipMemcpyHostToDevice, st));
+    char* d = d_blob_.as<char>();
+    HIP_TRY(bit::dev::partition_round(
+        a_, nops, reinterpret_cast<bit::dev::PartOpDev*>(d), nch,
+        reinterpret_cast<bit::dev::PartChunk*>(d + o_ch), reinterpret_cast<int*>(d + o_c0),
+        reinterpret_cast<int*>(d + o_nc), d_cres_.as<bit::dev::ChunkResult>(),
+        d_cbase_.as<int2>(), d_ores_.as<bit::dev::ChunkResult>(), st));
+    HIP_TRY(hipMemcpyAsync(down_.p, d_ores_.p, rbytes, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));  // the pinned tables are reused next round
+    std::memcpy(hres_.data(), down_.p, rbytes);
     return BIT_OK;
   }
+
+  // Page-locked host staging, grown on demand.
+  struct Pinned {
+    void* p = nullptr;
+    size_t cap = 0;
+    Pinned() = default;
+    Pinned(const Pinned&) = delete;
+    Pinned& operator=(const Pinned&) = delete;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+    int ensure(size_t need) {
+      if (need <= cap && p) return BIT_OK;
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return fail(BIT_E_OOM, "pinned host staging allocation failed");
+      }
+      cap = need;
+      return BIT_OK;
+    }
+  };
 
   bit_ctx* c_;
   bit::dev::SlotArrays a_;
@@ -279,7 +320,8 @@ class DeviceEngine : public bit::PartitionEngine {
   std::vector<bit::dev::PartChunk> chunks_;
   std::vector<int> c0_, nc_;
   std::vector<bit::dev::ChunkResult> hres_;
-  DevBuf d_ops_, d_chunks_, d_c0_, d_nc_, d_cres_, d_cbase_, d_ores_;
+  Pinned up_, down_;
+  DevBuf d_blob_, d_cres_, d_cbase_, d_ores_;
 };
 
 // Builds the tree of n particles whose coordinates are in HBM (original order)

@@ -419,31 +419,59 @@ __global__ __launch_bounds__(256) void k_part_count(SlotArrays a, const PartOpDe
 }
 
 // Per op (one thread): chunk bases in chunk order, totals and boxes.
-__global__ void k_part_offsets(int nops, const PartOpDev* __restrict__ ops,
-                               const PartChunk* __restrict__ chunks, const int* __restrict__ op_chunk0,
-                               const int* __restrict__ op_nchunk, const ChunkResult* __restrict__ cres,
-                               int2* __restrict__ cbase, ChunkResult* __restrict__ ores) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nops) return;
-  int c0 = op_chunk0[o], nc = op_nchunk[o];
-  int nl = 0, nr = 0;
-  ChunkResult r;
+// One wavefront per op: exclusive scan of the chunks' left/right counts (the
+// chunks' bases in the op's left and right runs) and the op's child boxes.
+__global__ __launch_bounds__(64) void k_part_offsets(const PartChunk* __restrict__ chunks,
+                                                     const int* __restrict__ op_chunk0,
+                                                     const int* __restrict__ op_nchunk,
+                                                     const ChunkResult* __restrict__ cres,
+                                                     int2* __restrict__ cbase,
+                                                     ChunkResult* __restrict__ ores) {
+  const int o = blockIdx.x, lane = threadIdx.x;
+  const int c0 = op_chunk0[o], nc = op_nchunk[o];
+  int nl_run = 0, nr_run = 0;
+  double v[12];
   for (int k = 0; k < 6; k++) {
-    r.v[k] = INFINITY;
-    r.v[6 + k] = -INFINITY;
+    v[k] = INFINITY;
+    v[6 + k] = -INFINITY;
   }
-  for (int c = c0; c < c0 + nc; c++) {
-    cbase[c] = make_int2(nl, nr);
-    const ChunkResult& x = cres[c];
-    nl += x.nl;
-    nr += (chunks[c].s1 - chunks[c].s0) - x.nl;
-    for (int k = 0; k < 6; k++) {
-      r.v[k] = fmin(r.v[k], x.v[k]);
-      r.v[6 + k] = fmax(r.v[6 + k], x.v[6 + k]);
+  for (int g = 0; g < nc; g += 64) {
+    const int c = c0 + g + lane;
+    const bool in = g + lane < nc;
+    int l = 0, r = 0;
+    if (in) {
+      const ChunkResult& x = cres[c];
+      l = x.nl;
+      r = (chunks[c].s1 - chunks[c].s0) - l;
+      for (int k = 0; k < 6; k++) {
+        v[k] = fmin(v[k], x.v[k]);
+        v[6 + k] = fmax(v[6 + k], x.v[6 + k]);
+      }
     }
+    int il = l, ir = r;  // inclusive scan across the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      const int tl = __shfl_up(il, off), tr = __shfl_up(ir, off);
+      if (lane >= off) {
+        il += tl;
+        ir += tr;
+      }
+    }
+    if (in) cbase[c] = make_int2(nl_run + il - l, nr_run + ir - r);
+    nl_run += __shfl(il, 63);
+    nr_run += __shfl(ir, 63);
   }
-  r.nl = nl;
-  ores[o] = r;
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 12; k++) {
+      const double w = __shfl_down(v[k], off);
+      v[k] = k < 6 ? fmin(v[k], w) : fmax(v[k], w);
+    }
+  if (lane == 0) {
+    ChunkResult r;
+    r.nl = nl_run;
+    r.pad = 0;
+    for (int k = 0; k < 12; k++) r.v[k] = v[k];
+    ores[o] = r;
+  }
 }
 
 // Stable scatter of each chunk into the temporary arrays: 256-slot tiles,
@@ -505,8 +533,8 @@ hipError_t partition_round(const SlotArrays& a, int nops, const PartOpDev* ops, 
                            ChunkResult* cres, int2* cbase, ChunkResult* ores, hipStream_t s) {
   if (nchunks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_part_count, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cres);
-  hipLaunchKernelGGL(k_part_offsets, dim3((nops + 63) / 64), dim3(64), 0, s, nops, ops, chunks,
-                     op_chunk0, op_nchunk, cres, cbase, ores);
+  hipLaunchKernelGGL(k_part_offsets, dim3(nops), dim3(64), 0, s, chunks, op_chunk0, op_nchunk,
+                     cres, cbase, ores);
   hipLaunchKernelGGL(k_part_scatter, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cbase, ores);
   hipLaunchKernelGGL(k_part_copy, dim3(nchunks), dim3(256), 0, s, a, chunks);
   return hipGetLastError();
