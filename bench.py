@@ -4,9 +4,8 @@
                     [--nterms P1] [--acc THETA] [--maxmemb N0] [--no-cpu-baseline]
 
 One step = one whole evaluation of the potential at all N particles of one
-system whose coordinates and charges are already resident in HBM: tree build
-(host, bit-exact with the reference treemake, after copying the coordinates
-down), upload of the node table and sorted sources, FP64 moments, batch
+system whose coordinates and charges are already resident in HBM: tree build on
+the device (bit-exact with the reference treemake), FP64 moments, batch
 interaction lists, far field and near field, result left in HBM.
 
 Prints ONE JSON line.  Extra keys: `accuracy` (relative L2 error of the last
@@ -32,6 +31,17 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32 = 157.3e12  # MI355X vector FP32, FLOP/s
 PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
+
+# HBM bytes per k_far launch, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this exact configuration (one pass per counter, each scaled by
+# its reading for a 1 GiB device-to-device copy in the same process: FETCH x2,
+# WRITE x1; the summary is kept under profiles/).  Counters cannot be read from
+# inside the timed run, so the figure is reported only when the configuration
+# matches the profiled one.
+MEASURED_TRAFFIC = {
+    # (n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
+    (4_000_000, 3, 11, 0.2, 1000, 0.5, 0.005, "fp32"): 1.005083e9,
+}
 
 
 def ncoef(nterms: int) -> int:
@@ -177,7 +187,10 @@ def main():
             f"{args.nterms},{str(screened).lower()}>" if kern == "far" else
             f"k_near{'32' if args.precision == 'fp32' else '64'}<{str(screened).lower()}>",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None,
+            "frac": achieved / peak,
+            "traffic": (MEASURED_TRAFFIC.get((args.n, args.seed, args.nterms, args.acc,
+                                              args.maxmemb, args.kappa, args.delta,
+                                              args.precision)) if kern == "far" else None),
             "flops_per_launch": kfl, "kernel_ms": kms,
             "other_kernel": {"name": "near" if kern == "far" else "far",
                              "tflops": (f_near / (st["ms_near"] * 1e-3) if kern == "far" else
