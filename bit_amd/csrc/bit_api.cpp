@@ -391,6 +391,38 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
   return BIT_OK;
 }
 
+// Host inputs of up to this many particles are built on the host (the same
+// split program and Morton leaf order, so the tree is identical) and uploaded
+// sorted: a small build is a few ms of host work, where the device build pays a
+// synchronised round trip per split round and contends for the GPU with other
+// work on it.
+constexpr int kHostBuildMax = 1 << 20;
+
+int build_on_host(bit_ctx* ctx, bit_system* s, int n, const double* x, const double* y,
+                  const double* z, const double* dq, int maxmemb) {
+  std::string err;
+  if (!bit::build_tree(n, x, y, z, maxmemb, s->tree, err, ctx->host_threads))
+    return fail(BIT_E_TREE, err);
+  const std::vector<int>& P = s->tree.perm;
+  std::vector<double> a(n), b(n), c(n);
+  for (int i = 0; i < n; i++) {
+    a[i] = x[P[i]];
+    b[i] = y[P[i]];
+    c[i] = z[P[i]];
+  }
+  const size_t nb = (size_t)n * sizeof(double);
+  BIT_TRY(upload(ctx, s->perm, P.data(), (size_t)n * sizeof(int)));
+  BIT_TRY(upload(ctx, s->sx, a.data(), nb));
+  BIT_TRY(upload(ctx, s->sy, b.data(), nb));
+  BIT_TRY(upload(ctx, s->sz, c.data(), nb));
+  BIT_TRY(s->sq.ensure(ctx, nb));
+  BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
+  HIP_TRY(bit::dev::set_charges(s->view(), dq, s->sq.as<double>(), s->sloc.as<float4>(),
+                                ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));  // the sorted host copies die here
+  return BIT_OK;
+}
+
 // Host copy of the permutation (downloaded on first use).
 int ensure_host_perm(const bit_system* cs) {
   bit_system* s = const_cast<bit_system*>(cs);
@@ -674,12 +706,13 @@ static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, 
       return fail(BIT_E_INVALID, "non-finite particle coordinate or charge");
     }
     const size_t nb = (size_t)n * sizeof(double);
-    if ((rc = upload(ctx, ux, x, nb)) == BIT_OK && (rc = upload(ctx, uy, y, nb)) == BIT_OK &&
-        (rc = upload(ctx, uz, z, nb)) == BIT_OK && (rc = upload(ctx, uq, q, nb)) == BIT_OK) {
+    if ((rc = upload(ctx, uq, q, nb)) == BIT_OK) dq = uq.as<double>();
+    if (rc == BIT_OK && n > kHostBuildMax &&  // the device build reads the coordinates in HBM
+        (rc = upload(ctx, ux, x, nb)) == BIT_OK && (rc = upload(ctx, uy, y, nb)) == BIT_OK &&
+        (rc = upload(ctx, uz, z, nb)) == BIT_OK) {
       dx = ux.as<double>();
       dy = uy.as<double>();
       dz = uz.as<double>();
-      dq = uq.as<double>();
     }
   }
   if (rc == BIT_OK && device_src) {  // inputs from HBM are checked on the device
@@ -694,7 +727,12 @@ static int create_common(bit_ctx* ctx, int n, const double* x, const double* y, 
       else if (h) rc = fail(BIT_E_INVALID, "non-finite particle coordinate or charge");
     }
   }
-  if (rc == BIT_OK) rc = build_on_device(ctx, s, n, dx, dy, dz, dq, maxmemb);
+  if (rc == BIT_OK) {
+    if (!device_src && n <= kHostBuildMax)
+      rc = build_on_host(ctx, s, n, x, y, z, dq, maxmemb);
+    else
+      rc = build_on_device(ctx, s, n, dx, dy, dz, dq, maxmemb);
+  }
   auto t1 = std::chrono::steady_clock::now();
   s->stats.ms_tree = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (rc == BIT_OK) rc = finish_system(ctx, s);
@@ -1186,33 +1224,122 @@ int bit_system_set_charges(bit_system* s, const double* q, int on_device) {
   return BIT_OK;
 }
 
+// GMRES(m) with every n-vector resident in HBM: the Krylov basis, the
+// residual and the iterate.  Arnoldi orthogonalises by classical Gram-Schmidt
+// applied twice (all projections of a pass in one launch, one scalar
+// read-back per pass); the Hessenberg columns, the Givens rotations and the
+// small least-squares solve stay on the host, all in FP64.  The stopping rule
+// is restarted_gmres's: ||b - A x|| <= tol ||b||, checked on the Givens
+// estimate inside a cycle and on the true residual at each restart.
 int bit_solve_second_kind(bit_system* s, const bit_params* p, double diag, double self,
                           const double* w, const double* f, double* sigma, int restart,
                           double tol, int maxit, int* iters, double* relres) {
   BIT_TRY(check_params(s, p));
   if (!w || !f || !sigma) return fail(BIT_E_INVALID, "null w, f or sigma");
   if (restart < 1 || maxit < 1 || !(tol > 0)) return fail(BIT_E_INVALID, "bad GMRES settings");
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
   const int n = s->tree.n;
-  std::vector<double> q(n), phi(n);
-  int rc = BIT_OK;
-  auto matvec = [&](const double* in, double* out) {
-    if (rc != BIT_OK) {
-      for (int i = 0; i < n; i++) out[i] = 0.0;
-      return;
-    }
-    for (int i = 0; i < n; i++) q[i] = w[i] * in[i];
-    rc = bit_system_set_charges(s, q.data(), 0);
-    if (rc == BIT_OK) rc = bit_eval_potential(s, p, phi.data(), 0);
-    if (rc != BIT_OK) {
-      for (int i = 0; i < n; i++) out[i] = 0.0;
-      return;
-    }
-    for (int i = 0; i < n; i++) out[i] = (diag + self * w[i]) * in[i] + phi[i];
+  if (!finite_all(n, w) || !finite_all(n, f) || !finite_all(n, sigma) || !std::isfinite(diag) ||
+      !std::isfinite(self))
+    return fail(BIT_E_INVALID, "non-finite weights, right-hand side, initial guess or diagonal");
+  const size_t nb = (size_t)n * sizeof(double);
+  DevBuf dw, db, dx, dv, dr, dq, dphi, dpart, ddots, dc;
+  BIT_TRY(upload(c, dw, w, nb));
+  BIT_TRY(upload(c, db, f, nb));
+  BIT_TRY(upload(c, dx, sigma, nb));
+  BIT_TRY(dv.ensure(c, (size_t)(restart + 1) * nb));
+  BIT_TRY(dr.ensure(c, nb));
+  BIT_TRY(dq.ensure(c, nb));
+  BIT_TRY(dphi.ensure(c, nb));
+  BIT_TRY(dpart.ensure(c, (size_t)bit::dev::kDotBlocks * (restart + 1) * sizeof(double)));
+  BIT_TRY(ddots.ensure(c, (size_t)(restart + 1) * sizeof(double)));
+  BIT_TRY(dc.ensure(c, (size_t)(restart + 1) * sizeof(double)));
+  double* V = dv.as<double>();
+  double* r = dr.as<double>();
+  // y = (diag + self w) v + phi(w v)
+  auto matvec = [&](const double* v, double* y) -> int {
+    HIP_TRY(bit::dev::vec_op(n, 0, 0.0, 0.0, dw.as<double>(), v, nullptr, dq.as<double>(), st));
+    BIT_TRY(bit_system_set_charges(s, dq.as<double>(), 1));
+    BIT_TRY(bit_eval_potential(s, p, dphi.as<double>(), 1));
+    HIP_TRY(bit::dev::vec_op(n, 1, diag, self, dw.as<double>(), v, dphi.as<double>(), y, st));
+    return BIT_OK;
   };
-  bit::GmresResult r = bit::restarted_gmres(matvec, f, sigma, n, restart, tol, maxit);
-  if (rc != BIT_OK) return rc;
-  if (iters) *iters = r.iters;
-  if (relres) *relres = r.relres;
+  auto dotv = [&](const double* Vc, int ncols, const double* u, double* host) -> int {
+    HIP_TRY(bit::dev::dots(n, Vc, (size_t)n, ncols, u, dpart.as<double>(), ddots.as<double>(),
+                           st));
+    HIP_TRY(hipMemcpyAsync(host, ddots.p, (size_t)ncols * sizeof(double), hipMemcpyDeviceToHost,
+                           st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return BIT_OK;
+  };
+  // u += sign * sum_j coef[j] V_j
+  auto combine = [&](int ncols, const double* coef, double sign, double* u) -> int {
+    HIP_TRY(hipMemcpyAsync(dc.p, coef, (size_t)ncols * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(bit::dev::combine_cols(n, V, (size_t)n, ncols, dc.as<double>(), sign, u, st));
+    HIP_TRY(hipStreamSynchronize(st));  // coef is host memory the caller may reuse
+    return BIT_OK;
+  };
+  bit::GmresResult res;
+  double bb = 0.0;
+  BIT_TRY(dotv(db.as<double>(), 1, db.as<double>(), &bb));
+  const double bnorm = std::sqrt(bb);
+  if (bnorm == 0.0) {
+    for (int i = 0; i < n; i++) sigma[i] = 0.0;
+    if (iters) *iters = 0;
+    if (relres) *relres = 0.0;
+    return BIT_OK;
+  }
+  const int ldh = restart;
+  std::vector<double> H((size_t)(restart + 1) * ldh), cs(restart), sn(restart), g(restart + 1),
+      h1(restart + 1), h2(restart + 1), y;
+  while (true) {
+    BIT_TRY(matvec(dx.as<double>(), r));
+    HIP_TRY(bit::dev::vec_op(n, 2, 0.0, 0.0, db.as<double>(), r, nullptr, r, st));
+    double rr = 0.0;
+    BIT_TRY(dotv(r, 1, r, &rr));
+    const double beta = std::sqrt(rr);
+    res.relres = beta / bnorm;
+    if (res.relres <= tol) {
+      res.converged = true;
+      break;
+    }
+    if (res.iters >= maxit) break;
+    HIP_TRY(bit::dev::vec_op(n, 3, 1.0 / beta, 0.0, nullptr, r, nullptr, V, st));
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    for (int j = 0; j < restart && res.iters < maxit; j++) {
+      BIT_TRY(matvec(V + (size_t)j * n, r));
+      res.iters++;
+      BIT_TRY(dotv(V, j + 1, r, h1.data()));
+      BIT_TRY(combine(j + 1, h1.data(), -1.0, r));
+      BIT_TRY(dotv(V, j + 1, r, h2.data()));
+      BIT_TRY(combine(j + 1, h2.data(), -1.0, r));
+      for (int i = 0; i <= j; i++) H[(size_t)i * ldh + j] = h1[i] + h2[i];
+      double hh = 0.0;
+      BIT_TRY(dotv(r, 1, r, &hh));
+      const double hn = std::sqrt(hh);
+      H[(size_t)(j + 1) * ldh + j] = hn;
+      if (hn > 0)
+        HIP_TRY(bit::dev::vec_op(n, 3, 1.0 / hn, 0.0, nullptr, r, nullptr,
+                                 V + (size_t)(j + 1) * n, st));
+      bit::detail::givens_column(H, ldh, j, cs, sn, g);
+      k = j + 1;
+      res.relres = std::fabs(g[j + 1]) / bnorm;
+      if (hn == 0.0) res.breakdown = true;  // the Krylov space is invariant: exact
+      if (res.relres <= tol || hn == 0.0) break;
+    }
+    bit::detail::back_solve(H, ldh, g, k, y);
+    BIT_TRY(combine(k, y.data(), 1.0, dx.as<double>()));
+    res.restarts++;
+  }
+  HIP_TRY(hipMemcpyAsync(sigma, dx.p, nb, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (iters) *iters = res.iters;
+  if (relres) *relres = res.relres;
   return BIT_OK;
 }
 

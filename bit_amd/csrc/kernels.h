@@ -144,6 +144,18 @@ hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nt
 hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool fp32,
                       bool screened, hipStream_t s);
 
+// Krylov vectors (FP64): out[j] = V_j . w for j < ncols (deterministic order;
+// partial holds kDotBlocks * ncols doubles), y += sign * sum_j c[j] V_j, and the
+// elementwise steps of vec_op: 0 y = u v; 1 y = (a + b u) v + z; 2 y = u - v;
+// 3 y = a v.
+constexpr int kDotBlocks = 256;
+hipError_t dots(int n, const double* V, size_t ld, int ncols, const double* w, double* partial,
+                double* out, hipStream_t s);
+hipError_t combine_cols(int n, const double* V, size_t ld, int ncols, const double* c,
+                        double sign, double* y, hipStream_t s);
+hipError_t vec_op(int n, int op, double a, double b, const double* u, const double* v,
+                  const double* z, double* y, hipStream_t s);
+
 // field (Coulomb, regularized; korder >= 1 adds the terms l = 1..korder of the
 // regularized family): E_far sorted [3n], then near + scatter [3n]
 hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, int korder,

@@ -1333,6 +1333,90 @@ hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, const 
 }
 
 // ---------------------------------------------------------------------------
+// Krylov vectors of the GMRES solve (FP64, resident in HBM).  Dot products are
+// deterministic: kDotBlocks blocks each sum one fixed contiguous slice in a
+// fixed tree order, then one fixed-order sum over the blocks.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dots(int n, const double* __restrict__ V, size_t ld,
+                                              const double* __restrict__ w,
+                                              double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int b = blockIdx.x, j = blockIdx.y;
+  const int per = (n + kDotBlocks - 1) / kDotBlocks;
+  const int i0 = min(n, b * per), i1 = min(n, i0 + per);
+  const double* v = V + (size_t)j * ld;
+  double s = 0.0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) s = fma(v[i], w[i], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)j * kDotBlocks + b] = red[0];
+}
+
+__global__ __launch_bounds__(kDotBlocks) void k_dots_final(const double* __restrict__ partial,
+                                                           double* __restrict__ out) {
+  __shared__ double red[kDotBlocks];
+  const int j = blockIdx.x;
+  red[threadIdx.x] = partial[(size_t)j * kDotBlocks + threadIdx.x];
+  __syncthreads();
+  for (int k = kDotBlocks / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+hipError_t dots(int n, const double* V, size_t ld, int ncols, const double* w, double* partial,
+                double* out, hipStream_t s) {
+  if (ncols < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, ncols), dim3(256), 0, s, n, V, ld, w, partial);
+  hipLaunchKernelGGL(k_dots_final, dim3(ncols), dim3(kDotBlocks), 0, s, partial, out);
+  return hipGetLastError();
+}
+
+// y += sign * sum_{j < ncols} c[j] V_j, j ascending.
+__global__ void k_combine_cols(int n, const double* __restrict__ V, size_t ld, int ncols,
+                               const double* __restrict__ c, double sign, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = y[i];
+  for (int j = 0; j < ncols; j++) acc = fma(sign * c[j], V[(size_t)j * ld + i], acc);
+  y[i] = acc;
+}
+
+hipError_t combine_cols(int n, const double* V, size_t ld, int ncols, const double* c,
+                        double sign, double* y, hipStream_t s) {
+  if (ncols < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_combine_cols, dim3((n + 255) / 256), dim3(256), 0, s, n, V, ld, ncols, c,
+                     sign, y);
+  return hipGetLastError();
+}
+
+// Elementwise steps of the second-kind operator y = (diag + self w) v + phi(w v)
+// and of the Arnoldi process.
+__global__ void k_vec(int n, int op, double a, double b, const double* __restrict__ u,
+                      const double* __restrict__ v, const double* __restrict__ z,
+                      double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  switch (op) {
+    case 0: y[i] = u[i] * v[i]; break;                   // charges w v
+    case 1: y[i] = (a + b * u[i]) * v[i] + z[i]; break;  // (diag + self w) v + phi
+    case 2: y[i] = u[i] - v[i]; break;                   // residual b - A x
+    default: y[i] = v[i] * a; break;                     // scaled copy
+  }
+}
+
+hipError_t vec_op(int n, int op, double a, double b, const double* u, const double* v,
+                  const double* z, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec, dim3((n + 255) / 256), dim3(256), 0, s, n, op, a, b, u, v, z, y);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // FP64 direct sum (shielded_pot_particle_ds, tree3d.cpp:1966-2016): 256 targets
 // per block, sources split into nsplit slices whose partial sums are added in
 // slice order.

@@ -329,14 +329,19 @@ def main():
     ap.add_argument("--builders", type=int, default=8)
     ap.add_argument("--hbm-budget-gb", type=float, default=16.0)
     ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--warmup-requests", type=int, default=64,
+                    help="untimed stream (seed + 1) served first")
     a = ap.parse_args()
     reqs = make_requests(a.requests, a.rate, a.seed)
     srv = Server(hbm_budget=int(a.hbm_budget_gb * (1 << 30)), builders=a.builders)
+    if a.warmup_requests > 0:
+        srv.run(make_requests(a.warmup_requests, a.rate, a.seed + 1))
     m = srv.run(reqs)
     srv.close()
     errors = sum("error" in r.extra for r in reqs)
     line = {"workload": "serving stream", "requests": a.requests, "rate_per_s": a.rate,
-            "seed": a.seed, "server": m, "errors": errors}
+            "seed": a.seed, "warmup_requests": a.warmup_requests, "server": m,
+            "errors": errors}
     if not a.no_baseline:
         base = make_requests(a.requests, a.rate, a.seed)
         line["one_at_a_time"] = run_one_at_a_time(base)

@@ -44,3 +44,39 @@ def test_fp32_matvec_reaches_tolerance(ctx):
     r = bie.solve_sphere(ctx, level=4, kappa=1.0, nterms=7, acc=0.6, restart=50, tol=1e-6,
                          lmax=8, seed=6, maxmemb=200, precision="fp32")
     assert r["relres"] <= 1e-6
+
+
+def test_resident_krylov_equals_host_gmres(ctx):
+    """The device-resident solve (CGS2 Arnoldi in HBM) against the host GMRES
+    core driven through the callback entry point with the same treecode
+    operator; restart 4 forces several restarts."""
+    V, F = bie.icosphere(3)
+    pts, area, h = bie.collocation(V, F)
+    w = area / (4 * np.pi)
+    kappa, d2 = 1.0, h * h
+    self_coef = float(np.exp(-kappa * h) / h)
+    f = bie.random_rhs(6, seed=5)(pts)
+    s = bit_amd.System(ctx, pts[:, 0], pts[:, 1], pts[:, 2], w, 40)
+    args = (10, 0.3, kappa, d2, None, "fp64")
+
+    def op(v):
+        s.set_charges(w * v)
+        return (0.5 + self_coef * w) * v + s.potential(*args)
+
+    x0 = np.linspace(-1.0, 1.0, len(f))
+    sig, it, rr = s.solve_second_kind(w, f, 0.5, self_coef, x0, 4, 1e-10, 400, *args)
+    ref, it_h, rr_h = bit_amd.gmres(len(f), op, f, x0, restart=4, tol=1e-10, maxit=400)
+    assert rr <= 1e-10 and rr_h <= 1e-10
+    assert abs(it - it_h) <= 2, (it, it_h)
+    assert np.linalg.norm(sig - ref) / np.linalg.norm(ref) < 1e-8
+    # the true residual of the device solution
+    res = np.linalg.norm(op(sig) - f) / np.linalg.norm(f)
+    assert res < 2e-10
+    # zero right-hand side and non-finite inputs
+    z, it0, rr0 = s.solve_second_kind(w, np.zeros_like(f), 0.5, self_coef, None, 4, 1e-10, 10,
+                                      *args)
+    assert it0 == 0 and rr0 == 0.0 and not z.any()
+    bad = f.copy()
+    bad[3] = np.nan
+    with pytest.raises(bit_amd.BitError):
+        s.solve_second_kind(w, bad, 0.5, self_coef, None, 4, 1e-10, 10, *args)

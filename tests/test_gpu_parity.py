@@ -26,10 +26,17 @@ def _system(ctx, fx):
     return bit_amd.System(ctx, fx["x"], fx["y"], fx["z"], fx["q"], int(fx["maxmemb"]))
 
 
+@pytest.mark.parametrize("src", ["host", "device"])
 @pytest.mark.parametrize("name", FIXTURES)
-def test_system_tree_bit_exact(ctx, name):
+def test_system_tree_bit_exact(ctx, name, src):
+    """Host inputs of this size take the host build, device inputs the device
+    build: both give the reference's tree."""
     fx = load_fixture(name)
-    s = _system(ctx, fx)
+    if src == "host":
+        s = _system(ctx, fx)
+    else:
+        dev = [bit_amd.DeviceArray.from_numpy(ctx, fx[k]) for k in ("x", "y", "z", "q")]
+        s = bit_amd.System(ctx, *dev, int(fx["maxmemb"]))
     t = s.export_tree()
     assert np.array_equal(t.children, fx["children"])
     assert np.array_equal(t.boxes.view(np.uint64), fx["boxes"].view(np.uint64))
