@@ -77,10 +77,30 @@ inline constexpr Tab<NT> kTab{};
 //   a_k = fac [ sum_e (2 x_e a_{k-e} - a_{k-2e}) - V/k_d
 //               + (kappa/k_d)(x_d b_{k-e_d} - b_{k-2e_d}) ]
 // b holds the coefficients of exp(-kappa R); a those of exp(-kappa R)/R.
+// The direction-d term of the sum is 2 x_d a_{k-e_d} - a_{k-2e_d} =
+// 2V + a_{k-2e_d}, so the bracket is evaluated as
+//   S' + a_{k-2e_d} + (2 - 1/k_d) V + (kappa/k_d) W,   S' = sum over e != d,
+// one operation fewer per coefficient than summing all e and subtracting V/k_d.
 // Each coefficient is folded into the result as soon as it exists, so only
 // the last two degree shells stay live in registers:
 //   GRAD = false: acc0 += M_k a_k                      (potential)
 //   GRAD = true:  acc_e += k_e M_{k-e} a_k, e = x,y,z   (field, M of degree < NT-1)
+template <typename T, int NT, int C, int E, int D>
+__device__ __forceinline__ void add_other(const T* a, const T (&x2)[3], T& S, bool& has) {
+  constexpr const Tab<NT>& tb = kTab<NT>;
+  if constexpr (E != D) {
+    constexpr int m1 = tb.m1[C][E], m2 = tb.m2[C][E];
+    if constexpr (m1 >= 0) {
+      S = has ? fma(x2[E], a[m1], S) : x2[E] * a[m1];
+      has = true;
+    }
+    if constexpr (m2 >= 0) {
+      S = has ? S - a[m2] : -a[m2];
+      has = true;
+    }
+  }
+}
+
 template <typename T, int NT, bool SCR, bool GRAD, int C, typename MP>
 __device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (&x2)[3],
                                           T fac, const T* kinv, MP M, T (&acc)[3]) {
@@ -88,27 +108,21 @@ __device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (
   constexpr int d = tb.d[C];
   constexpr int kd = tb.k[C][d];
   constexpr int e0 = tb.m1[C][0], e1 = tb.m1[C][1], e2 = tb.m1[C][2];
-  constexpr int f0 = tb.m2[C][0], f1 = tb.m2[C][1], f2 = tb.m2[C][2];
   constexpr int md1 = tb.m1[C][d], md2 = tb.m2[C][d];
-  T S;
-  if constexpr (e0 >= 0) {
-    S = x2[0] * a[e0];
-    if constexpr (e1 >= 0) S = fma(x2[1], a[e1], S);
-    if constexpr (e2 >= 0) S = fma(x2[2], a[e2], S);
-  } else if constexpr (e1 >= 0) {
-    S = x2[1] * a[e1];
-    if constexpr (e2 >= 0) S = fma(x2[2], a[e2], S);
-  } else {
-    S = x2[2] * a[e2];
+  T S = T(0);
+  bool has = false;  // resolved at compile time
+  add_other<T, NT, C, 0, d>(a, x2, S, has);
+  add_other<T, NT, C, 1, d>(a, x2, S, has);
+  add_other<T, NT, C, 2, d>(a, x2, S, has);
+  if constexpr (md2 >= 0) {
+    S = has ? S + a[md2] : a[md2];
+    has = true;
   }
-  if constexpr (f0 >= 0) S -= a[f0];
-  if constexpr (f1 >= 0) S -= a[f1];
-  if constexpr (f2 >= 0) S -= a[f2];
   T V;
   if constexpr (md2 >= 0) V = fma(x[d], a[md1], -a[md2]);
   else V = x[d] * a[md1];
-  constexpr double invk = 1.0 / kd;
-  T t = fma(T(-invk), V, S);
+  constexpr double ck = 2.0 - 1.0 / kd;
+  T t = has ? fma(T(ck), V, S) : T(ck) * V;
   if constexpr (SCR) {
     T W;
     if constexpr (md2 >= 0) W = fma(x[d], b[md1], -b[md2]);

@@ -113,11 +113,16 @@ def solve_alone(ctx, r: Request, precision="fp32"):
     x, y, z, q, w = _particles(r)
     s = bit_amd.System(ctx, x, y, z, q, r.maxmemb)
     try:
+        args = (r.nterms, r.acc, r.kappa, r.delta ** 2, None, precision)
         if r.kind == "eval":
-            return s.potential(r.nterms, r.acc, r.kappa, r.delta ** 2, None, precision)
-        sig, it, rr = s.solve_second_kind(w, r.extra["f"], 0.5, r.extra["self"], None, 50, r.tol,
-                                          500, r.nterms, r.acc, r.kappa, r.delta ** 2, None,
-                                          precision)
+            return s.potential(*args)
+        f, selfc = r.extra["f"], r.extra["self"]
+
+        def matvec(v):  # the server's operator and GMRES core, one system alone
+            s.set_charges(w * v)
+            return (0.5 + selfc * w) * v + s.potential(*args)
+
+        sig, it, rr = bit_amd.gmres(s.n, matvec, f, None, 50, r.tol, 500)
         return sig
     finally:
         s.close()
