@@ -71,20 +71,23 @@ struct Tab {
 template <int NT>
 inline constexpr Tab<NT> kTab{};
 
-// One coefficient of the recurrence (scaled variables, see far_eval):
-//   V   = x_d a_{k-e_d} - a_{k-2e_d}
-//   b_k = (kappa/k_d) V
-//   a_k = fac [ sum_e (2 x_e a_{k-e} - a_{k-2e}) - V/k_d
-//               + (kappa/k_d)(x_d b_{k-e_d} - b_{k-2e_d}) ]
-// b holds the coefficients of exp(-kappa R); a those of exp(-kappa R)/R.
-// The direction-d term of the sum is 2 x_d a_{k-e_d} - a_{k-2e_d} =
-// 2V + a_{k-2e_d}, so the bracket is evaluated as
-//   S' + a_{k-2e_d} + (2 - 1/k_d) V + (kappa/k_d) W,   S' = sum over e != d,
-// one operation fewer per coefficient than summing all e and subtracting V/k_d.
-// Each coefficient is folded into the result as soon as it exists, so only
-// the last two degree shells stay live in registers:
-//   GRAD = false: acc0 += M_k a_k                      (potential)
-//   GRAD = true:  acc_e += k_e M_{k-e} a_k, e = x,y,z   (field, M of degree < NT-1)
+// One coefficient of the recurrence in normalised form.  With R the target's
+// (regularized) distance to the cluster centre, u = (x - centre)/R and
+// lam = kappa R, the Taylor coefficients of exp(-kappa R)/R in the source
+// offset (in cluster units s) are a_k = rho^{|k|+1} c_k, rho = s/R, and those of
+// exp(-kappa R) are b_k = rho^{|k|} beta_k, where
+//   V   = u_d c_{k-e_d} - c_{k-2e_d}
+//   beta_k = (lam/k_d) V
+//   c_k = sum_e (2 u_e c_{k-e} - c_{k-2e}) - V/k_d
+//         + (lam/k_d)(u_d beta_{k-e_d} - beta_{k-2e_d}),
+// the recurrence of the unnormalised a, b (x -> u, kappa s -> lam, no 1/R^2
+// factor per coefficient).  The direction-d term of the sum is
+// 2 u_d c_{k-e_d} - c_{k-2e_d} = 2V + c_{k-2e_d}, so the bracket is evaluated as
+//   S' + c_{k-2e_d} + (2 - 1/k_d) V + (lam/k_d) W,   S' = sum over e != d.
+// Each coefficient is folded into a per-degree-shell sum as soon as it
+// exists, and each finished shell n into the total with rho^{n+1}:
+//   GRAD = false: sh0 += M_k c_k                      (potential)
+//   GRAD = true:  sh_e += k_e M_{k-e} c_k, e = x,y,z   (field, M of degree < NT-1)
 template <typename T, int NT, int C, int E, int D>
 __device__ __forceinline__ void add_other(const T* a, const T (&x2)[3], T& S, bool& has) {
   constexpr const Tab<NT>& tb = kTab<NT>;
@@ -101,14 +104,24 @@ __device__ __forceinline__ void add_other(const T* a, const T (&x2)[3], T& S, bo
   }
 }
 
+template <typename T>
+struct Shells {
+  T sh[3];   // current shell's partial sums
+  T tot[3];  // folded shells
+  T pw;      // rho^{n+1} of the last folded shell n
+  T rho;
+};
+
 template <typename T, int NT, bool SCR, bool GRAD, int C, typename MP>
 __device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (&x2)[3],
-                                          T fac, const T* kinv, MP M, T (&acc)[3]) {
+                                          const T* kinv, MP M, Shells<T>& z) {
   constexpr const Tab<NT>& tb = kTab<NT>;
   constexpr int d = tb.d[C];
   constexpr int kd = tb.k[C][d];
   constexpr int e0 = tb.m1[C][0], e1 = tb.m1[C][1], e2 = tb.m1[C][2];
   constexpr int md1 = tb.m1[C][d], md2 = tb.m2[C][d];
+  constexpr int deg = tb.k[C][0] + tb.k[C][1] + tb.k[C][2];
+  constexpr bool first = C == ncoef(deg), last = C + 1 == ncoef(deg + 1);
   T S = T(0);
   bool has = false;  // resolved at compile time
   add_other<T, NT, C, 0, d>(a, x2, S, has);
@@ -130,23 +143,33 @@ __device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (
     t = fma(kinv[kd], W, t);
     b[C] = kinv[kd] * V;
   }
-  a[C] = fac * t;
+  a[C] = t;
   if constexpr (!GRAD) {
-    acc[0] = fma(M[C], a[C], acc[0]);
+    z.sh[0] = first ? M[C] * t : fma(M[C], t, z.sh[0]);
   } else {
     // the integer factor multiplies the per-lane coefficient, so the moment
     // stays a scalar operand of the FMA (no per-lane copies of M)
-    if constexpr (e0 >= 0) acc[0] = fma(M[e0], tb.k[C][0] == 1 ? a[C] : T(tb.k[C][0]) * a[C], acc[0]);
-    if constexpr (e1 >= 0) acc[1] = fma(M[e1], tb.k[C][1] == 1 ? a[C] : T(tb.k[C][1]) * a[C], acc[1]);
-    if constexpr (e2 >= 0) acc[2] = fma(M[e2], tb.k[C][2] == 1 ? a[C] : T(tb.k[C][2]) * a[C], acc[2]);
+    constexpr int k0 = tb.k[C][0], k1 = tb.k[C][1], k2 = tb.k[C][2];
+    if constexpr (first) z.sh[0] = z.sh[1] = z.sh[2] = T(0);
+    if constexpr (e0 >= 0) z.sh[0] = fma(M[e0], k0 == 1 ? t : T(k0) * t, z.sh[0]);
+    if constexpr (e1 >= 0) z.sh[1] = fma(M[e1], k1 == 1 ? t : T(k1) * t, z.sh[1]);
+    if constexpr (e2 >= 0) z.sh[2] = fma(M[e2], k2 == 1 ? t : T(k2) * t, z.sh[2]);
+  }
+  if constexpr (last) {
+    z.pw *= z.rho;
+    z.tot[0] = fma(z.sh[0], z.pw, z.tot[0]);
+    if constexpr (GRAD) {
+      z.tot[1] = fma(z.sh[1], z.pw, z.tot[1]);
+      z.tot[2] = fma(z.sh[2], z.pw, z.tot[2]);
+    }
   }
 }
 
 template <typename T, int NT, bool SCR, bool GRAD, typename MP, int... C>
 __device__ __forceinline__ void coef_all(std::integer_sequence<int, C...>, T* a, T* b,
-                                         const T (&x)[3], const T (&x2)[3], T fac,
-                                         const T* kinv, MP M, T (&acc)[3]) {
-  (coef_step<T, NT, SCR, GRAD, C + 1, MP>(a, b, x, x2, fac, kinv, M, acc), ...);
+                                         const T (&x)[3], const T (&x2)[3], const T* kinv, MP M,
+                                         Shells<T>& z) {
+  (coef_step<T, NT, SCR, GRAD, C + 1, MP>(a, b, x, x2, kinv, M, z), ...);
 }
 
 // Pointers read from a descriptor are generic (flat) to the compiler; loads
@@ -159,47 +182,49 @@ __device__ __forceinline__ gptr<T> as_global(const T* p) {
   return (gptr<T>)(p);
 }
 
-// Far field of one cluster at one target, in units of the cluster scale s:
-// x = (target - centre)/s, fac = s^2/R^2, a0 = exp(-kh Rh)/Rh, b0 = exp(-kh Rh),
-// kh = kappa s.  Returns sum_k Mhat_k ahat_k (times 1/s = the potential).
+// Far field of one cluster at one target in the normalised variables of
+// coef_step: u = (target - centre)/R, rho = s/R, c0 = exp(-lam), lam = kappa R.
+// Returns sum_k Mhat_k ahat_k with ahat_k = rho^{|k|+1} c_k (times 1/s = the
+// potential).
 template <typename T, int NT, bool SCR, typename MP>
-__device__ __forceinline__ T far_eval(T x0, T x1, T x2_, T fac, T a0, T b0, T kh, MP M) {
+__device__ __forceinline__ T far_eval(T u0, T u1, T u2, T rho, T c0, T lam, MP M) {
   constexpr int NC = ncoef(NT);
   T a[NC];
   T b[SCR ? NC : 1];
-  a[0] = a0;
-  if constexpr (SCR) b[0] = b0;
+  a[0] = c0;
+  if constexpr (SCR) b[0] = c0;
   T kinv[NT > 1 ? NT : 2];
   if constexpr (SCR) {
 #pragma unroll
-    for (int m = 1; m < NT; m++) kinv[m] = kh * T(1.0 / m);
+    for (int m = 1; m < NT; m++) kinv[m] = lam * T(1.0 / m);
   }
-  const T x[3] = {x0, x1, x2_};
-  const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
-  T acc[3] = {M[0] * a0, T(0), T(0)};
+  const T x[3] = {u0, u1, u2};
+  const T xx[3] = {u0 + u0, u1 + u1, u2 + u2};
+  Shells<T> z{{T(0), T(0), T(0)}, {M[0] * c0 * rho, T(0), T(0)}, rho, rho};
   if constexpr (NC > 1)
-    coef_all<T, NT, SCR, false, MP>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, fac,
-                                    kinv, M, acc);
-  return acc[0];
+    coef_all<T, NT, SCR, false, MP>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, kinv,
+                                    M, z);
+  return z.tot[0];
 }
 
-// Field: E_e = sum_k (k_e+1) M_k a_{k+e}, coefficients to degree NT (Coulomb).
+// Field: E_e = sum_k (k_e+1) M_k a_{k+e}, coefficients to degree NT (Coulomb),
+// in the normalised variables of far_eval (c0 = 1).
 template <typename T, int NT, typename MP>
-__device__ __forceinline__ void far_eval_grad(T x0, T x1, T x2_, T fac, T a0, MP M, T& ex, T& ey,
+__device__ __forceinline__ void far_eval_grad(T u0, T u1, T u2, T rho, MP M, T& ex, T& ey,
                                               T& ez) {
   constexpr int NC = ncoef(NT + 1);
   T a[NC];
   T b[1];
-  a[0] = a0;
-  const T x[3] = {x0, x1, x2_};
-  const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
+  a[0] = T(1);
+  const T x[3] = {u0, u1, u2};
+  const T xx[3] = {u0 + u0, u1 + u1, u2 + u2};
   T kinv[2] = {T(0), T(0)};
-  T acc[3] = {T(0), T(0), T(0)};
+  Shells<T> z{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, rho, rho};
   coef_all<T, NT + 1, false, true, MP>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx,
-                                       fac, kinv, M, acc);
-  ex = acc[0];
-  ey = acc[1];
-  ez = acc[2];
+                                       kinv, M, z);
+  ex = z.tot[0];
+  ey = z.tot[1];
+  ez = z.tot[2];
 }
 
 // Term l >= 1 of the regularized field family, R^-(2l+1) (tree3d.cpp:614-791):
@@ -947,18 +972,15 @@ __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
     if ((mask >> lane) & 1ull) {
       const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
       const double s = scale[node];
-      const double inv = 1.0 / s;
       const double dx = px - cx, dy = py - cy, dz = pz - cz;
-      const double R2 = dx * dx + dy * dy + dz * dz + delta2;
-      const double R2h = R2 * inv * inv;
-      const double Rh = sqrt(R2h);
-      const double kh = kappa * s;
-      const double b0 = SCR ? exp(-kh * Rh) : 1.0;
-      const double a0 = b0 / Rh;
-      T v = far_eval<T, NT, SCR, gptr<T>>((T)(dx * inv), (T)(dy * inv), (T)(dz * inv),
-                                          (T)(1.0 / R2h), (T)a0, (T)b0, (T)kh,
+      const double R = sqrt(dx * dx + dy * dy + dz * dz + delta2);
+      const double rinv = 1.0 / R;
+      const double lam = kappa * R;
+      const double c0 = SCR ? exp(-lam) : 1.0;
+      T v = far_eval<T, NT, SCR, gptr<T>>((T)(dx * rinv), (T)(dy * rinv), (T)(dz * rinv),
+                                          (T)(s * rinv), (T)c0, (T)lam,
                                           mom_hat + (size_t)node * NC);
-      acc.add(v * (T)inv);
+      acc.add(v * (T)(1.0 / s));
     }
   }
   if (valid) d.phi_far[t] = acc.value();
@@ -1203,9 +1225,9 @@ __global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2, int
       const double inv = 1.0 / s;
       const double dx = px - c.x, dy = py - c.y, dz = pz - c.z;
       const double R2h = (dx * dx + dy * dy + dz * dz + delta2) * inv * inv;
-      const double a0 = 1.0 / sqrt(R2h);
+      const double a0 = 1.0 / sqrt(R2h);  // s/R
       double gx, gy, gz;
-      far_eval_grad<double, NT, gptr<double>>(dx * inv, dy * inv, dz * inv, 1.0 / R2h, a0,
+      far_eval_grad<double, NT, gptr<double>>(dx * inv * a0, dy * inv * a0, dz * inv * a0, a0,
                                                as_global(mom_hat) + (size_t)node * NC, gx, gy,
                                                gz);
       if constexpr (FAM) {
