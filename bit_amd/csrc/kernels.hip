@@ -946,14 +946,21 @@ __device__ __forceinline__ const double* mom_of<double>(const EvalDesc& d) {
 }
 
 template <typename T, int NT, bool SCR>
+// The cluster's moments are staged in LDS: the wave loads the next item's
+// moments from HBM into registers while it evaluates the current item, so the
+// moment operands are LDS broadcasts (VGPR operands) instead of scalar loads
+// issued, and waited for, just before use.
 __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
                                             const int2* __restrict__ map) {
   constexpr int NC = ncoef(NT);
+  constexpr int PF = (NC + kBatch - 1) / kBatch;  // moments per lane per item
+  __shared__ T mbuf[2][PF * kBatch];
   int sys, b;
   which(map, sys, b);
   const EvalDesc& d = descs[sys];
   const TreeView& tv = d.tv;
   const double kappa = d.kappa, delta2 = d.delta2;
+  // global (not flat) loads: flat loads also count on the LDS wait counter
   const gptr<T> mom_hat = as_global(mom_of<T>(d));
   const gptr<Item> items = as_global(d.far_items);
   const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
@@ -966,9 +973,25 @@ __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
   const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   Acc<T> acc;
   const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
+  T pf[PF];
+  auto fetch = [&](int node) {
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+      const int c = j * kBatch + lane;
+      pf[j] = c < NC ? mom_hat[(size_t)node * NC + c] : T(0);
+    }
+  };
+  if (i0 < i1) fetch(uniform(items[i0].node));
+  int buf = 0;
   for (int it = i0; it < i1; it++) {
     const unsigned long long mask = uniform64(items[it].mask);
     const int node = uniform(items[it].node);
+#pragma unroll
+    for (int j = 0; j < PF; j++) mbuf[buf][j * kBatch + lane] = pf[j];
+    __syncthreads();
+    if (it + 1 < i1) fetch(uniform(items[it + 1].node));
+    const T* M = mbuf[buf];
+    buf ^= 1;
     if ((mask >> lane) & 1ull) {
       const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
       const double s = scale[node];
@@ -977,9 +1000,8 @@ __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
       const double rinv = 1.0 / R;
       const double lam = kappa * R;
       const double c0 = SCR ? exp(-lam) : 1.0;
-      T v = far_eval<T, NT, SCR, gptr<T>>((T)(dx * rinv), (T)(dy * rinv), (T)(dz * rinv),
-                                          (T)(s * rinv), (T)c0, (T)lam,
-                                          mom_hat + (size_t)node * NC);
+      T v = far_eval<T, NT, SCR, const T*>((T)(dx * rinv), (T)(dy * rinv), (T)(dz * rinv),
+                                           (T)(s * rinv), (T)c0, (T)lam, M);
       acc.add(v * (T)(1.0 / s));
     }
   }
