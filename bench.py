@@ -94,8 +94,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--nterms", type=int, default=11)
-    ap.add_argument("--acc", type=float, default=0.2)
+    ap.add_argument("--nterms", type=int, default=12)
+    ap.add_argument("--acc", type=float, default=0.25)
     ap.add_argument("--maxmemb", type=int, default=1000)
     ap.add_argument("--kappa", type=float, default=0.5)
     ap.add_argument("--delta", type=float, default=0.005)

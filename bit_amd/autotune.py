@@ -24,7 +24,7 @@ import bit_amd
 from bit_amd import workloads
 
 DEFAULT_N0 = (500, 750, 1000, 1500, 2000)
-DEFAULT_NTERMS = (9, 10, 11, 12)
+DEFAULT_NTERMS = (9, 10, 11, 12, 13)
 DEFAULT_ACC = (0.4, 0.35, 0.3, 0.25, 0.2, 0.15)
 
 
