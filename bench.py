@@ -40,7 +40,7 @@ PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 # matches the profiled one.
 MEASURED_TRAFFIC = {
     # (n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
-    (4_000_000, 3, 11, 0.2, 1000, 0.5, 0.005, "fp32"): 1.005083e9,
+    (4_000_000, 3, 12, 0.25, 1000, 0.5, 0.005, "fp32"): 8.46324e8,
 }
 
 
