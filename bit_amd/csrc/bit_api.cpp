@@ -155,6 +155,9 @@ struct bit_system {
   int mom_nterms = 0;
   int nchunks = 0;
   DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, mom_hat32;
+  // internal nodes grouped by height (leaves 0): up_off[h-1]..up_off[h] in up_nodes
+  DevBuf up_nodes;
+  std::vector<int> up_off;
   // lists: pl for the particles as targets (cached per MAC parameters), ql for
   // the last external point set
   ListSet pl, ql;
@@ -831,23 +834,43 @@ int bit_moments(bit_system* s, int nterms) {
   // call (e.g. BIT_E_OOM) leaves the system as it was, never half-initialised.
   s->mom_nterms = 0;
   if (s->nchunks == 0) {
-    std::vector<int> cn, c0(nt), ck(nt), cr;
+    // leaves: sums over their particles in chunks; internal nodes: upward pass
+    // from their children, height by height (a child's index exceeds its
+    // parent's, so one backward sweep gives every height)
+    std::vector<int> cn, c0(nt), ck(nt), cr, height(nt, 0);
+    int hmax = 0;
+    for (int i = nt - 1; i >= 0; i--)
+      for (int q = 0; q < 8; q++) {
+        const int ch = T.children[(size_t)i * 8 + q];
+        if (ch >= 0) height[i] = std::max(height[i], height[ch] + 1);
+      }
     for (int i = 0; i < nt; i++) {
       c0[i] = (int)cn.size();
-      int b = T.begin[i], e = T.end[i];
       int k = 0;
-      for (int x = b; x < e; x += kMomChunk, k++) {
-        cn.push_back(i);
-        cr.push_back(x);
-        cr.push_back(std::min(e, x + kMomChunk));
+      if (height[i] == 0) {
+        int b = T.begin[i], e = T.end[i];
+        for (int x = b; x < e; x += kMomChunk, k++) {
+          cn.push_back(i);
+          cr.push_back(x);
+          cr.push_back(std::min(e, x + kMomChunk));
+        }
       }
       ck[i] = k;
+      hmax = std::max(hmax, height[i]);
+    }
+    std::vector<int> up, off(1, 0);
+    for (int h = 1; h <= hmax; h++) {
+      for (int i = 0; i < nt; i++)
+        if (height[i] == h) up.push_back(i);
+      off.push_back((int)up.size());
     }
     BIT_TRY(upload(c, s->chunk_node, cn.data(), cn.size() * sizeof(int)));
     BIT_TRY(upload(c, s->chunk_rng, cr.data(), cr.size() * sizeof(int)));
     BIT_TRY(upload(c, s->node_chunk0, c0.data(), c0.size() * sizeof(int)));
     BIT_TRY(upload(c, s->node_nchunk, ck.data(), ck.size() * sizeof(int)));
+    if (!up.empty()) BIT_TRY(upload(c, s->up_nodes, up.data(), up.size() * sizeof(int)));
     HIP_TRY(hipStreamSynchronize(c->stream));  // host tables die at scope end
+    s->up_off = off;
     s->nchunks = (int)cn.size();
   }
   BIT_TRY(s->partial.ensure(c, (size_t)s->nchunks * nc * sizeof(double)));
@@ -862,6 +885,11 @@ int bit_moments(bit_system* s, int nterms) {
                                    s->node_nchunk.as<int>(), s->partial.as<double>(),
                                    s->mom.as<double>(), s->mom_hat64.as<double>(),
                                    s->mom_hat32.as<float>(), c->stream));
+  for (size_t h = 1; h < s->up_off.size(); h++)
+    HIP_TRY(bit::dev::moments_upward(s->view(), nterms, s->up_off[h] - s->up_off[h - 1],
+                                     s->up_nodes.as<int>() + s->up_off[h - 1],
+                                     s->mom.as<double>(), s->mom_hat64.as<double>(),
+                                     s->mom_hat32.as<float>(), c->stream));
   s->t_mom.stop(c->stream);
   s->mom_nterms = nterms;
   s->stats.ms_moments = -1.0;

@@ -107,6 +107,10 @@ hipError_t leaf_local(const TreeView& tv, const int* leaves, int nleaves, float4
 
 hipError_t moments_partial(const TreeView& tv, int nterms, int nchunks, const int* chunk_node,
                            const int2* chunk_rng, double* partial, hipStream_t s);
+// Upward pass: moments of `nodes` (internal nodes whose children are done)
+// from their children's raw moments; writes raw and scaled copies.
+hipError_t moments_upward(const TreeView& tv, int nterms, int nnodes, const int* nodes,
+                          double* mom, double* mom_hat64, float* mom_hat32, hipStream_t s);
 hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0,
                           const int* node_nchunk, const double* partial, double* mom,
                           double* mom_hat64, float* mom_hat32, hipStream_t s);

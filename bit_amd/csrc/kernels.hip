@@ -771,6 +771,96 @@ hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0
 }
 
 // ---------------------------------------------------------------------------
+// Upward pass for the internal nodes of one height: node P's moments about its
+// centre are the sum over its children C (stored order) of C's moments
+// shifted by h = centre_C - centre_P, from the binomial expansion
+//   (t + h)^i = sum_{a<=i} C(i,a) h^{i-a} t^a,
+// applied one axis at a time in LDS.  Mathematically the same sums as the
+// per-particle moments (tree3d.cpp:2631-2711), at a cost per child instead of
+// per particle and level.
+// ---------------------------------------------------------------------------
+constexpr int kMaxCoef = ncoef(kMaxNterms);
+
+__device__ __forceinline__ double shift_1d(const double* src, int e, int i, int j, int k,
+                                           double h) {
+  // sum over a <= idx_e of C(idx_e, a) h^{idx_e - a} src[idx with idx_e = a]
+  int v[3] = {i, j, k};
+  const int n = v[e];
+  double binom = 1.0, hp = 1.0, sum = 0.0;
+  for (int a = n; a >= 0; a--) {
+    v[e] = a;
+    sum = fma(binom * hp, src[cidx(v[0], v[1], v[2])], sum);
+    binom = binom * a / (n - a + 1);
+    hp *= h;
+  }
+  return sum;
+}
+
+__global__ __launch_bounds__(256) void k_m2m(int nterms, const int* __restrict__ nodes,
+                                             const int* __restrict__ child,
+                                             const double4* __restrict__ center,
+                                             const double* __restrict__ scale,
+                                             double* __restrict__ mom,
+                                             double* __restrict__ mom_hat64,
+                                             float* __restrict__ mom_hat32) {
+  __shared__ double A[kMaxCoef], B[kMaxCoef];
+  const int P = nodes[blockIdx.x];
+  const int nc = ncoef(nterms);
+  const int tid = threadIdx.x;
+  const double4 cp = center[P];
+  int ci[2][3];
+  int cnt = 0;
+  for (int cc = tid; cc < nc && cnt < 2; cc += 256, cnt++) {
+    int n = 0;
+    while (ncoef(n + 1) <= cc) n++;
+    int rem = cc - ncoef(n);
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= rem) m++;
+    int k = rem - m * (m + 1) / 2;
+    ci[cnt][0] = n - m;
+    ci[cnt][1] = m - k;
+    ci[cnt][2] = k;
+  }
+  double acc[2] = {0.0, 0.0};
+  for (int q = 0; q < 8; q++) {
+    const int C = child[8 * P + q];
+    if (C < 0) continue;
+    const double4 cc = center[C];
+    const double hx = cc.x - cp.x, hy = cc.y - cp.y, hz = cc.z - cp.z;
+    for (int c = tid; c < nc; c += 256) A[c] = mom[(size_t)C * nc + c];
+    __syncthreads();
+    for (int u = 0; u < cnt; u++)
+      B[cidx(ci[u][0], ci[u][1], ci[u][2])] = shift_1d(A, 0, ci[u][0], ci[u][1], ci[u][2], hx);
+    __syncthreads();
+    for (int u = 0; u < cnt; u++)
+      A[cidx(ci[u][0], ci[u][1], ci[u][2])] = shift_1d(B, 1, ci[u][0], ci[u][1], ci[u][2], hy);
+    __syncthreads();
+    for (int u = 0; u < cnt; u++) acc[u] += shift_1d(A, 2, ci[u][0], ci[u][1], ci[u][2], hz);
+    __syncthreads();  // A is reloaded for the next child
+  }
+  const double inv = 1.0 / scale[P];
+  for (int u = 0; u < cnt; u++) {
+    const int c = cidx(ci[u][0], ci[u][1], ci[u][2]);
+    const int n = ci[u][0] + ci[u][1] + ci[u][2];
+    double f = 1.0;
+    for (int i = 0; i < n; i++) f *= inv;
+    const size_t g = (size_t)P * nc + c;
+    mom[g] = acc[u];
+    mom_hat64[g] = acc[u] * f;
+    mom_hat32[g] = (float)(acc[u] * f);
+  }
+}
+
+hipError_t moments_upward(const TreeView& tv, int nterms, int nnodes, const int* nodes,
+                          double* mom, double* mom_hat64, float* mom_hat32, hipStream_t s) {
+  if (nnodes == 0) return hipSuccess;
+  if (ncoef(nterms) > 2 * 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_m2m, dim3(nnodes), dim3(256), 0, s, nterms, nodes, tv.child, tv.center,
+                     tv.scale, mom, mom_hat64, mom_hat32);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Batch interaction lists.  One wavefront walks the tree for its 64 targets
 // with a shared depth-first stack of (node, mask) entries, children pushed in
 // reverse so they are visited in children[0..7] order.  At each node every
