@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <map>
 #include <atomic>
 #include <chrono>
 #include <tuple>
@@ -64,7 +66,69 @@ struct bit_ctx {
   int host_threads = 0;           // tree-build threads, 0 = all
   std::vector<std::pair<void*, size_t>> user_bufs;  // bit_device_alloc
   std::atomic<int> live_systems{0};
+  // Device blocks released by the library's own buffers, kept for reuse (a
+  // system's buffers are freed and requested again at every rebuild).  Every
+  // API call synchronises its stream before returning, so a released block has
+  // no work in flight.  Guarded by pool_mu.
+  std::mutex pool_mu;
+  std::multimap<size_t, void*> pool;
+  size_t pooled = 0;
 };
+
+namespace {
+
+constexpr size_t kPoolGranule = 1 << 16;       // block sizes are multiples of 64 KiB
+constexpr size_t kPoolMax = (size_t)8 << 30;    // bytes kept cached at most
+
+size_t pool_round(size_t n) { return (n + kPoolGranule - 1) / kPoolGranule * kPoolGranule; }
+
+// Frees cached blocks, largest first, until at most `keep` bytes stay cached.
+void pool_trim(bit_ctx* c, size_t keep) {
+  while (c->pooled > keep && !c->pool.empty()) {
+    auto it = std::prev(c->pool.end());
+    (void)hipFree(it->second);
+    c->pooled -= it->first;
+    c->pool.erase(it);
+  }
+}
+
+// A block of at least `need` bytes (size returned in `got`): a cached block of
+// the same rounded size class (up to 2x) if there is one, else a new one.
+hipError_t pool_get(bit_ctx* c, size_t need, void** p, size_t* got) {
+  const size_t want = pool_round(need);
+  {
+    std::lock_guard<std::mutex> lk(c->pool_mu);
+    auto it = c->pool.lower_bound(want);
+    if (it != c->pool.end() && it->first <= 2 * want) {
+      *p = it->second;
+      *got = it->first;
+      c->pooled -= it->first;
+      c->pool.erase(it);
+      return hipSuccess;
+    }
+    if (c->budget && c->in_use + c->pooled + want > c->budget) pool_trim(c, 0);
+  }
+  hipError_t e = hipMalloc(p, want);
+  if (e != hipSuccess) {  // cached blocks may be what fills the device
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> lk(c->pool_mu);
+      pool_trim(c, 0);
+    }
+    e = hipMalloc(p, want);
+  }
+  if (e == hipSuccess) *got = want;
+  return e;
+}
+
+void pool_put(bit_ctx* c, void* p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(c->pool_mu);
+  c->pool.emplace(bytes, p);
+  c->pooled += bytes;
+  pool_trim(c, kPoolMax);
+}
+
+}  // namespace
 
 namespace {
 
@@ -79,8 +143,8 @@ struct DevBuf {
   ~DevBuf() { release(); }
   void release() {
     if (p) {
-      (void)hipFree(p);
       ctx->in_use -= bytes;
+      pool_put(ctx, p, bytes);
     }
     p = nullptr;
     bytes = 0;
@@ -97,14 +161,15 @@ struct DevBuf {
       return fail(BIT_E_OOM, "HBM budget exceeded: need " + std::to_string(need) +
                                  " bytes, in use " + std::to_string(c->in_use) + " of " +
                                  std::to_string(c->budget));
-    hipError_t e = hipMalloc(&p, need);
+    size_t got = 0;
+    hipError_t e = pool_get(c, need, &p, &got);
     if (e != hipSuccess) {
       p = nullptr;
       (void)hipGetLastError();
       return fail(BIT_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
-    bytes = need;
-    c->in_use += need;
+    bytes = got;
+    c->in_use += got;
     return BIT_OK;
   }
   template <typename T>
@@ -603,6 +668,10 @@ int bit_ctx_destroy(bit_ctx* c) {
                                  " system(s) still alive: destroy them before their context");
   (void)hipStreamSynchronize(c->stream);
   for (auto& b : c->user_bufs) (void)hipFree(b.first);
+  {
+    std::lock_guard<std::mutex> lk(c->pool_mu);
+    pool_trim(c, 0);
+  }
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
   return BIT_OK;
