@@ -32,15 +32,15 @@ sys.path.insert(0, ROOT)
 PEAK_FP32 = 157.3e12  # MI355X vector FP32, FLOP/s
 PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 
-# HBM bytes per k_far launch, measured with rocprofv3 --pmc FETCH_SIZE and
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this exact configuration (one pass per counter, each scaled by
 # its reading for a 1 GiB device-to-device copy in the same process: FETCH x2,
 # WRITE x1; the summary is kept under profiles/).  Counters cannot be read from
 # inside the timed run, so the figure is reported only when the configuration
 # matches the profiled one.
 MEASURED_TRAFFIC = {
-    # (n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
-    (4_000_000, 3, 12, 0.25, 1000, 0.5, 0.005, "fp32"): 8.46324e8,
+    # (kernel, n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
+    ("near", 4_000_000, 3, 12, 0.25, 1000, 0.5, 0.005, "fp32"): 3.31436e9,
 }
 
 
@@ -48,11 +48,22 @@ def ncoef(nterms: int) -> int:
     return nterms * (nterms + 1) * (nterms + 2) // 6
 
 
-def flops(stats: dict, nterms: int, screened: bool) -> tuple[float, float]:
-    """Algorithmic FLOPs of the near and far kernels: 16 (screened) / 12 per
-    near pair; 24 T(p) (screened) / 12 T(p) per far (cluster, target) pair."""
+def nqcoef(nterms: int) -> int:
+    """Coefficients per cluster of the FP32 far kernel's radial-derivative form."""
+    p = nterms - 1
+    return sum((d + 1) * (d + 2) // 2 * ((p - d) // 2 + 1) for d in range(p + 1))
+
+
+def flops(stats: dict, nterms: int, screened: bool, precision: str) -> tuple[float, float]:
+    """Algorithmic FLOPs of the near and far kernels.  Near: 16 (screened) / 12
+    per pair.  Far, per (cluster, target) pair: FP64 (coefficient recurrence)
+    24 T(p) (screened) / 12 T(p); FP32 (radial-derivative form) one FMA per
+    cluster coefficient and per monomial of the Horner sweep, 2 (NQ + T(p))."""
     fn = 16.0 if screened else 12.0
-    ff = (24.0 if screened else 12.0) * ncoef(nterms)
+    if precision == "fp32":
+        ff = 2.0 * (nqcoef(nterms) + ncoef(nterms))
+    else:
+        ff = (24.0 if screened else 12.0) * ncoef(nterms)
     return stats["near_pairs"] * fn, stats["far_pairs"] * ff
 
 
@@ -177,20 +188,22 @@ def main():
         log(f"[bench] one fp64 step: {fp64_ms:.1f} ms (far {st64['ms_far']:.1f}, near "
             f"{st64['ms_near']:.1f})")
 
-    f_near, f_far = flops(st, args.nterms, screened)
+    f_near, f_far = flops(st, args.nterms, screened, args.precision)
     kern = "far" if st["ms_far"] >= st["ms_near"] else "near"
     kms = st["ms_far"] if kern == "far" else st["ms_near"]
     kfl = f_far if kern == "far" else f_near
     peak = PEAK_FP32 if args.precision == "fp32" else PEAK_FP64
     achieved = kfl / (kms * 1e-3)
-    roof = {"bound": "valu", "kernel": f"k_far<{'float' if args.precision == 'fp32' else 'double'},"
-            f"{args.nterms},{str(screened).lower()}>" if kern == "far" else
+    roof = {"bound": "valu", "kernel": (f"k_farq<{args.nterms},{str(screened).lower()}>"
+                                        if args.precision == "fp32" else
+                                        f"k_far<double,{args.nterms},{str(screened).lower()}>")
+            if kern == "far" else
             f"k_near{'32' if args.precision == 'fp32' else '64'}<{str(screened).lower()}>",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": (MEASURED_TRAFFIC.get((args.n, args.seed, args.nterms, args.acc,
-                                              args.maxmemb, args.kappa, args.delta,
-                                              args.precision)) if kern == "far" else None),
+            "traffic": MEASURED_TRAFFIC.get((kern, args.n, args.seed, args.nterms, args.acc,
+                                             args.maxmemb, args.kappa, args.delta,
+                                             args.precision)),
             "flops_per_launch": kfl, "kernel_ms": kms,
             "other_kernel": {"name": "near" if kern == "far" else "far",
                              "tflops": (f_near / (st["ms_near"] * 1e-3) if kern == "far" else

@@ -219,7 +219,7 @@ struct bit_system {
   // moments
   int mom_nterms = 0;
   int nchunks = 0;
-  DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, mom_hat32;
+  DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, qc32;
   // internal nodes grouped by height (leaves 0): up_off[h-1]..up_off[h] in up_nodes
   DevBuf up_nodes;
   std::vector<int> up_off;
@@ -945,7 +945,7 @@ int bit_moments(bit_system* s, int nterms) {
   BIT_TRY(s->partial.ensure(c, (size_t)s->nchunks * nc * sizeof(double)));
   BIT_TRY(s->mom.ensure(c, (size_t)nt * nc * sizeof(double)));
   BIT_TRY(s->mom_hat64.ensure(c, (size_t)nt * nc * sizeof(double)));
-  BIT_TRY(s->mom_hat32.ensure(c, (size_t)nt * nc * sizeof(float)));
+  BIT_TRY(s->qc32.ensure(c, (size_t)nt * bit::dev::nqcoef(nterms) * sizeof(float)));
   s->t_mom.start(c->stream);
   HIP_TRY(bit::dev::moments_partial(s->view(), nterms, s->nchunks, s->chunk_node.as<int>(),
                                     s->chunk_rng.as<int2>(), s->partial.as<double>(),
@@ -953,12 +953,14 @@ int bit_moments(bit_system* s, int nterms) {
   HIP_TRY(bit::dev::moments_reduce(s->view(), nterms, s->node_chunk0.as<int>(),
                                    s->node_nchunk.as<int>(), s->partial.as<double>(),
                                    s->mom.as<double>(), s->mom_hat64.as<double>(),
-                                   s->mom_hat32.as<float>(), c->stream));
+                                   c->stream));
   for (size_t h = 1; h < s->up_off.size(); h++)
     HIP_TRY(bit::dev::moments_upward(s->view(), nterms, s->up_off[h] - s->up_off[h - 1],
                                      s->up_nodes.as<int>() + s->up_off[h - 1],
                                      s->mom.as<double>(), s->mom_hat64.as<double>(),
-                                     s->mom_hat32.as<float>(), c->stream));
+                                     c->stream));
+  HIP_TRY(bit::dev::far_coeffs32(nt, nterms, s->mom_hat64.as<double>(), s->qc32.as<float>(),
+                                 c->stream));
   s->t_mom.stop(c->stream);
   s->mom_nterms = nterms;
   s->stats.ms_moments = -1.0;
@@ -1015,7 +1017,7 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
   d.kappa = p->kappa;
   d.delta2 = p->delta2;
   d.mom64 = s->mom_hat64.as<double>();
-  d.mom32 = s->mom_hat32.as<float>();
+  d.qc32 = s->qc32.as<float>();
   d.far_off = s->pl.far_off.as<int>();
   d.far_items = s->pl.far_items.as<bit::dev::Item>();
   d.near_off = s->pl.near_off.as<int>();
@@ -1215,7 +1217,7 @@ int bit_eval_potential_points(bit_system* s, const bit_params* p, int m, const d
   d.kappa = p->kappa;
   d.delta2 = p->delta2;
   d.mom64 = s->mom_hat64.as<double>();
-  d.mom32 = s->mom_hat32.as<float>();
+  d.qc32 = s->qc32.as<float>();
   d.far_off = s->ql.far_off.as<int>();
   d.far_items = s->ql.far_items.as<bit::dev::Item>();
   d.near_off = s->ql.near_off.as<int>();

@@ -110,10 +110,21 @@ hipError_t moments_partial(const TreeView& tv, int nterms, int nchunks, const in
 // Upward pass: moments of `nodes` (internal nodes whose children are done)
 // from their children's raw moments; writes raw and scaled copies.
 hipError_t moments_upward(const TreeView& tv, int nterms, int nnodes, const int* nodes,
-                          double* mom, double* mom_hat64, float* mom_hat32, hipStream_t s);
+                          double* mom, double* mom_hat64, hipStream_t s);
 hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0,
                           const int* node_nchunk, const double* partial, double* mom,
-                          double* mom_hat64, float* mom_hat32, hipStream_t s);
+                          double* mom_hat64, hipStream_t s);
+
+// Far-field coefficients of the FP32 far kernel per node (radial-derivative
+// form of the Taylor sum): nqcoef(nterms) floats per node from the scaled
+// FP64 moments.
+constexpr int nqcoef(int nt) {
+  int n = 0;
+  for (int d = 0; d < nt; d++) n += (d + 1) * (d + 2) / 2 * ((nt - 1 - d) / 2 + 1);
+  return n;
+}
+hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qc32,
+                        hipStream_t s);
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
                        int* near_cnt, ListStats* stats, int* err_flag, hipStream_t s);
@@ -130,7 +141,7 @@ struct EvalDesc {
   double kappa;
   double delta2;
   const double* mom64;   // moments scaled by s^-|k|, FP64
-  const float* mom32;    // same, FP32
+  const float* qc32;     // far coefficients of the FP32 kernel (far_coeffs32)
   const int* far_off;    // [nbatch+1]
   const Item* far_items;
   const int* near_off;

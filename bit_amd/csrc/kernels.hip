@@ -6,7 +6,8 @@
 //   * node table SoA: double4 centre+sqradius, int[8] children, int2 range,
 //     double scale s = sqrt(sqradius);
 //   * moments packed by total-degree shell, T(p) = C(p+3,3) per node, FP64,
-//     plus copies scaled by s^-|k| in FP64 and FP32;
+//     plus a copy scaled by s^-|k| (FP64) and the FP32 far kernel's
+//     coefficients (radial-derivative form, see far_q);
 //   * interaction lists per batch of 64 consecutive sorted targets: (mask,
 //     node) items, far and near, exactly the reference traversal's choices.
 // Work mapping: one wavefront per batch, one lane per target.  Every lane owns
@@ -15,6 +16,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
 #include <utility>
 
 #include "kernels.h"
@@ -276,6 +278,179 @@ __device__ __forceinline__ void far_eval_grad_pow(T x0, T x1, T x2_, T fac, T a0
   ex = acc[0];
   ey = acc[1];
   ez = acc[2];
+}
+
+// ---------------------------------------------------------------------------
+// Radial-derivative form of the far field (FP32 kernel).  The same truncated
+// Taylor sum sum_{|k|<=p} M_k a_k, evaluated through the radial derivatives of
+// the kernel instead of the coefficient recurrence.  With G(R) = exp(-kappa R)/R,
+// g_m = (R^-1 d/dR)^m G and d = target - centre, every Taylor coefficient is
+//   a_k = (-1)^|k| sum_{2j<=k} d^(k-2j) g_{|k|-|j|} / (j! (k-2j)! 2^|j|)
+// (the chain rule for G(sqrt(2 S)), S = (|d|^2 + delta^2)/2, one axis at a
+// time).  In the variables u = d/R, rho = s/R, w = rho u, z = rho^2 and
+// ghat_m = g_m R^(2m+1) this regroups as
+//   sum_k M_k a_k = (1/R) sum_{|l|<=p} w^l sum_t ghat_{|l|+t} z^t Q_{|l|+t,l},
+//   Q_{m,l} = (-1)^|l| / l! * sum_{|j|=m-|l|} Mhat_{l+2j} / (j! 2^|j|),
+// with Mhat_k = M_k / s^|k| and |l| + 2t <= p.  Q depends on the cluster only
+// and is computed once per node (k_qcoef).  Per (target, cluster) pair the
+// work is one FMA per Q plus a Horner sweep over the C(p+3,3) monomials w^l,
+// against about nine operations per Taylor coefficient for the recurrence.
+// ghat obeys ghat_0 = exp(-lam), ghat_1 = -(1 + lam) exp(-lam),
+//   ghat_{m+1} = lam^2 ghat_{m-1} - (2m+1) ghat_m,   lam = kappa R,
+// whose terms share one sign, so it is stable forward.  The Horner sweep
+// multiplies the partial sums by |w| <= rho < 1 at every step.
+// Q order = consumption order: l0 descending, l1 descending, l2 descending,
+// then t ascending.
+// ---------------------------------------------------------------------------
+template <int NT>
+struct QTab {
+  static constexpr int P = NT - 1;
+  static constexpr int NQ = nqcoef(NT);
+  int l[NQ][3];
+  int t[NQ];
+  constexpr QTab() : l(), t() {
+    int q = 0;
+    for (int l0 = P; l0 >= 0; l0--)
+      for (int l1 = P - l0; l1 >= 0; l1--)
+        for (int l2 = P - l0 - l1; l2 >= 0; l2--)
+          for (int tt = 0; tt <= (P - l0 - l1 - l2) / 2; tt++) {
+            l[q][0] = l0;
+            l[q][1] = l1;
+            l[q][2] = l2;
+            t[q] = tt;
+            q++;
+          }
+  }
+};
+template <int NT>
+inline constexpr QTab<NT> kQTab{};
+template <int NT>
+__constant__ QTab<NT> kQTabDev = QTab<NT>();
+
+// Index of ghat_{d+t} z^t in the per-pair table H (degree d, t <= (p-d)/2).
+constexpr int hidx(int P, int d, int t) {
+  int o = 0;
+  for (int e = 0; e < d; e++) o += (P - e) / 2 + 1;
+  return o + t;
+}
+constexpr int nhcoef(int NT) { return hidx(NT - 1, NT, 0); }
+
+__constant__ double kInvFact[kMaxNterms + 1] = {
+    1.0,           1.0,           1.0 / 2,        1.0 / 6,         1.0 / 24,
+    1.0 / 120,     1.0 / 720,     1.0 / 5040,     1.0 / 40320,     1.0 / 362880,
+    1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0};
+
+// One thread per (node, Q entry), from the FP64 scaled moments.
+template <int NT>
+__global__ __launch_bounds__(256) void k_qcoef(int numtree, const double* __restrict__ mom_hat,
+                                               float* __restrict__ qc) {
+  constexpr int NQ = nqcoef(NT), NC = ncoef(NT);
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)numtree * NQ) return;
+  const int node = (int)(g / NQ), q = (int)(g - (long long)node * NQ);
+  const int l0 = kQTabDev<NT>.l[q][0], l1 = kQTabDev<NT>.l[q][1], l2 = kQTabDev<NT>.l[q][2];
+  const int t = kQTabDev<NT>.t[q];
+  const double* M = mom_hat + (size_t)node * NC;
+  double sum = 0.0;
+  for (int j0 = 0; j0 <= t; j0++)
+    for (int j1 = 0; j0 + j1 <= t; j1++) {
+      const int j2 = t - j0 - j1;
+      sum = fma(M[cidx(l0 + 2 * j0, l1 + 2 * j1, l2 + 2 * j2)],
+                kInvFact[j0] * kInvFact[j1] * kInvFact[j2], sum);
+    }
+  const double w = ldexp(kInvFact[l0] * kInvFact[l1] * kInvFact[l2], -t);
+  qc[g] = (float)(((l0 + l1 + l2) & 1 ? -w : w) * sum);
+}
+
+template <int NT>
+static void launch_qcoef(int numtree, const double* mom_hat, float* qc, hipStream_t s) {
+  const long long total = (long long)numtree * nqcoef(NT);
+  hipLaunchKernelGGL(k_qcoef<NT>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     numtree, mom_hat, qc);
+}
+
+template <int... NTs>
+static bool dispatch_qcoef(int nterms, std::integer_sequence<int, NTs...>, int numtree,
+                           const double* mom_hat, float* qc, hipStream_t s) {
+  bool done = false;
+  ((nterms == NTs + 1 ? (launch_qcoef<NTs + 1>(numtree, mom_hat, qc, s), done = true) : false),
+   ...);
+  return done;
+}
+
+hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qc32,
+                        hipStream_t s) {
+  if (numtree == 0) return hipSuccess;
+  if (!dispatch_qcoef(nterms, std::make_integer_sequence<int, kMaxNterms>{}, numtree, mom_hat64,
+                      qc32, s))
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+struct QState {
+  float p, b, a, r;  // current P_l, Horner sums over l2, l1, l0
+};
+
+template <int NT, int C>
+__device__ __forceinline__ void qstep(const float* H, float w0, float w1, float w2,
+                                      const float* M, QState& z) {
+  constexpr const QTab<NT>& tb = kQTab<NT>;
+  constexpr int P = NT - 1;
+  constexpr int l0 = tb.l[C][0], l1 = tb.l[C][1], l2 = tb.l[C][2], t = tb.t[C];
+  constexpr int d = l0 + l1 + l2;
+  constexpr int h = hidx(P, d, t);
+  if constexpr (t == 0) z.p = H[h] * M[C];
+  else z.p = fmaf(H[h], M[C], z.p);
+  if constexpr (t == (P - d) / 2) {
+    if constexpr (l2 == P - l0 - l1) z.b = z.p;
+    else z.b = fmaf(z.b, w2, z.p);
+    if constexpr (l2 == 0) {
+      if constexpr (l1 == P - l0) z.a = z.b;
+      else z.a = fmaf(z.a, w1, z.b);
+      if constexpr (l1 == 0) {
+        if constexpr (l0 == P) z.r = z.a;
+        else z.r = fmaf(z.r, w0, z.a);
+      }
+    }
+  }
+}
+
+template <int NT, int... C>
+__device__ __forceinline__ void qall(std::integer_sequence<int, C...>, const float* H, float w0,
+                                     float w1, float w2, const float* M, QState& z) {
+  (qstep<NT, C>(H, w0, w1, w2, M, z), ...);
+}
+
+// R times the far field of one cluster at one target: w = rho u, z = rho^2,
+// c0 = exp(-lam) (screened), M = the cluster's Q.
+template <int NT, bool SCR>
+__device__ __forceinline__ float far_q(float w0, float w1, float w2, float z, float c0, float lam,
+                                       const float* M) {
+  constexpr int P = NT - 1;
+  float g[NT];
+  if constexpr (SCR) {
+    g[0] = c0;
+    if constexpr (NT > 1) g[1] = -c0 * (1.0f + lam);
+    const float lam2 = lam * lam;
+#pragma unroll
+    for (int m = 1; m < P; m++) g[m + 1] = fmaf(lam2, g[m - 1], -float(2 * m + 1) * g[m]);
+  } else {
+    g[0] = 1.0f;
+#pragma unroll
+    for (int m = 0; m < P; m++) g[m + 1] = -float(2 * m + 1) * g[m];
+  }
+  float zp[P / 2 + 1];
+  zp[0] = 1.0f;
+#pragma unroll
+  for (int t = 1; t <= P / 2; t++) zp[t] = zp[t - 1] * z;
+  float H[nhcoef(NT)];
+#pragma unroll
+  for (int d = 0; d <= P; d++)
+#pragma unroll
+    for (int t = 0; t <= (P - d) / 2; t++) H[hidx(P, d, t)] = t == 0 ? g[d] : g[d + t] * zp[t];
+  QState s{0.0f, 0.0f, 0.0f, 0.0f};
+  qall<NT>(std::make_integer_sequence<int, nqcoef(NT)>{}, H, w0, w1, w2, M, s);
+  return s.r;
 }
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
@@ -743,7 +918,7 @@ __global__ void k_moments_reduce(int numtree, int nterms, const int* __restrict_
                                  const int* __restrict__ node_nchunk,
                                  const double* __restrict__ scale,
                                  const double* __restrict__ partial, double* mom,
-                                 double* mom_hat64, float* mom_hat32) {
+                                 double* mom_hat64) {
   const int nc = ncoef(nterms);
   long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (long long)numtree * nc) return;
@@ -757,16 +932,15 @@ __global__ void k_moments_reduce(int numtree, int nterms, const int* __restrict_
   for (int i = 0; i < n; i++) f *= inv;
   mom[g] = s;
   mom_hat64[g] = s * f;
-  mom_hat32[g] = (float)(s * f);
 }
 
 hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0,
                           const int* node_nchunk, const double* partial, double* mom,
-                          double* mom_hat64, float* mom_hat32, hipStream_t s) {
+                          double* mom_hat64, hipStream_t s) {
   long long total = (long long)tv.numtree * ncoef(nterms);
   int blocks = (int)((total + 255) / 256);
   hipLaunchKernelGGL(k_moments_reduce, dim3(blocks), dim3(256), 0, s, tv.numtree, nterms,
-                     node_chunk0, node_nchunk, tv.scale, partial, mom, mom_hat64, mom_hat32);
+                     node_chunk0, node_nchunk, tv.scale, partial, mom, mom_hat64);
   return hipGetLastError();
 }
 
@@ -801,8 +975,7 @@ __global__ __launch_bounds__(256) void k_m2m(int nterms, const int* __restrict__
                                              const double4* __restrict__ center,
                                              const double* __restrict__ scale,
                                              double* __restrict__ mom,
-                                             double* __restrict__ mom_hat64,
-                                             float* __restrict__ mom_hat32) {
+                                             double* __restrict__ mom_hat64) {
   __shared__ double A[kMaxCoef], B[kMaxCoef];
   const int P = nodes[blockIdx.x];
   const int nc = ncoef(nterms);
@@ -847,16 +1020,15 @@ __global__ __launch_bounds__(256) void k_m2m(int nterms, const int* __restrict__
     const size_t g = (size_t)P * nc + c;
     mom[g] = acc[u];
     mom_hat64[g] = acc[u] * f;
-    mom_hat32[g] = (float)(acc[u] * f);
   }
 }
 
 hipError_t moments_upward(const TreeView& tv, int nterms, int nnodes, const int* nodes,
-                          double* mom, double* mom_hat64, float* mom_hat32, hipStream_t s) {
+                          double* mom, double* mom_hat64, hipStream_t s) {
   if (nnodes == 0) return hipSuccess;
   if (ncoef(nterms) > 2 * 256) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_m2m, dim3(nnodes), dim3(256), 0, s, nterms, nodes, tv.child, tv.center,
-                     tv.scale, mom, mom_hat64, mom_hat32);
+                     tv.scale, mom, mom_hat64);
   return hipGetLastError();
 }
 
@@ -986,10 +1158,10 @@ hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp
 
 // ---------------------------------------------------------------------------
 // Far field.  Per item the wave's lanes in the mask evaluate the cluster's
-// Taylor expansion; the recurrence is fully unrolled for the compile-time order
-// so all coefficients live in registers, and the moments are read with
-// wave-uniform addresses (scalar loads).  FP32: the per-cluster value is added
-// with Kahan compensation; FP64: plain sum.
+// Taylor sum.  FP64 (k_far): the coefficient recurrence, fully unrolled for
+// the compile-time order so all coefficients live in registers.  FP32
+// (k_farq): the radial-derivative form (far_q), per-cluster values added with
+// Kahan compensation.  Both stage the cluster's data in LDS.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Acc;
@@ -1024,16 +1196,6 @@ __device__ __forceinline__ void which(const int2* __restrict__ map, int& sys, in
   }
 }
 
-template <typename T>
-__device__ __forceinline__ const T* mom_of(const EvalDesc& d);
-template <>
-__device__ __forceinline__ const float* mom_of<float>(const EvalDesc& d) {
-  return d.mom32;
-}
-template <>
-__device__ __forceinline__ const double* mom_of<double>(const EvalDesc& d) {
-  return d.mom64;
-}
 
 template <typename T, int NT, bool SCR>
 // The cluster's moments are staged in LDS: the wave loads the next item's
@@ -1051,7 +1213,7 @@ __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
   const TreeView& tv = d.tv;
   const double kappa = d.kappa, delta2 = d.delta2;
   // global (not flat) loads: flat loads also count on the LDS wait counter
-  const gptr<T> mom_hat = as_global(mom_of<T>(d));
+  const gptr<T> mom_hat = as_global(d.mom64);
   const gptr<Item> items = as_global(d.far_items);
   const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
   const gptr<double> scale = as_global(tv.scale);
@@ -1098,9 +1260,74 @@ __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
   if (valid) d.phi_far[t] = acc.value();
 }
 
+// FP32 far field in the radial-derivative form: the cluster's Q (far_q) is
+// staged in LDS like the FP64 kernel's moments.  The target's offset from the
+// centre, R and exp(-lam) are formed in FP64 once per item.
+template <int NT, bool SCR>
+__global__ __launch_bounds__(64) void k_farq(const EvalDesc* __restrict__ descs,
+                                             const int2* __restrict__ map) {
+  constexpr int NQ = nqcoef(NT);
+  constexpr int PF = (NQ + kBatch - 1) / kBatch;  // coefficients per lane per item
+  __shared__ float mbuf[2][PF * kBatch];
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const TreeView& tv = d.tv;
+  const double kappa = d.kappa, delta2 = d.delta2;
+  const gptr<float> qc = as_global(d.qc32);
+  const gptr<Item> items = as_global(d.far_items);
+  const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
+  const gptr<double> scale = as_global(tv.scale);
+  const gptr<int> far_off = as_global(d.far_off);
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
+  Acc<float> acc;
+  const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
+  float pf[PF];
+  auto fetch = [&](int node) {
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+      const int c = j * kBatch + lane;
+      pf[j] = c < NQ ? qc[(size_t)node * NQ + c] : 0.0f;
+    }
+  };
+  if (i0 < i1) fetch(uniform(items[i0].node));
+  int buf = 0;
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+#pragma unroll
+    for (int j = 0; j < PF; j++) mbuf[buf][j * kBatch + lane] = pf[j];
+    __syncthreads();
+    if (it + 1 < i1) fetch(uniform(items[it + 1].node));
+    const float* M = mbuf[buf];
+    buf ^= 1;
+    if ((mask >> lane) & 1ull) {
+      const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
+      const double s = scale[node];
+      const double dx = px - cx, dy = py - cy, dz = pz - cz;
+      const double R = sqrt(dx * dx + dy * dy + dz * dz + delta2);
+      const double rinv = 1.0 / R;
+      const double rho = s * rinv, f = rho * rinv;
+      const double lam = kappa * R;
+      const double c0 = SCR ? exp(-lam) : 1.0;
+      const float v = far_q<NT, SCR>((float)(dx * f), (float)(dy * f), (float)(dz * f),
+                                     (float)(rho * rho), (float)c0, (float)lam, M);
+      acc.add(v * (float)rinv);
+    }
+  }
+  if (valid) d.phi_far[t] = acc.value();
+}
+
 template <typename T, bool SCR, int NT>
 static void launch_far(const EvalDesc* descs, const int2* map, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((k_far<T, NT, SCR>), dim3(nblocks), dim3(kBatch), 0, s, descs, map);
+  if constexpr (std::is_same_v<T, float>)
+    hipLaunchKernelGGL((k_farq<NT, SCR>), dim3(nblocks), dim3(kBatch), 0, s, descs, map);
+  else
+    hipLaunchKernelGGL((k_far<T, NT, SCR>), dim3(nblocks), dim3(kBatch), 0, s, descs, map);
 }
 
 template <typename T, bool SCR, int... NTs>

@@ -1,0 +1,122 @@
+"""The FP32 far kernel's radial-derivative form against the oracle's Taylor sum.
+
+The FP32 far kernel (k_farq) evaluates sum_{|k|<=p} M_k a_k as
+    (1/R) sum_l w^l sum_t ghat_{|l|+t} z^t Q_{|l|+t,l}
+with w = rho u, z = rho^2, ghat_m = R^(2m+1) (R^-1 d/dR)^m G and the
+per-cluster Q of k_qcoef.  This restates that form in FP64 numpy, in the
+kernel's Horner order, and checks it against the oracle's coefficient
+recurrence (ora_taylor_coeffs, pinned to the reference) on random clusters
+and targets that pass the MAC, screened and Coulomb, with and without delta.
+It also checks the consumption-order count nqcoef against bench.py's copy.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle
+
+
+def monos(p):
+    return [(n - m, m - k, k) for n in range(p + 1) for m in range(n + 1) for k in range(m + 1)]
+
+
+def qcoef(mhat, p):
+    """Q_{m,l} = (-1)^|l| / l! * sum_{|j|=m-|l|} Mhat_{l+2j} / (j! 2^|j|)."""
+    f = math.factorial
+    q = {}
+    for l in monos(p):
+        d = sum(l)
+        for t in range((p - d) // 2 + 1):
+            s = 0.0
+            for j0 in range(t + 1):
+                for j1 in range(t - j0 + 1):
+                    j2 = t - j0 - j1
+                    k = (l[0] + 2 * j0, l[1] + 2 * j1, l[2] + 2 * j2)
+                    s += mhat[k] / (f(j0) * f(j1) * f(j2) * 2 ** t)
+            q[(d + t, l)] = (-1) ** d * s / (f(l[0]) * f(l[1]) * f(l[2]))
+    return q
+
+
+def far_q(q, p, d, s, kappa, delta2):
+    """(1/R) sum_l w^l P_l by Horner over l0, l1, l2 (descending), as k_farq."""
+    R = math.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + delta2)
+    lam, rho = kappa * R, s / R
+    w = [rho * d[e] / R for e in range(3)]
+    z = rho * rho
+    g = [math.exp(-lam), -math.exp(-lam) * (1.0 + lam)]
+    for m in range(1, p):
+        g.append(lam * lam * g[m - 1] - (2 * m + 1) * g[m])
+
+    def P(l):
+        dl = sum(l)
+        return sum(g[dl + t] * z ** t * q[(dl + t, l)] for t in range((p - dl) // 2 + 1))
+
+    r = 0.0
+    for l0 in range(p, -1, -1):
+        a = 0.0
+        for l1 in range(p - l0, -1, -1):
+            b = 0.0
+            for l2 in range(p - l0 - l1, -1, -1):
+                b = b * w[2] + P((l0, l1, l2))
+            a = a * w[1] + b
+        r = r * w[0] + a
+    return r / R
+
+
+@pytest.fixture(scope="module")
+def ora():
+    return Oracle()
+
+
+@pytest.mark.parametrize("p,kappa,delta2", [(0, 0.5, 0.0), (1, 0.5, 0.0), (5, 0.0, 0.0),
+                                            (8, 0.5, 2.5e-5), (11, 0.5, 2.5e-5),
+                                            (11, 2.0, 4e-4), (12, 0.0, 1e-4)])
+def test_radial_form_equals_taylor_sum(ora, p, kappa, delta2):
+    rng = np.random.default_rng(100 + p)
+    worst = 0.0
+    for _ in range(12):
+        h = 10 ** rng.uniform(-3, -1)
+        ys = rng.uniform(-h, h, (40, 3))
+        qs = rng.uniform(-1, 1, 40)
+        s = math.sqrt(3.0) * h
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        d = u * s * (2.0 + 10 ** rng.uniform(-2, 2))
+        M, mhat = {}, {}
+        for k in monos(p):
+            M[k] = float(np.sum(qs * ys[:, 0] ** k[0] * ys[:, 1] ** k[1] * ys[:, 2] ** k[2]))
+            mhat[k] = M[k] / s ** sum(k)
+        a = ora.taylor_coeffs(p + 1, d[0], d[1], d[2], float(d @ d) + delta2, kappa)
+        ref = sum(M[k] * a[k] for k in monos(p))
+        got = far_q(qcoef(mhat, p), p, d, s, kappa, delta2)
+        scale = float(np.sum(np.abs(qs))) / math.sqrt(float(d @ d) + delta2)
+        worst = max(worst, abs(got - ref) / scale)
+    assert worst < 1e-13
+
+
+def test_plant_dropped_term_fails(ora):
+    """A Q with one j-term dropped must fail the same comparison."""
+    p, kappa, delta2 = 6, 0.5, 0.0
+    rng = np.random.default_rng(7)
+    ys = rng.uniform(-0.01, 0.01, (30, 3))
+    qs = rng.uniform(-1, 1, 30)
+    s = math.sqrt(3.0) * 0.01
+    d = np.array([0.05, -0.03, 0.02])
+    M = {k: float(np.sum(qs * ys[:, 0] ** k[0] * ys[:, 1] ** k[1] * ys[:, 2] ** k[2]))
+         for k in monos(p)}
+    mhat = {k: v / s ** sum(k) for k, v in M.items()}
+    a = ora.taylor_coeffs(p + 1, d[0], d[1], d[2], float(d @ d), kappa)
+    ref = sum(M[k] * a[k] for k in monos(p))
+    q = qcoef(mhat, p)
+    q[(3, (1, 0, 0))] -= mhat[(3, 0, 0)] / 2  # drop j = e_x from Q_{3,(1,0,0)}
+    got = far_q(q, p, d, s, kappa, delta2)
+    assert abs(got - ref) / abs(ref) > 1e-9
+
+
+def test_nqcoef_matches_consumption_order():
+    import bench
+    for nt in range(1, 14):
+        p = nt - 1
+        n = sum((p - sum(l)) // 2 + 1 for l in monos(p))
+        assert bench.nqcoef(nt) == n
