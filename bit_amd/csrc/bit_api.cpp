@@ -945,7 +945,7 @@ int bit_moments(bit_system* s, int nterms) {
   BIT_TRY(s->partial.ensure(c, (size_t)s->nchunks * nc * sizeof(double)));
   BIT_TRY(s->mom.ensure(c, (size_t)nt * nc * sizeof(double)));
   BIT_TRY(s->mom_hat64.ensure(c, (size_t)nt * nc * sizeof(double)));
-  BIT_TRY(s->qc32.ensure(c, (size_t)nt * bit::dev::nqcoef(nterms) * sizeof(float)));
+  BIT_TRY(s->qc32.ensure(c, (size_t)nt * bit::dev::nqpad(nterms) * sizeof(float)));
   s->t_mom.start(c->stream);
   HIP_TRY(bit::dev::moments_partial(s->view(), nterms, s->nchunks, s->chunk_node.as<int>(),
                                     s->chunk_rng.as<int2>(), s->partial.as<double>(),

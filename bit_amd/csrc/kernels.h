@@ -116,13 +116,15 @@ hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0
                           double* mom_hat64, hipStream_t s);
 
 // Far-field coefficients of the FP32 far kernel per node (radial-derivative
-// form of the Taylor sum): nqcoef(nterms) floats per node from the scaled
-// FP64 moments.
+// form of the Taylor sum): nqcoef(nterms) values per node from the scaled
+// FP64 moments, stored in tiles of nqpad(nterms) floats.
 constexpr int nqcoef(int nt) {
   int n = 0;
   for (int d = 0; d < nt; d++) n += (d + 1) * (d + 2) / 2 * ((nt - 1 - d) / 2 + 1);
   return n;
 }
+// Floats per node in the coefficient buffer (tiles padded to whole wavefronts).
+constexpr int nqpad(int nt) { return (nqcoef(nt) + 63) / 64 * 64; }
 hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qc32,
                         hipStream_t s);
 

@@ -289,18 +289,19 @@ __device__ __forceinline__ void far_eval_grad_pow(T x0, T x1, T x2_, T fac, T a0
 // (the chain rule for G(sqrt(2 S)), S = (|d|^2 + delta^2)/2, one axis at a
 // time).  In the variables u = d/R, rho = s/R, w = rho u, z = rho^2 and
 // ghat_m = g_m R^(2m+1) this regroups as
-//   sum_k M_k a_k = (1/R) sum_{|l|<=p} w^l sum_t ghat_{|l|+t} z^t Q_{|l|+t,l},
+//   sum_k M_k a_k = (1/R) sum_t z^t sum_{|l|<=p-2t} ghat_{|l|+t} Q_{|l|+t,l} w^l,
 //   Q_{m,l} = (-1)^|l| / l! * sum_{|j|=m-|l|} Mhat_{l+2j} / (j! 2^|j|),
-// with Mhat_k = M_k / s^|k| and |l| + 2t <= p.  Q depends on the cluster only
-// and is computed once per node (k_qcoef).  Per (target, cluster) pair the
-// work is one FMA per Q plus a Horner sweep over the C(p+3,3) monomials w^l,
-// against about nine operations per Taylor coefficient for the recurrence.
-// ghat obeys ghat_0 = exp(-lam), ghat_1 = -(1 + lam) exp(-lam),
-//   ghat_{m+1} = lam^2 ghat_{m-1} - (2m+1) ghat_m,   lam = kappa R,
-// whose terms share one sign, so it is stable forward.  The Horner sweep
-// multiplies the partial sums by |w| <= rho < 1 at every step.
-// Q order = consumption order: l0 descending, l1 descending, l2 descending,
-// then t ascending.
+// with Mhat_k = M_k / s^|k|.  Q depends on the cluster only and is computed
+// once per node (k_qcoef).  With the ratios r_m = ghat_{m+1}/ghat_m the inner
+// sum is ghat_t times a nested Horner sweep (l0, l1, l2 descending) whose step
+// in direction e at degree n multiplies by r_{n+t} w_e, so every Q costs one
+// FMA per (target, cluster) pair:
+//   B = fma(B, r_{l0+l1+l2+t} w_2, Q),  A = fma(A, r_{l0+l1+t} w_1, B),
+//   Phi = fma(Phi, r_{l0+t} w_0, A),    S = fma(S, z, ghat_t Phi).
+// The ratios obey r_0 = -(1 + lam), r_m = lam^2 / r_{m-1} - (2m+1), lam =
+// kappa R (from ghat_{m+1} = lam^2 ghat_{m-1} - (2m+1) ghat_m, whose terms
+// share one sign); Coulomb: r_m = -(2m+1).
+// Q order = consumption order: t descending, then l0, l1, l2 descending.
 // ---------------------------------------------------------------------------
 template <int NT>
 struct QTab {
@@ -310,16 +311,18 @@ struct QTab {
   int t[NQ];
   constexpr QTab() : l(), t() {
     int q = 0;
-    for (int l0 = P; l0 >= 0; l0--)
-      for (int l1 = P - l0; l1 >= 0; l1--)
-        for (int l2 = P - l0 - l1; l2 >= 0; l2--)
-          for (int tt = 0; tt <= (P - l0 - l1 - l2) / 2; tt++) {
+    for (int tt = P / 2; tt >= 0; tt--) {
+      const int L = P - 2 * tt;
+      for (int l0 = L; l0 >= 0; l0--)
+        for (int l1 = L - l0; l1 >= 0; l1--)
+          for (int l2 = L - l0 - l1; l2 >= 0; l2--) {
             l[q][0] = l0;
             l[q][1] = l1;
             l[q][2] = l2;
             t[q] = tt;
             q++;
           }
+    }
   }
 };
 template <int NT>
@@ -327,27 +330,24 @@ inline constexpr QTab<NT> kQTab{};
 template <int NT>
 __constant__ QTab<NT> kQTabDev = QTab<NT>();
 
-// Index of ghat_{d+t} z^t in the per-pair table H (degree d, t <= (p-d)/2).
-constexpr int hidx(int P, int d, int t) {
-  int o = 0;
-  for (int e = 0; e < d; e++) o += (P - e) / 2 + 1;
-  return o + t;
-}
-constexpr int nhcoef(int NT) { return hidx(NT - 1, NT, 0); }
-
 __constant__ double kInvFact[kMaxNterms + 1] = {
     1.0,           1.0,           1.0 / 2,        1.0 / 6,         1.0 / 24,
     1.0 / 120,     1.0 / 720,     1.0 / 5040,     1.0 / 40320,     1.0 / 362880,
     1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0};
 
-// One thread per (node, Q entry), from the FP64 scaled moments.
+// One thread per (node, Q entry), from the FP64 scaled moments.  Each node's
+// tile is padded with zeros to nqpad(NT) floats.
 template <int NT>
 __global__ __launch_bounds__(256) void k_qcoef(int numtree, const double* __restrict__ mom_hat,
                                                float* __restrict__ qc) {
-  constexpr int NQ = nqcoef(NT), NC = ncoef(NT);
+  constexpr int NQ = nqcoef(NT), NQP = nqpad(NT), NC = ncoef(NT);
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (long long)numtree * NQ) return;
-  const int node = (int)(g / NQ), q = (int)(g - (long long)node * NQ);
+  if (g >= (long long)numtree * NQP) return;
+  const int node = (int)(g / NQP), q = (int)(g - (long long)node * NQP);
+  if (q >= NQ) {
+    qc[g] = 0.0f;
+    return;
+  }
   const int l0 = kQTabDev<NT>.l[q][0], l1 = kQTabDev<NT>.l[q][1], l2 = kQTabDev<NT>.l[q][2];
   const int t = kQTabDev<NT>.t[q];
   const double* M = mom_hat + (size_t)node * NC;
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void k_qcoef(int numtree, const double* __rest
 
 template <int NT>
 static void launch_qcoef(int numtree, const double* mom_hat, float* qc, hipStream_t s) {
-  const long long total = (long long)numtree * nqcoef(NT);
+  const long long total = (long long)numtree * nqpad(NT);
   hipLaunchKernelGGL(k_qcoef<NT>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      numtree, mom_hat, qc);
 }
@@ -387,70 +387,76 @@ hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float*
   return hipGetLastError();
 }
 
-struct QState {
-  float p, b, a, r;  // current P_l, Horner sums over l2, l1, l0
+// Per-pair values of the sweep: mult[e][m] = r_m w_e, ghat_t, z.
+template <int NT>
+struct QPair {
+  float mult[3][NT > 1 ? NT - 1 : 1];
+  float g[NT / 2 + 1];
+  float z;
 };
 
-template <int NT, int C>
-__device__ __forceinline__ void qstep(const float* H, float w0, float w1, float w2,
-                                      const float* M, QState& z) {
-  constexpr const QTab<NT>& tb = kQTab<NT>;
+// w = rho u, z = rho^2, c0 = exp(-lam) (screened), lam = kappa R.
+template <int NT, bool SCR>
+__device__ __forceinline__ void qpair_init(QPair<NT>& v, float w0, float w1, float w2, float z,
+                                           float c0, float lam) {
   constexpr int P = NT - 1;
+  float r[P > 0 ? P : 1];
+  if constexpr (SCR) {
+    if constexpr (P > 0) r[0] = -(1.0f + lam);
+    const float lam2 = lam * lam;
+#pragma unroll
+    for (int m = 1; m < P; m++)
+      r[m] = fmaf(lam2, __builtin_amdgcn_rcpf(r[m - 1]), -float(2 * m + 1));
+    v.g[0] = c0;
+  } else {
+#pragma unroll
+    for (int m = 0; m < P; m++) r[m] = -float(2 * m + 1);
+    v.g[0] = 1.0f;
+  }
+#pragma unroll
+  for (int t = 1; t <= P / 2; t++) v.g[t] = v.g[t - 1] * r[t - 1];
+#pragma unroll
+  for (int m = 0; m < P; m++) {
+    v.mult[0][m] = r[m] * w0;
+    v.mult[1][m] = r[m] * w1;
+    v.mult[2][m] = r[m] * w2;
+  }
+  v.z = z;
+}
+
+struct QState {
+  float b, a, phi, s;  // Horner sums over l2, l1, l0 and t
+};
+
+// Entry C of the sweep for one target (q = the cluster's Q_C).
+template <int NT, int C>
+__device__ __forceinline__ void qstep(const QPair<NT>& v, float q, QState& z) {
+  constexpr const QTab<NT>& tb = kQTab<NT>;
   constexpr int l0 = tb.l[C][0], l1 = tb.l[C][1], l2 = tb.l[C][2], t = tb.t[C];
-  constexpr int d = l0 + l1 + l2;
-  constexpr int h = hidx(P, d, t);
-  if constexpr (t == 0) z.p = H[h] * M[C];
-  else z.p = fmaf(H[h], M[C], z.p);
-  if constexpr (t == (P - d) / 2) {
-    if constexpr (l2 == P - l0 - l1) z.b = z.p;
-    else z.b = fmaf(z.b, w2, z.p);
-    if constexpr (l2 == 0) {
-      if constexpr (l1 == P - l0) z.a = z.b;
-      else z.a = fmaf(z.a, w1, z.b);
-      if constexpr (l1 == 0) {
-        if constexpr (l0 == P) z.r = z.a;
-        else z.r = fmaf(z.r, w0, z.a);
+  constexpr int L = NT - 1 - 2 * t;
+  if constexpr (l2 == L - l0 - l1) z.b = q;
+  else z.b = fmaf(z.b, v.mult[2][l0 + l1 + l2 + t], q);
+  if constexpr (l2 == 0) {
+    if constexpr (l1 == L - l0) z.a = z.b;
+    else z.a = fmaf(z.a, v.mult[1][l0 + l1 + t], z.b);
+    if constexpr (l1 == 0) {
+      if constexpr (l0 == L) z.phi = z.a;
+      else z.phi = fmaf(z.phi, v.mult[0][l0 + t], z.a);
+      if constexpr (l0 == 0) {
+        if constexpr (t == (NT - 1) / 2) z.s = v.g[t] * z.phi;
+        else z.s = fmaf(z.s, v.z, v.g[t] * z.phi);
       }
     }
   }
 }
 
+// The sweep over the cluster's Q (in LDS) for one target.
 template <int NT, int... C>
-__device__ __forceinline__ void qall(std::integer_sequence<int, C...>, const float* H, float w0,
-                                     float w1, float w2, const float* M, QState& z) {
-  (qstep<NT, C>(H, w0, w1, w2, M, z), ...);
-}
-
-// R times the far field of one cluster at one target: w = rho u, z = rho^2,
-// c0 = exp(-lam) (screened), M = the cluster's Q.
-template <int NT, bool SCR>
-__device__ __forceinline__ float far_q(float w0, float w1, float w2, float z, float c0, float lam,
-                                       const float* M) {
-  constexpr int P = NT - 1;
-  float g[NT];
-  if constexpr (SCR) {
-    g[0] = c0;
-    if constexpr (NT > 1) g[1] = -c0 * (1.0f + lam);
-    const float lam2 = lam * lam;
-#pragma unroll
-    for (int m = 1; m < P; m++) g[m + 1] = fmaf(lam2, g[m - 1], -float(2 * m + 1) * g[m]);
-  } else {
-    g[0] = 1.0f;
-#pragma unroll
-    for (int m = 0; m < P; m++) g[m + 1] = -float(2 * m + 1) * g[m];
-  }
-  float zp[P / 2 + 1];
-  zp[0] = 1.0f;
-#pragma unroll
-  for (int t = 1; t <= P / 2; t++) zp[t] = zp[t - 1] * z;
-  float H[nhcoef(NT)];
-#pragma unroll
-  for (int d = 0; d <= P; d++)
-#pragma unroll
-    for (int t = 0; t <= (P - d) / 2; t++) H[hidx(P, d, t)] = t == 0 ? g[d] : g[d + t] * zp[t];
-  QState s{0.0f, 0.0f, 0.0f, 0.0f};
-  qall<NT>(std::make_integer_sequence<int, nqcoef(NT)>{}, H, w0, w1, w2, M, s);
-  return s.r;
+__device__ __forceinline__ float qall(std::integer_sequence<int, C...>, const QPair<NT>& v,
+                                      const float* M) {
+  QState z{0.0f, 0.0f, 0.0f, 0.0f};
+  (qstep<NT, C>(v, M[C], z), ...);
+  return z.s;
 }
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
@@ -1260,20 +1266,40 @@ __global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
   if (valid) d.phi_far[t] = acc.value();
 }
 
-// FP32 far field in the radial-derivative form: the cluster's Q (far_q) is
-// staged in LDS like the FP64 kernel's moments.  The target's offset from the
-// centre, R and exp(-lam) are formed in FP64 once per item.
+// Per-target set-up of the sweep for one cluster: the offset from the centre
+// is formed in FP64 and rounded, the rest is FP32 (hardware rsqrt and exp2).
+// Returns 1/R.
 template <int NT, bool SCR>
-__global__ __launch_bounds__(64) void k_farq(const EvalDesc* __restrict__ descs,
-                                             const int2* __restrict__ map) {
+__device__ __forceinline__ float qsetup(QPair<NT>& v, double px, double py, double pz, double cx,
+                                        double cy, double cz, float s, float kappa,
+                                        float delta2) {
+  const float dx = (float)(px - cx), dy = (float)(py - cy), dz = (float)(pz - cz);
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, delta2)));
+  const float ri = __builtin_amdgcn_rsqf(r2);
+  const float R = r2 * ri;
+  const float rho = s * ri, f = rho * ri;
+  const float lam = kappa * R;
+  const float c0 = SCR ? __builtin_amdgcn_exp2f(-1.4426950408889634f * lam) : 1.0f;
+  qpair_init<NT, SCR>(v, dx * f, dy * f, dz * f, rho * rho, c0, lam);
+  return ri;
+}
+
+// FP32 far field in the radial-derivative form: the cluster's Q is staged in
+// LDS like the FP64 kernel's moments (the next item's Q is loaded into
+// registers while the current one is evaluated) and read as broadcasts.
+// Four waves per SIMD: the sweep is chains of dependent FMAs, and the issue
+// rate of one wave is well below the SIMD's.
+template <int NT, bool SCR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_farq(
+    const EvalDesc* __restrict__ descs, const int2* __restrict__ map) {
   constexpr int NQ = nqcoef(NT);
-  constexpr int PF = (NQ + kBatch - 1) / kBatch;  // coefficients per lane per item
-  __shared__ float mbuf[2][PF * kBatch];
+  constexpr int PF = nqpad(NT) / kBatch;  // Q values per lane per item
+  __shared__ __align__(16) float mbuf[PF * kBatch];
   int sys, b;
   which(map, sys, b);
   const EvalDesc& d = descs[sys];
   const TreeView& tv = d.tv;
-  const double kappa = d.kappa, delta2 = d.delta2;
+  const float kappa32 = (float)d.kappa, delta2_32 = (float)d.delta2;
   const gptr<float> qc = as_global(d.qc32);
   const gptr<Item> items = as_global(d.far_items);
   const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
@@ -1287,37 +1313,26 @@ __global__ __launch_bounds__(64) void k_farq(const EvalDesc* __restrict__ descs,
   Acc<float> acc;
   const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
   float pf[PF];
-  auto fetch = [&](int node) {
+  auto fetch = [&](int node) {  // node tiles are padded to PF * 64 floats
 #pragma unroll
-    for (int j = 0; j < PF; j++) {
-      const int c = j * kBatch + lane;
-      pf[j] = c < NQ ? qc[(size_t)node * NQ + c] : 0.0f;
-    }
+    for (int j = 0; j < PF; j++) pf[j] = qc[(size_t)node * (PF * kBatch) + j * kBatch + lane];
   };
   if (i0 < i1) fetch(uniform(items[i0].node));
-  int buf = 0;
   for (int it = i0; it < i1; it++) {
     const unsigned long long mask = uniform64(items[it].mask);
     const int node = uniform(items[it].node);
 #pragma unroll
-    for (int j = 0; j < PF; j++) mbuf[buf][j * kBatch + lane] = pf[j];
+    for (int j = 0; j < PF; j++) mbuf[j * kBatch + lane] = pf[j];
     __syncthreads();
     if (it + 1 < i1) fetch(uniform(items[it + 1].node));
-    const float* M = mbuf[buf];
-    buf ^= 1;
     if ((mask >> lane) & 1ull) {
       const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
-      const double s = scale[node];
-      const double dx = px - cx, dy = py - cy, dz = pz - cz;
-      const double R = sqrt(dx * dx + dy * dy + dz * dz + delta2);
-      const double rinv = 1.0 / R;
-      const double rho = s * rinv, f = rho * rinv;
-      const double lam = kappa * R;
-      const double c0 = SCR ? exp(-lam) : 1.0;
-      const float v = far_q<NT, SCR>((float)(dx * f), (float)(dy * f), (float)(dz * f),
-                                     (float)(rho * rho), (float)c0, (float)lam, M);
-      acc.add(v * (float)rinv);
+      QPair<NT> v;
+      const float rinv = qsetup<NT, SCR>(v, px, py, pz, cx, cy, cz, (float)scale[node], kappa32,
+                                         delta2_32);
+      acc.add(qall<NT>(std::make_integer_sequence<int, NQ>{}, v, mbuf) * rinv);
     }
+    __syncthreads();  // the tile is overwritten next item
   }
   if (valid) d.phi_far[t] = acc.value();
 }

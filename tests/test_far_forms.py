@@ -1,10 +1,10 @@
 """The FP32 far kernel's radial-derivative form against the oracle's Taylor sum.
 
 The FP32 far kernel (k_farq) evaluates sum_{|k|<=p} M_k a_k as
-    (1/R) sum_l w^l sum_t ghat_{|l|+t} z^t Q_{|l|+t,l}
+    (1/R) sum_t z^t sum_{|l|<=p-2t} ghat_{|l|+t} Q_{|l|+t,l} w^l
 with w = rho u, z = rho^2, ghat_m = R^(2m+1) (R^-1 d/dR)^m G and the
-per-cluster Q of k_qcoef.  This restates that form in FP64 numpy, in the
-kernel's Horner order, and checks it against the oracle's coefficient
+per-cluster Q of k_qcoef.  This restates that form in FP64, in the kernel's
+sweep order (ratio Horner), and checks it against the oracle's coefficient
 recurrence (ora_taylor_coeffs, pinned to the reference) on random clusters
 and targets that pass the MAC, screened and Coulomb, with and without delta.
 It also checks the consumption-order count nqcoef against bench.py's copy.
@@ -39,29 +39,34 @@ def qcoef(mhat, p):
 
 
 def far_q(q, p, d, s, kappa, delta2):
-    """(1/R) sum_l w^l P_l by Horner over l0, l1, l2 (descending), as k_farq."""
+    """(1/R) sum_t z^t ghat_t Phi_t, Phi_t by the ratio Horner sweep of k_farq:
+    t descending, then l0, l1, l2 descending, a step in direction e at degree
+    n multiplying by r_{n+t} w_e, r_m = ghat_{m+1} / ghat_m."""
     R = math.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + delta2)
     lam, rho = kappa * R, s / R
     w = [rho * d[e] / R for e in range(3)]
     z = rho * rho
-    g = [math.exp(-lam), -math.exp(-lam) * (1.0 + lam)]
+    r = [-(1.0 + lam)]
     for m in range(1, p):
-        g.append(lam * lam * g[m - 1] - (2 * m + 1) * g[m])
-
-    def P(l):
-        dl = sum(l)
-        return sum(g[dl + t] * z ** t * q[(dl + t, l)] for t in range((p - dl) // 2 + 1))
-
-    r = 0.0
-    for l0 in range(p, -1, -1):
-        a = 0.0
-        for l1 in range(p - l0, -1, -1):
-            b = 0.0
-            for l2 in range(p - l0 - l1, -1, -1):
-                b = b * w[2] + P((l0, l1, l2))
-            a = a * w[1] + b
-        r = r * w[0] + a
-    return r / R
+        r.append(lam * lam / r[m - 1] - (2 * m + 1))
+    g = [math.exp(-lam)]
+    for t in range(p // 2):
+        g.append(g[t] * r[t])
+    S = 0.0
+    for t in range(p // 2, -1, -1):
+        L = p - 2 * t
+        phi = 0.0
+        for l0 in range(L, -1, -1):
+            a = 0.0
+            for l1 in range(L - l0, -1, -1):
+                b = 0.0
+                for l2 in range(L - l0 - l1, -1, -1):
+                    qv = q[(l0 + l1 + l2 + t, (l0, l1, l2))]
+                    b = qv if l2 == L - l0 - l1 else b * r[l0 + l1 + l2 + t] * w[2] + qv
+                a = b if l1 == L - l0 else a * r[l0 + l1 + t] * w[1] + b
+            phi = a if l0 == L else phi * r[l0 + t] * w[0] + a
+        S = g[t] * phi if t == p // 2 else S * z + g[t] * phi
+    return S / R
 
 
 @pytest.fixture(scope="module")
