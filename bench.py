@@ -40,7 +40,8 @@ PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 # matches the profiled one.
 MEASURED_TRAFFIC = {
     # (kernel, n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
-    ("near", 4_000_000, 3, 12, 0.25, 1000, 0.5, 0.005, "fp32"): 3.31436e9,
+    ("far", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 4.04204e9,
+    ("near", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 1.96254e9,
 }
 
 
@@ -107,7 +108,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--nterms", type=int, default=12)
     ap.add_argument("--acc", type=float, default=0.25)
-    ap.add_argument("--maxmemb", type=int, default=1000)
+    ap.add_argument("--maxmemb", type=int, default=500)
     ap.add_argument("--kappa", type=float, default=0.5)
     ap.add_argument("--delta", type=float, default=0.005)
     ap.add_argument("--seed", type=int, default=3)

@@ -23,8 +23,8 @@ import numpy as np
 import bit_amd
 from bit_amd import workloads
 
-DEFAULT_N0 = (500, 750, 1000, 1500, 2000)
-DEFAULT_NTERMS = (9, 10, 11, 12, 13)
+DEFAULT_N0 = (350, 500, 750, 1000, 2000)
+DEFAULT_NTERMS = (10, 11, 12, 13)
 DEFAULT_ACC = (0.4, 0.35, 0.3, 0.25, 0.2, 0.15)
 
 
@@ -78,12 +78,20 @@ def main():
     ap.add_argument("--delta", type=float, default=0.005)
     ap.add_argument("--target", type=float, default=1e-6)
     ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--n0s", default=",".join(map(str, DEFAULT_N0)),
+                    help="comma-separated leaf sizes")
+    ap.add_argument("--nterms", default=",".join(map(str, DEFAULT_NTERMS)),
+                    help="comma-separated expansion sizes (p + 1)")
+    ap.add_argument("--accs", default=",".join(map(str, DEFAULT_ACC)),
+                    help="comma-separated MAC parameters, largest first")
     a = ap.parse_args()
     P = workloads.make(a.kind, a.n, a.seed)
     ctx = bit_amd.Context(0)
     t0 = time.perf_counter()
     best, table = autotune(ctx, P, a.kappa, a.delta, a.target, precision=a.precision,
-                           verbose=True)
+                           n0s=[int(v) for v in a.n0s.split(",")],
+                           nterms_list=[int(v) for v in a.nterms.split(",")],
+                           accs=[float(v) for v in a.accs.split(",")], verbose=True)
     print(json.dumps({"best": best, "candidates": len(table),
                       "tuning_s": time.perf_counter() - t0}), flush=True)
     ctx.close()
