@@ -113,7 +113,7 @@ def main():
     ap.add_argument("--delta", type=float, default=0.005)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--accuracy-targets", type=int, default=10_000)
-    ap.add_argument("--cpu-sample", type=int, default=3000)
+    ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=800,
