@@ -1403,6 +1403,23 @@ __device__ __forceinline__ double pair64(double px, double py, double pz, double
   else return s.w / R;
 }
 
+#ifndef BIT_NEAR_SPARSE
+#define BIT_NEAR_SPARSE 16
+#endif
+// Near items with at most this many targets are evaluated source-parallel.
+constexpr int kNearSparse = BIT_NEAR_SPARSE;
+
+__device__ __forceinline__ float bcast_lane(float x, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+
+// Sum over the wave, the same value in every lane (fixed butterfly order).
+__device__ __forceinline__ float wave_sum(float g) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o);
+  return g;
+}
+
 template <bool SCR>
 __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ descs,
                                                const int2* __restrict__ map) {
@@ -1431,6 +1448,35 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
     // particle targets: the self pair lies in leaves holding some of the
     // batch's targets; point targets: pairs closer than eps are skipped instead
     const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
+    if (__popcll(mask) <= kNearSparse) {
+      // Few targets: lanes take the leaf's sources 64 at a time and the wave
+      // walks the item's targets in lane order; a target's sum over the leaf
+      // is one wave reduction, added to its lane's accumulator.
+      unsigned long long m = mask;
+      while (m) {
+        const int tl = __builtin_ctzll(m);
+        m &= m - 1;
+        const float sx = bcast_lane(ox, tl), sy = bcast_lane(oy, tl), sz = bcast_lane(oz, tl);
+        const int me = t0 + tl;  // the target's own slot (particle targets)
+        float g = 0.0f;
+        for (int j = r.x + lane; j < r.y; j += 64) {
+          const float4 q = tv.sloc[j];
+          const float v = pair32<SCR>(0.0f, sx, sy, sz, q, d2, kneg2);
+          if (!selfp) {
+            g += v;
+          } else if (tv.self) {
+            g += (j == me) ? 0.0f : v;
+          } else {  // point targets: skip coincident sources (R^2 <= eps)
+            const float dx = sx - q.x, dy = sy - q.y, dz = sz - q.z;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
+            g += r2 > 1e-14f ? v : 0.0f;
+          }
+        }
+        g = wave_sum(g);
+        if (lane == tl) acc.add(g);
+      }
+      continue;
+    }
     for (int base = r.x; base < r.y; base += 64) {
       const int cnt = min(64, r.y - base);
       if (lane < cnt) tile[lane] = tv.sloc[base + lane];
