@@ -1285,16 +1285,21 @@ __device__ __forceinline__ float qsetup(QPair<NT>& v, double px, double py, doub
 }
 
 // FP32 far field in the radial-derivative form: the cluster's Q is staged in
-// LDS like the FP64 kernel's moments (the next item's Q is loaded into
-// registers while the current one is evaluated) and read as broadcasts.
+// LDS like the FP64 kernel's moments (the next tiles are loaded into registers
+// while the current ones are evaluated) and read as broadcasts.  Two
+// consecutive items whose target masks are disjoint share one sweep, each lane
+// reading its own item's tile: a target still receives its items in list
+// order, so the results are those of one item at a time.
 // Four waves per SIMD: the sweep is chains of dependent FMAs, and the issue
 // rate of one wave is well below the SIMD's.
 template <int NT, bool SCR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_farq(
     const EvalDesc* __restrict__ descs, const int2* __restrict__ map) {
   constexpr int NQ = nqcoef(NT);
-  constexpr int PF = nqpad(NT) / kBatch;  // Q values per lane per item
-  __shared__ __align__(16) float mbuf[PF * kBatch];
+  constexpr int NQP = nqpad(NT);
+  constexpr int PF = NQP / kBatch;   // Q values per lane per tile
+  constexpr int T2 = NQP + 16;       // second tile's offset: other LDS banks
+  __shared__ __align__(16) float mbuf[T2 + NQP];
   int sys, b;
   which(map, sys, b);
   const EvalDesc& d = descs[sys];
@@ -1312,27 +1317,58 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   Acc<float> acc;
   const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
-  float pf[PF];
-  auto fetch = [&](int node) {  // node tiles are padded to PF * 64 floats
-#pragma unroll
-    for (int j = 0; j < PF; j++) pf[j] = qc[(size_t)node * (PF * kBatch) + j * kBatch + lane];
+  // items [u, u + n): n = 2 when item u+1 exists and the masks are disjoint
+  auto unit_len = [&](int u) -> int {
+    if (u + 1 >= i1) return 1;
+    const unsigned long long m0 = uniform64(items[u].mask), m1 = uniform64(items[u + 1].mask);
+    return (m0 & m1) ? 1 : 2;
   };
-  if (i0 < i1) fetch(uniform(items[i0].node));
-  for (int it = i0; it < i1; it++) {
-    const unsigned long long mask = uniform64(items[it].mask);
-    const int node = uniform(items[it].node);
+  float pa[PF], pb[PF];
+  auto fetch = [&](int u, int n) {  // node tiles are padded to NQP floats
+    const int na = uniform(items[u].node);
 #pragma unroll
-    for (int j = 0; j < PF; j++) mbuf[j * kBatch + lane] = pf[j];
+    for (int j = 0; j < PF; j++) pa[j] = qc[(size_t)na * NQP + j * kBatch + lane];
+    if (n == 2) {
+      const int nb = uniform(items[u + 1].node);
+#pragma unroll
+      for (int j = 0; j < PF; j++) pb[j] = qc[(size_t)nb * NQP + j * kBatch + lane];
+    }
+  };
+  int u = i0, n = 0;
+  if (u < i1) {
+    n = unit_len(u);
+    fetch(u, n);
+  }
+  while (u < i1) {
+    const unsigned long long m0 = uniform64(items[u].mask);
+    const unsigned long long m1 = n == 2 ? uniform64(items[u + 1].mask) : 0ull;
+    const int n0 = uniform(items[u].node);
+    const int n1 = n == 2 ? uniform(items[u + 1].node) : n0;
+#pragma unroll
+    for (int j = 0; j < PF; j++) mbuf[j * kBatch + lane] = pa[j];
+    if (n == 2) {
+#pragma unroll
+      for (int j = 0; j < PF; j++) mbuf[T2 + j * kBatch + lane] = pb[j];
+    }
     __syncthreads();
-    if (it + 1 < i1) fetch(uniform(items[it + 1].node));
-    if ((mask >> lane) & 1ull) {
+    const int un = u + n;
+    int nn = 0;
+    if (un < i1) {
+      nn = unit_len(un);
+      fetch(un, nn);
+    }
+    const bool in0 = (m0 >> lane) & 1ull, in1 = (m1 >> lane) & 1ull;
+    if (in0 || in1) {
+      const int node = in1 ? n1 : n0;
       const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
       QPair<NT> v;
       const float rinv = qsetup<NT, SCR>(v, px, py, pz, cx, cy, cz, (float)scale[node], kappa32,
                                          delta2_32);
-      acc.add(qall<NT>(std::make_integer_sequence<int, NQ>{}, v, mbuf) * rinv);
+      acc.add(qall<NT>(std::make_integer_sequence<int, NQ>{}, v, mbuf + (in1 ? T2 : 0)) * rinv);
     }
-    __syncthreads();  // the tile is overwritten next item
+    __syncthreads();  // the tiles are overwritten next unit
+    u = un;
+    n = nn;
   }
   if (valid) d.phi_far[t] = acc.value();
 }
