@@ -92,6 +92,37 @@ def test_lists_bit_exact(ctx, oracle, name, acc, d2, sub):
         assert near[slot] == list(nr), slot
 
 
+def _tree_depth(children):
+    depth = np.zeros(len(children), dtype=np.int64)
+    for i in range(len(children)):
+        for c in children[i]:
+            if c >= 0:
+                depth[c] = depth[i] + 1
+    return int(depth.max())
+
+
+def test_lists_deep_tree_bit_exact(ctx, oracle):
+    """About 100 levels (points at geometric distances along a line, maxmemb 1):
+    the traversal's stack, sized 8 entries per level, holds every path and the
+    lists equal the oracle's."""
+    rng = np.random.default_rng(77)
+    k = np.arange(1, 101)
+    x = np.concatenate([0.5 * 2.0 ** (-k), rng.uniform(-0.5, 0.5, 60)])
+    y = np.concatenate([np.zeros(100), rng.uniform(-0.5, 0.5, 60)])
+    z = np.concatenate([np.zeros(100), rng.uniform(-0.5, 0.5, 60)])
+    q = rng.uniform(-1, 1, x.size)
+    s = bit_amd.System(ctx, x, y, z, q, 1)
+    t = s.export_tree()
+    assert _tree_depth(t.children) > 75
+    lst = s.lists(4, 0.5, 0.0, 0.0)
+    far, near = _per_target_lists(lst, s.n)
+    o = oracle.tree(x, y, z, 1)
+    for slot in range(s.n):
+        f, nr = o.lists(int(t.perm[slot]), 4, 0.5, 0.0, 0.0)
+        assert far[slot] == list(f), slot
+        assert near[slot] == list(nr), slot
+
+
 @pytest.mark.parametrize("name", ["uniform10k", "uniform12k_reg", "blobs12k_screened", "aniso6k",
                                   "tiny1", "tiny5", "tiny300"])
 def test_potential_fp64_matches_reference(ctx, name):
