@@ -325,7 +325,9 @@ class DeviceEngine : public bit::PartitionEngine {
   static size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
 
   // One round: the op and chunk tables go up in one copy from pinned memory,
-  // the per-op results come back into pinned memory.
+  // the per-op results come back into pinned memory as soon as the counting
+  // phase is done, and the host works out the next round while the slots move
+  // (the next round's work is queued behind the move on the same stream).
   int step(int nops, int nch) {
     hipStream_t st = c_->stream;
     const size_t o_ch = al(nops * sizeof(bit::dev::PartOpDev));
@@ -346,13 +348,20 @@ class DeviceEngine : public bit::PartitionEngine {
     BIT_TRY(d_ores_.ensure(c_, rbytes));
     HIP_TRY(hipMemcpyAsync(d_blob_.p, h, total, hipMemcpyHostToDevice, st));
     char* d = d_blob_.as<char>();
-    HIP_TRY(bit::dev::partition_round(
-        a_, nops, reinterpret_cast<bit::dev::PartOpDev*>(d), nch,
-        reinterpret_cast<bit::dev::PartChunk*>(d + o_ch), reinterpret_cast<int*>(d + o_c0),
-        reinterpret_cast<int*>(d + o_nc), d_cres_.as<bit::dev::ChunkResult>(),
-        d_cbase_.as<int2>(), d_ores_.as<bit::dev::ChunkResult>(), st));
+    auto* dops = reinterpret_cast<bit::dev::PartOpDev*>(d);
+    auto* dch = reinterpret_cast<bit::dev::PartChunk*>(d + o_ch);
+    HIP_TRY(bit::dev::partition_count(a_, nops, dops, nch, dch, reinterpret_cast<int*>(d + o_c0),
+                                      reinterpret_cast<int*>(d + o_nc),
+                                      d_cres_.as<bit::dev::ChunkResult>(), d_cbase_.as<int2>(),
+                                      d_ores_.as<bit::dev::ChunkResult>(), st));
     HIP_TRY(hipMemcpyAsync(down_.p, d_ores_.p, rbytes, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));  // the pinned tables are reused next round
+    if (!counted_) HIP_TRY(hipEventCreateWithFlags(&counted_, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(counted_, st));
+    HIP_TRY(bit::dev::partition_move(a_, dops, nch, dch, d_cbase_.as<int2>(),
+                                     d_ores_.as<bit::dev::ChunkResult>(), st));
+    // The pinned tables are reused next round: the upload has run before the
+    // counting phase, and the results are read here.
+    HIP_TRY(hipEventSynchronize(counted_));
     std::memcpy(hres_.data(), down_.p, rbytes);
     return BIT_OK;
   }
@@ -390,6 +399,12 @@ class DeviceEngine : public bit::PartitionEngine {
   std::vector<bit::dev::ChunkResult> hres_;
   Pinned up_, down_;
   DevBuf d_blob_, d_cres_, d_cbase_, d_ores_;
+  hipEvent_t counted_ = nullptr;
+
+ public:
+  ~DeviceEngine() {
+    if (counted_) (void)hipEventDestroy(counted_);
+  }
 };
 
 // Builds the tree of n particles whose coordinates are in HBM (original order)

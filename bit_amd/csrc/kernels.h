@@ -89,11 +89,16 @@ hipError_t nonfinite(int n, const double* x, const double* y, const double* z, c
                      int* flag, hipStream_t s);
 hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int blocks,
                   double* part, hipStream_t s);
-// All split ops of one round; after it, ores[o] holds op o's count and boxes
-// and the slot arrays hold the split ranges.
-hipError_t partition_round(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
+// All split ops of one round, in two phases on one stream.  After
+// partition_count, ores[o] holds op o's count and boxes (what the host needs
+// for the next round); partition_move then moves the slots, and after it the
+// slot arrays hold the split ranges.
+hipError_t partition_count(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
                            const PartChunk* chunks, const int* op_chunk0, const int* op_nchunk,
                            ChunkResult* cres, int2* cbase, ChunkResult* ores, hipStream_t s);
+hipError_t partition_move(const SlotArrays& a, const PartOpDev* ops, int nchunks,
+                          const PartChunk* chunks, const int2* cbase, const ChunkResult* ores,
+                          hipStream_t s);
 // Morton order inside leaves, written to the t* arrays (every slot covered).
 hipError_t leaf_morton(const SlotArrays& a, int nleaves, const LeafBox* leaves, hipStream_t s);
 

@@ -748,13 +748,20 @@ __global__ __launch_bounds__(256) void k_part_copy(SlotArrays a, const PartChunk
   }
 }
 
-hipError_t partition_round(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
+hipError_t partition_count(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
                            const PartChunk* chunks, const int* op_chunk0, const int* op_nchunk,
                            ChunkResult* cres, int2* cbase, ChunkResult* ores, hipStream_t s) {
   if (nchunks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_part_count, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cres);
   hipLaunchKernelGGL(k_part_offsets, dim3(nops), dim3(64), 0, s, chunks, op_chunk0, op_nchunk,
                      cres, cbase, ores);
+  return hipGetLastError();
+}
+
+hipError_t partition_move(const SlotArrays& a, const PartOpDev* ops, int nchunks,
+                          const PartChunk* chunks, const int2* cbase, const ChunkResult* ores,
+                          hipStream_t s) {
+  if (nchunks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_part_scatter, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cbase, ores);
   hipLaunchKernelGGL(k_part_copy, dim3(nchunks), dim3(256), 0, s, a, chunks);
   return hipGetLastError();
