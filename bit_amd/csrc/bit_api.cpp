@@ -295,7 +295,7 @@ class DeviceEngine : public bit::PartitionEngine {
     nc_.resize(nops);
     chunks_.clear();
     for (int o = 0; o < nops; o++) {
-      hops_[o] = bit::dev::PartOpDev{ops[o].b, ops[o].e, ops[o].dir, 0, ops[o].mid};
+      hops_[o] = bit::dev::PartOpDev{ops[o].b, ops[o].e, ops[o].dir, ops[o].par, ops[o].mid};
       c0_[o] = (int)chunks_.size();
       for (int s0 = ops[o].b; s0 < ops[o].e; s0 += bit::dev::kPartChunk)
         chunks_.push_back(bit::dev::PartChunk{o, s0, std::min(ops[o].e, s0 + bit::dev::kPartChunk), 0});
@@ -450,22 +450,20 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
   std::string err;
   if (!bit::build_tree_generic(eng, n, mn, mx, maxmemb, s->tree, err))
     return eng.status != BIT_OK ? eng.status : fail(BIT_E_TREE, err);
-  // Morton order inside leaves (same keys as the host build), via t* arrays
+  // Morton order inside leaves (same keys as the host build), into the sorted
+  // arrays from the buffer each leaf's slots are in
   std::vector<bit::dev::LeafBox> lb;
   const bit::HostTree& T = s->tree;
   for (int i = 0; i < T.numtree; i++) {
     if (T.children[(size_t)i * 8] >= 0) continue;
     const double* f = &T.box[(size_t)i * bit::BOX_FIELDS];
-    lb.push_back(bit::dev::LeafBox{T.begin[i], T.end[i], f[bit::BX0], f[bit::BX1], f[bit::BY0],
+    lb.push_back(bit::dev::LeafBox{T.begin[i], T.end[i], (int)T.par[i], 0, f[bit::BX0],
+                                   f[bit::BX1], f[bit::BY0],
                                    f[bit::BY1], f[bit::BZ0], f[bit::BZ1]});
   }
   DevBuf d_lb;
   BIT_TRY(upload(ctx, d_lb, lb.data(), lb.size() * sizeof(bit::dev::LeafBox)));
   HIP_TRY(bit::dev::leaf_morton(a, (int)lb.size(), d_lb.as<bit::dev::LeafBox>(), st));
-  HIP_TRY(hipMemcpyAsync(s->perm.p, tP.p, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(s->sx.p, tX.p, nb, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(s->sy.p, tY.p, nb, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(s->sz.p, tZ.p, nb, hipMemcpyDeviceToDevice, st));
   // charges in slot order (the FP32 copy is completed by leaf_local)
   BIT_TRY(s->sq.ensure(ctx, nb));
   BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));

@@ -55,7 +55,10 @@ struct ListStats {
 };
 
 // ---- tree build on the device ----
-struct SlotArrays {  // slot order: P (original index), coordinates; t* = scratch
+// Slot order: P (original index), coordinates.  Two buffers: a split moves a
+// range from the buffer its node's data is in to the other one (the node's
+// parity), so no copy back is needed; the leaf pass writes the first buffer.
+struct SlotArrays {
   int* P;
   double* X;
   double* Y;
@@ -66,7 +69,7 @@ struct SlotArrays {  // slot order: P (original index), coordinates; t* = scratc
   double* tZ;
 };
 struct PartOpDev {
-  int b, e, dir, pad;
+  int b, e, dir, src;  // src: buffer holding the range (0: P..Z, 1: tP..tZ)
   double mid;
 };
 struct PartChunk {
@@ -77,7 +80,7 @@ struct ChunkResult {
   double v[12];  // min side0 xyz, min side1 xyz, max side0 xyz, max side1 xyz
 };
 struct LeafBox {
-  int b, e;
+  int b, e, src, pad;  // src: buffer holding the leaf's slots
   double x0, x1, y0, y1, z0, z1;
 };
 constexpr int kPartChunk = 4096;  // slots per work-group of a split
@@ -99,7 +102,8 @@ hipError_t partition_count(const SlotArrays& a, int nops, const PartOpDev* ops, 
 hipError_t partition_move(const SlotArrays& a, const PartOpDev* ops, int nchunks,
                           const PartChunk* chunks, const int2* cbase, const ChunkResult* ores,
                           hipStream_t s);
-// Morton order inside leaves, written to the t* arrays (every slot covered).
+// Morton order inside leaves, written to the first buffer (P..Z) from the
+// buffer each leaf's slots are in (every slot covered).
 hipError_t leaf_morton(const SlotArrays& a, int nleaves, const LeafBox* leaves, hipStream_t s);
 
 hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,

@@ -538,7 +538,17 @@ hipError_t leaf_local(const TreeView& tv, const int* leaves, int nleaves, float4
 // leaves.  Only comparisons and min/max: the results are exact, so the tree is
 // the one the host build (and the reference) produces.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double coord_of(const SlotArrays& a, int dir, int i) {
+// One of the two slot buffers.
+struct SlotBuf {
+  int* P;
+  double* X;
+  double* Y;
+  double* Z;
+};
+__device__ __forceinline__ SlotBuf slot_buf(const SlotArrays& a, int which) {
+  return which ? SlotBuf{a.tP, a.tX, a.tY, a.tZ} : SlotBuf{a.P, a.X, a.Y, a.Z};
+}
+__device__ __forceinline__ double coord_of(const SlotBuf& a, int dir, int i) {
   return dir == 0 ? a.X[i] : (dir == 1 ? a.Y[i] : a.Z[i]);
 }
 
@@ -596,13 +606,14 @@ hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int 
 }
 
 // Per chunk: how many slots go to the `<=` side, and both sides' boxes.
-__global__ __launch_bounds__(256) void k_part_count(SlotArrays a, const PartOpDev* __restrict__ ops,
+__global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpDev* __restrict__ ops,
                                                     const PartChunk* __restrict__ chunks,
                                                     ChunkResult* __restrict__ cres) {
   __shared__ double red[12][256];
   __shared__ int cnt[256];
   const PartChunk ch = chunks[blockIdx.x];
   const PartOpDev op = ops[ch.op];
+  const SlotBuf a = slot_buf(sa, op.src);
   int c = 0;
   double v[12];
   for (int k = 0; k < 6; k++) {
@@ -694,15 +705,16 @@ __global__ __launch_bounds__(64) void k_part_offsets(const PartChunk* __restrict
   }
 }
 
-// Stable scatter of each chunk into the temporary arrays: 256-slot tiles,
-// ranks from a work-group scan of the side flags.
-__global__ __launch_bounds__(256) void k_part_scatter(SlotArrays a, const PartOpDev* __restrict__ ops,
+// Stable scatter of each chunk into the other buffer: 256-slot tiles, ranks
+// from a work-group scan of the side flags.
+__global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartOpDev* __restrict__ ops,
                                                       const PartChunk* __restrict__ chunks,
                                                       const int2* __restrict__ cbase,
                                                       const ChunkResult* __restrict__ ores) {
   __shared__ int wsum[4];
   const PartChunk ch = chunks[blockIdx.x];
   const PartOpDev op = ops[ch.op];
+  const SlotBuf a = slot_buf(sa, op.src), t = slot_buf(sa, op.src ^ 1);
   const int opnl = ores[ch.op].nl;
   const int2 base = cbase[blockIdx.x];
   int lrun = base.x, rrun = base.y;
@@ -727,24 +739,14 @@ __global__ __launch_bounds__(256) void k_part_scatter(SlotArrays a, const PartOp
       int dst;
       if (left) dst = op.b + lrun + lrank;
       else dst = op.b + opnl + rrun + (rank_in_tile - lrank);
-      a.tP[dst] = a.P[i];
-      a.tX[dst] = a.X[i];
-      a.tY[dst] = a.Y[i];
-      a.tZ[dst] = a.Z[i];
+      t.P[dst] = a.P[i];
+      t.X[dst] = a.X[i];
+      t.Y[dst] = a.Y[i];
+      t.Z[dst] = a.Z[i];
     }
     lrun += tot;
     rrun += ntile - tot;
     __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(256) void k_part_copy(SlotArrays a, const PartChunk* __restrict__ chunks) {
-  const PartChunk ch = chunks[blockIdx.x];
-  for (int i = ch.s0 + threadIdx.x; i < ch.s1; i += 256) {
-    a.P[i] = a.tP[i];
-    a.X[i] = a.tX[i];
-    a.Y[i] = a.tY[i];
-    a.Z[i] = a.tZ[i];
   }
 }
 
@@ -763,13 +765,13 @@ hipError_t partition_move(const SlotArrays& a, const PartOpDev* ops, int nchunks
                           hipStream_t s) {
   if (nchunks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_part_scatter, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cbase, ores);
-  hipLaunchKernelGGL(k_part_copy, dim3(nchunks), dim3(256), 0, s, a, chunks);
   return hipGetLastError();
 }
 
 // Morton order inside each leaf (same key as the host build): bitonic sort of
-// (key, position) pairs in LDS, then a gather into the temporary arrays.
-// Leaves larger than kMortonMax keep their order.
+// (key, position) pairs in LDS, then a gather into the first buffer (through
+// the second for leaves held in the first).  Leaves larger than kMortonMax keep
+// their order.
 __device__ __forceinline__ unsigned spread3_dev(unsigned v) {
   v &= 0x3ffu;
   v = (v | (v << 16)) & 0x030000ffu;
@@ -787,17 +789,20 @@ __device__ __forceinline__ unsigned quant_dev(double v, double lo, double hi) {
   return (unsigned)t;
 }
 
-__global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays a, const LeafBox* __restrict__ leaves) {
+__global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays sa,
+                                                     const LeafBox* __restrict__ leaves) {
   __shared__ unsigned long long key[kMortonMax];
   const LeafBox L = leaves[blockIdx.x];
   const int n = L.e - L.b;
+  const SlotBuf a = slot_buf(sa, L.src), out = slot_buf(sa, 0);
   if (n <= 1 || n > kMortonMax) {
-    for (int i = threadIdx.x; i < n; i += 256) {
-      a.tP[L.b + i] = a.P[L.b + i];
-      a.tX[L.b + i] = a.X[L.b + i];
-      a.tY[L.b + i] = a.Y[L.b + i];
-      a.tZ[L.b + i] = a.Z[L.b + i];
-    }
+    if (L.src)
+      for (int i = threadIdx.x; i < n; i += 256) {
+        out.P[L.b + i] = a.P[L.b + i];
+        out.X[L.b + i] = a.X[L.b + i];
+        out.Y[L.b + i] = a.Y[L.b + i];
+        out.Z[L.b + i] = a.Z[L.b + i];
+      }
     return;
   }
   int m = 1;
@@ -830,12 +835,23 @@ __global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays a, const LeafBox
       __syncthreads();
     }
   }
+  // gather into the other buffer, then (leaves held in the first buffer) back
+  const SlotBuf g = slot_buf(sa, L.src ^ 1);
   for (int i = threadIdx.x; i < n; i += 256) {
     const int src = L.b + (int)(key[i] & 0xffffffffu);
-    a.tP[L.b + i] = a.P[src];
-    a.tX[L.b + i] = a.X[src];
-    a.tY[L.b + i] = a.Y[src];
-    a.tZ[L.b + i] = a.Z[src];
+    g.P[L.b + i] = a.P[src];
+    g.X[L.b + i] = a.X[src];
+    g.Y[L.b + i] = a.Y[src];
+    g.Z[L.b + i] = a.Z[src];
+  }
+  if (L.src == 0) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) {
+      out.P[L.b + i] = g.P[L.b + i];
+      out.X[L.b + i] = g.X[L.b + i];
+      out.Y[L.b + i] = g.Y[L.b + i];
+      out.Z[L.b + i] = g.Z[L.b + i];
+    }
   }
 }
 

@@ -162,6 +162,7 @@ int stable_partition(Arrays& a, int b, int e, int dir, double mid, MinMax mm[2],
 struct Slot {
   bool used = false;
   int b = 0, e = 0;
+  int par = 0;  // splits that moved these slots, mod 2
   double f[BOX_FIELDS];
   int nummemb() const { return used ? e - b : 0; }
 };
@@ -208,6 +209,7 @@ class SplitProgram {
         t.used = true;
         t.b = lo[sd];
         t.e = hi[sd];
+        t.par = src_.par ^ 1;
         box_init(t.f, r.mn[sd][0], r.mx[sd][0], r.mn[sd][1], r.mx[sd][1], r.mn[sd][2],
                  r.mx[sd][2]);
         made_++;
@@ -236,7 +238,7 @@ class SplitProgram {
     src_ = src;
     c1_ = c1;
     c2_ = c2;
-    op_ = PartOp{src.b, src.e, dir, src.f[BXMID + dir]};
+    op_ = PartOp{src.b, src.e, dir, src.f[BXMID + dir], src.par};
   }
   bool add(int slot) {
     if (B.nchild >= 8) {
@@ -492,6 +494,7 @@ bool build_tree_generic(PartitionEngine& eng, int n, const double mn[3], const d
   out.begin.push_back(0);
   out.end.push_back(n);
   out.depth.push_back(0);
+  out.par.push_back(0);
 
   int gen_lo = 0, gen_hi = 1;
   std::vector<PartOp> ops;
@@ -508,6 +511,7 @@ bool build_tree_generic(PartitionEngine& eng, int n, const double mn[3], const d
       parent.used = true;
       parent.b = out.begin[i];
       parent.e = out.end[i];
+      parent.par = out.par[i];
       std::memcpy(parent.f, &out.box[(size_t)i * BOX_FIELDS], sizeof(parent.f));
       prog[idx].start(parent, maxmemb, lim);
       active.push_back(idx);
@@ -548,6 +552,7 @@ bool build_tree_generic(PartitionEngine& eng, int n, const double mn[3], const d
     out.begin.resize(next);
     out.end.resize(next);
     out.depth.resize(next);
+    out.par.resize(next);
     for (int idx = 0; idx < cnt; idx++) {
       const Block& B = prog[idx].B;
       int parent = gen_lo + idx;
@@ -557,6 +562,7 @@ bool build_tree_generic(PartitionEngine& eng, int n, const double mn[3], const d
         out.begin[g] = B.s[sl].b;
         out.end[g] = B.s[sl].e;
         out.depth[g] = out.depth[parent] + 1;
+        out.par[g] = (unsigned char)B.s[sl].par;
         out.maxdepth = std::max(out.maxdepth, out.depth[g]);
       }
       for (int c = 0; c < B.nchild; c++)

@@ -24,6 +24,9 @@ struct HostTree {
   std::vector<int> end;
   std::vector<int> depth;
   std::vector<int> perm;      // sorted slot -> original particle index
+  // Number of splits that moved node i's slots, mod 2 (an engine that moves
+  // slots between two buffers finds node i's data in buffer par[i]).
+  std::vector<unsigned char> par;
 };
 
 // Builds the tree over n particles (coordinates in original order).  Returns
@@ -33,6 +36,7 @@ struct HostTree {
 struct PartOp {
   int b, e, dir;
   double mid;
+  int par;  // splits that moved these slots so far, mod 2
 };
 // Its outcome: nl slots on the `<=` side, then each side's bounding box
 // (mn/mx[side][axis], meaningful for a non-empty side).
