@@ -179,6 +179,10 @@ __device__ __forceinline__ void coef_all(std::integer_sequence<int, C...>, T* a,
 // scalar loads (the values then live in SGPRs, not VGPRs).
 template <typename T>
 using gptr = const __attribute__((address_space(1))) T*;
+// Read-only data: loads with wave-uniform addresses through a constant
+// address-space pointer are scalar loads.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
 template <typename T>
 __device__ __forceinline__ gptr<T> as_global(const T* p) {
   return (gptr<T>)(p);
@@ -1099,6 +1103,12 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   const int tt = valid ? t : tv.nt - 1;
   const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   const unsigned long long active = ballot(valid);
+  // node data through constant-address-space pointers (the tree is read-only
+  // here): the wave-uniform addresses become scalar loads, issued together
+  // once per visit
+  const cptr<double> gcenter = (cptr<double>)(tv.center);
+  const cptr<int> gchild = (cptr<int>)(tv.child);
+  const cptr<int> grange = (cptr<int>)(tv.range);
   if (lane == 0) {
     st_node[0] = 0;
     st_mask[0] = active;
@@ -1114,7 +1124,11 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
     const int node = uniform(st_node[sp]);
     const unsigned long long m = uniform64(st_mask[sp]);
     __syncthreads();
-    const double4 c = tv.center[node];
+    const double4 c = make_double4(gcenter[4 * node], gcenter[4 * node + 1],
+                                   gcenter[4 * node + 2], gcenter[4 * node + 3]);
+    int kids[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) kids[q] = gchild[8 * node + q];
     const bool acc = mac_accept(px, py, pz, c, mp);
     const unsigned long long am = ballot(acc) & m;
     if (am) {
@@ -1124,25 +1138,27 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
     }
     const unsigned long long rm = m & ~am;
     if (rm) {
-      const int* ch = tv.child + (size_t)node * 8;
-      if (ch[0] < 0) {
+      if (kids[0] < 0) {
         if (FILL && lane == 0) near_items[nbase + nn] = Item{rm, node, 0};
         nn++;
-        const int2 r = tv.range[node];
+        const int2 r = make_int2(grange[2 * node], grange[2 * node + 1]);
         near_pairs += (unsigned long long)__popcll(rm) * (unsigned long long)(r.y - r.x);
         near_slots += (unsigned long long)(r.y - r.x);
       } else {
-        int nch = 0;
-        while (nch < 8 && ch[nch] >= 0) nch++;
+        int nch = 0;  // children fill the first slots, -1 after
+#pragma unroll
+        for (int q = 0; q < 8; q++) nch += (kids[q] >= 0 && nch == q) ? 1 : 0;
         if (sp + nch > stack_cap) {
           if (lane == 0) atomicOr(err_flag, 1);
           break;
         }
-        if (lane == 0)
-          for (int k = nch - 1; k >= 0; k--) {
-            st_node[sp + (nch - 1 - k)] = ch[k];
-            st_mask[sp + (nch - 1 - k)] = rm;
-          }
+        if (lane < nch) {  // child `lane` goes to slot sp + nch - 1 - lane
+          int kid = kids[0];
+#pragma unroll
+          for (int q = 1; q < 8; q++) kid = (lane == q) ? kids[q] : kid;
+          st_node[sp + (nch - 1 - lane)] = kid;
+          st_mask[sp + (nch - 1 - lane)] = rm;
+        }
         sp += nch;
       }
     }
