@@ -1479,20 +1479,13 @@ __device__ __forceinline__ double pair64(double px, double py, double pz, double
 }
 
 #ifndef BIT_NEAR_SPARSE
-#define BIT_NEAR_SPARSE 16
+#define BIT_NEAR_SPARSE 32
 #endif
 // Near items with at most this many targets are evaluated source-parallel.
 constexpr int kNearSparse = BIT_NEAR_SPARSE;
 
 __device__ __forceinline__ float bcast_lane(float x, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
-}
-
-// Sum over the wave, the same value in every lane (fixed butterfly order).
-__device__ __forceinline__ float wave_sum(float g) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o);
-  return g;
 }
 
 template <bool SCR>
@@ -1525,30 +1518,47 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
     const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
     if (__popcll(mask) <= kNearSparse) {
       // Few targets: lanes take the leaf's sources 64 at a time and the wave
-      // walks the item's targets in lane order; a target's sum over the leaf
-      // is one wave reduction, added to its lane's accumulator.
+      // walks the item's targets in lane order, four at a time (one source
+      // load feeds four pairs); a target's sum over the leaf is one wave
+      // reduction, added to its lane's accumulator.
       unsigned long long m = mask;
       while (m) {
-        const int tl = __builtin_ctzll(m);
-        m &= m - 1;
-        const float sx = bcast_lane(ox, tl), sy = bcast_lane(oy, tl), sz = bcast_lane(oz, tl);
-        const int me = t0 + tl;  // the target's own slot (particle targets)
-        float g = 0.0f;
+        int tl[4];
+        bool on[4];
+        float sx[4], sy[4], sz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          on[u] = m != 0ull;
+          tl[u] = on[u] ? __builtin_ctzll(m) : 0;
+          m &= m - 1;
+          sx[u] = bcast_lane(ox, tl[u]);
+          sy[u] = bcast_lane(oy, tl[u]);
+          sz[u] = bcast_lane(oz, tl[u]);
+        }
+        float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int j = r.x + lane; j < r.y; j += 64) {
           const float4 q = tv.sloc[j];
-          const float v = pair32<SCR>(0.0f, sx, sy, sz, q, d2, kneg2);
-          if (!selfp) {
-            g += v;
-          } else if (tv.self) {
-            g += (j == me) ? 0.0f : v;
-          } else {  // point targets: skip coincident sources (R^2 <= eps)
-            const float dx = sx - q.x, dy = sy - q.y, dz = sz - q.z;
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
-            g += r2 > 1e-14f ? v : 0.0f;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const float v = pair32<SCR>(0.0f, sx[u], sy[u], sz[u], q, d2, kneg2);
+            if (!selfp) {
+              g[u] += v;
+            } else if (tv.self) {
+              g[u] += (j == t0 + tl[u]) ? 0.0f : v;  // the target's own slot
+            } else {  // point targets: skip coincident sources (R^2 <= eps)
+              const float dx = sx[u] - q.x, dy = sy[u] - q.y, dz = sz[u] - q.z;
+              const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
+              g[u] += r2 > 1e-14f ? v : 0.0f;
+            }
           }
         }
-        g = wave_sum(g);
-        if (lane == tl) acc.add(g);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+          for (int u = 0; u < 4; u++) g[u] += __shfl_xor(g[u], o);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (on[u] && lane == tl[u]) acc.add(g[u]);
       }
       continue;
     }
