@@ -610,6 +610,8 @@ hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int 
 }
 
 // Per chunk: how many slots go to the `<=` side, and both sides' boxes.
+// Four slots per thread and pass (their loads in flight together); the side
+// decides by select which box a slot widens.
 __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpDev* __restrict__ ops,
                                                     const PartChunk* __restrict__ chunks,
                                                     ChunkResult* __restrict__ cres) {
@@ -617,28 +619,53 @@ __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpD
   __shared__ int cnt[256];
   const PartChunk ch = chunks[blockIdx.x];
   const PartOpDev op = ops[ch.op];
-  const SlotBuf a = slot_buf(sa, op.src);
+  const double* __restrict__ X = op.src ? sa.tX : sa.X;
+  const double* __restrict__ Y = op.src ? sa.tY : sa.Y;
+  const double* __restrict__ Z = op.src ? sa.tZ : sa.Z;
   int c = 0;
+  // v[k]: min side 0 xyz, min side 1 xyz, max side 0 xyz, max side 1 xyz
   double v[12];
+#pragma unroll
   for (int k = 0; k < 6; k++) {
-    v[k] = INFINITY;      // min: side 0 x,y,z then side 1 x,y,z
-    v[6 + k] = -INFINITY; // max
+    v[k] = INFINITY;
+    v[6 + k] = -INFINITY;
   }
-  for (int i = ch.s0 + threadIdx.x; i < ch.s1; i += 256) {
-    const double p[3] = {a.X[i], a.Y[i], a.Z[i]};
-    const int side = coord_of(a, op.dir, i) <= op.mid ? 0 : 1;
-    c += side == 0;
-    for (int d = 0; d < 3; d++) {
-      v[3 * side + d] = fmin(v[3 * side + d], p[d]);
-      v[6 + 3 * side + d] = fmax(v[6 + 3 * side + d], p[d]);
+  for (int i0 = ch.s0 + threadIdx.x; i0 < ch.s1; i0 += 4 * 256) {
+    double px[4], py[4], pz[4];
+    bool in[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + u * 256;
+      in[u] = i < ch.s1;
+      const int j = in[u] ? i : ch.s0;
+      px[u] = X[j];
+      py[u] = Y[j];
+      pz[u] = Z[j];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (!in[u]) continue;
+      const double p[3] = {px[u], py[u], pz[u]};
+      const double cv = op.dir == 0 ? p[0] : (op.dir == 1 ? p[1] : p[2]);
+      const bool left = cv <= op.mid;
+      c += left;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        v[d] = left ? fmin(v[d], p[d]) : v[d];
+        v[3 + d] = left ? v[3 + d] : fmin(v[3 + d], p[d]);
+        v[6 + d] = left ? fmax(v[6 + d], p[d]) : v[6 + d];
+        v[9 + d] = left ? v[9 + d] : fmax(v[9 + d], p[d]);
+      }
     }
   }
   cnt[threadIdx.x] = c;
+#pragma unroll
   for (int k = 0; k < 12; k++) red[k][threadIdx.x] = v[k];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       cnt[threadIdx.x] += cnt[threadIdx.x + w];
+#pragma unroll
       for (int k = 0; k < 12; k++)
         red[k][threadIdx.x] = k < 6 ? fmin(red[k][threadIdx.x], red[k][threadIdx.x + w])
                                     : fmax(red[k][threadIdx.x], red[k][threadIdx.x + w]);
@@ -648,6 +675,7 @@ __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpD
   if (threadIdx.x == 0) {
     ChunkResult r;
     r.nl = cnt[0];
+    r.pad = 0;
     for (int k = 0; k < 12; k++) r.v[k] = red[k][0];
     cres[blockIdx.x] = r;
   }
@@ -709,48 +737,71 @@ __global__ __launch_bounds__(64) void k_part_offsets(const PartChunk* __restrict
   }
 }
 
-// Stable scatter of each chunk into the other buffer: 256-slot tiles, ranks
-// from a work-group scan of the side flags.
+// Stable scatter of each chunk into the other buffer.  Tiles of 1024 slots,
+// four rows of 256 (slot t0 + 256 u + thread); a slot's rank among its tile's
+// `<=` slots comes from the wave ballots of all four rows, one barrier per
+// tile, and every load of the tile is in flight before it.
 __global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartOpDev* __restrict__ ops,
                                                       const PartChunk* __restrict__ chunks,
                                                       const int2* __restrict__ cbase,
                                                       const ChunkResult* __restrict__ ores) {
-  __shared__ int wsum[4];
+  __shared__ int wsum[4][4];  // [row][wave] counts of `<=` slots
   const PartChunk ch = chunks[blockIdx.x];
   const PartOpDev op = ops[ch.op];
   const SlotBuf a = slot_buf(sa, op.src), t = slot_buf(sa, op.src ^ 1);
+  const double* __restrict__ C = op.dir == 0 ? a.X : (op.dir == 1 ? a.Y : a.Z);
   const int opnl = ores[ch.op].nl;
   const int2 base = cbase[blockIdx.x];
   int lrun = base.x, rrun = base.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t0 = ch.s0; t0 < ch.s1; t0 += 256) {
-    const int i = t0 + threadIdx.x;
-    const bool in = i < ch.s1;
-    const bool left = in && coord_of(a, op.dir, i) <= op.mid;
-    const unsigned long long bl = __ballot(left);
-    const int before = __popcll(bl & ((1ull << lane) - 1ull));
-    if (lane == 0) wsum[wave] = __popcll(bl);
-    __syncthreads();
-    int woff = 0, tot = 0;
-    for (int w = 0; w < 4; w++) {
-      if (w < wave) woff += wsum[w];
-      tot += wsum[w];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int t0 = ch.s0; t0 < ch.s1; t0 += 1024) {
+    bool in[4], left[4];
+    int before[4], pv[4];
+    double xv[4], yv[4], zv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = t0 + u * 256 + threadIdx.x;
+      in[u] = i < ch.s1;
+      const int j = in[u] ? i : ch.s0;
+      pv[u] = a.P[j];
+      xv[u] = a.X[j];
+      yv[u] = a.Y[j];
+      zv[u] = a.Z[j];
+      left[u] = in[u] && C[j] <= op.mid;
     }
-    const int ntile = min(256, ch.s1 - t0);
-    if (in) {
-      const int lrank = woff + before;
-      const int rank_in_tile = threadIdx.x;
-      int dst;
-      if (left) dst = op.b + lrun + lrank;
-      else dst = op.b + opnl + rrun + (rank_in_tile - lrank);
-      t.P[dst] = a.P[i];
-      t.X[dst] = a.X[i];
-      t.Y[dst] = a.Y[i];
-      t.Z[dst] = a.Z[i];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const unsigned long long bl = __ballot(left[u]);
+      before[u] = __popcll(bl & below);
+      if (lane == 0) wsum[u][wave] = __popcll(bl);
     }
-    lrun += tot;
-    rrun += ntile - tot;
     __syncthreads();
+    int rowoff = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int woff = 0, tot = 0;
+#pragma unroll
+      for (int w = 0; w < 4; w++) {
+        const int c = wsum[u][w];
+        woff += w < wave ? c : 0;
+        tot += c;
+      }
+      if (in[u]) {
+        const int lrank = rowoff + woff + before[u];
+        const int pos = u * 256 + threadIdx.x;  // slot's place in the tile
+        const int dst = left[u] ? op.b + lrun + lrank : op.b + opnl + rrun + (pos - lrank);
+        t.P[dst] = pv[u];
+        t.X[dst] = xv[u];
+        t.Y[dst] = yv[u];
+        t.Z[dst] = zv[u];
+      }
+      rowoff += tot;
+    }
+    const int ntile = min(1024, ch.s1 - t0);
+    lrun += rowoff;
+    rrun += ntile - rowoff;
+    __syncthreads();  // wsum is rewritten by the next tile
   }
 }
 
