@@ -226,7 +226,7 @@ struct bit_system {
   // lists: pl for the particles as targets (cached per MAC parameters), ql for
   // the last external point set
   ListSet pl, ql;
-  DevBuf scan_tmp, lstats, err_flag;
+  DevBuf scan_tmp, lstats, pstat, err_flag;
   bit::dev::ListStats hstats{};
   // external point targets (batch order) and their results
   DevBuf qx, qy, qz, qperm, qphi, qout, qdesc;
@@ -577,15 +577,18 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   BIT_TRY(L.far_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
   BIT_TRY(L.near_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
   BIT_TRY(s->lstats.ensure(c, sizeof(bit::dev::ListStats)));
+  BIT_TRY(s->pstat.ensure(c, (size_t)nb * 3 * sizeof(unsigned long long)));
   BIT_TRY(s->err_flag.ensure(c, sizeof(int)));
   HIP_TRY(hipMemsetAsync(L.far_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
   HIP_TRY(hipMemsetAsync(L.near_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
-  HIP_TRY(hipMemsetAsync(s->lstats.p, 0, sizeof(bit::dev::ListStats), c->stream));
   HIP_TRY(hipMemsetAsync(s->err_flag.p, 0, sizeof(int), c->stream));
   s->t_lists.start(c->stream);
   HIP_TRY(bit::dev::lists_count(tv, mp, s->stack_cap, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
-                                s->lstats.as<bit::dev::ListStats>(), s->err_flag.as<int>(),
+                                s->pstat.as<unsigned long long>(), s->err_flag.as<int>(),
                                 c->stream));
+  HIP_TRY(bit::dev::list_stats(nb, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
+                               s->pstat.as<unsigned long long>(),
+                               s->lstats.as<bit::dev::ListStats>(), c->stream));
   size_t tmp = 0;
   HIP_TRY(bit::dev::exclusive_scan(L.far_cnt.as<int>(), L.far_off.as<int>(), nb + 1, nullptr, tmp,
                                    c->stream));

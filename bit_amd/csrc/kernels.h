@@ -138,7 +138,11 @@ hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float*
                         hipStream_t s);
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
-                       int* near_cnt, ListStats* stats, int* err_flag, hipStream_t s);
+                       int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s);
+// Totals over nb batches of the counts and of pstat (3 per batch: far pairs,
+// near pairs, near slots) written by lists_count.
+hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
+                      const unsigned long long* pstat, ListStats* out, hipStream_t s);
 hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,
                       const int* near_off, Item* far_items, Item* near_items, int* err_flag,
                       hipStream_t s);

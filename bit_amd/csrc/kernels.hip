@@ -1139,7 +1139,7 @@ template <bool FILL>
 __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int stack_cap,
                                               int* __restrict__ far_cnt,
                                               int* __restrict__ near_cnt,
-                                              ListStats* __restrict__ stats,
+                                              unsigned long long* __restrict__ pstat,
                                               const int* __restrict__ far_off,
                                               const int* __restrict__ near_off,
                                               Item* __restrict__ far_items,
@@ -1218,20 +1218,49 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   if (!FILL && lane == 0) {
     far_cnt[b] = nf;
     near_cnt[b] = nn;
-    atomicAdd(&stats->far_items, (unsigned long long)nf);
-    atomicAdd(&stats->near_items, (unsigned long long)nn);
-    atomicAdd(&stats->far_pairs, far_pairs);
-    atomicAdd(&stats->near_pairs, near_pairs);
-    atomicAdd(&stats->near_slots, near_slots);
+    pstat[3 * b] = far_pairs;
+    pstat[3 * b + 1] = near_pairs;
+    pstat[3 * b + 2] = near_slots;
   }
 }
 
+// Totals of the per-batch list counts (one work-group, fixed order).
+__global__ __launch_bounds__(1024) void k_list_stats(int nb, const int* __restrict__ far_cnt,
+                                                     const int* __restrict__ near_cnt,
+                                                     const unsigned long long* __restrict__ pstat,
+                                                     ListStats* __restrict__ out) {
+  __shared__ unsigned long long red[5][1024];
+  unsigned long long v[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += 1024) {
+    v[0] += (unsigned long long)far_cnt[b];
+    v[1] += (unsigned long long)near_cnt[b];
+    v[2] += pstat[3 * b];
+    v[3] += pstat[3 * b + 1];
+    v[4] += pstat[3 * b + 2];
+  }
+  for (int k = 0; k < 5; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    *out = ListStats{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0]};
+}
+
+hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
+                      const unsigned long long* pstat, ListStats* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(1024), 0, s, nb, far_cnt, near_cnt, pstat, out);
+  return hipGetLastError();
+}
+
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
-                       int* near_cnt, ListStats* stats, int* err_flag, hipStream_t s) {
+                       int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s) {
   int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
   hipLaunchKernelGGL(k_lists<false>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap, far_cnt,
-                     near_cnt, stats, (const int*)nullptr, (const int*)nullptr,
+                     near_cnt, pstat, (const int*)nullptr, (const int*)nullptr,
                      (Item*)nullptr, (Item*)nullptr, err_flag);
   return hipGetLastError();
 }
@@ -1242,7 +1271,8 @@ hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int
   int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
   hipLaunchKernelGGL(k_lists<true>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
-                     (int*)nullptr, (int*)nullptr, (ListStats*)nullptr, far_off, near_off,
+                     (int*)nullptr, (int*)nullptr, (unsigned long long*)nullptr, far_off,
+                     near_off,
                      far_items, near_items, err_flag);
   return hipGetLastError();
 }
