@@ -219,7 +219,8 @@ struct bit_system {
   // moments
   int mom_nterms = 0;
   int nchunks = 0;
-  DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, qc32;
+  DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, qc32, qc64;
+  int qc64_nterms = 0;  // nterms of qc64, 0 = not computed for the current moments
   // internal nodes grouped by height (leaves 0): up_off[h-1]..up_off[h] in up_nodes
   DevBuf up_nodes;
   std::vector<int> up_off;
@@ -918,6 +919,7 @@ int bit_moments(bit_system* s, int nterms) {
   // State changes only after every allocation and upload succeeded: a failed
   // call (e.g. BIT_E_OOM) leaves the system as it was, never half-initialised.
   s->mom_nterms = 0;
+  s->qc64_nterms = 0;
   if (s->nchunks == 0) {
     // leaves: sums over their particles in chunks; internal nodes: upward pass
     // from their children, height by height (a child's index exceeds its
@@ -983,6 +985,22 @@ int bit_moments(bit_system* s, int nterms) {
   return BIT_OK;
 }
 
+}  // extern "C"
+
+// FP64 cluster coefficients for the current moments, computed on first use.
+static int ensure_qc64(bit_system* s, int nterms) {
+  if (s->qc64_nterms == nterms) return BIT_OK;
+  bit_ctx* c = s->ctx;
+  const int nt = s->tree.numtree;
+  BIT_TRY(s->qc64.ensure(c, (size_t)nt * bit::dev::nqpad(nterms) * sizeof(double)));
+  HIP_TRY(bit::dev::far_coeffs64(nt, nterms, s->mom_hat64.as<double>(), s->qc64.as<double>(),
+                                 c->stream));
+  s->qc64_nterms = nterms;
+  return BIT_OK;
+}
+
+extern "C" {
+
 int bit_moments_download(const bit_system* s, double* out) {
   if (!s || !out) return fail(BIT_E_INVALID, "null system or output");
   if (s->mom_nterms == 0) return fail(BIT_E_STATE, "moments not computed");
@@ -1032,8 +1050,9 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
   d.tv = s->view();
   d.kappa = p->kappa;
   d.delta2 = p->delta2;
-  d.mom64 = s->mom_hat64.as<double>();
   d.qc32 = s->qc32.as<float>();
+  if (p->precision == BIT_FP64) BIT_TRY(ensure_qc64(s, p->nterms));
+  d.qc64 = s->qc64.as<double>();
   d.far_off = s->pl.far_off.as<int>();
   d.far_items = s->pl.far_items.as<bit::dev::Item>();
   d.near_off = s->pl.near_off.as<int>();
@@ -1232,8 +1251,9 @@ int bit_eval_potential_points(bit_system* s, const bit_params* p, int m, const d
   d.tv = tv;
   d.kappa = p->kappa;
   d.delta2 = p->delta2;
-  d.mom64 = s->mom_hat64.as<double>();
   d.qc32 = s->qc32.as<float>();
+  if (p->precision == BIT_FP64) BIT_TRY(ensure_qc64(s, p->nterms));
+  d.qc64 = s->qc64.as<double>();
   d.far_off = s->ql.far_off.as<int>();
   d.far_items = s->ql.far_items.as<bit::dev::Item>();
   d.near_off = s->ql.near_off.as<int>();

@@ -136,6 +136,8 @@ constexpr int nqcoef(int nt) {
 constexpr int nqpad(int nt) { return (nqcoef(nt) + 63) / 64 * 64; }
 hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qc32,
                         hipStream_t s);
+hipError_t far_coeffs64(int numtree, int nterms, const double* mom_hat64, double* qc64,
+                        hipStream_t s);
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
                        int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s);
@@ -155,8 +157,8 @@ struct EvalDesc {
   TreeView tv;
   double kappa;
   double delta2;
-  const double* mom64;   // moments scaled by s^-|k|, FP64
-  const float* qc32;     // far coefficients of the FP32 kernel (far_coeffs32)
+  const float* qc32;     // far coefficients, FP32 (far_coeffs32)
+  const double* qc64;    // far coefficients, FP64 (far_coeffs64)
   const int* far_off;    // [nbatch+1]
   const Item* far_items;
   const int* near_off;

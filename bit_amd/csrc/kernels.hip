@@ -188,31 +188,6 @@ __device__ __forceinline__ gptr<T> as_global(const T* p) {
   return (gptr<T>)(p);
 }
 
-// Far field of one cluster at one target in the normalised variables of
-// coef_step: u = (target - centre)/R, rho = s/R, c0 = exp(-lam), lam = kappa R.
-// Returns sum_k Mhat_k ahat_k with ahat_k = rho^{|k|+1} c_k (times 1/s = the
-// potential).
-template <typename T, int NT, bool SCR, typename MP>
-__device__ __forceinline__ T far_eval(T u0, T u1, T u2, T rho, T c0, T lam, MP M) {
-  constexpr int NC = ncoef(NT);
-  T a[NC];
-  T b[SCR ? NC : 1];
-  a[0] = c0;
-  if constexpr (SCR) b[0] = c0;
-  T kinv[NT > 1 ? NT : 2];
-  if constexpr (SCR) {
-#pragma unroll
-    for (int m = 1; m < NT; m++) kinv[m] = lam * T(1.0 / m);
-  }
-  const T x[3] = {u0, u1, u2};
-  const T xx[3] = {u0 + u0, u1 + u1, u2 + u2};
-  Shells<T> z{{T(0), T(0), T(0)}, {M[0] * c0 * rho, T(0), T(0)}, rho, rho};
-  if constexpr (NC > 1)
-    coef_all<T, NT, SCR, false, MP>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx, kinv,
-                                    M, z);
-  return z.tot[0];
-}
-
 // Field: E_e = sum_k (k_e+1) M_k a_{k+e}, coefficients to degree NT (Coulomb),
 // in the normalised variables of far_eval (c0 = 1).
 template <typename T, int NT, typename MP>
@@ -340,16 +315,16 @@ __constant__ double kInvFact[kMaxNterms + 1] = {
     1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0};
 
 // One thread per (node, Q entry), from the FP64 scaled moments.  Each node's
-// tile is padded with zeros to nqpad(NT) floats.
-template <int NT>
+// tile is padded with zeros to nqpad(NT) values.
+template <int NT, typename T>
 __global__ __launch_bounds__(256) void k_qcoef(int numtree, const double* __restrict__ mom_hat,
-                                               float* __restrict__ qc) {
+                                               T* __restrict__ qc) {
   constexpr int NQ = nqcoef(NT), NQP = nqpad(NT), NC = ncoef(NT);
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (long long)numtree * NQP) return;
   const int node = (int)(g / NQP), q = (int)(g - (long long)node * NQP);
   if (q >= NQ) {
-    qc[g] = 0.0f;
+    qc[g] = T(0);
     return;
   }
   const int l0 = kQTabDev<NT>.l[q][0], l1 = kQTabDev<NT>.l[q][1], l2 = kQTabDev<NT>.l[q][2];
@@ -363,59 +338,78 @@ __global__ __launch_bounds__(256) void k_qcoef(int numtree, const double* __rest
                 kInvFact[j0] * kInvFact[j1] * kInvFact[j2], sum);
     }
   const double w = ldexp(kInvFact[l0] * kInvFact[l1] * kInvFact[l2], -t);
-  qc[g] = (float)(((l0 + l1 + l2) & 1 ? -w : w) * sum);
+  qc[g] = (T)(((l0 + l1 + l2) & 1 ? -w : w) * sum);
 }
 
-template <int NT>
-static void launch_qcoef(int numtree, const double* mom_hat, float* qc, hipStream_t s) {
+template <int NT, typename T>
+static void launch_qcoef(int numtree, const double* mom_hat, T* qc, hipStream_t s) {
   const long long total = (long long)numtree * nqpad(NT);
-  hipLaunchKernelGGL(k_qcoef<NT>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL((k_qcoef<NT, T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      numtree, mom_hat, qc);
 }
 
-template <int... NTs>
+template <typename T, int... NTs>
 static bool dispatch_qcoef(int nterms, std::integer_sequence<int, NTs...>, int numtree,
-                           const double* mom_hat, float* qc, hipStream_t s) {
+                           const double* mom_hat, T* qc, hipStream_t s) {
   bool done = false;
-  ((nterms == NTs + 1 ? (launch_qcoef<NTs + 1>(numtree, mom_hat, qc, s), done = true) : false),
+  ((nterms == NTs + 1 ? (launch_qcoef<NTs + 1, T>(numtree, mom_hat, qc, s), done = true)
+                      : false),
    ...);
   return done;
 }
 
-hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qc32,
-                        hipStream_t s) {
+template <typename T>
+static hipError_t far_coeffs(int numtree, int nterms, const double* mom_hat64, T* qc,
+                             hipStream_t s) {
   if (numtree == 0) return hipSuccess;
   if (!dispatch_qcoef(nterms, std::make_integer_sequence<int, kMaxNterms>{}, numtree, mom_hat64,
-                      qc32, s))
+                      qc, s))
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
+hipError_t far_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qc32,
+                        hipStream_t s) {
+  return far_coeffs(numtree, nterms, mom_hat64, qc32, s);
+}
+
+hipError_t far_coeffs64(int numtree, int nterms, const double* mom_hat64, double* qc64,
+                        hipStream_t s) {
+  return far_coeffs(numtree, nterms, mom_hat64, qc64, s);
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return fma(a, b, c); }
+
 // Per-pair values of the sweep: mult[e][m] = r_m w_e, ghat_t, z.
-template <int NT>
+template <int NT, typename T>
 struct QPair {
-  float mult[3][NT > 1 ? NT - 1 : 1];
-  float g[NT / 2 + 1];
-  float z;
+  T mult[3][NT > 1 ? NT - 1 : 1];
+  T g[NT / 2 + 1];
+  T z;
 };
 
-// w = rho u, z = rho^2, c0 = exp(-lam) (screened), lam = kappa R.
-template <int NT, bool SCR>
-__device__ __forceinline__ void qpair_init(QPair<NT>& v, float w0, float w1, float w2, float z,
-                                           float c0, float lam) {
+// w = rho u, z = rho^2, c0 = exp(-lam) (screened), lam = kappa R.  FP32 uses
+// the hardware reciprocal for the ratio recurrence, FP64 the division.
+template <int NT, bool SCR, typename T>
+__device__ __forceinline__ void qpair_init(QPair<NT, T>& v, T w0, T w1, T w2, T z, T c0, T lam) {
   constexpr int P = NT - 1;
-  float r[P > 0 ? P : 1];
+  T r[P > 0 ? P : 1];
   if constexpr (SCR) {
-    if constexpr (P > 0) r[0] = -(1.0f + lam);
-    const float lam2 = lam * lam;
+    if constexpr (P > 0) r[0] = -(T(1) + lam);
+    const T lam2 = lam * lam;
 #pragma unroll
-    for (int m = 1; m < P; m++)
-      r[m] = fmaf(lam2, __builtin_amdgcn_rcpf(r[m - 1]), -float(2 * m + 1));
+    for (int m = 1; m < P; m++) {
+      if constexpr (std::is_same_v<T, float>)
+        r[m] = fmaf(lam2, __builtin_amdgcn_rcpf(r[m - 1]), -float(2 * m + 1));
+      else
+        r[m] = lam2 / r[m - 1] - double(2 * m + 1);
+    }
     v.g[0] = c0;
   } else {
 #pragma unroll
-    for (int m = 0; m < P; m++) r[m] = -float(2 * m + 1);
-    v.g[0] = 1.0f;
+    for (int m = 0; m < P; m++) r[m] = -T(2 * m + 1);
+    v.g[0] = T(1);
   }
 #pragma unroll
   for (int t = 1; t <= P / 2; t++) v.g[t] = v.g[t - 1] * r[t - 1];
@@ -428,38 +422,39 @@ __device__ __forceinline__ void qpair_init(QPair<NT>& v, float w0, float w1, flo
   v.z = z;
 }
 
+template <typename T>
 struct QState {
-  float b, a, phi, s;  // Horner sums over l2, l1, l0 and t
+  T b, a, phi, s;  // Horner sums over l2, l1, l0 and t
 };
 
 // Entry C of the sweep for one target (q = the cluster's Q_C).
-template <int NT, int C>
-__device__ __forceinline__ void qstep(const QPair<NT>& v, float q, QState& z) {
+template <int NT, int C, typename T>
+__device__ __forceinline__ void qstep(const QPair<NT, T>& v, T q, QState<T>& z) {
   constexpr const QTab<NT>& tb = kQTab<NT>;
   constexpr int l0 = tb.l[C][0], l1 = tb.l[C][1], l2 = tb.l[C][2], t = tb.t[C];
   constexpr int L = NT - 1 - 2 * t;
   if constexpr (l2 == L - l0 - l1) z.b = q;
-  else z.b = fmaf(z.b, v.mult[2][l0 + l1 + l2 + t], q);
+  else z.b = fmaT(z.b, v.mult[2][l0 + l1 + l2 + t], q);
   if constexpr (l2 == 0) {
     if constexpr (l1 == L - l0) z.a = z.b;
-    else z.a = fmaf(z.a, v.mult[1][l0 + l1 + t], z.b);
+    else z.a = fmaT(z.a, v.mult[1][l0 + l1 + t], z.b);
     if constexpr (l1 == 0) {
       if constexpr (l0 == L) z.phi = z.a;
-      else z.phi = fmaf(z.phi, v.mult[0][l0 + t], z.a);
+      else z.phi = fmaT(z.phi, v.mult[0][l0 + t], z.a);
       if constexpr (l0 == 0) {
         if constexpr (t == (NT - 1) / 2) z.s = v.g[t] * z.phi;
-        else z.s = fmaf(z.s, v.z, v.g[t] * z.phi);
+        else z.s = fmaT(z.s, v.z, v.g[t] * z.phi);
       }
     }
   }
 }
 
 // The sweep over the cluster's Q (in LDS) for one target.
-template <int NT, int... C>
-__device__ __forceinline__ float qall(std::integer_sequence<int, C...>, const QPair<NT>& v,
-                                      const float* M) {
-  QState z{0.0f, 0.0f, 0.0f, 0.0f};
-  (qstep<NT, C>(v, M[C], z), ...);
+template <int NT, typename T, int... C>
+__device__ __forceinline__ T qall(std::integer_sequence<int, C...>, const QPair<NT, T>& v,
+                                  const T* M) {
+  QState<T> z{T(0), T(0), T(0), T(0)};
+  (qstep<NT, C, T>(v, M[C], z), ...);
   return z.s;
 }
 
@@ -1323,75 +1318,12 @@ __device__ __forceinline__ void which(const int2* __restrict__ map, int& sys, in
 }
 
 
-template <typename T, int NT, bool SCR>
-// The cluster's moments are staged in LDS: the wave loads the next item's
-// moments from HBM into registers while it evaluates the current item, so the
-// moment operands are LDS broadcasts (VGPR operands) instead of scalar loads
-// issued, and waited for, just before use.
-__global__ __launch_bounds__(64) void k_far(const EvalDesc* __restrict__ descs,
-                                            const int2* __restrict__ map) {
-  constexpr int NC = ncoef(NT);
-  constexpr int PF = (NC + kBatch - 1) / kBatch;  // moments per lane per item
-  __shared__ T mbuf[2][PF * kBatch];
-  int sys, b;
-  which(map, sys, b);
-  const EvalDesc& d = descs[sys];
-  const TreeView& tv = d.tv;
-  const double kappa = d.kappa, delta2 = d.delta2;
-  // global (not flat) loads: flat loads also count on the LDS wait counter
-  const gptr<T> mom_hat = as_global(d.mom64);
-  const gptr<Item> items = as_global(d.far_items);
-  const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
-  const gptr<double> scale = as_global(tv.scale);
-  const gptr<int> far_off = as_global(d.far_off);
-  const int lane = threadIdx.x;
-  const int t = b * kBatch + lane;
-  const bool valid = t < tv.nt;
-  const int tt = valid ? t : tv.nt - 1;
-  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
-  Acc<T> acc;
-  const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
-  T pf[PF];
-  auto fetch = [&](int node) {
-#pragma unroll
-    for (int j = 0; j < PF; j++) {
-      const int c = j * kBatch + lane;
-      pf[j] = c < NC ? mom_hat[(size_t)node * NC + c] : T(0);
-    }
-  };
-  if (i0 < i1) fetch(uniform(items[i0].node));
-  int buf = 0;
-  for (int it = i0; it < i1; it++) {
-    const unsigned long long mask = uniform64(items[it].mask);
-    const int node = uniform(items[it].node);
-#pragma unroll
-    for (int j = 0; j < PF; j++) mbuf[buf][j * kBatch + lane] = pf[j];
-    __syncthreads();
-    if (it + 1 < i1) fetch(uniform(items[it + 1].node));
-    const T* M = mbuf[buf];
-    buf ^= 1;
-    if ((mask >> lane) & 1ull) {
-      const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
-      const double s = scale[node];
-      const double dx = px - cx, dy = py - cy, dz = pz - cz;
-      const double R = sqrt(dx * dx + dy * dy + dz * dz + delta2);
-      const double rinv = 1.0 / R;
-      const double lam = kappa * R;
-      const double c0 = SCR ? exp(-lam) : 1.0;
-      T v = far_eval<T, NT, SCR, const T*>((T)(dx * rinv), (T)(dy * rinv), (T)(dz * rinv),
-                                           (T)(s * rinv), (T)c0, (T)lam, M);
-      acc.add(v * (T)(1.0 / s));
-    }
-  }
-  if (valid) d.phi_far[t] = acc.value();
-}
-
 // Per-target set-up of the sweep for one cluster: the offset from the centre
 // is formed in FP64 and rounded, the rest is FP32 (hardware rsqrt and exp2).
 // Returns 1/R.
 template <int NT, bool SCR>
-__device__ __forceinline__ float qsetup(QPair<NT>& v, double px, double py, double pz, double cx,
-                                        double cy, double cz, float s, float kappa,
+__device__ __forceinline__ float qsetup(QPair<NT, float>& v, double px, double py, double pz,
+                                        double cx, double cy, double cz, float s, float kappa,
                                         float delta2) {
   const float dx = (float)(px - cx), dy = (float)(py - cy), dz = (float)(pz - cz);
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, delta2)));
@@ -1400,8 +1332,23 @@ __device__ __forceinline__ float qsetup(QPair<NT>& v, double px, double py, doub
   const float rho = s * ri, f = rho * ri;
   const float lam = kappa * R;
   const float c0 = SCR ? __builtin_amdgcn_exp2f(-1.4426950408889634f * lam) : 1.0f;
-  qpair_init<NT, SCR>(v, dx * f, dy * f, dz * f, rho * rho, c0, lam);
+  qpair_init<NT, SCR, float>(v, dx * f, dy * f, dz * f, rho * rho, c0, lam);
   return ri;
+}
+
+// The same in FP64 throughout.
+template <int NT, bool SCR>
+__device__ __forceinline__ double qsetup64(QPair<NT, double>& v, double px, double py, double pz,
+                                           double cx, double cy, double cz, double s,
+                                           double kappa, double delta2) {
+  const double dx = px - cx, dy = py - cy, dz = pz - cz;
+  const double R = sqrt(dx * dx + dy * dy + dz * dz + delta2);
+  const double rinv = 1.0 / R;
+  const double rho = s * rinv, f = rho * rinv;
+  const double lam = kappa * R;
+  const double c0 = SCR ? exp(-lam) : 1.0;
+  qpair_init<NT, SCR, double>(v, dx * f, dy * f, dz * f, rho * rho, c0, lam);
+  return rinv;
 }
 
 // FP32 far field in the radial-derivative form: the cluster's Q is staged in
@@ -1481,14 +1428,66 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (in0 || in1) {
       const int node = in1 ? n1 : n0;
       const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
-      QPair<NT> v;
+      QPair<NT, float> v;
       const float rinv = qsetup<NT, SCR>(v, px, py, pz, cx, cy, cz, (float)scale[node], kappa32,
                                          delta2_32);
-      acc.add(qall<NT>(std::make_integer_sequence<int, NQ>{}, v, mbuf + (in1 ? T2 : 0)) * rinv);
+      acc.add(qall<NT, float>(std::make_integer_sequence<int, NQ>{}, v, mbuf + (in1 ? T2 : 0)) *
+              rinv);
     }
     __syncthreads();  // the tiles are overwritten next unit
     u = un;
     n = nn;
+  }
+  if (valid) d.phi_far[t] = acc.value();
+}
+
+// FP64 far field: the same radial-derivative sweep in FP64 over FP64 cluster
+// coefficients, one item at a time (a tile is 6.5 KB at p = 11).
+template <int NT, bool SCR>
+__global__ __launch_bounds__(64) void k_farq64(const EvalDesc* __restrict__ descs,
+                                               const int2* __restrict__ map) {
+  constexpr int NQ = nqcoef(NT);
+  constexpr int NQP = nqpad(NT);
+  constexpr int PF = NQP / kBatch;  // Q values per lane per tile
+  __shared__ __align__(16) double mbuf[NQP];
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const TreeView& tv = d.tv;
+  const double kappa = d.kappa, delta2 = d.delta2;
+  const gptr<double> qc = as_global(d.qc64);
+  const gptr<Item> items = as_global(d.far_items);
+  const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
+  const gptr<double> scale = as_global(tv.scale);
+  const gptr<int> far_off = as_global(d.far_off);
+  const int lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
+  Acc<double> acc;
+  const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
+  double pf[PF];
+  auto fetch = [&](int node) {  // node tiles are padded to NQP values
+#pragma unroll
+    for (int j = 0; j < PF; j++) pf[j] = qc[(size_t)node * NQP + j * kBatch + lane];
+  };
+  if (i0 < i1) fetch(uniform(items[i0].node));
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+#pragma unroll
+    for (int j = 0; j < PF; j++) mbuf[j * kBatch + lane] = pf[j];
+    __syncthreads();
+    if (it + 1 < i1) fetch(uniform(items[it + 1].node));
+    if ((mask >> lane) & 1ull) {
+      const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
+      QPair<NT, double> v;
+      const double rinv =
+          qsetup64<NT, SCR>(v, px, py, pz, cx, cy, cz, scale[node], kappa, delta2);
+      acc.add(qall<NT, double>(std::make_integer_sequence<int, NQ>{}, v, mbuf) * rinv);
+    }
+    __syncthreads();  // the tile is overwritten next item
   }
   if (valid) d.phi_far[t] = acc.value();
 }
@@ -1498,7 +1497,7 @@ static void launch_far(const EvalDesc* descs, const int2* map, int nblocks, hipS
   if constexpr (std::is_same_v<T, float>)
     hipLaunchKernelGGL((k_farq<NT, SCR>), dim3(nblocks), dim3(kBatch), 0, s, descs, map);
   else
-    hipLaunchKernelGGL((k_far<T, NT, SCR>), dim3(nblocks), dim3(kBatch), 0, s, descs, map);
+    hipLaunchKernelGGL((k_farq64<NT, SCR>), dim3(nblocks), dim3(kBatch), 0, s, descs, map);
 }
 
 template <typename T, bool SCR, int... NTs>
@@ -1554,9 +1553,9 @@ __device__ __forceinline__ double pair64(double px, double py, double pz, double
                                          double kappa) {
   double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
   double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, d2)));
-  double R = sqrt(r2);
-  if constexpr (SCR) return s.w * exp(-kappa * R) / R;
-  else return s.w / R;
+  double ri = rsqrt(r2);  // one reciprocal square root instead of sqrt and a division
+  if constexpr (SCR) return s.w * exp(-kappa * (r2 * ri)) * ri;
+  else return s.w * ri;
 }
 
 #ifndef BIT_NEAR_SPARSE
