@@ -57,14 +57,11 @@ def nqcoef(nterms: int) -> int:
 
 def flops(stats: dict, nterms: int, screened: bool, precision: str) -> tuple[float, float]:
     """Algorithmic FLOPs of the near and far kernels.  Near: 16 (screened) / 12
-    per pair.  Far, per (cluster, target) pair: FP64 (coefficient recurrence)
-    24 T(p) (screened) / 12 T(p); FP32 (radial-derivative form) one FMA per
-    cluster coefficient and per monomial of the Horner sweep, 2 (NQ + T(p))."""
+    per pair.  Far, per (cluster, target) pair, both precisions (radial-derivative
+    form): one FMA per cluster coefficient and per monomial of the Horner sweep,
+    2 (NQ + T(p))."""
     fn = 16.0 if screened else 12.0
-    if precision == "fp32":
-        ff = 2.0 * (nqcoef(nterms) + ncoef(nterms))
-    else:
-        ff = (24.0 if screened else 12.0) * ncoef(nterms)
+    ff = 2.0 * (nqcoef(nterms) + ncoef(nterms))
     return stats["near_pairs"] * fn, stats["far_pairs"] * ff
 
 
@@ -197,7 +194,7 @@ def main():
     achieved = kfl / (kms * 1e-3)
     roof = {"bound": "valu", "kernel": (f"k_farq<{args.nterms},{str(screened).lower()}>"
                                         if args.precision == "fp32" else
-                                        f"k_far<double,{args.nterms},{str(screened).lower()}>")
+                                        f"k_farq64<{args.nterms},{str(screened).lower()}>")
             if kern == "far" else
             f"k_near{'32' if args.precision == 'fp32' else '64'}<{str(screened).lower()}>",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
