@@ -1567,6 +1567,12 @@ constexpr int kNearSparse = BIT_NEAR_SPARSE;
 __device__ __forceinline__ float bcast_lane(float x, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
 }
+__device__ __forceinline__ double bcast_lane64(double x, int l) {
+  const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
 
 template <bool SCR>
 __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ descs,
@@ -1719,6 +1725,46 @@ __global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ desc
     // particle targets: the self pair lies in leaves holding some of the
     // batch's targets; point targets: pairs closer than eps are skipped instead
     const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
+    if (__popcll(mask) <= kNearSparse) {
+      // Few targets: source-parallel as in k_near32, four targets per pass.
+      unsigned long long m = mask;
+      while (m) {
+        int tl[4];
+        bool on[4];
+        double sx[4], sy[4], sz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          on[u] = m != 0ull;
+          tl[u] = on[u] ? __builtin_ctzll(m) : 0;
+          m &= m - 1;
+          sx[u] = bcast_lane64(px, tl[u]);
+          sy[u] = bcast_lane64(py, tl[u]);
+          sz[u] = bcast_lane64(pz, tl[u]);
+        }
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int j = r.x + lane; j < r.y; j += 64) {
+          const double4 q = make_double4(tv.sx[j], tv.sy[j], tv.sz[j], tv.sq[j]);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (tv.self) {
+              if (selfp && j == t0 + tl[u]) continue;  // the target's own slot
+            } else {  // point targets: skip coincident sources (R^2 <= eps)
+              const double dx = sx[u] - q.x, dy = sy[u] - q.y, dz = sz[u] - q.z;
+              if (dx * dx + dy * dy + dz * dz + d2 <= 1e-14) continue;
+            }
+            g[u] += pair64<SCR>(sx[u], sy[u], sz[u], q, d2, kappa);
+          }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+          for (int u = 0; u < 4; u++) g[u] += __shfl_xor(g[u], o);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (on[u] && lane == tl[u]) acc += g[u];
+      }
+      continue;
+    }
     for (int base = r.x; base < r.y; base += 64) {
       const int cnt = min(64, r.y - base);
       if (lane < cnt) {
