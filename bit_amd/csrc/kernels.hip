@@ -6,8 +6,8 @@
 //   * node table SoA: double4 centre+sqradius, int[8] children, int2 range,
 //     double scale s = sqrt(sqradius);
 //   * moments packed by total-degree shell, T(p) = C(p+3,3) per node, FP64,
-//     plus a copy scaled by s^-|k| (FP64) and the FP32 far kernel's
-//     coefficients (radial-derivative form, see far_q);
+//     plus a copy scaled by s^-|k| (FP64) and the far kernels' cluster
+//     coefficients (radial-derivative form, FP32 and FP64; see qstep);
 //   * interaction lists per batch of 64 consecutive sorted targets: (mask,
 //     node) items, far and near, exactly the reference traversal's choices.
 // Work mapping: one wavefront per batch, one lane per target.  Every lane owns
@@ -1279,10 +1279,10 @@ hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp
 
 // ---------------------------------------------------------------------------
 // Far field.  Per item the wave's lanes in the mask evaluate the cluster's
-// Taylor sum.  FP64 (k_far): the coefficient recurrence, fully unrolled for
-// the compile-time order so all coefficients live in registers.  FP32
-// (k_farq): the radial-derivative form (far_q), per-cluster values added with
-// Kahan compensation.  Both stage the cluster's data in LDS.
+// Taylor sum in the radial-derivative form (qall): FP32 (k_farq, per-cluster
+// values added with Kahan compensation) and FP64 (k_farq64).  Both stage the
+// cluster's coefficients in LDS.  The coefficient recurrence (coef_step) is
+// used by the field kernel.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Acc;
