@@ -255,7 +255,8 @@ def main():
         "breakdown": {k: st[k] for k in ("ms_tree", "ms_upload", "ms_moments", "ms_lists",
                                          "ms_far", "ms_near")},
         "interactions": {k: st[k] for k in ("near_pairs", "far_pairs", "far_items",
-                                            "near_items", "lane_far", "lane_near_pairs")},
+                                            "near_items", "lane_far", "lane_near_pairs",
+                                            "list_passes")},
         "fp64_ms_per_step": fp64_ms,
         "roofline": roof,
         "serving": serving,

@@ -65,7 +65,7 @@ class Stats(C.Structure):
                 ("near_items", C.c_int64), ("lane_far", C.c_int64),
                 ("lane_near_pairs", C.c_int64), ("ms_tree", C.c_double),
                 ("ms_upload", C.c_double), ("ms_moments", C.c_double), ("ms_lists", C.c_double),
-                ("ms_far", C.c_double), ("ms_near", C.c_double)]
+                ("ms_far", C.c_double), ("ms_near", C.c_double), ("list_passes", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

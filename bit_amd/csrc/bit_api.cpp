@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <mutex>
 #include <map>
 #include <atomic>
@@ -73,6 +74,10 @@ struct bit_ctx {
   std::mutex pool_mu;
   std::multimap<size_t, void*> pool;
   size_t pooled = 0;
+  // Per-batch list staging capacities from the context's last list build,
+  // [0] particles as targets, [1] external points; far then near.  A hint
+  // only: a batch that outgrows them is rebuilt by count-then-fill.
+  std::atomic<int> list_cap[2][2] = {{{0}, {0}}, {{0}, {0}}};
 };
 
 namespace {
@@ -206,6 +211,7 @@ struct ListSet {
   bool valid = false;
   double acc = 0, delta2 = 0, sub = 0;
   int nfar = 0, nnear = 0;
+  int passes = 0;  // traversals of the last build
   DevBuf far_cnt, near_cnt, far_off, near_off, far_items, near_items;
 };
 
@@ -227,7 +233,7 @@ struct bit_system {
   // lists: pl for the particles as targets (cached per MAC parameters), ql for
   // the last external point set
   ListSet pl, ql;
-  DevBuf scan_tmp, lstats, pstat, err_flag;
+  DevBuf scan_tmp, lstats, pstat, err_flag, stage_far, stage_near;
   bit::dev::ListStats hstats{};
   // external point targets (batch order) and their results
   DevBuf qx, qy, qz, qperm, qphi, qout, qdesc;
@@ -567,7 +573,18 @@ int check_params(const bit_system* s, const bit_params* p, bool field = false) {
   return BIT_OK;
 }
 
-// Two passes over the tree per target batch (count, exclusive scan, fill).
+// Staging capacity for a batch's next build: the largest count seen plus an
+// eighth, or 0 (two traversals) when the staging regions would exceed four
+// times the entries the lists actually hold.
+static int stage_cap(unsigned long long mx, int nb, long long total) {
+  const long long cap = (long long)mx + (long long)(mx / 8) + 4;
+  if (cap > INT_MAX / 2 || cap * nb > 4 * total + 64LL * nb) return 0;
+  return (int)cap;
+}
+
+// One traversal when the capacities from the context's last build of this
+// kind hold every batch (count and stage, exclusive scan, compaction); two
+// otherwise (count, exclusive scan, fill).  Both give the same lists.
 int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const bit_params* p) {
   L.valid = false;  // set again only when both passes were issued
   bit_ctx* c = s->ctx;
@@ -583,10 +600,31 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   HIP_TRY(hipMemsetAsync(L.far_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
   HIP_TRY(hipMemsetAsync(L.near_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
   HIP_TRY(hipMemsetAsync(s->err_flag.p, 0, sizeof(int), c->stream));
+  const int kind = &L == &s->ql ? 1 : 0;  // the context's staging hint for these lists
+  const int cap_far = c->list_cap[kind][0].load();
+  const int cap_near = c->list_cap[kind][1].load();
+  bool staged = cap_far > 0 && cap_near > 0;
+  if (staged && c->budget) {  // staging is optional: never the allocation that breaks a budget
+    const size_t nf = (size_t)nb * cap_far * sizeof(bit::dev::Item);
+    const size_t nn = (size_t)nb * cap_near * sizeof(bit::dev::Item);
+    const size_t extra = (nf > s->stage_far.bytes ? nf : 0) + (nn > s->stage_near.bytes ? nn : 0);
+    staged = c->in_use + extra <= c->budget;
+  }
+  if (staged) {
+    BIT_TRY(s->stage_far.ensure(c, (size_t)nb * cap_far * sizeof(bit::dev::Item)));
+    BIT_TRY(s->stage_near.ensure(c, (size_t)nb * cap_near * sizeof(bit::dev::Item)));
+  }
   s->t_lists.start(c->stream);
-  HIP_TRY(bit::dev::lists_count(tv, mp, s->stack_cap, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
-                                s->pstat.as<unsigned long long>(), s->err_flag.as<int>(),
-                                c->stream));
+  if (staged)
+    HIP_TRY(bit::dev::lists_stage(tv, mp, s->stack_cap, L.far_cnt.as<int>(),
+                                  L.near_cnt.as<int>(), s->pstat.as<unsigned long long>(),
+                                  cap_far, cap_near, s->stage_far.as<bit::dev::Item>(),
+                                  s->stage_near.as<bit::dev::Item>(), s->err_flag.as<int>(),
+                                  c->stream));
+  else
+    HIP_TRY(bit::dev::lists_count(tv, mp, s->stack_cap, L.far_cnt.as<int>(),
+                                  L.near_cnt.as<int>(), s->pstat.as<unsigned long long>(),
+                                  s->err_flag.as<int>(), c->stream));
   HIP_TRY(bit::dev::list_stats(nb, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
                                s->pstat.as<unsigned long long>(),
                                s->lstats.as<bit::dev::ListStats>(), c->stream));
@@ -615,10 +653,25 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   L.nnear = tot[1];
   BIT_TRY(L.far_items.ensure(c, (size_t)std::max(1, tot[0]) * sizeof(bit::dev::Item)));
   BIT_TRY(L.near_items.ensure(c, (size_t)std::max(1, tot[1]) * sizeof(bit::dev::Item)));
-  HIP_TRY(bit::dev::lists_fill(tv, mp, s->stack_cap, L.far_off.as<int>(), L.near_off.as<int>(),
-                               L.far_items.as<bit::dev::Item>(), L.near_items.as<bit::dev::Item>(),
-                               s->err_flag.as<int>(), c->stream));
+  const bool fits = staged && s->hstats.far_max <= (unsigned long long)cap_far &&
+                    s->hstats.near_max <= (unsigned long long)cap_near;
+  if (fits)
+    HIP_TRY(bit::dev::lists_compact(nb, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
+                                    L.far_off.as<int>(), L.near_off.as<int>(), cap_far,
+                                    cap_near, s->stage_far.as<bit::dev::Item>(),
+                                    s->stage_near.as<bit::dev::Item>(),
+                                    L.far_items.as<bit::dev::Item>(),
+                                    L.near_items.as<bit::dev::Item>(), c->stream));
+  else
+    HIP_TRY(bit::dev::lists_fill(tv, mp, s->stack_cap, L.far_off.as<int>(),
+                                 L.near_off.as<int>(), L.far_items.as<bit::dev::Item>(),
+                                 L.near_items.as<bit::dev::Item>(), s->err_flag.as<int>(),
+                                 c->stream));
   s->t_lists.stop(c->stream);
+  L.passes = fits ? 1 : 2;
+  const long long total = (long long)tot[0] + tot[1];
+  c->list_cap[kind][0].store(stage_cap(s->hstats.far_max, nb, total));
+  c->list_cap[kind][1].store(stage_cap(s->hstats.near_max, nb, total));
   L.valid = true;
   L.acc = p->acc;
   L.delta2 = p->delta2;
@@ -630,6 +683,7 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   s->stats.lane_far = 64 * (int64_t)tot[0];
   s->stats.lane_near_pairs = 64 * (int64_t)s->hstats.near_slots;
   s->stats.ms_lists = -1.0;  // filled after the next synchronisation
+  s->stats.list_passes = L.passes;
   return BIT_OK;
 }
 

@@ -52,6 +52,7 @@ struct MacParams {
 // counters accumulated by the list-count pass
 struct ListStats {
   unsigned long long far_items, near_items, far_pairs, near_pairs, near_slots;
+  unsigned long long far_max, near_max;  // largest per-batch entry counts
 };
 
 // ---- tree build on the device ----
@@ -148,6 +149,16 @@ hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
 hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,
                       const int* near_off, Item* far_items, Item* near_items, int* err_flag,
                       hipStream_t s);
+// One traversal that counts like lists_count and writes batch b's first
+// cap_far / cap_near entries to far_stage[b*cap_far ..] / near_stage[b*cap_near ..].
+hipError_t lists_stage(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
+                       int* near_cnt, unsigned long long* pstat, int cap_far, int cap_near,
+                       Item* far_stage, Item* near_stage, int* err_flag, hipStream_t s);
+// Staged entries to the scanned offsets (valid when no count exceeded its cap).
+hipError_t lists_compact(int nb, const int* far_cnt, const int* near_cnt, const int* far_off,
+                         const int* near_off, int cap_far, int cap_near, const Item* far_stage,
+                         const Item* near_stage, Item* far_items, Item* near_items,
+                         hipStream_t s);
 hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp_bytes,
                           hipStream_t s);
 

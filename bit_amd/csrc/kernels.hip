@@ -16,6 +16,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
 #include <type_traits>
 #include <utility>
 
@@ -1130,14 +1131,19 @@ __device__ __forceinline__ bool mac_accept(double px, double py, double pz, doub
   return c.w < mp.acc * (rs - mp.mac_sub);
 }
 
-template <bool FILL>
+// Modes of k_lists: count only; fill at the scanned offsets; count and stage
+// the items into fixed per-batch regions of cap_far / cap_near entries
+// (entries beyond a region's capacity are counted, not written).
+enum { kListCount = 0, kListFill = 1, kListStage = 2 };
+
+template <int MODE>
 __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int stack_cap,
                                               int* __restrict__ far_cnt,
                                               int* __restrict__ near_cnt,
                                               unsigned long long* __restrict__ pstat,
                                               const int* __restrict__ far_off,
-                                              const int* __restrict__ near_off,
-                                              Item* __restrict__ far_items,
+                                              const int* __restrict__ near_off, int cap_far,
+                                              int cap_near, Item* __restrict__ far_items,
                                               Item* __restrict__ near_items, int* err_flag) {
   extern __shared__ unsigned char smem[];
   unsigned long long* st_mask = reinterpret_cast<unsigned long long*>(smem);
@@ -1162,8 +1168,16 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   int sp = 1;
   int nf = 0, nn = 0;
   unsigned long long far_pairs = 0, near_pairs = 0, near_slots = 0;
-  const int fbase = FILL ? far_off[b] : 0;
-  const int nbase = FILL ? near_off[b] : 0;
+  constexpr bool kWrite = MODE != kListCount;
+  const size_t fbase = MODE == kListFill    ? (size_t)far_off[b]
+                       : MODE == kListStage ? (size_t)b * (size_t)cap_far
+                                            : 0;
+  const size_t nbase = MODE == kListFill    ? (size_t)near_off[b]
+                       : MODE == kListStage ? (size_t)b * (size_t)cap_near
+                                            : 0;
+  // entries a batch may write: all at the scanned offsets, cap when staging
+  const int flim = MODE == kListStage ? cap_far : INT_MAX;
+  const int nlim = MODE == kListStage ? cap_near : INT_MAX;
   __syncthreads();
   while (sp > 0) {
     sp--;
@@ -1178,14 +1192,14 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
     const bool acc = mac_accept(px, py, pz, c, mp);
     const unsigned long long am = ballot(acc) & m;
     if (am) {
-      if (FILL && lane == 0) far_items[fbase + nf] = Item{am, node, 0};
+      if (kWrite && lane == 0 && nf < flim) far_items[fbase + nf] = Item{am, node, 0};
       nf++;
       far_pairs += __popcll(am);
     }
     const unsigned long long rm = m & ~am;
     if (rm) {
       if (kids[0] < 0) {
-        if (FILL && lane == 0) near_items[nbase + nn] = Item{rm, node, 0};
+        if (kWrite && lane == 0 && nn < nlim) near_items[nbase + nn] = Item{rm, node, 0};
         nn++;
         const int2 r = make_int2(grange[2 * node], grange[2 * node + 1]);
         near_pairs += (unsigned long long)__popcll(rm) * (unsigned long long)(r.y - r.x);
@@ -1210,7 +1224,7 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
     }
     __syncthreads();
   }
-  if (!FILL && lane == 0) {
+  if (MODE != kListFill && lane == 0) {
     far_cnt[b] = nf;
     near_cnt[b] = nn;
     pstat[3 * b] = far_pairs;
@@ -1224,24 +1238,34 @@ __global__ __launch_bounds__(1024) void k_list_stats(int nb, const int* __restri
                                                      const int* __restrict__ near_cnt,
                                                      const unsigned long long* __restrict__ pstat,
                                                      ListStats* __restrict__ out) {
-  __shared__ unsigned long long red[5][1024];
-  unsigned long long v[5] = {0, 0, 0, 0, 0};
+  __shared__ unsigned long long red[7][1024];
+  unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};  // five sums, two maxima
   for (int b = threadIdx.x; b < nb; b += 1024) {
-    v[0] += (unsigned long long)far_cnt[b];
-    v[1] += (unsigned long long)near_cnt[b];
+    const unsigned long long f = (unsigned long long)far_cnt[b];
+    const unsigned long long n = (unsigned long long)near_cnt[b];
+    v[0] += f;
+    v[1] += n;
     v[2] += pstat[3 * b];
     v[3] += pstat[3 * b + 1];
     v[4] += pstat[3 * b + 2];
+    v[5] = f > v[5] ? f : v[5];
+    v[6] = n > v[6] ? n : v[6];
   }
-  for (int k = 0; k < 5; k++) red[k][threadIdx.x] = v[k];
+  for (int k = 0; k < 7; k++) red[k][threadIdx.x] = v[k];
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
+    if ((int)threadIdx.x < w) {
       for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+      for (int k = 5; k < 7; k++) {
+        const unsigned long long o = red[k][threadIdx.x + w];
+        red[k][threadIdx.x] = o > red[k][threadIdx.x] ? o : red[k][threadIdx.x];
+      }
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    *out = ListStats{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0]};
+    *out = ListStats{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0], red[5][0],
+                     red[6][0]};
 }
 
 hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
@@ -1254,9 +1278,21 @@ hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far
                        int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s) {
   int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
-  hipLaunchKernelGGL(k_lists<false>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap, far_cnt,
-                     near_cnt, pstat, (const int*)nullptr, (const int*)nullptr,
+  hipLaunchKernelGGL(k_lists<kListCount>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
+                     far_cnt, near_cnt, pstat, (const int*)nullptr, (const int*)nullptr, 0, 0,
                      (Item*)nullptr, (Item*)nullptr, err_flag);
+  return hipGetLastError();
+}
+
+hipError_t lists_stage(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
+                       int* near_cnt, unsigned long long* pstat, int cap_far, int cap_near,
+                       Item* far_stage, Item* near_stage, int* err_flag, hipStream_t s) {
+  if (cap_far <= 0 || cap_near <= 0) return hipErrorInvalidValue;
+  int nb = (tv.nt + kBatch - 1) / kBatch;
+  size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
+  hipLaunchKernelGGL(k_lists<kListStage>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
+                     far_cnt, near_cnt, pstat, (const int*)nullptr, (const int*)nullptr, cap_far,
+                     cap_near, far_stage, near_stage, err_flag);
   return hipGetLastError();
 }
 
@@ -1265,10 +1301,46 @@ hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int
                       hipStream_t s) {
   int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
-  hipLaunchKernelGGL(k_lists<true>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
+  hipLaunchKernelGGL(k_lists<kListFill>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
                      (int*)nullptr, (int*)nullptr, (unsigned long long*)nullptr, far_off,
-                     near_off,
-                     far_items, near_items, err_flag);
+                     near_off, 0, 0, far_items, near_items, err_flag);
+  return hipGetLastError();
+}
+
+// Staged items to their scanned offsets: one wavefront per batch and list,
+// 16-byte items copied as uint4 (far lists in blocks [0, nb), near after).
+__global__ __launch_bounds__(256) void k_list_compact(int nb, const int* __restrict__ far_cnt,
+                                                      const int* __restrict__ near_cnt,
+                                                      const int* __restrict__ far_off,
+                                                      const int* __restrict__ near_off,
+                                                      int cap_far, int cap_near,
+                                                      const uint4* __restrict__ far_stage,
+                                                      const uint4* __restrict__ near_stage,
+                                                      uint4* __restrict__ far_items,
+                                                      uint4* __restrict__ near_items) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= 2 * nb) return;
+  const bool isfar = w < nb;
+  const int b = isfar ? w : w - nb;
+  const int n = isfar ? far_cnt[b] : near_cnt[b];
+  const uint4* src = isfar ? far_stage + (size_t)b * (size_t)cap_far
+                           : near_stage + (size_t)b * (size_t)cap_near;
+  uint4* dst = isfar ? far_items + far_off[b] : near_items + near_off[b];
+  for (int i = lane; i < n; i += 64) dst[i] = src[i];
+}
+
+hipError_t lists_compact(int nb, const int* far_cnt, const int* near_cnt, const int* far_off,
+                         const int* near_off, int cap_far, int cap_near, const Item* far_stage,
+                         const Item* near_stage, Item* far_items, Item* near_items,
+                         hipStream_t s) {
+  static_assert(sizeof(Item) == sizeof(uint4), "items are copied as 16-byte words");
+  if (nb == 0) return hipSuccess;
+  const int nblk = (2 * nb + 3) / 4;
+  hipLaunchKernelGGL(k_list_compact, dim3(nblk), dim3(256), 0, s, nb, far_cnt, near_cnt, far_off,
+                     near_off, cap_far, cap_near, reinterpret_cast<const uint4*>(far_stage),
+                     reinterpret_cast<const uint4*>(near_stage),
+                     reinterpret_cast<uint4*>(far_items), reinterpret_cast<uint4*>(near_items));
   return hipGetLastError();
 }
 

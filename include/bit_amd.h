@@ -71,6 +71,9 @@ typedef struct bit_stats {
   int64_t lane_far;    /* 64 x far_items: lane slots the far kernel ran */
   int64_t lane_near_pairs; /* 64 x sum over near items of leaf size */
   double ms_tree, ms_upload, ms_moments, ms_lists, ms_far, ms_near; /* last timings */
+  /* tree traversals of the last list build: 1 when every batch fit the
+   * per-batch regions sized from the previous build, else 2 (count, fill) */
+  int64_t list_passes;
 } bit_stats;
 
 void bit_params_default(bit_params *p);

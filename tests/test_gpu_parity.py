@@ -123,6 +123,34 @@ def test_lists_deep_tree_bit_exact(ctx, oracle):
         assert near[slot] == list(nr), slot
 
 
+def test_lists_single_traversal_equals_two(ctx):
+    """A rebuild stages items into per-batch regions sized by the context's
+    previous build (one traversal) and falls back to count-then-fill when a
+    batch outgrows them; either way the lists equal a first, two-traversal
+    build on a fresh context."""
+    fx = load_fixture("blobs12k_screened")
+    own = bit_amd.Context(0)  # no hint from earlier builds
+    try:
+        s = _system(own, fx)
+        seen = []
+        for acc in (0.5, 0.7, 0.3, 0.3001, 0.9, 0.5):
+            got = s.lists(4, acc, 2.5e-5, 2.5e-5)
+            passes = s.stats()["list_passes"]
+            seen.append(passes)
+            fresh = bit_amd.Context(0)
+            try:
+                r = _system(fresh, fx)
+                ref = r.lists(4, acc, 2.5e-5, 2.5e-5)
+                assert r.stats()["list_passes"] == 2
+            finally:
+                fresh.close()
+            for k in ref:
+                assert np.array_equal(got[k], ref[k]), (acc, k, passes)
+    finally:
+        own.close()
+    assert seen[0] == 2 and set(seen) == {1, 2}, seen
+
+
 @pytest.mark.parametrize("name", ["uniform10k", "uniform12k_reg", "blobs12k_screened", "aniso6k",
                                   "tiny1", "tiny5", "tiny300"])
 def test_potential_fp64_matches_reference(ctx, name):
