@@ -1646,6 +1646,43 @@ __device__ __forceinline__ double bcast_lane64(double x, int l) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// A float4 through a global-address-space pointer (one 16-byte global load;
+// HIP's vector types do not bind references in other address spaces).
+__device__ __forceinline__ float4 load4(gptr<float4> p, int i) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = ((const __attribute__((address_space(1))) v4*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// One source-parallel sweep of the sparse near mode: lane l takes sources
+// j0 + l, j0 + l + 64, ... below j1, each against four broadcast targets
+// (sx, sy, sz), and sums per target into g.  MODE 0: every pair; 1: the
+// target's own slot (own[u]) skipped; 2: sources with R^2 <= eps skipped
+// (point targets).  The mode is fixed per item, outside the source loop.
+template <bool SCR, int MODE>
+__device__ __forceinline__ void near_sparse_sweep(gptr<float4> sloc, int j0, int j1,
+                                                  const float (&sx)[4], const float (&sy)[4],
+                                                  const float (&sz)[4], const int (&own)[4],
+                                                  float d2, float kneg2, float (&g)[4]) {
+  for (int j = j0; j < j1; j += 64) {
+    const float4 q = load4(sloc, j);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if constexpr (MODE == 0) {
+        g[u] = pair32<SCR>(g[u], sx[u], sy[u], sz[u], q, d2, kneg2);
+      } else if constexpr (MODE == 1) {
+        const float v = pair32<SCR>(0.0f, sx[u], sy[u], sz[u], q, d2, kneg2);
+        g[u] += j == own[u] ? 0.0f : v;
+      } else {
+        const float dx = sx[u] - q.x, dy = sy[u] - q.y, dz = sz[u] - q.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
+        const float v = pair32<SCR>(0.0f, sx[u], sy[u], sz[u], q, d2, kneg2);
+        g[u] += r2 > 1e-14f ? v : 0.0f;
+      }
+    }
+  }
+}
+
 template <bool SCR>
 __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ descs,
                                                const int2* __restrict__ map) {
@@ -1662,6 +1699,7 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
   const int tt = valid ? t : tv.nt - 1;
   const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   const int t0 = b * kBatch, t1 = min(tv.nt, t0 + kBatch);
+  const gptr<float4> sloc = as_global(tv.sloc);
   Acc<float> acc;
   const int i0 = uniform(d.near_off[b]), i1 = uniform(d.near_off[b + 1]);
   for (int it = i0; it < i1; it++) {
@@ -1694,22 +1732,13 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
           sz[u] = bcast_lane(oz, tl[u]);
         }
         float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int j = r.x + lane; j < r.y; j += 64) {
-          const float4 q = tv.sloc[j];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const float v = pair32<SCR>(0.0f, sx[u], sy[u], sz[u], q, d2, kneg2);
-            if (!selfp) {
-              g[u] += v;
-            } else if (tv.self) {
-              g[u] += (j == t0 + tl[u]) ? 0.0f : v;  // the target's own slot
-            } else {  // point targets: skip coincident sources (R^2 <= eps)
-              const float dx = sx[u] - q.x, dy = sy[u] - q.y, dz = sz[u] - q.z;
-              const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
-              g[u] += r2 > 1e-14f ? v : 0.0f;
-            }
-          }
-        }
+        const int own[4] = {t0 + tl[0], t0 + tl[1], t0 + tl[2], t0 + tl[3]};
+        if (!selfp)
+          near_sparse_sweep<SCR, 0>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2, kneg2, g);
+        else if (tv.self)
+          near_sparse_sweep<SCR, 1>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2, kneg2, g);
+        else
+          near_sparse_sweep<SCR, 2>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2, kneg2, g);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
@@ -1722,7 +1751,9 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
     }
     for (int base = r.x; base < r.y; base += 64) {
       const int cnt = min(64, r.y - base);
-      if (lane < cnt) tile[lane] = tv.sloc[base + lane];
+      if (lane < cnt) {
+        tile[lane] = load4(sloc, base + lane);
+      }
       __syncthreads();
       if (act) {
         if (!selfp) {
