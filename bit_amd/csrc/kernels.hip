@@ -1646,6 +1646,14 @@ __device__ __forceinline__ double bcast_lane64(double x, int l) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// A source in the near kernel's coordinates: positions times csf when the
+// coordinates are scaled (charge unchanged).
+template <bool SCALED>
+__device__ __forceinline__ float4 scale_src(float4 q, float csf) {
+  if constexpr (SCALED) return make_float4(q.x * csf, q.y * csf, q.z * csf, q.w);
+  else return q;
+}
+
 // A float4 through a global-address-space pointer (one 16-byte global load;
 // HIP's vector types do not bind references in other address spaces).
 __device__ __forceinline__ float4 load4(gptr<float4> p, int i) {
@@ -1659,13 +1667,14 @@ __device__ __forceinline__ float4 load4(gptr<float4> p, int i) {
 // (sx, sy, sz), and sums per target into g.  MODE 0: every pair; 1: the
 // target's own slot (own[u]) skipped; 2: sources with R^2 <= eps skipped
 // (point targets).  The mode is fixed per item, outside the source loop.
-template <bool SCR, int MODE>
+template <bool SCR, bool SCALED, int MODE>
 __device__ __forceinline__ void near_sparse_sweep(gptr<float4> sloc, int j0, int j1,
                                                   const float (&sx)[4], const float (&sy)[4],
                                                   const float (&sz)[4], const int (&own)[4],
-                                                  float d2, float kneg2, float (&g)[4]) {
+                                                  float d2, float kneg2, float csf, float eps2,
+                                                  float (&g)[4]) {
   for (int j = j0; j < j1; j += 64) {
-    const float4 q = load4(sloc, j);
+    const float4 q = scale_src<SCALED>(load4(sloc, j), csf);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       if constexpr (MODE == 0) {
@@ -1677,22 +1686,24 @@ __device__ __forceinline__ void near_sparse_sweep(gptr<float4> sloc, int j0, int
         const float dx = sx[u] - q.x, dy = sy[u] - q.y, dz = sz[u] - q.z;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
         const float v = pair32<SCR>(0.0f, sx[u], sy[u], sz[u], q, d2, kneg2);
-        g[u] += r2 > 1e-14f ? v : 0.0f;
+        g[u] += r2 > eps2 ? v : 0.0f;
       }
     }
   }
 }
 
-template <bool SCR>
-__global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ descs,
-                                               const int2* __restrict__ map) {
-  __shared__ float4 tile[64];
-  int sys, b;
-  which(map, sys, b);
-  const EvalDesc& d = descs[sys];
+// One batch of the FP32 near field.  SCALED (screened kernel): positions are
+// multiplied by cs = kappa log2(e), so that exp(-kappa R) = exp2(-R') with
+// R' = cs R needs no multiplication per pair; 1/R' = 1/(cs R) is corrected by
+// one multiplication by cs per target.  Otherwise cs = 1 and the exponent is
+// kneg2 R.
+template <bool SCR, bool SCALED>
+__device__ __forceinline__ void near32_batch(const EvalDesc& d, int b, float4* tile, double cs) {
   const TreeView& tv = d.tv;
-  const float d2 = (float)d.delta2;
-  const float kneg2 = (float)(-d.kappa * 1.4426950408889634);
+  const float csf = (float)cs;
+  const float d2 = (float)(d.delta2 * cs * cs);
+  const float eps2 = SCALED ? (float)(1e-14 * cs * cs) : 1e-14f;  // point targets' R^2 cut
+  const float kneg2 = SCALED ? -1.0f : (float)(-d.kappa * 1.4426950408889634);
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
   const bool valid = t < tv.nt;
@@ -1708,7 +1719,8 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
     const int2 r = tv.range[node];
     const double4 c = tv.center[node];
     const bool act = (mask >> lane) & 1ull;
-    const float ox = (float)(px - c.x), oy = (float)(py - c.y), oz = (float)(pz - c.z);
+    const float ox = (float)((px - c.x) * cs), oy = (float)((py - c.y) * cs),
+                oz = (float)((pz - c.z) * cs);
     // particle targets: the self pair lies in leaves holding some of the
     // batch's targets; point targets: pairs closer than eps are skipped instead
     const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
@@ -1734,11 +1746,14 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
         float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const int own[4] = {t0 + tl[0], t0 + tl[1], t0 + tl[2], t0 + tl[3]};
         if (!selfp)
-          near_sparse_sweep<SCR, 0>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2, kneg2, g);
+          near_sparse_sweep<SCR, SCALED, 0>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2,
+                                                 kneg2, csf, eps2, g);
         else if (tv.self)
-          near_sparse_sweep<SCR, 1>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2, kneg2, g);
+          near_sparse_sweep<SCR, SCALED, 1>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2,
+                                                 kneg2, csf, eps2, g);
         else
-          near_sparse_sweep<SCR, 2>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2, kneg2, g);
+          near_sparse_sweep<SCR, SCALED, 2>(sloc, r.x + lane, r.y, sx, sy, sz, own, d2,
+                                                 kneg2, csf, eps2, g);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
@@ -1752,7 +1767,7 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
     for (int base = r.x; base < r.y; base += 64) {
       const int cnt = min(64, r.y - base);
       if (lane < cnt) {
-        tile[lane] = load4(sloc, base + lane);
+        tile[lane] = scale_src<SCALED>(load4(sloc, base + lane), csf);
       }
       __syncthreads();
       if (act) {
@@ -1792,7 +1807,7 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
             float dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
             float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, d2)));
             float v = pair32<SCR>(0.0f, ox, oy, oz, q, d2, kneg2);
-            g += r2 > 1e-14f ? v : 0.0f;
+            g += r2 > eps2 ? v : 0.0f;
           }
           acc.add(g);
         }
@@ -1800,7 +1815,27 @@ __global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ desc
       __syncthreads();
     }
   }
-  if (valid) d.out[tv.tperm[t]] = d.phi_far[t] + acc.value();
+  if (valid)
+    d.out[tv.tperm[t]] = d.phi_far[t] + (SCALED ? (double)acc.value() * cs : acc.value());
+}
+
+template <bool SCR>
+__global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ descs,
+                                               const int2* __restrict__ map) {
+  __shared__ float4 tile[64];
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  if constexpr (SCR) {
+    // scaled coordinates while cs keeps leaf-size offsets and their squares
+    // far from FP32's range limits
+    const double cs = d.kappa * 1.4426950408889634;
+    if (cs >= 0x1p-20 && cs <= 0x1p20) {
+      near32_batch<true, true>(d, b, tile, cs);
+      return;
+    }
+  }
+  near32_batch<SCR, false>(d, b, tile, 1.0);
 }
 
 template <bool SCR>

@@ -181,6 +181,20 @@ def test_potential_fp32_within_tolerance(ctx, name):
     assert rel_l2(p32, p64) < FP32_TOL
 
 
+@pytest.mark.parametrize("kappa", [1e-7, 0.5, 40.0])
+def test_potential_fp32_near_scaled_and_unscaled(ctx, kappa):
+    """The screened FP32 near field runs in coordinates scaled by kappa*log2(e)
+    when that factor lies in [2^-20, 2^20] (0.5, 40) and unscaled below it
+    (1e-7); all three stay within the FP32 tolerance of the FP64 result."""
+    fx = load_fixture("blobs12k_screened")
+    s = _system(ctx, fx)
+    args = (int(fx["nterms"]), float(fx["acc"]), kappa, 0.0, 0.01)
+    p64 = s.potential(*args, precision="fp64")
+    p32 = s.potential(*args, precision="fp32")
+    assert np.all(np.isfinite(p32))
+    assert rel_l2(p32, p64) < FP32_TOL
+
+
 def test_field_fp64_matches_treeforce(ctx):
     fx = load_fixture("box16k")
     s = _system(ctx, fx)
