@@ -45,10 +45,6 @@ MEASURED_TRAFFIC = {
 }
 
 
-def ncoef(nterms: int) -> int:
-    return nterms * (nterms + 1) * (nterms + 2) // 6
-
-
 def nqcoef(nterms: int) -> int:
     """Coefficients per cluster of the FP32 far kernel's radial-derivative form."""
     p = nterms - 1
@@ -56,12 +52,14 @@ def nqcoef(nterms: int) -> int:
 
 
 def flops(stats: dict, nterms: int, screened: bool, precision: str) -> tuple[float, float]:
-    """Algorithmic FLOPs of the near and far kernels.  Near: 16 (screened) / 12
-    per pair.  Far, per (cluster, target) pair, both precisions (radial-derivative
-    form): one FMA per cluster coefficient and per monomial of the Horner sweep,
-    2 (NQ + T(p))."""
-    fn = 16.0 if screened else 12.0
-    ff = 2.0 * (nqcoef(nterms) + ncoef(nterms))
+    """Algorithmic FLOPs of the near and far kernels (FMA = 2, transcendental = 1).
+    Near, per pair: 15 screened in FP32 (3 differences, 3 FMAs for R^2, rsqrt, R,
+    exp2, q*e, the FMA into the sum; the screening factor is folded into scaled
+    coordinates), 16 in FP64 (kappa*R), 12 Coulomb.  Far, per (cluster, target) pair, both precisions
+    (radial-derivative form): the ratio-Horner sweep takes one FMA per cluster
+    coefficient, 2 NQ(p); its O(p) per-pair set-up is not counted."""
+    fn = (15.0 if precision == "fp32" else 16.0) if screened else 12.0
+    ff = 2.0 * nqcoef(nterms)
     return stats["near_pairs"] * fn, stats["far_pairs"] * ff
 
 

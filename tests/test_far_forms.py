@@ -125,3 +125,25 @@ def test_nqcoef_matches_consumption_order():
         p = nt - 1
         n = sum((p - sum(l)) // 2 + 1 for l in monos(p))
         assert bench.nqcoef(nt) == n
+
+
+def test_sweep_takes_one_fma_per_coefficient():
+    """bench.py prices a far pair at 2 NQ flops: the ratio-Horner sweep of
+    far_q (t descending, then l0, l1, l2) takes NQ - 1 multiply-adds (every
+    coefficient but each t's first, plus the joins of rows, planes and t
+    levels) and one multiplication g_t * phi per t."""
+    import bench
+    for nt in range(1, 14):
+        p = nt - 1
+        fma = mul = 0
+        for t in range(p // 2, -1, -1):
+            L = p - 2 * t
+            for l0 in range(L, -1, -1):
+                for l1 in range(L - l0, -1, -1):
+                    fma += L - l0 - l1          # l2 steps after the row's first
+                    fma += l1 != L - l0         # row joined to the plane
+                fma += l0 != L                  # plane joined to phi
+            mul += 1                            # g_t * phi
+            fma += t != p // 2                  # S * z + g_t phi
+        assert fma == bench.nqcoef(nt) - 1, nt
+        assert mul == p // 2 + 1
