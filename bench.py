@@ -53,12 +53,12 @@ def nqcoef(nterms: int) -> int:
 
 def flops(stats: dict, nterms: int, screened: bool, precision: str) -> tuple[float, float]:
     """Algorithmic FLOPs of the near and far kernels (FMA = 2, transcendental = 1).
-    Near, per pair: 15 screened in FP32 (3 differences, 3 FMAs for R^2, rsqrt, R,
-    exp2, q*e, the FMA into the sum; the screening factor is folded into scaled
-    coordinates), 16 in FP64 (kappa*R), 12 Coulomb.  Far, per (cluster, target) pair, both precisions
-    (radial-derivative form): the ratio-Horner sweep takes one FMA per cluster
-    coefficient, 2 NQ(p); its O(p) per-pair set-up is not counted."""
-    fn = (15.0 if precision == "fp32" else 16.0) if screened else 12.0
+    Near, per pair: 16 screened, 12 Coulomb (3 differences, R^2 by FMAs with
+    delta^2, rsqrt, the FMA into the sum; screened adds R, -kappa R, exp, q*e).
+    Far, per (cluster, target) pair, both precisions (radial-derivative form):
+    the ratio-Horner sweep takes one FMA per cluster coefficient, 2 NQ(p); its
+    O(p) per-pair set-up is not counted."""
+    fn = 16.0 if screened else 12.0
     ff = 2.0 * nqcoef(nterms)
     return stats["near_pairs"] * fn, stats["far_pairs"] * ff
 
@@ -201,6 +201,12 @@ def main():
                                              args.maxmemb, args.kappa, args.delta,
                                              args.precision)),
             "flops_per_launch": kfl, "kernel_ms": kms,
+            # per far pair: what is priced (the sweep's FMAs) and what the
+            # reference's coefficient recurrence would cost (12 or 24 per Taylor
+            # coefficient), which this far kernel does not execute
+            "far_flops_per_pair": {"priced": 2 * nqcoef(args.nterms),
+                                   "coefficient_recurrence": (24 if screened else 12) *
+                                   (args.nterms * (args.nterms + 1) * (args.nterms + 2) // 6)},
             "other_kernel": {"name": "near" if kern == "far" else "far",
                              "tflops": (f_near / (st["ms_near"] * 1e-3) if kern == "far" else
                                         f_far / (st["ms_far"] * 1e-3)) / 1e12}}
