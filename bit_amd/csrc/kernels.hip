@@ -1035,78 +1035,111 @@ hipError_t moments_reduce(const TreeView& tv, int nterms, const int* node_chunk0
 // ---------------------------------------------------------------------------
 constexpr int kMaxCoef = ncoef(kMaxNterms);
 
+// C(n, a) for n < kMaxNterms (exact in FP64) and the (i, j, k) of every
+// packed coefficient index.
+struct M2MTab {
+  double binom[kMaxNterms][kMaxNterms];
+  unsigned char ijk[kMaxCoef][3];
+  constexpr M2MTab() : binom(), ijk() {
+    for (int n = 0; n < kMaxNterms; n++) {
+      binom[n][n] = 1.0;
+      for (int a = n; a > 0; a--) binom[n][a - 1] = binom[n][a] * a / (n - a + 1);
+    }
+    for (int n = 0; n < kMaxNterms; n++)
+      for (int m = 0; m <= n; m++)
+        for (int kk = 0; kk <= m; kk++) {
+          const int c = cidx(n - m, m - kk, kk);
+          ijk[c][0] = (unsigned char)(n - m);
+          ijk[c][1] = (unsigned char)(m - kk);
+          ijk[c][2] = (unsigned char)kk;
+        }
+  }
+};
+__constant__ M2MTab kM2M = M2MTab();
+
+// sum over a <= idx_e of C(idx_e, a) h^{idx_e - a} src[idx with idx_e = a],
+// a descending
 __device__ __forceinline__ double shift_1d(const double* src, int e, int i, int j, int k,
                                            double h) {
-  // sum over a <= idx_e of C(idx_e, a) h^{idx_e - a} src[idx with idx_e = a]
   int v[3] = {i, j, k};
   const int n = v[e];
-  double binom = 1.0, hp = 1.0, sum = 0.0;
+  double hp = 1.0, sum = 0.0;
   for (int a = n; a >= 0; a--) {
     v[e] = a;
-    sum = fma(binom * hp, src[cidx(v[0], v[1], v[2])], sum);
-    binom = binom * a / (n - a + 1);
+    sum = fma(kM2M.binom[n][a] * hp, src[cidx(v[0], v[1], v[2])], sum);
     hp *= h;
   }
   return sum;
 }
 
+// One work-group per parent; the eight children are shifted together, one
+// axis per pass over all (child, coefficient) pairs, then summed per
+// coefficient in child order.
 __global__ __launch_bounds__(256) void k_m2m(int nterms, const int* __restrict__ nodes,
                                              const int* __restrict__ child,
                                              const double4* __restrict__ center,
                                              const double* __restrict__ scale,
                                              double* __restrict__ mom,
                                              double* __restrict__ mom_hat64) {
-  __shared__ double A[kMaxCoef], B[kMaxCoef];
+  __shared__ double A[8 * kMaxCoef], B[8 * kMaxCoef];
+  __shared__ double H[8][3];
+  __shared__ int K[8];
   const int P = nodes[blockIdx.x];
   const int nc = ncoef(nterms);
   const int tid = threadIdx.x;
-  const double4 cp = center[P];
-  int ci[2][3];
-  int cnt = 0;
-  for (int cc = tid; cc < nc && cnt < 2; cc += 256, cnt++) {
-    int n = 0;
-    while (ncoef(n + 1) <= cc) n++;
-    int rem = cc - ncoef(n);
-    int m = 0;
-    while ((m + 1) * (m + 2) / 2 <= rem) m++;
-    int k = rem - m * (m + 1) / 2;
-    ci[cnt][0] = n - m;
-    ci[cnt][1] = m - k;
-    ci[cnt][2] = k;
+  if (tid < 8) {
+    const int C = child[8 * P + tid];
+    K[tid] = C;
+    if (C >= 0) {
+      const double4 cp = center[P], cc = center[C];
+      H[tid][0] = cc.x - cp.x;
+      H[tid][1] = cc.y - cp.y;
+      H[tid][2] = cc.z - cp.z;
+    }
   }
-  double acc[2] = {0.0, 0.0};
-  for (int q = 0; q < 8; q++) {
-    const int C = child[8 * P + q];
-    if (C < 0) continue;
-    const double4 cc = center[C];
-    const double hx = cc.x - cp.x, hy = cc.y - cp.y, hz = cc.z - cp.z;
-    for (int c = tid; c < nc; c += 256) A[c] = mom[(size_t)C * nc + c];
-    __syncthreads();
-    for (int u = 0; u < cnt; u++)
-      B[cidx(ci[u][0], ci[u][1], ci[u][2])] = shift_1d(A, 0, ci[u][0], ci[u][1], ci[u][2], hx);
-    __syncthreads();
-    for (int u = 0; u < cnt; u++)
-      A[cidx(ci[u][0], ci[u][1], ci[u][2])] = shift_1d(B, 1, ci[u][0], ci[u][1], ci[u][2], hy);
-    __syncthreads();
-    for (int u = 0; u < cnt; u++) acc[u] += shift_1d(A, 2, ci[u][0], ci[u][1], ci[u][2], hz);
-    __syncthreads();  // A is reloaded for the next child
+  __syncthreads();
+  const int tot = 8 * nc;
+  for (int x = tid; x < tot; x += 256) {
+    const int q = x / nc, c = x - q * nc;
+    A[x] = K[q] >= 0 ? mom[(size_t)K[q] * nc + c] : 0.0;
   }
+  __syncthreads();
+  for (int x = tid; x < tot; x += 256) {
+    const int q = x / nc, c = x - q * nc;
+    if (K[q] >= 0)
+      B[x] = shift_1d(A + q * nc, 0, kM2M.ijk[c][0], kM2M.ijk[c][1], kM2M.ijk[c][2], H[q][0]);
+  }
+  __syncthreads();
+  for (int x = tid; x < tot; x += 256) {
+    const int q = x / nc, c = x - q * nc;
+    if (K[q] >= 0)
+      A[x] = shift_1d(B + q * nc, 1, kM2M.ijk[c][0], kM2M.ijk[c][1], kM2M.ijk[c][2], H[q][1]);
+  }
+  __syncthreads();
+  for (int x = tid; x < tot; x += 256) {
+    const int q = x / nc, c = x - q * nc;
+    if (K[q] >= 0)
+      B[x] = shift_1d(A + q * nc, 2, kM2M.ijk[c][0], kM2M.ijk[c][1], kM2M.ijk[c][2], H[q][2]);
+  }
+  __syncthreads();
   const double inv = 1.0 / scale[P];
-  for (int u = 0; u < cnt; u++) {
-    const int c = cidx(ci[u][0], ci[u][1], ci[u][2]);
-    const int n = ci[u][0] + ci[u][1] + ci[u][2];
+  for (int c = tid; c < nc; c += 256) {
+    double acc = 0.0;
+    for (int q = 0; q < 8; q++)
+      if (K[q] >= 0) acc += B[q * nc + c];
+    const int n = kM2M.ijk[c][0] + kM2M.ijk[c][1] + kM2M.ijk[c][2];
     double f = 1.0;
     for (int i = 0; i < n; i++) f *= inv;
     const size_t g = (size_t)P * nc + c;
-    mom[g] = acc[u];
-    mom_hat64[g] = acc[u] * f;
+    mom[g] = acc;
+    mom_hat64[g] = acc * f;
   }
 }
 
 hipError_t moments_upward(const TreeView& tv, int nterms, int nnodes, const int* nodes,
                           double* mom, double* mom_hat64, hipStream_t s) {
   if (nnodes == 0) return hipSuccess;
-  if (ncoef(nterms) > 2 * 256) return hipErrorInvalidValue;
+  if (nterms < 1 || nterms > kMaxNterms) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_m2m, dim3(nnodes), dim3(256), 0, s, nterms, nodes, tv.child, tv.center,
                      tv.scale, mom, mom_hat64);
   return hipGetLastError();
