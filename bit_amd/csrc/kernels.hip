@@ -1459,9 +1459,10 @@ __device__ __forceinline__ double qsetup64(QPair<NT, double>& v, double px, doub
 // FP32 far field in the radial-derivative form: the cluster's Q is staged in
 // LDS like the FP64 kernel's moments (the next tiles are loaded into registers
 // while the current ones are evaluated) and read as broadcasts.  Two
-// consecutive items whose target masks are disjoint share one sweep, each lane
-// reading its own item's tile: a target still receives its items in list
-// order, so the results are those of one item at a time.
+// consecutive items with at most 64 targets together share one sweep: lanes
+// are remapped to the first item's targets, then the second's, each reading
+// its item's tile, and the results return to the targets' lanes, which add
+// them in list order, so the results are those of one item at a time.
 // Four waves per SIMD: the sweep is chains of dependent FMAs, and the issue
 // rate of one wave is well below the SIMD's.
 template <int NT, bool SCR>
@@ -1472,6 +1473,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   constexpr int PF = NQP / kBatch;   // Q values per lane per tile
   constexpr int T2 = NQP + 16;       // second tile's offset: other LDS banks
   __shared__ __align__(16) float mbuf[T2 + NQP];
+  __shared__ int tslot[kBatch];      // packed unit: lane j's target lane
+  __shared__ float res[2][kBatch];   // packed unit: per-item results by target lane
   int sys, b;
   which(map, sys, b);
   const EvalDesc& d = descs[sys];
@@ -1489,12 +1492,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
   Acc<float> acc;
   const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
-  // items [u, u + n): n = 2 when item u+1 exists and the masks are disjoint
+  // items [u, u + n): n = 2 when item u+1 exists and the two items have at
+  // most 64 targets together (one sweep, lanes remapped to their targets)
   auto unit_len = [&](int u) -> int {
     if (u + 1 >= i1) return 1;
     const unsigned long long m0 = uniform64(items[u].mask), m1 = uniform64(items[u + 1].mask);
-    return (m0 & m1) ? 1 : 2;
+    return __popcll(m0) + __popcll(m1) <= kBatch ? 2 : 1;
   };
+  const unsigned long long below = (1ull << lane) - 1ull;
   float pa[PF], pb[PF];
   auto fetch = [&](int u, int n) {  // node tiles are padded to NQP floats
     const int na = uniform(items[u].node);
@@ -1530,16 +1535,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       fetch(un, nn);
     }
     const bool in0 = (m0 >> lane) & 1ull, in1 = (m1 >> lane) & 1ull;
-    if (in0 || in1) {
-      const int node = in1 ? n1 : n0;
-      const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
-      QPair<NT, float> v;
-      const float rinv = qsetup<NT, SCR>(v, px, py, pz, cx, cy, cz, (float)scale[node], kappa32,
-                                         delta2_32);
-      acc.add(qall<NT, float>(std::make_integer_sequence<int, NQ>{}, v, mbuf + (in1 ? T2 : 0)) *
-              rinv);
+    if (n == 1) {
+      if (in0) {
+        const double cx = center[4 * n0], cy = center[4 * n0 + 1], cz = center[4 * n0 + 2];
+        QPair<NT, float> v;
+        const float rinv = qsetup<NT, SCR>(v, px, py, pz, cx, cy, cz, (float)scale[n0],
+                                           kappa32, delta2_32);
+        acc.add(qall<NT, float>(std::make_integer_sequence<int, NQ>{}, v, mbuf) * rinv);
+      }
+    } else {
+      // Packed unit: lane j takes the j-th target of the first item, then the
+      // targets of the second; each result returns to its target's lane, which
+      // adds the first item's before the second's (list order, as one item at
+      // a time).
+      const int k0 = __popcll(m0), k1 = __popcll(m1);
+      if (in0) tslot[__popcll(m0 & below)] = lane;
+      if (in1) tslot[k0 + __popcll(m1 & below)] = lane;
+      __syncthreads();
+      const bool act = lane < k0 + k1, second = lane >= k0;
+      const int tl = act ? tslot[lane] : lane;
+      const double qx = __shfl(px, tl), qy = __shfl(py, tl), qz = __shfl(pz, tl);
+      if (act) {
+        const int node = second ? n1 : n0;
+        const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
+        QPair<NT, float> v;
+        const float rinv = qsetup<NT, SCR>(v, qx, qy, qz, cx, cy, cz, (float)scale[node],
+                                           kappa32, delta2_32);
+        res[second ? 1 : 0][tl] =
+            qall<NT, float>(std::make_integer_sequence<int, NQ>{}, v, mbuf + (second ? T2 : 0)) *
+            rinv;
+      }
+      __syncthreads();
+      if (in0) acc.add(res[0][lane]);
+      if (in1) acc.add(res[1][lane]);
     }
-    __syncthreads();  // the tiles are overwritten next unit
+    __syncthreads();  // the tiles and slots are overwritten next unit
     u = un;
     n = nn;
   }
