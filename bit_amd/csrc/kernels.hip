@@ -1882,9 +1882,12 @@ __device__ __forceinline__ void near32_batch(const EvalDesc& d, int b, float4* t
     d.out[tv.tperm[t]] = d.phi_far[t] + (SCALED ? (double)acc.value() * cs : acc.value());
 }
 
+// At least six waves per SIMD: left alone the compiler keeps a whole group of
+// 16 sources in flight (114 VGPRs, 4 waves); with the bound it keeps fewer
+// (66 VGPRs, 7 waves), which hides more of the LDS and transcendental latency.
 template <bool SCR>
-__global__ __launch_bounds__(64) void k_near32(const EvalDesc* __restrict__ descs,
-                                               const int2* __restrict__ map) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_near32(
+    const EvalDesc* __restrict__ descs, const int2* __restrict__ map) {
   __shared__ float4 tile[64];
   int sys, b;
   which(map, sys, b);
