@@ -1211,6 +1211,10 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   // entries a batch may write: all at the scanned offsets, cap when staging
   const int flim = MODE == kListStage ? cap_far : INT_MAX;
   const int nlim = MODE == kListStage ? cap_near : INT_MAX;
+  // Items are gathered in registers, entry k of each run of 64 in lane k, and
+  // written a run at a time (one coalesced store instead of 64 single-lane ones).
+  unsigned long long fbm = 0ull, nbm = 0ull;
+  int fbn = 0, nbn = 0;
   __syncthreads();
   while (sp > 0) {
     sp--;
@@ -1225,14 +1229,28 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
     const bool acc = mac_accept(px, py, pz, c, mp);
     const unsigned long long am = ballot(acc) & m;
     if (am) {
-      if (kWrite && lane == 0 && nf < flim) far_items[fbase + nf] = Item{am, node, 0};
+      if (kWrite) {
+        if (lane == (nf & 63)) {
+          fbm = am;
+          fbn = node;
+        }
+        if ((nf & 63) == 63 && nf - 63 + lane < flim)
+          far_items[fbase + (nf - 63 + lane)] = Item{fbm, fbn, 0};
+      }
       nf++;
       far_pairs += __popcll(am);
     }
     const unsigned long long rm = m & ~am;
     if (rm) {
       if (kids[0] < 0) {
-        if (kWrite && lane == 0 && nn < nlim) near_items[nbase + nn] = Item{rm, node, 0};
+        if (kWrite) {
+          if (lane == (nn & 63)) {
+            nbm = rm;
+            nbn = node;
+          }
+          if ((nn & 63) == 63 && nn - 63 + lane < nlim)
+            near_items[nbase + (nn - 63 + lane)] = Item{nbm, nbn, 0};
+        }
         nn++;
         const int2 r = make_int2(grange[2 * node], grange[2 * node + 1]);
         near_pairs += (unsigned long long)__popcll(rm) * (unsigned long long)(r.y - r.x);
@@ -1256,6 +1274,12 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
       }
     }
     __syncthreads();
+  }
+  if (kWrite) {  // the last, partial runs
+    const int rf = nf & 63, rn = nn & 63;
+    if (lane < rf && nf - rf + lane < flim) far_items[fbase + (nf - rf + lane)] = Item{fbm, fbn, 0};
+    if (lane < rn && nn - rn + lane < nlim)
+      near_items[nbase + (nn - rn + lane)] = Item{nbm, nbn, 0};
   }
   if (MODE != kListFill && lane == 0) {
     far_cnt[b] = nf;
