@@ -84,6 +84,7 @@ EXPORTS = [
     "bit_copy_to_host", "bit_system_set_charges", "bit_solve_second_kind", "bit_gmres_dense",
     "bit_system_prepare", "bit_eval_potential_multi", "bit_gmres", "bit_ctx_set_host_threads",
     "bit_copy_on_device", "bit_eval_potential_points", "bit_eval_field_points",
+    "bit_direct_field", "bit_direct_field_points",
 ]
 
 
@@ -122,6 +123,8 @@ def _load():
     L.bit_eval_field.argtypes = [vp, C.POINTER(Params), vp, C.c_int]
     L.bit_system_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bit_direct_potential.argtypes = [vp, C.c_double, C.c_double, C.c_int, vp, vp]
+    L.bit_direct_field.argtypes = [vp, C.c_double, C.c_int, C.c_int, vp, vp]
+    L.bit_direct_field_points.argtypes = [vp, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp]
     L.bit_system_export_lists.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp,
                                           vp, vp]
     L.bit_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
@@ -477,6 +480,23 @@ class System:
         _check(lib.bit_direct_potential(self.h, float(kappa), float(delta2), len(t), _ptr(t),
                                         _ptr(out)))
         return out
+
+    def direct_field(self, delta2, tidx, kernel_order=0):
+        """FP64 GPU direct-sum field at particle targets tidx (own particle
+        skipped), [len(tidx), 3]."""
+        t = np.ascontiguousarray(tidx, dtype=np.int32)
+        out = np.zeros(3 * len(t))
+        _check(lib.bit_direct_field(self.h, float(delta2), int(kernel_order), len(t), _ptr(t),
+                                    _ptr(out)))
+        return out.reshape(-1, 3)
+
+    def direct_field_at(self, px, py, pz, delta2, kernel_order=0):
+        """FP64 GPU direct-sum field at arbitrary points (every source), [m, 3]."""
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        out = np.zeros(3 * len(px))
+        _check(lib.bit_direct_field_points(self.h, float(delta2), int(kernel_order), len(px),
+                                           _ptr(px), _ptr(py), _ptr(pz), _ptr(out)))
+        return out.reshape(-1, 3)
 
     def lists(self, nterms=6, acc=ACC, delta2=0.0, mac_sub=None):
         """Batch interaction lists for the MAC parameters: dict of far/near

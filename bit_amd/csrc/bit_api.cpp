@@ -78,6 +78,9 @@ struct bit_ctx {
   // [0] particles as targets, [1] external points; far then near.  A hint
   // only: a batch that outgrows them is rebuilt by count-then-fill.
   std::atomic<int> list_cap[2][2] = {{{0}, {0}}, {{0}, {0}}};
+  // this device's scratch of bit_eval_potential_noalloc (a static device
+  // array shared by every context on the device)
+  void* scratch = nullptr;
 };
 
 namespace {
@@ -240,8 +243,8 @@ struct bit_system {
   bit::dev::EvalDesc qhdesc{};
   // outputs
   DevBuf phi_far, out;
-  // direct sum
-  DevBuf tslot, dpart, dout;
+  // direct sum (particle targets tslot; point targets dpx, dpy, dpz)
+  DevBuf tslot, dpart, dout, dpx, dpy, dpz;
   // charge updates
   DevBuf qin;
   // evaluation descriptor (host copy kept alive for the async upload)
@@ -722,7 +725,12 @@ int bit_ctx_create(int device, size_t hbm_budget_bytes, bit_ctx** out) {
   bit_ctx* c = new bit_ctx();
   c->device = device;
   c->budget = hbm_budget_bytes;
-  hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  hipError_t e = bit::dev::noalloc_scratch(device, &c->scratch);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(BIT_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
     return fail(BIT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -1152,6 +1160,13 @@ int bit_eval_potential_multi(bit_ctx* ctx, int count, bit_system* const* systems
     return fail(BIT_E_INVALID, "bad multi-evaluation arguments");
   if (count == 0) return BIT_OK;
   HIP_TRY(hipSetDevice(ctx->device));
+  {
+    // a system's moments, lists and result buffers serve one entry of a launch
+    std::vector<const bit_system*> seen(systems, systems + count);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+      return fail(BIT_E_INVALID, "a system appears more than once in a mixed launch");
+  }
   for (int i = 0; i < count; i++) {
     if (!systems[i] || !outs[i]) return fail(BIT_E_INVALID, "null system or output");
     if (systems[i]->ctx->device != ctx->device)
@@ -1558,6 +1573,34 @@ int bit_gmres(int n, bit_matvec_fn matvec, void* user, const double* b, double* 
   return BIT_OK;
 }
 
+}  // extern "C"
+
+// Sorted slots of the particle targets tidx (original indices) in s->tslot.
+static int upload_target_slots(bit_system* s, int ntargets, const int* tidx) {
+  bit_ctx* c = s->ctx;
+  const int n = s->tree.n;
+  BIT_TRY(ensure_host_perm(s));
+  std::vector<int> inv(n), slot(ntargets);
+  for (int i = 0; i < n; i++) inv[s->tree.perm[i]] = i;
+  for (int t = 0; t < ntargets; t++) {
+    if (tidx[t] < 0 || tidx[t] >= n) return fail(BIT_E_INVALID, "target index out of range");
+    slot[t] = inv[tidx[t]];
+  }
+  BIT_TRY(upload(c, s->tslot, slot.data(), slot.size() * sizeof(int)));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // `slot` is a host temporary
+  return BIT_OK;
+}
+
+// Source slices of a direct sum: enough work-groups to fill the device, each
+// slice at least 1024 sources.
+static int direct_split(int n, int ntargets) {
+  const int tblocks = (ntargets + 255) / 256;
+  const int nsplit = std::max(1, std::min(1024, 4096 / tblocks));
+  return std::min(nsplit, std::max(1, n / 1024));
+}
+
+extern "C" {
+
 int bit_direct_potential(bit_system* s, double kappa, double delta2, int ntargets,
                          const int* tidx, double* out) {
   if (!s || (!tidx && ntargets > 0) || (!out && ntargets > 0))
@@ -1567,18 +1610,8 @@ int bit_direct_potential(bit_system* s, double kappa, double delta2, int ntarget
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   const int n = s->tree.n;
-  BIT_TRY(ensure_host_perm(s));
-  std::vector<int> inv(n), slot(ntargets);
-  for (int i = 0; i < n; i++) inv[s->tree.perm[i]] = i;
-  for (int t = 0; t < ntargets; t++) {
-    if (tidx[t] < 0 || tidx[t] >= n) return fail(BIT_E_INVALID, "target index out of range");
-    slot[t] = inv[tidx[t]];
-  }
-  int tblocks = (ntargets + 255) / 256;
-  int nsplit = std::max(1, std::min(1024, 4096 / tblocks));
-  nsplit = std::min(nsplit, std::max(1, n / 1024));
-  BIT_TRY(upload(c, s->tslot, slot.data(), slot.size() * sizeof(int)));
-  HIP_TRY(hipStreamSynchronize(c->stream));  // `slot` is a host temporary
+  BIT_TRY(upload_target_slots(s, ntargets, tidx));
+  const int nsplit = direct_split(n, ntargets);
   BIT_TRY(s->dpart.ensure(c, (size_t)nsplit * ntargets * sizeof(double)));
   BIT_TRY(s->dout.ensure(c, (size_t)ntargets * sizeof(double)));
   HIP_TRY(bit::dev::direct_potential(s->view(), kappa, delta2, ntargets, s->tslot.as<int>(),
@@ -1586,6 +1619,63 @@ int bit_direct_potential(bit_system* s, double kappa, double delta2, int ntarget
                                      c->stream));
   HIP_TRY(hipMemcpyAsync(out, s->dout.p, (size_t)ntargets * sizeof(double),
                          hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
+
+int bit_direct_field(bit_system* s, double delta2, int kernel_order, int ntargets,
+                     const int* tidx, double* out) {
+  if (!s || (!tidx && ntargets > 0) || (!out && ntargets > 0))
+    return fail(BIT_E_INVALID, "null argument");
+  if (ntargets < 0) return fail(BIT_E_INVALID, "ntargets < 0");
+  if (!std::isfinite(delta2) || delta2 < 0 || kernel_order < 0 ||
+      kernel_order > BIT_MAX_KERNEL_ORDER)
+    return fail(BIT_E_INVALID, "delta2 must be finite and >= 0, kernel_order in [0, " +
+                                   std::to_string(BIT_MAX_KERNEL_ORDER) + "]");
+  if (ntargets == 0) return BIT_OK;
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = s->tree.n;
+  BIT_TRY(upload_target_slots(s, ntargets, tidx));
+  const int nsplit = direct_split(n, ntargets);
+  BIT_TRY(s->dpart.ensure(c, (size_t)3 * nsplit * ntargets * sizeof(double)));
+  BIT_TRY(s->dout.ensure(c, (size_t)3 * ntargets * sizeof(double)));
+  HIP_TRY(bit::dev::direct_field(s->view(), delta2, kernel_order, ntargets, s->tslot.as<int>(),
+                                 nullptr, nullptr, nullptr, nsplit, s->dpart.as<double>(),
+                                 s->dout.as<double>(), c->stream));
+  HIP_TRY(hipMemcpyAsync(out, s->dout.p, (size_t)3 * ntargets * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
+int bit_direct_field_points(bit_system* s, double delta2, int kernel_order, int m,
+                            const double* x, const double* y, const double* z, double* out) {
+  if (!s) return fail(BIT_E_INVALID, "null system");
+  if (m < 0) return fail(BIT_E_INVALID, "negative point count");
+  if (!std::isfinite(delta2) || delta2 < 0 || kernel_order < 0 ||
+      kernel_order > BIT_MAX_KERNEL_ORDER)
+    return fail(BIT_E_INVALID, "delta2 must be finite and >= 0, kernel_order in [0, " +
+                                   std::to_string(BIT_MAX_KERNEL_ORDER) + "]");
+  if (m == 0) return BIT_OK;
+  if (!x || !y || !z || !out) return fail(BIT_E_INVALID, "null point or output array");
+  if (!finite_all(m, x) || !finite_all(m, y) || !finite_all(m, z))
+    return fail(BIT_E_INVALID, "non-finite point coordinate");
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = s->tree.n;
+  const size_t mb = (size_t)m * sizeof(double);
+  BIT_TRY(upload(c, s->dpx, x, mb));
+  BIT_TRY(upload(c, s->dpy, y, mb));
+  BIT_TRY(upload(c, s->dpz, z, mb));
+  const int nsplit = direct_split(n, m);
+  BIT_TRY(s->dpart.ensure(c, (size_t)3 * nsplit * m * sizeof(double)));
+  BIT_TRY(s->dout.ensure(c, 3 * mb));
+  HIP_TRY(bit::dev::direct_field(s->view(), delta2, kernel_order, m, nullptr,
+                                 s->dpx.as<double>(), s->dpy.as<double>(), s->dpz.as<double>(),
+                                 nsplit, s->dpart.as<double>(), s->dout.as<double>(), c->stream));
+  HIP_TRY(hipMemcpyAsync(out, s->dout.p, 3 * mb, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return BIT_OK;
 }

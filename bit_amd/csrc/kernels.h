@@ -213,6 +213,27 @@ hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, const 
 hipError_t direct_potential(const TreeView& tv, double kappa, double delta2, int ntargets,
                             const int* tslot, int nsplit, double* partial, double* out,
                             hipStream_t s);
+// FP64 direct-sum field E_t = sum_j q_j (x_t - x_j) G'(rs), rs = r^2 + delta2, the
+// regularized family up to korder, at sorted particle slots tslot (own slot
+// skipped) or, with tslot == nullptr, at points (qx, qy, qz) (every source).
+// partial holds 3 * nsplit * ntargets doubles; out[3 * ntargets].
+hipError_t direct_field(const TreeView& tv, double delta2, int korder, int ntargets,
+                        const int* tslot, const double* qx, const double* qy, const double* qz,
+                        int nsplit, double* partial, double* out, hipStream_t s);
+
+// ---- allocation-free evaluation ----
+// Scratch of bit_eval_potential_noalloc: the evaluation descriptor, then the
+// far-field results of up to kNoAllocMaxN targets.
+constexpr size_t kNoAllocDescBytes = 1024;
+constexpr int kNoAllocMaxN = 1 << 22;
+constexpr size_t kNoAllocScratchBytes = kNoAllocDescBytes + (size_t)kNoAllocMaxN * sizeof(double);
+static_assert(sizeof(EvalDesc) <= kNoAllocDescBytes, "descriptor region too small");
+// Device address of the calling device's scratch (resolved once per device,
+// with `device` current).
+hipError_t noalloc_scratch(int device, void** out);
+// *dst = d, stored by a kernel that takes d by value (stream-ordered, no copy
+// from host memory).
+hipError_t put_desc(const EvalDesc& d, EvalDesc* dst, hipStream_t s);
 
 }  // namespace dev
 }  // namespace bit

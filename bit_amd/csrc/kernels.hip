@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -1808,8 +1809,10 @@ __device__ __forceinline__ void near32_batch(const EvalDesc& d, int b, float4* t
     const int2 r = tv.range[node];
     const double4 c = tv.center[node];
     const bool act = (mask >> lane) & 1ull;
-    const float ox = (float)((px - c.x) * cs), oy = (float)((py - c.y) * cs),
-                oz = (float)((pz - c.z) * cs);
+    // the sources' operation sequence (offset rounded to FP32, then scaled in
+    // FP32), so that a target on a source gives a zero offset exactly
+    const float ox = (float)(px - c.x) * csf, oy = (float)(py - c.y) * csf,
+                oz = (float)(pz - c.z) * csf;
     // particle targets: the self pair lies in leaves holding some of the
     // batch's targets; point targets: pairs closer than eps are skipped instead
     const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
@@ -2347,6 +2350,137 @@ hipError_t direct_potential(const TreeView& tv, double kappa, double delta2, int
                      nsplit, partial);
   hipLaunchKernelGGL(k_direct_reduce, dim3((ntargets + 255) / 256), dim3(256), 0, s, ntargets,
                      nsplit, partial, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct-sum field (force_particle_ds_noreg/_reg, force_point_ds_noreg/_reg,
+// tree3d.cpp:125-330).  Per pair, in the reference's operation order:
+//   d = x_source - x_target, rs = |d|^2 + d2, temp = 1/(sqrt(rs) rs) (plus the
+//   regularized family's terms, eval_grad_kernel_reg tree3d.cpp:80-91),
+//   E += q (-d temp).
+// Particle targets (tslot != nullptr) skip their own slot only; point targets
+// take every source, so a point on a source gives what the reference gives
+// there (0/0 with d2 = 0).  Partial sums over nsplit source slices, added in
+// slice order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_direct_field(TreeView tv, double d2, int korder,
+                                                      int ntargets, const int* __restrict__ tslot,
+                                                      const double* __restrict__ qx,
+                                                      const double* __restrict__ qy,
+                                                      const double* __restrict__ qz, int nsplit,
+                                                      double* __restrict__ partial) {
+  __shared__ double4 tile[256];
+  const int tb = blockIdx.x, sl = blockIdx.y;
+  const int ti = tb * 256 + threadIdx.x;
+  const bool valid = ti < ntargets;
+  const int tc = valid ? ti : 0;
+  int me = -1;
+  double px, py, pz;
+  if (tslot) {
+    me = tslot[tc];
+    px = tv.sx[me];
+    py = tv.sy[me];
+    pz = tv.sz[me];
+  } else {
+    px = qx[tc];
+    py = qy[tc];
+    pz = qz[tc];
+  }
+  const long long per = ((long long)tv.n + nsplit - 1) / nsplit;
+  const int s0 = (int)min((long long)tv.n, per * sl), s1 = (int)min((long long)tv.n, per * (sl + 1));
+  double ex = 0.0, ey = 0.0, ez = 0.0;
+  for (int base = s0; base < s1; base += 256) {
+    const int cnt = min(256, s1 - base);
+    if ((int)threadIdx.x < cnt) {
+      int j = base + threadIdx.x;
+      tile[threadIdx.x] = make_double4(tv.sx[j], tv.sy[j], tv.sz[j], tv.sq[j]);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; j++) {
+      if (base + j == me) continue;
+      const double4 s = tile[j];
+      const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+      const double rs = dx * dx + dy * dy + dz * dz + d2;
+      const double r = sqrt(rs);
+      double temp = 1.0 / (r * rs), f = temp;
+      for (int l = 1; l <= korder; l++) {
+        f = f * (l + 0.5) / l * d2 / rs;
+        temp += f;
+      }
+      ex += s.w * (-dx * temp);
+      ey += s.w * (-dy * temp);
+      ez += s.w * (-dz * temp);
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    double* p = partial + 3 * ((size_t)sl * ntargets + ti);
+    p[0] = ex;
+    p[1] = ey;
+    p[2] = ez;
+  }
+}
+
+__global__ void k_direct_field_reduce(int ntargets, int nsplit, const double* __restrict__ partial,
+                                      double* out) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= ntargets) return;
+  double e[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < nsplit; k++)
+    for (int c = 0; c < 3; c++) e[c] += partial[3 * ((size_t)k * ntargets + ti) + c];
+  for (int c = 0; c < 3; c++) out[3 * (size_t)ti + c] = e[c];
+}
+
+hipError_t direct_field(const TreeView& tv, double delta2, int korder, int ntargets,
+                        const int* tslot, const double* qx, const double* qy, const double* qz,
+                        int nsplit, double* partial, double* out, hipStream_t s) {
+  if (ntargets == 0) return hipSuccess;
+  dim3 grid((ntargets + 255) / 256, nsplit);
+  hipLaunchKernelGGL(k_direct_field, grid, dim3(256), 0, s, tv, delta2, korder, ntargets, tslot,
+                     qx, qy, qz, nsplit, partial);
+  hipLaunchKernelGGL(k_direct_field_reduce, dim3((ntargets + 255) / 256), dim3(256), 0, s,
+                     ntargets, nsplit, partial, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free evaluation: its scratch (the evaluation descriptor and the
+// far-field results of up to kNoAllocMaxN targets) lives in this file-scope
+// device array, one copy per GPU, held until the process ends.  Host code never
+// uses the array's name as an address: noalloc_scratch() resolves each device's
+// copy once with hipGetSymbolAddress and hands out that pointer.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // device is current
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+// The descriptor arrives by value as a kernel argument and is stored to the
+// scratch by one lane: no host-to-device copy, nothing allocated.
+__global__ void k_put_desc(EvalDesc d, EvalDesc* dst) {
+  if (threadIdx.x == 0) *dst = d;
+}
+
+hipError_t put_desc(const EvalDesc& d, EvalDesc* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_put_desc, dim3(1), dim3(64), 0, s, d, dst);
   return hipGetLastError();
 }
 

@@ -8,7 +8,9 @@
 #include "tree3d.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <set>
 #include <tuple>
@@ -331,8 +333,21 @@ void treemoments(int numtree, int nterms, TREE* tree, PARTICLE* part) {
     return;
   }
   Bound& b = it->second;
-  if (!b.sys) b.sys = make_system(part, b.npart, b.maxmemb);
-  if (!b.sys) return;
+  if (!b.sys) {
+    b.sys = make_system(part, b.npart, b.maxmemb);
+    if (!b.sys) return;
+  } else {
+    // the reference recomputes the moments from part[].tot_charge on every
+    // call (tree3d.cpp:2685-2707): upload the current charges, and drop the
+    // results computed with the previous ones
+    std::vector<double> q(b.npart);
+    for (int i = 0; i < b.npart; i++) q[i] = part[i].tot_charge;
+    if (bit_system_set_charges(b.sys, q.data(), 0) != BIT_OK) {
+      cout << "[treemoments] " << bit_last_error() << endl;
+      return;
+    }
+    b.cache.clear();
+  }
   if (bit_moments(b.sys, nterms) != BIT_OK) {
     cout << "[treemoments] " << bit_last_error() << endl;
     return;
@@ -431,20 +446,282 @@ void shielded_freespace_pot_tree(int npart, PARTICLE* part, double alpha, int nt
         false);
 }
 
-void shielded_freespace_pot_ds(int npart, PARTICLE* part, double alpha) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  bit_system* s = make_system(part, npart, std::max(1, npart));
-  if (!s) return;
+// ---------------------------------------------------------------------------
+// direct sums (no tree): GPU FP64 sums over every particle
+// ---------------------------------------------------------------------------
+namespace {
+
+enum DirectKind { kDsNoreg = 0, kDsReg = 1, kDsShieldedPot = 2 };
+
+// The particle array as uploaded for the direct sums: a one-leaf system and a
+// snapshot of the fields the sums read.  Results per (kind, alpha), computed
+// for all particles at once, are served while the array still holds the
+// snapshot's values, so per-particle calls from many threads cost one GPU
+// evaluation and a comparison each.
+struct DirectState {
+  const PARTICLE* part = nullptr;
+  int npart = 0;
+  std::vector<uint64_t> snap;  // bit patterns of x, y, z, tot_charge per particle
+  bit_system* sys = nullptr;
+  std::map<std::pair<int, double>, std::shared_ptr<const std::vector<double>>> results;
+  ~DirectState() {
+    if (sys) bit_system_destroy(sys);
+  }
+};
+// guarded by g_mu; never destroyed, so no device call runs at process exit
+std::shared_ptr<DirectState>& g_direct = *new std::shared_ptr<DirectState>();
+
+uint64_t bits(double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, sizeof u);
+  return u;
+}
+
+bool same_particles(const DirectState& st, const PARTICLE* part, int npart) {
+  if (st.part != part || st.npart != npart) return false;
+  const uint64_t* s = st.snap.data();
+  for (int i = 0; i < npart; i++, s += 4)
+    if (s[0] != bits(part[i].x) || s[1] != bits(part[i].y) || s[2] != bits(part[i].z) ||
+        s[3] != bits(part[i].tot_charge))
+      return false;
+  return true;
+}
+
+// The current direct-sum state of `part` (made anew when the array changed),
+// with g_mu held by the caller's lock `lk` on return; nullptr on failure
+// (message printed).
+std::shared_ptr<DirectState> direct_state(const char* fn, const PARTICLE* part, int npart,
+                                          std::unique_lock<std::mutex>& lk) {
+  for (;;) {
+    std::shared_ptr<DirectState> st = g_direct;
+    lk.unlock();
+    const bool same = st && same_particles(*st, part, npart);  // O(n), outside the lock
+    lk.lock();
+    if (g_direct != st) continue;  // replaced meanwhile: compare again
+    if (same) return st;
+    auto ns = std::make_shared<DirectState>();
+    ns->part = part;
+    ns->npart = npart;
+    ns->snap.resize(4 * (size_t)npart);
+    for (int i = 0; i < npart; i++) {
+      ns->snap[4 * (size_t)i] = bits(part[i].x);
+      ns->snap[4 * (size_t)i + 1] = bits(part[i].y);
+      ns->snap[4 * (size_t)i + 2] = bits(part[i].z);
+      ns->snap[4 * (size_t)i + 3] = bits(part[i].tot_charge);
+    }
+    ns->sys = make_system(const_cast<PARTICLE*>(part), npart, std::max(1, npart));
+    if (!ns->sys) {
+      cout << "[" << fn << "] direct sum: no device copy of the particles" << endl;
+      return nullptr;
+    }
+    g_direct = ns;
+    return ns;
+  }
+}
+
+// All-particle result of a direct-sum kind: 3 per particle (fields) or 1
+// (shielded potential, the reference's -phi).
+std::shared_ptr<const std::vector<double>> direct_all(const char* fn, const PARTICLE* part,
+                                                      int npart, int kind, double alpha) {
+  std::unique_lock<std::mutex> lk(g_mu);
+  std::shared_ptr<DirectState> st = direct_state(fn, part, npart, lk);
+  if (!st) return nullptr;
+  auto key = std::make_pair(kind, kind == kDsShieldedPot ? alpha : 0.0);
+  auto it = st->results.find(key);
+  if (it != st->results.end()) return it->second;
   std::vector<int> idx(npart);
   for (int i = 0; i < npart; i++) idx[i] = i;
-  std::vector<double> phi(npart);
-  if (bit_direct_potential(s, alpha, 0.0, npart, idx.data(), phi.data()) != BIT_OK) {
-    cout << "[shielded_freespace_pot_ds] " << bit_last_error() << endl;
+  auto v = std::make_shared<std::vector<double>>();
+  int rc;
+  if (kind == kDsShieldedPot) {
+    v->resize(npart);
+    rc = bit_direct_potential(st->sys, alpha, 0.0, npart, idx.data(), v->data());
+    for (double& x : *v) x = -x;  // shielded_pot_particle_ds returns -phi
   } else {
-    for (int i = 0; i < npart; i++)  // shielded_pot_particle_ds returns -phi
-      part[i].xforce = -phi[i] * charge_constant * part[i].tot_charge;
+    v->resize(3 * (size_t)npart);
+    rc = kind == kDsReg ? bit_direct_field(st->sys, DEL, KERNEL_ORDER, npart, idx.data(), v->data())
+                        : bit_direct_field(st->sys, 0.0, 0, npart, idx.data(), v->data());
   }
-  bit_system_destroy(s);
+  if (rc != BIT_OK) {
+    cout << "[" << fn << "] " << bit_last_error() << endl;
+    return nullptr;
+  }
+  st->results.emplace(key, v);
+  return v;
+}
+
+void particle_field_ds(const char* fn, int kind, int ind, int npart, PARTICLE* part,
+                       double* xfor, double* yfor, double* zfor) {
+  if (ind < 0 || ind >= npart) {
+    cout << "[" << fn << "] particle index " << ind << " outside [0, " << npart << ")" << endl;
+    return;
+  }
+  auto v = direct_all(fn, part, npart, kind, 0.0);
+  if (!v) return;
+  *xfor = (*v)[3 * (size_t)ind];
+  *yfor = (*v)[3 * (size_t)ind + 1];
+  *zfor = (*v)[3 * (size_t)ind + 2];
+}
+
+void point_field_ds(const char* fn, bool reg, double x, double y, double z, int npart,
+                    PARTICLE* part, double* xfor, double* yfor, double* zfor) {
+  if (npart < 1) {
+    *xfor = *yfor = *zfor = 0.0;
+    return;
+  }
+  std::unique_lock<std::mutex> lk(g_mu);
+  std::shared_ptr<DirectState> st = direct_state(fn, part, npart, lk);
+  if (!st) return;
+  double E[3];
+  int rc = reg ? bit_direct_field_points(st->sys, DEL, KERNEL_ORDER, 1, &x, &y, &z, E)
+               : bit_direct_field_points(st->sys, 0.0, 0, 1, &x, &y, &z, E);
+  if (rc != BIT_OK) {
+    cout << "[" << fn << "] " << bit_last_error() << endl;
+    return;
+  }
+  *xfor = E[0];
+  *yfor = E[1];
+  *zfor = E[2];
+}
+
+void all_fields_ds(const char* fn, int kind, int npart, PARTICLE* part) {
+  if (npart < 1) return;
+  auto v = direct_all(fn, part, npart, kind, 0.0);
+  if (!v) return;
+  for (int i = 0; i < npart; i++) {  // the reference's product order
+    part[i].xforce = (*v)[3 * (size_t)i] * charge_constant * part[i].tot_charge;
+    part[i].yforce = (*v)[3 * (size_t)i + 1] * charge_constant * part[i].tot_charge;
+    part[i].zforce = (*v)[3 * (size_t)i + 2] * charge_constant * part[i].tot_charge;
+  }
+}
+
+}  // namespace
+
+void eval_grad_kernel_noreg(double dx, double dy, double dz, double rs, double* xfor,
+                            double* yfor, double* zfor) {
+  const double t = 1 / (rs * sqrt(rs));
+  *xfor = -dx * t;
+  *yfor = -dy * t;
+  *zfor = -dz * t;
+}
+
+void eval_grad_kernel_reg(double dx, double dy, double dz, double rs, double* xfor,
+                          double* yfor, double* zfor) {
+  // the regularized family's factor chain, in the reference's operation order
+  // (tree3d.cpp:80-91)
+  const double r = sqrt(rs);
+  double t = 1.0 / (r * rs), f = t;
+  for (int l = 1; l <= KERNEL_ORDER; l++) {
+    f = f * (l + 0.5) / l * DEL / rs;
+    t += f;
+  }
+  *xfor = -dx * t;
+  *yfor = -dy * t;
+  *zfor = -dz * t;
+}
+
+void force_point_ds_noreg(double x, double y, double z, int npart, PARTICLE* part,
+                          double* xfor, double* yfor, double* zfor) {
+  point_field_ds("force_point_ds_noreg", false, x, y, z, npart, part, xfor, yfor, zfor);
+}
+
+void force_point_ds_reg(double x, double y, double z, int npart, PARTICLE* part,
+                        double* xfor, double* yfor, double* zfor) {
+  point_field_ds("force_point_ds_reg", true, x, y, z, npart, part, xfor, yfor, zfor);
+}
+
+void force_particle_ds_noreg(int ind, int npart, PARTICLE* part, double* xfor, double* yfor,
+                             double* zfor) {
+  particle_field_ds("force_particle_ds_noreg", kDsNoreg, ind, npart, part, xfor, yfor, zfor);
+}
+
+void force_particle_ds_reg(int ind, int npart, PARTICLE* part, double* xfor, double* yfor,
+                           double* zfor) {
+  particle_field_ds("force_particle_ds_reg", kDsReg, ind, npart, part, xfor, yfor, zfor);
+}
+
+void freespace_particles_ds_noreg(int npart, PARTICLE* part) {
+  all_fields_ds("freespace_particles_ds_noreg", kDsNoreg, npart, part);
+}
+
+void freespace_particles_ds_reg(int npart, PARTICLE* part) {
+  all_fields_ds("freespace_particles_ds_reg", kDsReg, npart, part);
+}
+
+void shielded_pot_particle_ds(int ind, int npart, PARTICLE* part, double* pot, double alpha) {
+  if (ind < 0 || ind >= npart) {
+    cout << "[shielded_pot_particle_ds] particle index " << ind << " outside [0, " << npart
+         << ")" << endl;
+    return;
+  }
+  auto v = direct_all("shielded_pot_particle_ds", part, npart, kDsShieldedPot, alpha);
+  if (v) *pot = (*v)[ind];
+}
+
+void shielded_freespace_pot_ds(int npart, PARTICLE* part, double alpha) {
+  if (npart < 1) return;
+  auto v = direct_all("shielded_freespace_pot_ds", part, npart, kDsShieldedPot, alpha);
+  if (!v) return;
+  for (int i = 0; i < npart; i++) part[i].xforce = (*v)[i] * charge_constant * part[i].tot_charge;
+}
+
+// ---------------------------------------------------------------------------
+// far-field primitives: one cluster's Taylor sum at one offset
+// ---------------------------------------------------------------------------
+namespace {
+
+// sum_{|k| < nterms} M_k a_k(d) for G(R) = exp(-kappa R)/R, R^2 = dist, with
+// a_k = (-1)^|k| / k! d^k G the Taylor coefficients about the cluster centre
+// (moment[i][j][k] the reference's raw moments).  Radial-derivative form:
+// with g_m = (R^-1 d/dR)^m G,
+//   a_k = (-1)^|k| sum_{2l <= k} d^(k-2l) g_(|k|-|l|) / (l! (k-2l)! 2^|l|)
+// (chain rule for G(sqrt(2S)), S = dist/2, one axis at a time), and
+//   R^2 g_(m+1) = kappa^2 g_(m-1) - (2m+1) g_m,  g_-1 = -exp(-kappa R)/kappa
+// (for kappa = 0 the first term vanishes).
+double cluster_sum(int nterms, double dx, double dy, double dz, double dist, double*** moment,
+                   double kappa) {
+  if (nterms < 1) return 0.0;
+  const int p = nterms - 1;
+  const double R = sqrt(dist), e = exp(-kappa * R);
+  std::vector<double> g(nterms + 1);
+  g[0] = e / R;
+  if (p >= 1) g[1] = -e * (kappa * R + 1.0) / (R * dist);
+  for (int m = 1; m + 1 <= p; m++) g[m + 1] = (kappa * kappa * g[m - 1] - (2 * m + 1) * g[m]) / dist;
+  // powers d^a / a! and 1 / (b! 2^b) per axis
+  std::vector<double> px(nterms), py(nterms), pz(nterms), h(nterms);
+  px[0] = py[0] = pz[0] = h[0] = 1.0;
+  for (int a = 1; a < nterms; a++) {
+    px[a] = px[a - 1] * dx / a;
+    py[a] = py[a - 1] * dy / a;
+    pz[a] = pz[a - 1] * dz / a;
+    h[a] = h[a - 1] / (2.0 * a);
+  }
+  double sum = 0.0;
+  for (int n = 0; n <= p; n++)
+    for (int i = n; i >= 0; i--)
+      for (int j = n - i; j >= 0; j--) {
+        const int k = n - i - j;
+        double a = 0.0;
+        for (int l0 = 0; 2 * l0 <= i; l0++)
+          for (int l1 = 0; 2 * l1 <= j; l1++)
+            for (int l2 = 0; 2 * l2 <= k; l2++)
+              a += px[i - 2 * l0] * py[j - 2 * l1] * pz[k - 2 * l2] * h[l0] * h[l1] * h[l2] *
+                   g[n - l0 - l1 - l2];
+        sum += moment[i][j][k] * ((n & 1) ? -a : a);
+      }
+  return sum;
+}
+
+}  // namespace
+
+void shielded_treetaylorpot(int nterms, double dx, double dy, double dz, double dist,
+                            double* pot, double*** moment, double kappa) {
+  *pot -= cluster_sum(nterms, dx, dy, dz, dist, moment, kappa);
+}
+
+void treetaylorpotential(int nterms, double dx, double dy, double dz, double rs, double* pot,
+                         double*** moment) {
+  *pot += cluster_sum(nterms, dx, dy, dz, rs, moment, 0.0);
 }
 
 // ---------------------------------------------------------------------------

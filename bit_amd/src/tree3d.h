@@ -16,7 +16,13 @@
 //                       calls, from any thread, read that result
 //   freespace_particles_tree, freespace_pot_tree, shielded_freespace_pot_tree
 //                       batched GPU evaluation
-//   shielded_freespace_pot_ds                    GPU FP64 direct sum
+//   direct sums (force_*_ds_*, freespace_particles_ds_*, shielded_pot_particle_ds,
+//   shielded_freespace_pot_ds)
+//                       GPU FP64 direct sum over all particles; per-particle
+//                       calls are served from one evaluation of all particles
+//                       while the array holds the same positions and charges
+//   eval_grad_kernel_*, treetaylorpotential, shielded_treetaylorpot
+//                       one pair / one cluster on the host (primitives)
 #ifndef tree3D_h
 #define tree3D_h 1
 
@@ -77,6 +83,32 @@ void treeinit(TREE* tree1, double x0, double x1, double y0, double y1, double z0
 int treedivide(int dir, TREE* split, TREE* c1, TREE* c2, PARTICLE* part);
 void treeshrink(TREE* tree1, PARTICLE* part);
 void treemoments(int numtree, int nterms, TREE* tree, PARTICLE* part);
+
+// ---- kernel primitives (one pair; the outputs are set) ----
+void eval_grad_kernel_noreg(double dx, double dy, double dz, double rs, double* xfor,
+                            double* yfor, double* zfor);
+void eval_grad_kernel_reg(double dx, double dy, double dz, double rs, double* xfor,
+                          double* yfor, double* zfor);
+
+// ---- direct sums (outputs set, as the reference) ----
+void force_point_ds_noreg(double x, double y, double z, int npart, PARTICLE* part,
+                          double* xfor, double* yfor, double* zfor);
+void force_particle_ds_noreg(int ind, int npart, PARTICLE* part, double* xfor, double* yfor,
+                             double* zfor);
+void force_point_ds_reg(double x, double y, double z, int npart, PARTICLE* part, double* xfor,
+                        double* yfor, double* zfor);
+void force_particle_ds_reg(int ind, int npart, PARTICLE* part, double* xfor, double* yfor,
+                           double* zfor);
+void freespace_particles_ds_noreg(int npart, PARTICLE* part);
+void freespace_particles_ds_reg(int npart, PARTICLE* part);
+void shielded_pot_particle_ds(int ind, int npart, PARTICLE* part, double* pot, double alpha);
+
+// ---- one cluster's Taylor sum at offset (dx,dy,dz) = target - centre ----
+// (degrees below nterms; pot -= for the screened kernel, += for 1/R)
+void shielded_treetaylorpot(int nterms, double dx, double dy, double dz, double rs,
+                            double* pot, double*** moment, double kappa);
+void treetaylorpotential(int nterms, double dx, double dy, double dz, double rs, double* pot,
+                         double*** moment);
 
 // ---- per-target evaluation (accumulate into the outputs) ----
 double treeforce(int ind, int p, int nterms, double x, double y, double z, double acc,

@@ -242,6 +242,23 @@ int bit_gmres(int n, bit_matvec_fn matvec, void *user, const double *b, double *
 int bit_direct_potential(bit_system *sys, double kappa, double delta2, int ntargets,
                          const int *tidx, double *out);
 
+/* FP64 direct-sum field at ntargets particle targets (original indices tidx,
+ * host array), own particle skipped:
+ *     E_t = sum_{j != tidx[t]} q_j (x_t - x_j) G(rs),  rs = |x_t - x_j|^2 + delta2,
+ *     G = rs^-3/2 (+ the regularized family up to kernel_order, as bit_params).
+ * out[3*ntargets] = Ex,Ey,Ez per target (host).  With delta2 = 0 and
+ * kernel_order 0 it replaces force_particle_ds_noreg (tree3d.cpp:182-222),
+ * with delta2 = DEL and kernel_order = KERNEL_ORDER force_particle_ds_reg
+ * (tree3d.cpp:298-330); freespace_particles_ds_noreg/_reg (359-426) are all
+ * particles. */
+int bit_direct_field(bit_system *sys, double delta2, int kernel_order, int ntargets,
+                     const int *tidx, double *out);
+/* The same at m arbitrary points (host arrays), every source included as in
+ * force_point_ds_noreg/_reg (tree3d.cpp:125-158, 241-274): a point on a
+ * source with delta2 = 0 gives the reference's 0/0.  m = 0 returns BIT_OK. */
+int bit_direct_field_points(bit_system *sys, double delta2, int kernel_order, int m,
+                            const double *x, const double *y, const double *z, double *out);
+
 #ifdef __cplusplus
 }
 #endif
