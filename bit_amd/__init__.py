@@ -84,7 +84,7 @@ EXPORTS = [
     "bit_copy_to_host", "bit_system_set_charges", "bit_solve_second_kind", "bit_gmres_dense",
     "bit_system_prepare", "bit_eval_potential_multi", "bit_gmres", "bit_ctx_set_host_threads",
     "bit_copy_on_device", "bit_eval_potential_points", "bit_eval_field_points",
-    "bit_direct_field", "bit_direct_field_points",
+    "bit_direct_field", "bit_direct_field_points", "bit_eval_potential_noalloc",
 ]
 
 
@@ -121,6 +121,7 @@ def _load():
     L.bit_moments_download.argtypes = [vp, vp]
     L.bit_eval_potential.argtypes = [vp, C.POINTER(Params), vp, C.c_int]
     L.bit_eval_field.argtypes = [vp, C.POINTER(Params), vp, C.c_int]
+    L.bit_eval_potential_noalloc.argtypes = [vp, C.POINTER(Params), vp]
     L.bit_system_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bit_direct_potential.argtypes = [vp, C.c_double, C.c_double, C.c_int, vp, vp]
     L.bit_direct_field.argtypes = [vp, C.c_double, C.c_int, C.c_int, vp, vp]
@@ -445,6 +446,19 @@ class System:
         res = np.zeros(self.n)
         _check(lib.bit_eval_potential(self.h, C.byref(prm), _ptr(res), 0))
         return res
+
+    def potential_noalloc(self, out, nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None,
+                          precision="fp64"):
+        """+phi into `out` (a float64 DeviceArray / torch device tensor of n
+        elements) with no device allocation; needs prepare() with the same
+        parameters.  Synchronises the context before returning."""
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+        ptr = _dev_ptr(out)
+        if ptr is None:
+            raise ValueError("potential_noalloc needs a device output array")
+        _check(lib.bit_eval_potential_noalloc(self.h, C.byref(prm), C.c_void_p(ptr)))
+        self.ctx.sync()
+        return out
 
     def field(self, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0):
         """E = sum q (x_i - x_j)/R^3 (regularized Coulomb), [n, 3], original order;

@@ -1147,6 +1147,43 @@ int bit_eval_potential(bit_system* s, const bit_params* p, double* out, int out_
   return BIT_OK;
 }
 
+int bit_eval_potential_noalloc(bit_system* s, const bit_params* p, double* d_out) {
+  BIT_TRY(check_params(s, p));
+  if (!d_out) return fail(BIT_E_INVALID, "null output");
+  const int n = s->tree.n;
+  if (n > bit::dev::kNoAllocMaxN)
+    return fail(BIT_E_INVALID, "the allocation-free evaluation takes at most " +
+                                   std::to_string(bit::dev::kNoAllocMaxN) + " particles (system has " +
+                                   std::to_string(n) + ")");
+  const ListSet& L = s->pl;
+  if (s->mom_nterms != p->nterms || !L.valid || L.acc != p->acc || L.delta2 != p->delta2 ||
+      L.sub != p->mac_sub || (p->precision == BIT_FP64 && s->qc64_nterms != p->nterms))
+    return fail(BIT_E_STATE, "system not prepared for these parameters: call bit_system_prepare");
+  bit_ctx* c = s->ctx;
+  if (!c->scratch) return fail(BIT_E_STATE, "context has no allocation-free scratch");
+  HIP_TRY(hipSetDevice(c->device));
+  char* base = static_cast<char*>(c->scratch);  // this device's copy, never the symbol
+  auto* dd = reinterpret_cast<bit::dev::EvalDesc*>(base);
+  bit::dev::EvalDesc d = s->hdesc;
+  d.tv = s->view();
+  d.kappa = p->kappa;
+  d.delta2 = p->delta2;
+  d.qc32 = s->qc32.as<float>();
+  d.qc64 = s->qc64.as<double>();
+  d.far_off = L.far_off.as<int>();
+  d.far_items = L.far_items.as<bit::dev::Item>();
+  d.near_off = L.near_off.as<int>();
+  d.near_items = L.near_items.as<bit::dev::Item>();
+  d.phi_far = reinterpret_cast<double*>(base + bit::dev::kNoAllocDescBytes);
+  d.out = d_out;
+  const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
+  const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
+  HIP_TRY(bit::dev::put_desc(d, dd, c->stream));
+  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
+  HIP_TRY(bit::dev::near_field(dd, nullptr, nb, fp32, scr, c->stream));
+  return BIT_OK;
+}
+
 int bit_system_prepare(bit_system* s, const bit_params* p) {
   BIT_TRY(prepare_eval(s, p, nullptr));
   HIP_TRY(hipStreamSynchronize(s->ctx->stream));

@@ -161,6 +161,20 @@ int bit_eval_potential(bit_system *sys, const bit_params *params, double *out,
  * evaluation, alone or mixed, only runs the far and near kernels). */
 int bit_system_prepare(bit_system *sys, const bit_params *params);
 
+/* No counterpart in the reference: bit_eval_potential for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It allocates nothing: the far-field
+ * results and the evaluation descriptor go to a static device array of
+ * 32 MiB + 1 KiB per GPU (held until the process ends), so the system may hold
+ * at most 4,194,304 particles (BIT_E_INVALID beyond).  The system must have
+ * been prepared for exactly these params (bit_system_prepare; BIT_E_STATE
+ * otherwise).  d_out is a DEVICE array of n doubles that receives +phi in the
+ * caller's original order, bit-identical to bit_eval_potential's.  Returns once
+ * the work is queued on the context's stream (read d_out after bit_ctx_sync or
+ * later work on that stream).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other contexts) race. */
+int bit_eval_potential_noalloc(bit_system *sys, const bit_params *params, double *d_out);
+
 /* Potentials of `count` independent systems (any sizes, per-system params) in
  * mixed launches on ctx's stream: one far-field and one near-field launch per
  * (nterms, precision, screened) group, each work-group serving one 64-target
