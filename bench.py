@@ -88,10 +88,26 @@ def cpu_baseline(P, nterms, acc, kappa, d2, maxmemb, sample: int, seed: int = 5)
     total = (t1 - t0) + (t2 - t1) + P.n * per_target
     return {
         "value": P.n / total, "unit": "targets/s", "cores": threads, "kind": "port",
+        "cpu_model": cpu_model(), "nproc": os.cpu_count(),
         "sample": f"tree+moments of all {P.n} particles timed in full ({t1 - t0:.2f}s + "
                   f"{t2 - t1:.2f}s); traversal timed on {len(tidx)} seeded targets "
                   f"({t3 - t2:.2f}s on {threads} threads) and extrapolated to N",
+        # oracle/time_vs_ref.py: the port against the compiled reference, one thread,
+        # 1M uniform, kappa 0.5, nterms 9, acc 0.5, MAC shift 0.01
+        "validation": "port 1.109 ms/target vs reference 1.035 ms/target (port/reference "
+                      "time 1.07), potentials equal to 7.3e-15 (build container, Xeon 2.7 GHz)",
     }
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -111,9 +127,14 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=800,
+    ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=400.0)
+    ap.add_argument("--serving-sweep", default="1600,6400",
+                    help="extra arrival rates (req/s) served by the same server, 1000 "
+                         "requests each (empty = none)")
+    ap.add_argument("--no-serving-baseline", action="store_true",
+                    help="skip the one-at-a-time run of the serving stream")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also copy this many MB device-to-device (a plain copy kernel of "
                          "known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -217,19 +238,14 @@ def main():
         log(f"[bench] serving leg: {args.serving_requests} requests at {args.serving_rate}/s")
         for d in dev + [out]:
             d.free()
-        reqs = srv_mod.make_requests(args.serving_requests, args.serving_rate, seed=2024)
-        srv = srv_mod.Server(hbm_budget=16 << 30, builders=8)
-        # untimed warm-up stream (other seed): first use of each kernel
-        # instantiation and of the builder contexts happens here
-        srv.run(srv_mod.make_requests(64, args.serving_rate, seed=2025))
-        serving = srv.run(reqs)
-        srv.close()
-        serving.update(rate_per_s=args.serving_rate, seed=2024, warmup_requests=64,
-                       errors=sum("error" in r.extra for r in reqs),
-                       workload="Poisson arrivals; N log-uniform [1e3,2e5]; uniform/blobs; "
-                                "delta U[0,0.02], kappa U[0,2], p in {4,6,8}; 20% sphere BIE "
-                                "GMRES solves (levels 3-5)")
-        log(f"[bench] serving p50 {serving['p50_ms']:.1f} ms p99 {serving['p99_ms']:.1f} ms")
+        sweep = [float(v) for v in args.serving_sweep.split(",") if v]
+        serving = srv_mod.benchmark(args.serving_requests, args.serving_rate, seed=2024,
+                                    builders=8, hbm_budget=16 << 30, warmup=64,
+                                    baseline=not args.no_serving_baseline, sweep=sweep,
+                                    sweep_requests=1000, log=log)
+        serving["workload"] = ("Poisson arrivals; N log-uniform [1e3,2e5]; uniform/blobs; "
+                               "delta U[0,0.02], kappa U[0,2], p in {4,6,8}; 20% sphere BIE "
+                               "GMRES(50) solves (levels 3-5), Krylov state resident in HBM")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -85,6 +85,8 @@ EXPORTS = [
     "bit_system_prepare", "bit_eval_potential_multi", "bit_gmres", "bit_ctx_set_host_threads",
     "bit_copy_on_device", "bit_eval_potential_points", "bit_eval_field_points",
     "bit_direct_field", "bit_direct_field_points", "bit_eval_potential_noalloc",
+    "bit_krylov_create", "bit_krylov_destroy", "bit_krylov_status", "bit_krylov_solution",
+    "bit_serve_round",
 ]
 
 
@@ -142,6 +144,12 @@ def _load():
     L.bit_gmres.argtypes = [C.c_int, MATVEC_FN, vp, vp, vp, C.c_int, C.c_double, C.c_int, vp,
                             vp]
     L.bit_ctx_set_host_threads.argtypes = [vp, C.c_int]
+    L.bit_krylov_create.argtypes = [vp, C.POINTER(Params), C.c_double, C.c_double, vp, vp, vp,
+                                    C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+    L.bit_krylov_destroy.argtypes = [vp]
+    L.bit_krylov_status.argtypes = [vp, vp, vp, vp]
+    L.bit_krylov_solution.argtypes = [vp, vp]
+    L.bit_serve_round.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp, vp]
     for f in ("bit_eval_potential_points", "bit_eval_field_points"):
         getattr(L, f).argtypes = [vp, C.POINTER(Params), C.c_int, vp, vp, vp, vp]
     L.bit_tree_build.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.POINTER(vp)]
@@ -377,6 +385,66 @@ def eval_multi(ctx: "Context", systems, params, precision_out=None):
     out_arr = (C.c_void_p * k)(*[o.ctypes.data for o in outs])
     ms = C.c_double()
     _check(lib.bit_eval_potential_multi(ctx.h, k, sys_arr, prm_arr, out_arr, 0, C.byref(ms)))
+    return outs, ms.value
+
+
+class Krylov:
+    """A second-kind GMRES solve on a System with its Krylov state resident in
+    HBM (bit_krylov_*), advanced one matrix-vector product per serve_round."""
+
+    def __init__(self, system: "System", w, f, diag, self_coef=0.0, sigma0=None, restart=50,
+                 tol=1e-6, maxit=500, nterms=7, acc=0.6, kappa=0.0, delta2=0.0, mac_sub=None,
+                 precision="fp64"):
+        self.system = system
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+        w, f = _f64(w), _f64(f)
+        s0 = None if sigma0 is None else _f64(sigma0)
+        h = C.c_void_p()
+        _check(lib.bit_krylov_create(system.h, C.byref(prm), float(diag), float(self_coef),
+                                     _ptr(w), _ptr(f), None if s0 is None else _ptr(s0),
+                                     int(restart), float(tol), int(maxit), C.byref(h)))
+        self.h = h
+
+    def status(self):
+        """(done, iterations, relative residual); done 0 running, 1 converged,
+        2 stopped at maxit."""
+        d, it, rr = C.c_int(), C.c_int(), C.c_double()
+        _check(lib.bit_krylov_status(self.h, C.byref(d), C.byref(it), C.byref(rr)))
+        return d.value, it.value, rr.value
+
+    @property
+    def done(self) -> bool:
+        return self.status()[0] != 0
+
+    def solution(self) -> np.ndarray:
+        out = np.zeros(self.system.n)
+        _check(lib.bit_krylov_solution(self.h, _ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self.system.ctx.h and self.system.h:
+                lib.bit_krylov_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_round(ctx: "Context", systems=(), params=(), solves=()):
+    """One serving round: potentials of `systems` (host outputs) and one step of
+    every running solve in `solves`; returns (outputs, GPU ms of the round)."""
+    k, m = len(systems), len(solves)
+    outs = [np.zeros(s.n) for s in systems]
+    sys_arr = (C.c_void_p * max(k, 1))(*[s.h.value for s in systems])
+    prm_arr = (Params * max(k, 1))(*params)
+    out_arr = (C.c_void_p * max(k, 1))(*[o.ctypes.data for o in outs])
+    kry_arr = (C.c_void_p * max(m, 1))(*[q.h.value for q in solves])
+    ms = C.c_double()
+    _check(lib.bit_serve_round(ctx.h, k, sys_arr, prm_arr, out_arr, m, kry_arr, C.byref(ms)))
     return outs, ms.value
 
 

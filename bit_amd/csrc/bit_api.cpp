@@ -969,7 +969,11 @@ int bit_system_export_tree(const bit_system* s, double* boxes, int* children, in
   return BIT_OK;
 }
 
-int bit_moments(bit_system* s, int nterms) {
+}  // extern "C"
+
+// Moments of the system's current charges, launched on `st` (the chunk tables
+// are made on the first call, on the context's stream).
+static int moments_on(bit_system* s, int nterms, hipStream_t st) {
   if (!s) return fail(BIT_E_INVALID, "null system");
   if (nterms < 1 || nterms > BIT_MAX_NTERMS)
     return fail(BIT_E_INVALID, "nterms must be in [1, " + std::to_string(BIT_MAX_NTERMS) + "]");
@@ -1026,37 +1030,40 @@ int bit_moments(bit_system* s, int nterms) {
   BIT_TRY(s->mom.ensure(c, (size_t)nt * nc * sizeof(double)));
   BIT_TRY(s->mom_hat64.ensure(c, (size_t)nt * nc * sizeof(double)));
   BIT_TRY(s->qc32.ensure(c, (size_t)nt * bit::dev::nqpad(nterms) * sizeof(float)));
-  s->t_mom.start(c->stream);
+  s->t_mom.start(st);
   HIP_TRY(bit::dev::moments_partial(s->view(), nterms, s->nchunks, s->chunk_node.as<int>(),
-                                    s->chunk_rng.as<int2>(), s->partial.as<double>(),
-                                    c->stream));
+                                    s->chunk_rng.as<int2>(), s->partial.as<double>(), st));
   HIP_TRY(bit::dev::moments_reduce(s->view(), nterms, s->node_chunk0.as<int>(),
                                    s->node_nchunk.as<int>(), s->partial.as<double>(),
-                                   s->mom.as<double>(), s->mom_hat64.as<double>(),
-                                   c->stream));
+                                   s->mom.as<double>(), s->mom_hat64.as<double>(), st));
   for (size_t h = 1; h < s->up_off.size(); h++)
     HIP_TRY(bit::dev::moments_upward(s->view(), nterms, s->up_off[h] - s->up_off[h - 1],
                                      s->up_nodes.as<int>() + s->up_off[h - 1],
-                                     s->mom.as<double>(), s->mom_hat64.as<double>(),
-                                     c->stream));
-  HIP_TRY(bit::dev::far_coeffs32(nt, nterms, s->mom_hat64.as<double>(), s->qc32.as<float>(),
-                                 c->stream));
-  s->t_mom.stop(c->stream);
+                                     s->mom.as<double>(), s->mom_hat64.as<double>(), st));
+  HIP_TRY(bit::dev::far_coeffs32(nt, nterms, s->mom_hat64.as<double>(), s->qc32.as<float>(), st));
+  s->t_mom.stop(st);
   s->mom_nterms = nterms;
   s->stats.ms_moments = -1.0;
   return BIT_OK;
 }
 
+extern "C" {
+
+int bit_moments(bit_system* s, int nterms) {
+  if (!s) return fail(BIT_E_INVALID, "null system");
+  return moments_on(s, nterms, s->ctx->stream);
+}
+
 }  // extern "C"
 
 // FP64 cluster coefficients for the current moments, computed on first use.
-static int ensure_qc64(bit_system* s, int nterms) {
+static int ensure_qc64(bit_system* s, int nterms, hipStream_t st = nullptr) {
   if (s->qc64_nterms == nterms) return BIT_OK;
   bit_ctx* c = s->ctx;
   const int nt = s->tree.numtree;
   BIT_TRY(s->qc64.ensure(c, (size_t)nt * bit::dev::nqpad(nterms) * sizeof(double)));
   HIP_TRY(bit::dev::far_coeffs64(nt, nterms, s->mom_hat64.as<double>(), s->qc64.as<double>(),
-                                 c->stream));
+                                 st ? st : c->stream));
   s->qc64_nterms = nterms;
   return BIT_OK;
 }
@@ -1187,6 +1194,7 @@ int bit_eval_potential_noalloc(bit_system* s, const bit_params* p, double* d_out
 int bit_system_prepare(bit_system* s, const bit_params* p) {
   BIT_TRY(prepare_eval(s, p, nullptr));
   HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  collect_times(s);
   return BIT_OK;
 }
 
@@ -1465,123 +1473,418 @@ int bit_system_set_charges(bit_system* s, const double* q, int on_device) {
   return BIT_OK;
 }
 
-// GMRES(m) with every n-vector resident in HBM: the Krylov basis, the
-// residual and the iterate.  Arnoldi orthogonalises by classical Gram-Schmidt
-// applied twice (all projections of a pass in one launch, one scalar
-// read-back per pass); the Hessenberg columns, the Givens rotations and the
-// small least-squares solve stay on the host, all in FP64.  The stopping rule
-// is restarted_gmres's: ||b - A x|| <= tol ||b||, checked on the Givens
-// estimate inside a cycle and on the true residual at each restart.
-int bit_solve_second_kind(bit_system* s, const bit_params* p, double diag, double self,
-                          const double* w, const double* f, double* sigma, int restart,
-                          double tol, int maxit, int* iters, double* relres) {
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Second-kind solves with their Krylov state resident in HBM, stepped in
+// serving rounds.
+//
+// GMRES(m) with every n-vector in HBM: the basis V, the residual/work vector,
+// the iterate, the weights and the right-hand side.  Arnoldi orthogonalises by
+// classical Gram-Schmidt applied twice; the Hessenberg columns, the Givens
+// rotations and the small least-squares solve stay on the host, all in FP64.
+// The stopping rule is restarted_gmres's: ||b - A x|| <= tol ||b||, checked on
+// the Givens estimate inside a cycle and on the true residual at each restart.
+//
+// A round advances every active solve by one matrix-vector product (a treecode
+// evaluation of w * u, u = x or V_j) and the Arnoldi step that follows it.  The
+// products of all solves go into the round's mixed far/near launches together
+// with the round's plain evaluations; each Gram-Schmidt pass, the norm and the
+// vector updates are one batched launch for all solves, with one read-back per
+// pass.  Every solve's arithmetic touches only its own vectors, with the same
+// partition of its n, so a solve gives the same iterates alone or in any mix.
+// ---------------------------------------------------------------------------
+struct bit_krylov {
+  bit_system* sys = nullptr;
+  bit_params p{};
+  double diag = 0, self = 0, tol = 0, bnorm = 0, relres = 0;
+  int n = 0, restart = 0, maxit = 0;
+  int phase = 0;  // 0: the next product is A x (true residual); 1: A V_j (Arnoldi)
+  int j = 0, iters = 0, restarts = 0;
+  bool done = false, converged = false, breakdown = false;
+  DevBuf w, b, x, V, r, q, phi, partial;
+  std::vector<double> H, cs, sn, g, y, h1, h2;
+};
+
+namespace {
+
+template <typename T>
+int upload_vec(bit_ctx* c, DevBuf& d, const std::vector<T>& v, hipStream_t st) {
+  BIT_TRY(d.ensure(c, std::max<size_t>(1, v.size()) * sizeof(T)));
+  if (!v.empty()) HIP_TRY(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return BIT_OK;
+}
+
+// Mixed far/near launches on `st` over systems whose descriptors (hdesc) were
+// set by prepare_eval: one launch per (nterms, precision, screened) group.
+// The host tables stay alive in `descs`/`map` until the caller synchronises.
+int mixed_launch(bit_ctx* ctx, const std::vector<bit_system*>& sys,
+                 const std::vector<bit_params>& prm, std::vector<bit::dev::EvalDesc>& descs,
+                 std::vector<int2>& map, DevBuf& d_desc, DevBuf& d_map, hipStream_t st) {
+  const int count = (int)sys.size();
+  if (count == 0) return BIT_OK;
+  std::vector<int> order(count);
+  for (int i = 0; i < count; i++) order[i] = i;
+  auto key = [&](int i) {
+    return std::make_tuple(prm[i].nterms, prm[i].precision, prm[i].kappa != 0.0);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  descs.assign(count, bit::dev::EvalDesc{});
+  map.clear();
+  std::vector<std::tuple<int, int, int>> groups;  // (first map entry, blocks, params index)
+  for (int k = 0; k < count;) {
+    int e = k;
+    while (e < count && key(order[e]) == key(order[k])) e++;
+    const int first = (int)map.size();
+    for (int j = k; j < e; j++) {
+      bit_system* s = sys[order[j]];
+      descs[j] = s->hdesc;
+      const int nb = (s->tree.n + bit::dev::kBatch - 1) / bit::dev::kBatch;
+      for (int b = 0; b < nb; b++) map.push_back(make_int2(j, b));
+    }
+    groups.emplace_back(first, (int)map.size() - first, order[k]);
+    k = e;
+  }
+  BIT_TRY(upload_vec(ctx, d_desc, descs, st));
+  BIT_TRY(upload_vec(ctx, d_map, map, st));
+  for (auto& gr : groups) {
+    const bit_params& p = prm[std::get<2>(gr)];
+    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
+                                std::get<1>(gr), p.nterms, p.precision == BIT_FP32,
+                                p.kappa != 0.0, st));
+  }
+  for (auto& gr : groups) {
+    const bit_params& p = prm[std::get<2>(gr)];
+    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
+                                 std::get<1>(gr), p.precision == BIT_FP32, p.kappa != 0.0, st));
+  }
+  return BIT_OK;
+}
+
+// One dot pass over the solves in `ks`: out[i][0..ncols) = V_j . r (basis
+// columns, ncols = j + 1) or r . r (norm), read back to the host.
+int dot_pass(bit_ctx* ctx, const std::vector<bit_krylov*>& ks, bool norm, DevBuf& d_jobs,
+             DevBuf& d_out, std::vector<double>& h_out, int ld, hipStream_t st) {
+  const int cnt = (int)ks.size();
+  if (cnt == 0) return BIT_OK;
+  std::vector<bit::dev::DotJob> jobs(cnt);
+  BIT_TRY(d_out.ensure(ctx, (size_t)cnt * ld * sizeof(double)));
+  int maxcols = 1;
+  for (int i = 0; i < cnt; i++) {
+    bit_krylov* k = ks[i];
+    const int nc = norm ? 1 : k->j + 1;
+    maxcols = std::max(maxcols, nc);
+    jobs[i] = bit::dev::DotJob{k->n, nc, norm ? k->r.as<double>() : k->V.as<double>(),
+                               k->r.as<double>(), k->partial.as<double>(),
+                               d_out.as<double>() + (size_t)i * ld};
+  }
+  BIT_TRY(upload_vec(ctx, d_jobs, jobs, st));
+  HIP_TRY(bit::dev::dots_multi(cnt, maxcols, d_jobs.as<bit::dev::DotJob>(), st));
+  h_out.resize((size_t)cnt * ld);
+  HIP_TRY(hipMemcpyAsync(h_out.data(), d_out.p, h_out.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));  // also ends the job upload's use of `jobs`
+  return BIT_OK;
+}
+
+// r -= V h (h per solve in coef[i*ld ..]) for the solves in `ks`.
+int project_pass(bit_ctx* ctx, const std::vector<bit_krylov*>& ks, const std::vector<double>& coef,
+                 int ld, DevBuf& d_jobs, DevBuf& d_coef, std::vector<bit::dev::CombJob>& jobs,
+                 hipStream_t st) {
+  const int cnt = (int)ks.size();
+  if (cnt == 0) return BIT_OK;
+  BIT_TRY(upload_vec(ctx, d_coef, coef, st));
+  jobs.resize(cnt);
+  int maxn = 1;
+  for (int i = 0; i < cnt; i++) {
+    bit_krylov* k = ks[i];
+    maxn = std::max(maxn, k->n);
+    jobs[i] = bit::dev::CombJob{k->n, k->j + 1, k->V.as<double>(),
+                                d_coef.as<double>() + (size_t)i * ld, -1.0, k->r.as<double>()};
+  }
+  BIT_TRY(upload_vec(ctx, d_jobs, jobs, st));
+  HIP_TRY(bit::dev::combine_multi(cnt, maxn, d_jobs.as<bit::dev::CombJob>(), st));
+  return BIT_OK;
+}
+
+int serve_round(bit_ctx* ctx, int ne, bit_system* const* es, const bit_params* ep,
+                double* const* eouts, int nk, bit_krylov* const* ks, double* kernel_ms) {
+  if (!ctx || ne < 0 || nk < 0 || (ne > 0 && (!es || !ep || !eouts)) || (nk > 0 && !ks))
+    return fail(BIT_E_INVALID, "bad serving-round arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  std::vector<bit_krylov*> act;
+  std::vector<const bit_system*> seen;
+  for (int i = 0; i < ne; i++) {
+    if (!es[i] || !eouts[i]) return fail(BIT_E_INVALID, "null system or output");
+    if (es[i]->ctx->device != ctx->device)
+      return fail(BIT_E_INVALID, "systems of a round must share the device");
+    seen.push_back(es[i]);
+  }
+  for (int i = 0; i < nk; i++) {
+    if (!ks[i]) return fail(BIT_E_INVALID, "null solve");
+    if (ks[i]->sys->ctx->device != ctx->device)
+      return fail(BIT_E_INVALID, "solves of a round must share the device");
+    seen.push_back(ks[i]->sys);
+    if (!ks[i]->done) act.push_back(ks[i]);
+  }
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+    return fail(BIT_E_INVALID, "a system appears more than once in a round");
+  const int na = (int)act.size();
+  std::vector<bit_system*> msys;
+  std::vector<bit_params> mprm;
+  for (int i = 0; i < ne; i++) {
+    BIT_TRY(prepare_eval(es[i], &ep[i], nullptr));
+    HIP_TRY(hipStreamSynchronize(es[i]->ctx->stream));  // moments/lists ready
+    msys.push_back(es[i]);
+    mprm.push_back(ep[i]);
+  }
+  Timer tm;
+  tm.start(st);
+  DevBuf d_vjobs, d_desc, d_map, d_djobs, d_dout, d_cjobs, d_coef, d_sjobs;
+  std::vector<bit::dev::VecJob> vjobs;
+  int maxn = 1;
+  // charges w * u of each active solve's product, u = x (phase 0) or V_j
+  for (bit_krylov* k : act) {
+    const double* u = k->phase == 0 ? k->x.as<double>() : k->V.as<double>() + (size_t)k->j * k->n;
+    vjobs.push_back(bit::dev::VecJob{k->n, 0, 0.0, 0.0, k->w.as<double>(), u, nullptr, nullptr,
+                                     k->q.as<double>()});
+    maxn = std::max(maxn, k->n);
+  }
+  BIT_TRY(upload_vec(ctx, d_vjobs, vjobs, st));
+  HIP_TRY(bit::dev::vec_multi(na, maxn, d_vjobs.as<bit::dev::VecJob>(), st));
+  for (bit_krylov* k : act) {
+    bit_system* s = k->sys;
+    HIP_TRY(bit::dev::set_charges(s->view(), k->q.as<double>(), s->sq.as<double>(),
+                                  s->sloc.as<float4>(), st));
+    s->mom_nterms = 0;
+    BIT_TRY(moments_on(s, k->p.nterms, st));
+    if (k->p.precision == BIT_FP64) BIT_TRY(ensure_qc64(s, k->p.nterms, st));
+    BIT_TRY(prepare_eval(s, &k->p, k->phi.as<double>()));  // cached lists; sets hdesc
+    msys.push_back(s);
+    mprm.push_back(k->p);
+  }
+  std::vector<bit::dev::EvalDesc> descs;
+  std::vector<int2> map;
+  BIT_TRY(mixed_launch(ctx, msys, mprm, descs, map, d_desc, d_map, st));
+  // y = (diag + self w) u + phi: into r (Arnoldi), or r = b - y (residual)
+  std::vector<bit::dev::VecJob> pjobs;
+  std::vector<bit_krylov*> arn;
+  for (bit_krylov* k : act) {
+    const double* u = k->phase == 0 ? k->x.as<double>() : k->V.as<double>() + (size_t)k->j * k->n;
+    pjobs.push_back(bit::dev::VecJob{k->n, k->phase == 0 ? 4 : 1, k->diag, k->self,
+                                     k->w.as<double>(), u, k->phi.as<double>(),
+                                     k->b.as<double>(), k->r.as<double>()});
+    if (k->phase == 1) arn.push_back(k);
+  }
+  DevBuf d_pjobs;
+  BIT_TRY(upload_vec(ctx, d_pjobs, pjobs, st));
+  HIP_TRY(bit::dev::vec_multi(na, maxn, d_pjobs.as<bit::dev::VecJob>(), st));
+  // classical Gram-Schmidt, twice
+  int ld = 1;
+  for (bit_krylov* k : arn) ld = std::max(ld, k->restart + 1);
+  std::vector<double> hd, coef;
+  std::vector<bit::dev::CombJob> cjobs;
+  for (int pass = 0; pass < 2 && !arn.empty(); pass++) {
+    BIT_TRY(dot_pass(ctx, arn, false, d_djobs, d_dout, hd, ld, st));
+    for (size_t i = 0; i < arn.size(); i++) {
+      std::vector<double>& h = pass == 0 ? arn[i]->h1 : arn[i]->h2;
+      h.assign(hd.begin() + i * ld, hd.begin() + i * ld + arn[i]->j + 1);
+    }
+    BIT_TRY(project_pass(ctx, arn, hd, ld, d_cjobs, d_coef, cjobs, st));
+  }
+  // norms of every active solve's r
+  std::vector<double> nrm;
+  BIT_TRY(dot_pass(ctx, act, true, d_djobs, d_dout, nrm, 1, st));
+  std::vector<bit::dev::VecJob> sjobs;   // V_0 = r / beta, V_{j+1} = r / hn
+  std::vector<bit::dev::CombJob> xjobs;  // x += V y at the end of a cycle
+  std::vector<std::vector<double>> ys;
+  int maxx = 1;
+  for (int i = 0; i < na; i++) {
+    bit_krylov* k = act[i];
+    const double rr = nrm[i];
+    if (k->phase == 0) {
+      const double beta = std::sqrt(rr);
+      k->relres = beta / k->bnorm;
+      if (k->relres <= k->tol) {
+        k->converged = true;
+        k->done = true;
+        continue;
+      }
+      if (k->iters >= k->maxit) {
+        k->done = true;
+        continue;
+      }
+      sjobs.push_back(bit::dev::VecJob{k->n, 3, 1.0 / beta, 0.0, nullptr, k->r.as<double>(),
+                                       nullptr, nullptr, k->V.as<double>()});
+      std::fill(k->H.begin(), k->H.end(), 0.0);
+      std::fill(k->g.begin(), k->g.end(), 0.0);
+      k->g[0] = beta;
+      k->j = 0;
+      k->phase = 1;
+      continue;
+    }
+    const int j = k->j, ldh = k->restart;
+    for (int t = 0; t <= j; t++) k->H[(size_t)t * ldh + j] = k->h1[t] + k->h2[t];
+    const double hn = std::sqrt(rr);
+    k->H[(size_t)(j + 1) * ldh + j] = hn;
+    if (hn > 0)
+      sjobs.push_back(bit::dev::VecJob{k->n, 3, 1.0 / hn, 0.0, nullptr, k->r.as<double>(),
+                                       nullptr, nullptr, k->V.as<double>() + (size_t)(j + 1) * k->n});
+    bit::detail::givens_column(k->H, ldh, j, k->cs, k->sn, k->g);
+    k->iters++;
+    k->relres = std::fabs(k->g[j + 1]) / k->bnorm;
+    if (hn == 0.0) k->breakdown = true;  // the Krylov space is invariant: exact
+    k->j = j + 1;
+    if (k->relres <= k->tol || hn == 0.0 || k->j == k->restart || k->iters >= k->maxit) {
+      bit::detail::back_solve(k->H, ldh, k->g, k->j, k->y);
+      ys.push_back(k->y);
+      xjobs.push_back(bit::dev::CombJob{k->n, k->j, k->V.as<double>(), nullptr, 1.0,
+                                        k->x.as<double>()});
+      maxx = std::max(maxx, k->n);
+      k->restarts++;
+      k->phase = 0;
+    }
+  }
+  DevBuf d_y;
+  std::vector<double> yflat;
+  for (size_t i = 0; i < xjobs.size(); i++) yflat.insert(yflat.end(), ys[i].begin(), ys[i].end());
+  BIT_TRY(upload_vec(ctx, d_y, yflat, st));
+  for (size_t i = 0, off = 0; i < xjobs.size(); off += ys[i].size(), i++)
+    xjobs[i].c = d_y.as<double>() + off;
+  BIT_TRY(upload_vec(ctx, d_sjobs, sjobs, st));
+  HIP_TRY(bit::dev::vec_multi((int)sjobs.size(), maxn, d_sjobs.as<bit::dev::VecJob>(), st));
+  BIT_TRY(upload_vec(ctx, d_cjobs, xjobs, st));
+  HIP_TRY(bit::dev::combine_multi((int)xjobs.size(), maxx, d_cjobs.as<bit::dev::CombJob>(), st));
+  tm.stop(st);
+  for (int i = 0; i < ne; i++)
+    HIP_TRY(hipMemcpyAsync(eouts[i], es[i]->hdesc.out, (size_t)es[i]->tree.n * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (kernel_ms) *kernel_ms = tm.ms();
+  return BIT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bit_krylov_create(bit_system* s, const bit_params* p, double diag, double self,
+                      const double* w, const double* f, const double* sigma0, int restart,
+                      double tol, int maxit, bit_krylov** out) {
+  if (!out) return fail(BIT_E_INVALID, "null out");
+  *out = nullptr;
   BIT_TRY(check_params(s, p));
-  if (!w || !f || !sigma) return fail(BIT_E_INVALID, "null w, f or sigma");
+  if (!w || !f) return fail(BIT_E_INVALID, "null w or f");
   if (restart < 1 || maxit < 1 || !(tol > 0)) return fail(BIT_E_INVALID, "bad GMRES settings");
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   hipStream_t st = c->stream;
   const int n = s->tree.n;
-  if (!finite_all(n, w) || !finite_all(n, f) || !finite_all(n, sigma) || !std::isfinite(diag) ||
-      !std::isfinite(self))
+  if (!finite_all(n, w) || !finite_all(n, f) || (sigma0 && !finite_all(n, sigma0)) ||
+      !std::isfinite(diag) || !std::isfinite(self))
     return fail(BIT_E_INVALID, "non-finite weights, right-hand side, initial guess or diagonal");
+  BIT_TRY(ensure_lists(s, p));  // the MAC's lists once; every product reuses them
+  auto* k = new bit_krylov();
+  auto cleanup = [&](int rc) {
+    (void)hipStreamSynchronize(st);
+    delete k;
+    return rc;
+  };
+  k->sys = s;
+  k->p = *p;
+  k->diag = diag;
+  k->self = self;
+  k->tol = tol;
+  k->n = n;
+  k->restart = restart;
+  k->maxit = maxit;
   const size_t nb = (size_t)n * sizeof(double);
-  DevBuf dw, db, dx, dv, dr, dq, dphi, dpart, ddots, dc;
-  BIT_TRY(upload(c, dw, w, nb));
-  BIT_TRY(upload(c, db, f, nb));
-  BIT_TRY(upload(c, dx, sigma, nb));
-  BIT_TRY(dv.ensure(c, (size_t)(restart + 1) * nb));
-  BIT_TRY(dr.ensure(c, nb));
-  BIT_TRY(dq.ensure(c, nb));
-  BIT_TRY(dphi.ensure(c, nb));
-  BIT_TRY(dpart.ensure(c, (size_t)bit::dev::kDotBlocks * (restart + 1) * sizeof(double)));
-  BIT_TRY(ddots.ensure(c, (size_t)(restart + 1) * sizeof(double)));
-  BIT_TRY(dc.ensure(c, (size_t)(restart + 1) * sizeof(double)));
-  double* V = dv.as<double>();
-  double* r = dr.as<double>();
-  // y = (diag + self w) v + phi(w v)
-  auto matvec = [&](const double* v, double* y) -> int {
-    HIP_TRY(bit::dev::vec_op(n, 0, 0.0, 0.0, dw.as<double>(), v, nullptr, dq.as<double>(), st));
-    BIT_TRY(bit_system_set_charges(s, dq.as<double>(), 1));
-    BIT_TRY(bit_eval_potential(s, p, dphi.as<double>(), 1));
-    HIP_TRY(bit::dev::vec_op(n, 1, diag, self, dw.as<double>(), v, dphi.as<double>(), y, st));
-    return BIT_OK;
-  };
-  auto dotv = [&](const double* Vc, int ncols, const double* u, double* host) -> int {
-    HIP_TRY(bit::dev::dots(n, Vc, (size_t)n, ncols, u, dpart.as<double>(), ddots.as<double>(),
-                           st));
-    HIP_TRY(hipMemcpyAsync(host, ddots.p, (size_t)ncols * sizeof(double), hipMemcpyDeviceToHost,
-                           st));
-    HIP_TRY(hipStreamSynchronize(st));
-    return BIT_OK;
-  };
-  // u += sign * sum_j coef[j] V_j
-  auto combine = [&](int ncols, const double* coef, double sign, double* u) -> int {
-    HIP_TRY(hipMemcpyAsync(dc.p, coef, (size_t)ncols * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_TRY(bit::dev::combine_cols(n, V, (size_t)n, ncols, dc.as<double>(), sign, u, st));
-    HIP_TRY(hipStreamSynchronize(st));  // coef is host memory the caller may reuse
-    return BIT_OK;
-  };
-  bit::GmresResult res;
+  int rc;
+  if ((rc = upload(c, k->w, w, nb)) != BIT_OK || (rc = upload(c, k->b, f, nb)) != BIT_OK ||
+      (rc = k->x.ensure(c, nb)) != BIT_OK || (rc = k->V.ensure(c, (size_t)(restart + 1) * nb)) != BIT_OK ||
+      (rc = k->r.ensure(c, nb)) != BIT_OK || (rc = k->q.ensure(c, nb)) != BIT_OK ||
+      (rc = k->phi.ensure(c, nb)) != BIT_OK ||
+      (rc = k->partial.ensure(c, (size_t)bit::dev::kDotBlocks * (restart + 1) * sizeof(double))) != BIT_OK)
+    return cleanup(rc);
+  hipError_t e = sigma0 ? hipMemcpyAsync(k->x.p, sigma0, nb, hipMemcpyHostToDevice, st)
+                        : hipMemsetAsync(k->x.p, 0, nb, st);
+  if (e != hipSuccess) return cleanup(fail(BIT_E_HIP, hipGetErrorString(e)));
+  // ||f||
+  DevBuf d_job, d_out;
+  bit::dev::DotJob job{n, 1, k->b.as<double>(), k->b.as<double>(), k->partial.as<double>(), nullptr};
+  if ((rc = d_out.ensure(c, sizeof(double))) != BIT_OK) return cleanup(rc);
+  job.out = d_out.as<double>();
   double bb = 0.0;
-  BIT_TRY(dotv(db.as<double>(), 1, db.as<double>(), &bb));
-  const double bnorm = std::sqrt(bb);
-  if (bnorm == 0.0) {
-    for (int i = 0; i < n; i++) sigma[i] = 0.0;
-    if (iters) *iters = 0;
-    if (relres) *relres = 0.0;
-    return BIT_OK;
+  if ((rc = upload(c, d_job, &job, sizeof(job))) != BIT_OK) return cleanup(rc);
+  e = bit::dev::dots_multi(1, 1, d_job.as<bit::dev::DotJob>(), st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&bb, d_out.p, sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return cleanup(fail(BIT_E_HIP, hipGetErrorString(e)));
+  collect_times(s);  // the list build's time
+  k->bnorm = std::sqrt(bb);
+  k->H.assign((size_t)(restart + 1) * restart, 0.0);
+  k->cs.assign(restart, 0.0);
+  k->sn.assign(restart, 0.0);
+  k->g.assign(restart + 1, 0.0);
+  if (k->bnorm == 0.0) {  // x = 0 solves it exactly
+    e = hipMemsetAsync(k->x.p, 0, nb, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return cleanup(fail(BIT_E_HIP, hipGetErrorString(e)));
+    k->done = k->converged = true;
+    k->relres = 0.0;
   }
-  const int ldh = restart;
-  std::vector<double> H((size_t)(restart + 1) * ldh), cs(restart), sn(restart), g(restart + 1),
-      h1(restart + 1), h2(restart + 1), y;
-  while (true) {
-    BIT_TRY(matvec(dx.as<double>(), r));
-    HIP_TRY(bit::dev::vec_op(n, 2, 0.0, 0.0, db.as<double>(), r, nullptr, r, st));
-    double rr = 0.0;
-    BIT_TRY(dotv(r, 1, r, &rr));
-    const double beta = std::sqrt(rr);
-    res.relres = beta / bnorm;
-    if (res.relres <= tol) {
-      res.converged = true;
-      break;
-    }
-    if (res.iters >= maxit) break;
-    HIP_TRY(bit::dev::vec_op(n, 3, 1.0 / beta, 0.0, nullptr, r, nullptr, V, st));
-    std::fill(H.begin(), H.end(), 0.0);
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
-    int k = 0;
-    for (int j = 0; j < restart && res.iters < maxit; j++) {
-      BIT_TRY(matvec(V + (size_t)j * n, r));
-      res.iters++;
-      BIT_TRY(dotv(V, j + 1, r, h1.data()));
-      BIT_TRY(combine(j + 1, h1.data(), -1.0, r));
-      BIT_TRY(dotv(V, j + 1, r, h2.data()));
-      BIT_TRY(combine(j + 1, h2.data(), -1.0, r));
-      for (int i = 0; i <= j; i++) H[(size_t)i * ldh + j] = h1[i] + h2[i];
-      double hh = 0.0;
-      BIT_TRY(dotv(r, 1, r, &hh));
-      const double hn = std::sqrt(hh);
-      H[(size_t)(j + 1) * ldh + j] = hn;
-      if (hn > 0)
-        HIP_TRY(bit::dev::vec_op(n, 3, 1.0 / hn, 0.0, nullptr, r, nullptr,
-                                 V + (size_t)(j + 1) * n, st));
-      bit::detail::givens_column(H, ldh, j, cs, sn, g);
-      k = j + 1;
-      res.relres = std::fabs(g[j + 1]) / bnorm;
-      if (hn == 0.0) res.breakdown = true;  // the Krylov space is invariant: exact
-      if (res.relres <= tol || hn == 0.0) break;
-    }
-    bit::detail::back_solve(H, ldh, g, k, y);
-    BIT_TRY(combine(k, y.data(), 1.0, dx.as<double>()));
-    res.restarts++;
-  }
-  HIP_TRY(hipMemcpyAsync(sigma, dx.p, nb, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (iters) *iters = res.iters;
-  if (relres) *relres = res.relres;
+  *out = k;
   return BIT_OK;
+}
+
+int bit_krylov_destroy(bit_krylov* k) {
+  if (!k) return BIT_OK;
+  (void)hipStreamSynchronize(k->sys->ctx->stream);
+  delete k;
+  return BIT_OK;
+}
+
+int bit_krylov_status(const bit_krylov* k, int* done, int* iters, double* relres) {
+  if (!k) return fail(BIT_E_INVALID, "null solve");
+  if (done) *done = k->done ? (k->converged ? 1 : 2) : 0;
+  if (iters) *iters = k->iters;
+  if (relres) *relres = k->relres;
+  return BIT_OK;
+}
+
+int bit_krylov_solution(const bit_krylov* k, double* sigma) {
+  if (!k || !sigma) return fail(BIT_E_INVALID, "null solve or output");
+  bit_ctx* c = k->sys->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(sigma, k->x.p, (size_t)k->n * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return BIT_OK;
+}
+
+int bit_serve_round(bit_ctx* ctx, int n_eval, bit_system* const* systems,
+                    const bit_params* params, double* const* outs, int n_solve,
+                    bit_krylov* const* solves, double* kernel_ms) {
+  return serve_round(ctx, n_eval, systems, params, outs, n_solve, solves, kernel_ms);
+}
+
+int bit_solve_second_kind(bit_system* s, const bit_params* p, double diag, double self,
+                          const double* w, const double* f, double* sigma, int restart,
+                          double tol, int maxit, int* iters, double* relres) {
+  if (!sigma) return fail(BIT_E_INVALID, "null sigma");
+  bit_krylov* k = nullptr;
+  BIT_TRY(bit_krylov_create(s, p, diag, self, w, f, sigma, restart, tol, maxit, &k));
+  int rc = BIT_OK;
+  while (rc == BIT_OK && !k->done) rc = serve_round(s->ctx, 0, nullptr, nullptr, nullptr, 1, &k, nullptr);
+  if (rc == BIT_OK) rc = bit_krylov_solution(k, sigma);
+  if (rc == BIT_OK) {
+    if (iters) *iters = k->iters;
+    if (relres) *relres = k->relres;
+  }
+  bit_krylov_destroy(k);
+  return rc;
 }
 
 int bit_gmres_dense(int n, const double* A, const double* b, double* x, int restart, double tol,

@@ -196,6 +196,32 @@ hipError_t dots(int n, const double* V, size_t ld, int ncols, const double* w, d
                 double* out, hipStream_t s);
 hipError_t combine_cols(int n, const double* V, size_t ld, int ncols, const double* c,
                         double sign, double* y, hipStream_t s);
+// Batched Krylov steps of many solves in one launch each: job i of a launch
+// works on solve i's own vectors only, with the same per-element arithmetic
+// (and the same kDotBlocks partition of its n) as the single-solve kernels.
+struct DotJob {      // out[j] = V_j . w, j < ncols (partial: kDotBlocks * ncols)
+  int n, ncols;
+  const double* V;   // ld = n
+  const double* w;
+  double* partial;
+  double* out;
+};
+struct CombJob {     // y += sign * sum_{j < ncols} c[j] V_j
+  int n, ncols;
+  const double* V;   // ld = n
+  const double* c;
+  double sign;
+  double* y;
+};
+struct VecJob {      // vec_op's elementwise steps; op 4: y = c - ((a + b u) v + z)
+  int n, op;
+  double a, b;
+  const double *u, *v, *z, *c;
+  double* y;
+};
+hipError_t dots_multi(int count, int maxcols, const DotJob* jobs, hipStream_t s);
+hipError_t combine_multi(int count, int maxn, const CombJob* jobs, hipStream_t s);
+hipError_t vec_multi(int count, int maxn, const VecJob* jobs, hipStream_t s);
 hipError_t vec_op(int n, int op, double a, double b, const double* u, const double* v,
                   const double* z, double* y, hipStream_t s);
 

@@ -2296,6 +2296,82 @@ hipError_t vec_op(int n, int op, double a, double b, const double* u, const doub
   return hipGetLastError();
 }
 
+// Batched forms: blockIdx.z (dots) or blockIdx.y picks the solve; blocks past a
+// solve's own extent return at once.
+__global__ __launch_bounds__(256) void k_dots_multi(const DotJob* __restrict__ jobs) {
+  __shared__ double red[256];
+  const DotJob J = jobs[blockIdx.z];
+  const int b = blockIdx.x, j = blockIdx.y;
+  if (j >= J.ncols) return;
+  const int per = (J.n + kDotBlocks - 1) / kDotBlocks;
+  const int i0 = min(J.n, b * per), i1 = min(J.n, i0 + per);
+  const double* v = J.V + (size_t)j * J.n;
+  double s = 0.0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) s = fma(v[i], J.w[i], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J.partial[(size_t)j * kDotBlocks + b] = red[0];
+}
+
+__global__ __launch_bounds__(kDotBlocks) void k_dots_final_multi(const DotJob* __restrict__ jobs) {
+  __shared__ double red[kDotBlocks];
+  const DotJob J = jobs[blockIdx.y];
+  const int j = blockIdx.x;
+  if (j >= J.ncols) return;
+  red[threadIdx.x] = J.partial[(size_t)j * kDotBlocks + threadIdx.x];
+  __syncthreads();
+  for (int k = kDotBlocks / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J.out[j] = red[0];
+}
+
+hipError_t dots_multi(int count, int maxcols, const DotJob* jobs, hipStream_t s) {
+  if (count < 1 || maxcols < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_dots_multi, dim3(kDotBlocks, maxcols, count), dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(k_dots_final_multi, dim3(maxcols, count), dim3(kDotBlocks), 0, s, jobs);
+  return hipGetLastError();
+}
+
+__global__ void k_combine_multi(const CombJob* __restrict__ jobs) {
+  const CombJob J = jobs[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= J.n) return;
+  double acc = J.y[i];
+  for (int j = 0; j < J.ncols; j++) acc = fma(J.sign * J.c[j], J.V[(size_t)j * J.n + i], acc);
+  J.y[i] = acc;
+}
+
+hipError_t combine_multi(int count, int maxn, const CombJob* jobs, hipStream_t s) {
+  if (count < 1 || maxn < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_combine_multi, dim3((maxn + 255) / 256, count), dim3(256), 0, s, jobs);
+  return hipGetLastError();
+}
+
+__global__ void k_vec_multi(const VecJob* __restrict__ jobs) {
+  const VecJob J = jobs[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= J.n) return;
+  switch (J.op) {
+    case 0: J.y[i] = J.u[i] * J.v[i]; break;
+    case 1: J.y[i] = (J.a + J.b * J.u[i]) * J.v[i] + J.z[i]; break;
+    case 2: J.y[i] = J.u[i] - J.v[i]; break;
+    case 4: J.y[i] = J.c[i] - ((J.a + J.b * J.u[i]) * J.v[i] + J.z[i]); break;
+    default: J.y[i] = J.v[i] * J.a; break;
+  }
+}
+
+hipError_t vec_multi(int count, int maxn, const VecJob* jobs, hipStream_t s) {
+  if (count < 1 || maxn < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_vec_multi, dim3((maxn + 255) / 256, count), dim3(256), 0, s, jobs);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // FP64 direct sum (shielded_pot_particle_ds, tree3d.cpp:1966-2016): 256 targets
 // per block, sources split into nsplit slices whose partial sums are added in

@@ -8,16 +8,20 @@ boundary-integral GMRES solves) arrive over time.  The server:
 * builds each admitted request's tree, moments and interaction lists on a pool
   of builder threads (each with its own context/stream, a few host threads for
   the tree);
-* runs a GPU loop that, each round, takes every ready evaluation and every
-  pending GMRES matrix-vector product and evaluates all of them in one mixed
-  launch per parameter group (bit_eval_potential_multi) -- requests arriving
-  while a round runs join the next one, and a GMRES solve occupies a slot for
-  as many rounds as it needs products;
-* keeps each solve's tree and lists resident between its products (charges are
-  swapped in place, bit_system_set_charges).
+* runs a GPU loop of rounds (bit_serve_round): each round takes every ready
+  evaluation and one step of every running GMRES solve -- its matrix-vector
+  product goes into the round's mixed far/near launches (one per parameter
+  group) with the evaluations, and its Arnoldi step (two Gram-Schmidt passes,
+  the norm, the updates) is batched with the other solves' -- so requests
+  arriving while a round runs join the next one, and a solve occupies a slot
+  for as many rounds as it needs products;
+* keeps each solve's tree, lists and Krylov state (basis, residual, iterate)
+  resident in HBM from admission to completion; only the Hessenberg column and
+  the Givens rotations are on the host.
 
-Every target's sums are formed the same way alone and in a mix, so results do
-not depend on what else was on the GPU (tests/test_gpu_serving.py).
+Every target's sums and every solve's vector arithmetic are formed the same
+way alone and in a mix, so results do not depend on what else was on the GPU
+(tests/test_gpu_serving.py).
 """
 from __future__ import annotations
 
@@ -63,7 +67,10 @@ class Request:
 
     @property
     def est_bytes(self) -> int:
-        return BYTES_PER_PARTICLE * self.n + BYTES_FIXED
+        b = BYTES_PER_PARTICLE * self.n + BYTES_FIXED
+        if self.kind == "bie":  # resident Krylov basis (restart 50) and work vectors
+            b += 8 * (51 + 6) * self.n
+        return b
 
     def params(self, precision="fp32"):
         return bit_amd.make_params(self.nterms, self.acc, self.kappa, self.delta ** 2, None,
@@ -109,35 +116,36 @@ def _particles(r: Request):
 
 
 def solve_alone(ctx, r: Request, precision="fp32"):
-    """The request evaluated by itself (reference result for the mix tests)."""
+    """The request served by itself: one evaluation, or the whole device-resident
+    GMRES solve (reference result for the mix tests, and the one-at-a-time
+    baseline)."""
     x, y, z, q, w = _particles(r)
     s = bit_amd.System(ctx, x, y, z, q, r.maxmemb)
     try:
         args = (r.nterms, r.acc, r.kappa, r.delta ** 2, None, precision)
         if r.kind == "eval":
             return s.potential(*args)
-        f, selfc = r.extra["f"], r.extra["self"]
-
-        def matvec(v):  # the server's operator and GMRES core, one system alone
-            s.set_charges(w * v)
-            return (0.5 + selfc * w) * v + s.potential(*args)
-
-        sig, it, rr = bit_amd.gmres(s.n, matvec, f, None, 50, r.tol, 500)
+        sig, it, rr = s.solve_second_kind(w, r.extra["f"], 0.5, r.extra["self"], None, 50,
+                                          r.tol, 500, *args)
+        r.iters, r.relres = it, rr
         return sig
     finally:
         s.close()
 
 
 class _Job:
-    __slots__ = ("req", "system", "params", "out", "done")
+    __slots__ = ("req", "system", "params")
 
     def __init__(self, req, system, params):
         self.req, self.system, self.params = req, system, params
-        self.out = None
-        self.done = threading.Event()
 
 
 class Server:
+    """Continuous batching on one GPU: builder threads make each admitted
+    request's system (tree, moments, lists; for a solve also its resident
+    Krylov state), and the GPU loop serves rounds (bit_serve_round): every
+    ready evaluation and one step of every running solve in shared launches."""
+
     def __init__(self, device: int = 0, hbm_budget: int = 16 << 30, builders: int = 8,
                  host_threads: int = 2, precision: str = "fp32", max_round_systems: int = 256):
         self.device = device
@@ -169,7 +177,7 @@ class Server:
     def run(self, requests, realtime: bool = True):
         """Serves `requests` (arrival times relative to the start); returns metrics."""
         reqs = sorted(requests, key=lambda r: r.arrival)
-        work: "queue.Queue[_Job]" = queue.Queue()
+        inbox: "queue.Queue" = queue.Queue()  # ("eval", job) or ("bie", (req, system, krylov))
         inflight = [0]
         cv = threading.Condition()
         pending = list(reversed(reqs))
@@ -177,13 +185,15 @@ class Server:
         total = len(reqs)
         t0 = time.perf_counter()
         pool = ThreadPoolExecutor(max_workers=self.builders)
-        solvers = []
         rounds = []
+        build_gpu_ms = []
 
         def now():
             return time.perf_counter() - t0
 
-        def release(r, s):
+        def release(r, s, k=None):
+            if k is not None:
+                k.close()
             s.close()
             with cv:
                 inflight[0] -= r.est_bytes
@@ -199,42 +209,31 @@ class Server:
                 cv.notify_all()
 
         def build(r):
+            s = None
             try:
                 ctx = self._builder_ctx()
                 x, y, z, q, w = _particles(r)
                 s = bit_amd.System(ctx, x, y, z, q, r.maxmemb)
                 prm = r.params(self.precision)
-                s.prepare(r.nterms, r.acc, r.kappa, r.delta ** 2, None, self.precision)
+                if r.kind == "eval":
+                    s.prepare(r.nterms, r.acc, r.kappa, r.delta ** 2, None, self.precision)
+                    k = None
+                else:
+                    k = bit_amd.Krylov(s, w, r.extra["f"], 0.5, r.extra["self"], None, 50, r.tol,
+                                       500, r.nterms, r.acc, r.kappa, r.delta ** 2, None,
+                                       self.precision)
+                st = s.stats()
+                build_gpu_ms.append(max(st["ms_moments"], 0.0) + max(st["ms_lists"], 0.0))
             except Exception as e:  # a failed request must not stall the loop
+                if s is not None:
+                    s.close()
                 fail(r, e)
                 return
             r.t_ready = now()
-            if r.kind == "eval":
-                work.put(_Job(r, s, prm))
-                return
-            th = threading.Thread(target=gmres_request, args=(r, s, w, prm), daemon=True)
-            solvers.append(th)
-            th.start()
-
-        def gmres_request(r, s, w, prm):
-            f, selfc = r.extra["f"], r.extra["self"]
-
-            def matvec(v):
-                s.set_charges(w * v)
-                job = _Job(r, s, prm)
-                work.put(job)
-                job.done.wait()
-                return (0.5 + selfc * w) * v + job.out
-
-            try:
-                sig, it, rr = bit_amd.gmres(s.n, matvec, f, None, 50, r.tol, 500)
-            except Exception as e:
-                s.close()
-                fail(r, e)
-                return
-            r.result, r.iters, r.relres = sig, it, rr
-            r.t_done = now()
-            release(r, s)
+            if k is None:
+                inbox.put(("eval", _Job(r, s, prm)))
+            else:
+                inbox.put(("bie", (r, s, k)))
 
         def admit():
             while pending:
@@ -254,41 +253,55 @@ class Server:
 
         adm = threading.Thread(target=admit, daemon=True)
         adm.start()
+        evals, solves = [], []
         while finished[0] < total:
-            try:
-                first = work.get(timeout=0.005)
-            except queue.Empty:
-                continue
-            jobs = [first]
-            while len(jobs) < self.max_round:
+            if not evals and not solves:
                 try:
-                    jobs.append(work.get_nowait())
+                    kind, item = inbox.get(timeout=0.005)
+                except queue.Empty:
+                    continue
+                (evals if kind == "eval" else solves).append(item)
+            while True:
+                try:
+                    kind, item = inbox.get_nowait()
                 except queue.Empty:
                     break
-            outs, ms = bit_amd.eval_multi(self.gpu_ctx, [j.system for j in jobs],
-                                          [j.params for j in jobs])
-            rounds.append((len(jobs), sum(j.system.n for j in jobs), ms))
+                (evals if kind == "eval" else solves).append(item)
+            jobs, evals = evals[:self.max_round], evals[self.max_round:]
+            outs, ms = bit_amd.serve_round(self.gpu_ctx, [j.system for j in jobs],
+                                           [j.params for j in jobs], [k for _, _, k in solves])
+            rounds.append((len(jobs) + len(solves),
+                           sum(j.system.n for j in jobs) + sum(s.n for _, s, _ in solves), ms))
             t = now()
             for j, o in zip(jobs, outs):
-                if j.req.kind == "eval":
-                    j.req.result = o
-                    j.req.t_done = t
-                    release(j.req, j.system)
-                else:
-                    j.out = o
-                    j.done.set()
+                j.req.result = o
+                j.req.t_done = t
+                release(j.req, j.system)
+            running = []
+            for r, s, k in solves:
+                done, it, rr = k.status()
+                if not done:
+                    running.append((r, s, k))
+                    continue
+                r.result, r.iters, r.relres = k.solution(), it, rr
+                r.t_done = now()
+                if done != 1:
+                    r.extra["error"] = f"GMRES stopped at maxit (relres {rr:.3e})"
+                release(r, s, k)
+            solves = running
         adm.join()
         pool.shutdown(wait=True)
-        for th in solvers:
-            th.join()
         makespan = max(r.t_done for r in reqs) - reqs[0].arrival
-        return summarize(reqs, makespan, rounds)
+        out = summarize(reqs, makespan, rounds)
+        out["gpu_build_s"] = float(sum(build_gpu_ms) / 1e3)
+        out["gpu_busy_frac"] = (out.get("gpu_kernel_s", 0.0) + out["gpu_build_s"]) / makespan
+        return out
 
 
 def run_one_at_a_time(requests, device: int = 0, precision: str = "fp32",
                       realtime: bool = True):
     """Baseline: the same arrivals served one request at a time (one system,
-    one evaluation or solve, on one stream)."""
+    one evaluation or one whole solve, on one stream)."""
     ctx = bit_amd.Context(device)
     reqs = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
@@ -324,9 +337,52 @@ def summarize(reqs, makespan, rounds):
     return out
 
 
+def benchmark(requests: int = 2000, rate: float = 400.0, seed: int = 2024, builders: int = 8,
+              hbm_budget: int = 16 << 30, warmup: int = 64, baseline: bool = True,
+              sweep=(), sweep_requests: int = 1000, log=None):
+    """The serving stream at `rate` (continuous batching, then the same arrivals
+    one at a time), and the server alone at each rate of `sweep`."""
+    srv = Server(hbm_budget=hbm_budget, builders=builders)
+    try:
+        if warmup > 0:  # first use of kernel instantiations and builder contexts
+            srv.run(make_requests(warmup, rate, seed + 1))
+        reqs = make_requests(requests, rate, seed)
+        m = srv.run(reqs)
+        out = {"requests": requests, "rate_per_s": rate, "seed": seed, "warmup_requests": warmup,
+               "server": m, "errors": sum("error" in r.extra for r in reqs)}
+        if log:
+            log(f"[serving] {requests} requests at {rate}/s: p50 {m['p50_ms']:.1f} ms, p99 "
+                f"{m['p99_ms']:.1f} ms, {m['mean_systems_per_round']:.2f} systems/round, "
+                f"GPU busy {m['gpu_busy_frac']:.2f}")
+        rows = []
+        for rt in sweep:
+            rq = make_requests(sweep_requests, rt, seed + 7)
+            mm = srv.run(rq)
+            rows.append({"rate_per_s": rt, "requests": sweep_requests, **{
+                k: mm[k] for k in ("p50_ms", "p99_ms", "targets_per_s", "mean_systems_per_round",
+                                   "gpu_busy_frac", "makespan_s")},
+                "errors": sum("error" in r.extra for r in rq)})
+            if log:
+                log(f"[serving] sweep {rt}/s: p50 {mm['p50_ms']:.1f} ms, p99 {mm['p99_ms']:.1f} ms,"
+                    f" {mm['mean_systems_per_round']:.2f} systems/round, GPU busy "
+                    f"{mm['gpu_busy_frac']:.2f}")
+        if rows:
+            out["rate_sweep"] = rows
+    finally:
+        srv.close()
+    if baseline:
+        base = make_requests(requests, rate, seed)
+        out["one_at_a_time"] = run_one_at_a_time(base)
+        if log:
+            b = out["one_at_a_time"]
+            log(f"[serving] one at a time: p50 {b['p50_ms']:.1f} ms, p99 {b['p99_ms']:.1f} ms")
+    return out
+
+
 def main():
     import argparse
     import json
+    import sys
     ap = argparse.ArgumentParser(description="serving stream benchmark (one JSON line)")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=400.0)
@@ -334,22 +390,16 @@ def main():
     ap.add_argument("--builders", type=int, default=8)
     ap.add_argument("--hbm-budget-gb", type=float, default=16.0)
     ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--sweep", default="", help="comma-separated extra rates (req/s)")
+    ap.add_argument("--sweep-requests", type=int, default=1000)
     ap.add_argument("--warmup-requests", type=int, default=64,
                     help="untimed stream (seed + 1) served first")
     a = ap.parse_args()
-    reqs = make_requests(a.requests, a.rate, a.seed)
-    srv = Server(hbm_budget=int(a.hbm_budget_gb * (1 << 30)), builders=a.builders)
-    if a.warmup_requests > 0:
-        srv.run(make_requests(a.warmup_requests, a.rate, a.seed + 1))
-    m = srv.run(reqs)
-    srv.close()
-    errors = sum("error" in r.extra for r in reqs)
-    line = {"workload": "serving stream", "requests": a.requests, "rate_per_s": a.rate,
-            "seed": a.seed, "warmup_requests": a.warmup_requests, "server": m,
-            "errors": errors}
-    if not a.no_baseline:
-        base = make_requests(a.requests, a.rate, a.seed)
-        line["one_at_a_time"] = run_one_at_a_time(base)
+    sweep = [float(v) for v in a.sweep.split(",") if v]
+    line = benchmark(a.requests, a.rate, a.seed, a.builders, int(a.hbm_budget_gb * (1 << 30)),
+                     a.warmup_requests, not a.no_baseline, sweep, a.sweep_requests,
+                     log=lambda m: print(m, file=sys.stderr, flush=True))
+    line["workload"] = "serving stream"
     print(json.dumps(line), flush=True)
 
 

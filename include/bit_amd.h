@@ -231,11 +231,39 @@ int bit_system_set_charges(bit_system *sys, const double *q, int on_device);
  *     diag * s_i + sum_{j != i} w_j s_j G(x_i - x_j) + self * w_i s_i = f_i
  * (G as bit_eval_potential with params; w = quadrature weights).  The
  * reference's solver::gmres (solver.h:157-260) is dense and unrestarted; this
- * is its matrix-free counterpart.  sigma holds the initial guess on entry.
+ * is its matrix-free counterpart, with the Krylov basis resident in HBM and
+ * classical Gram-Schmidt applied twice (bit_krylov_* stepped to completion).
+ * sigma holds the initial guess on entry.
  * iters/relres (may be NULL) receive the Arnoldi steps and ||r||/||f||. */
 int bit_solve_second_kind(bit_system *sys, const bit_params *params, double diag, double self,
                           const double *w, const double *f, double *sigma, int restart,
                           double tol, int maxit, int *iters, double *relres);
+
+/* The same solve as resumable state resident in HBM (basis, residual,
+ * iterate, weights, right-hand side), for a request server that steps many
+ * solves together: bit_serve_round advances it by one matrix-vector product
+ * and the Arnoldi step after it.  sigma0 (may be NULL = zero) is the initial
+ * guess.  No reference counterpart: the reference's solver::gmres
+ * (solver.h:157-260) runs one dense system to completion. */
+typedef struct bit_krylov bit_krylov;
+int bit_krylov_create(bit_system *sys, const bit_params *params, double diag, double self,
+                      const double *w, const double *f, const double *sigma0, int restart,
+                      double tol, int maxit, bit_krylov **out);
+int bit_krylov_destroy(bit_krylov *k);
+/* done: 0 running, 1 converged (||b - A x|| <= tol ||b||), 2 stopped at maxit */
+int bit_krylov_status(const bit_krylov *k, int *done, int *iters, double *relres);
+int bit_krylov_solution(const bit_krylov *k, double *sigma);
+
+/* One serving round on ctx's stream: the potentials of n_eval systems (host
+ * outputs, as bit_eval_potential_multi) and one step of each running solve,
+ * all products in shared mixed far/near launches; each Gram-Schmidt pass, the
+ * norms and the vector updates of all solves are one launch each, with one
+ * read-back per pass.  Every result is bit-identical to what the request gets
+ * alone (bit_eval_potential, bit_solve_second_kind).  A system may appear once
+ * per round.  kernel_ms (may be NULL) gets the GPU time of the round. */
+int bit_serve_round(bit_ctx *ctx, int n_eval, bit_system *const *systems,
+                    const bit_params *params, double *const *outs, int n_solve,
+                    bit_krylov *const *solves, double *kernel_ms);
 
 /* Host-only restarted GMRES on a dense row-major n x n matrix (same FP64
  * core as bit_solve_second_kind; no device needed). */

@@ -46,13 +46,13 @@ def test_server_results_equal_alone(ctx):
     srv.close()
     assert not any("error" in r.extra for r in reqs)
     assert m["requests"] == 40 and m["p99_ms"] >= m["p50_ms"] > 0
+    assert any(r.kind == "bie" for r in reqs)
     for r in reqs:
+        got_iters = r.iters
         ref = serving.solve_alone(ctx, r)
-        if r.kind == "eval":
-            assert np.array_equal(r.result, ref), r.rid
-        else:
-            assert r.relres <= r.tol
-            assert np.max(np.abs(r.result - ref)) <= 1e-12 * np.max(np.abs(ref)), r.rid
+        assert np.array_equal(r.result, ref), (r.rid, r.kind)
+        if r.kind == "bie":
+            assert r.relres <= r.tol and r.iters == got_iters
 
 
 def test_budget_admission_serialises_but_completes():
@@ -64,3 +64,48 @@ def test_budget_admission_serialises_but_completes():
     srv.close()
     assert not any("error" in r.extra for r in reqs)
     assert m["mean_systems_per_round"] == 1.0
+
+
+def test_solves_stepped_together_equal_alone(ctx):
+    """Several resident-Krylov solves of different sizes and parameters, stepped
+    together in serving rounds that also carry plain evaluations, give the same
+    iterates, iteration counts and residuals as each solve alone."""
+    from bit_amd import bie
+    cases = [(3, 1.0, "fp64", 7, 0.6), (4, 0.5, "fp32", 5, 0.5), (3, 2.0, "fp32", 9, 0.4),
+             (5, 1.0, "fp32", 7, 0.6)]
+    probs = []
+    for lv, kap, prec, nt, acc in cases:
+        V, F = bie.icosphere(lv)
+        pts, area, h = bie.collocation(V, F)
+        w = area / (4 * np.pi)
+        f = bie.random_rhs(6, seed=lv + int(10 * kap))(pts)
+        tol = 1e-9 if prec == "fp64" else 1e-6
+        probs.append((pts, w, f, float(np.exp(-kap * h) / h), tol,
+                      (nt, acc, kap, h * h, None, prec)))
+    alone = []
+    for pts, w, f, sc, tol, args in probs:
+        s = bit_amd.System(ctx, pts[:, 0], pts[:, 1], pts[:, 2], w, 64)
+        alone.append(s.solve_second_kind(w, f, 0.5, sc, None, 8, tol, 200, *args))
+        s.close()
+    systems, solves = [], []
+    for pts, w, f, sc, tol, args in probs:
+        s = bit_amd.System(ctx, pts[:, 0], pts[:, 1], pts[:, 2], w, 64)
+        systems.append(s)
+        solves.append(bit_amd.Krylov(s, w, f, 0.5, sc, None, 8, tol, 200, *args))
+    P = workloads.gaussian_blobs(20000, seed=77)
+    ev = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 100)
+    eargs = (7, 0.5, 0.8, 1e-4, None, "fp32")
+    want_eval = ev.potential(*eargs)
+    rounds = 0
+    while not all(k.done for k in solves):
+        outs, ms = bit_amd.serve_round(ctx, [ev], [bit_amd.make_params(*eargs)], solves)
+        assert np.array_equal(outs[0], want_eval)
+        rounds += 1
+        assert rounds < 500
+    for k, (sig, it, rr) in zip(solves, alone):
+        done, it2, rr2 = k.status()
+        assert done == 1 and it2 == it and rr2 == rr
+        assert np.array_equal(k.solution(), sig)
+        k.close()
+    for s in systems + [ev]:
+        s.close()
