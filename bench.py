@@ -40,8 +40,16 @@ PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 # matches the profiled one.
 MEASURED_TRAFFIC = {
     # (kernel, n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
-    ("far", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 3.79126e9,
-    ("near", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 2.14475e9,
+    # (profiles/pmc_traffic_blobs4m_all_kernels.md)
+    ("far", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 3.79212e9,
+    ("near", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 4.94937e9,
+}
+# Executed FP32 lane FLOPs per launch: 64 x rocprofv3's SQ_INSTS_VALU_FLOPS_FP32 (a
+# per-wave count, 2 FMA + ADD + MUL + TRANS) on this exact configuration
+# (profiles/pmc_flops_k_farq_k_near32_blobs4m.md); masked lanes are included.
+MEASURED_EXEC_FLOPS = {
+    ("far", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 64 * 3.215958e10,
+    ("near", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 64 * 1.344225e10,
 }
 
 
@@ -127,6 +135,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--fp64-point", default="12,0.25,350",
+                    help="nterms,acc,maxmemb of the FP64 evaluation compared with the FP32 "
+                         "step (the autotuner's FP64 point at 1e-6)")
+    ap.add_argument("--no-literal", action="store_true",
+                    help="skip the literal-parameter runs (p 8, acc 0.7, N0 2000)")
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=400.0)
@@ -195,15 +208,51 @@ def main():
     err = float(np.linalg.norm(phi[tidx] - ref) / np.linalg.norm(ref))
     log(f"[bench] rel L2 error vs FP64 direct sum on {len(tidx)} targets: {err:.3e}")
 
-    fp64_ms = None
-    if not args.no_fp64_compare and args.precision == "fp32":
+    def timed(nterms, acc, maxmemb, precision, mac_sub=None, steps=3):
+        """ms per step (one warm-up, then `steps` whole evaluations) and the rel
+        L2 error of the last one against the FP64 direct sum on tidx."""
+        def one():
+            s = bit_amd.System(ctx, *dev, maxmemb)
+            s.potential(nterms, acc, args.kappa, d2, mac_sub, precision, out=out)
+            r = s.stats()
+            s.close()
+            return r
+        one()
         ctx.sync()
         a = time.perf_counter()
-        st64 = step("fp64")
+        for _ in range(steps):
+            r = one()
         ctx.sync()
-        fp64_ms = 1e3 * (time.perf_counter() - a)
-        log(f"[bench] one fp64 step: {fp64_ms:.1f} ms (far {st64['ms_far']:.1f}, near "
-            f"{st64['ms_near']:.1f})")
+        t = 1e3 * (time.perf_counter() - a) / steps
+        e = float(np.linalg.norm(out.numpy()[tidx] - ref) / np.linalg.norm(ref))
+        return t, e, r
+
+    fp64 = None
+    if not args.no_fp64_compare and args.precision == "fp32":
+        # FP64 at the FP32 point, and at the autotuner's FP64 point for the same target
+        nt64, acc64, mm64 = args.fp64_point.split(",")
+        nt64, acc64, mm64 = int(nt64), float(acc64), int(mm64)
+        t_same, e_same, st64 = timed(args.nterms, args.acc, args.maxmemb, "fp64", steps=2)
+        t_tuned, e_tuned, _ = timed(nt64, acc64, mm64, "fp64", steps=2)
+        fp64 = {"same_point": {"ms_per_step": t_same, "rel_l2_vs_fp64_direct": e_same,
+                               "ms_far": st64["ms_far"], "ms_near": st64["ms_near"]},
+                "fp64_tuned_point": {"nterms": nt64, "acc": acc64, "maxmemb": mm64,
+                                     "ms_per_step": t_tuned, "rel_l2_vs_fp64_direct": e_tuned},
+                "fp32_speedup_vs_fp64_tuned": t_tuned / ms}
+        log(f"[bench] fp64 at the fp32 point: {t_same:.1f} ms (err {e_same:.2e}); at its own "
+            f"tuned point ({nt64},{acc64},{mm64}): {t_tuned:.1f} ms (err {e_tuned:.2e})")
+    fp64_ms = fp64["same_point"]["ms_per_step"] if fp64 else None
+
+    literal = None
+    if not args.no_literal:
+        # the configuration's literal parameters: p 8, acc 0.7, N0 2000, FP32, with
+        # the clean MAC (delta^2 subtracted) and with the reference's MAC shift 0.01
+        literal = {"nterms": 9, "p": 8, "acc": 0.7, "maxmemb": 2000, "precision": "fp32"}
+        for name, sub in (("clean_mac", None), ("reference_mac_shift_0.01", 0.01)):
+            t, e, _ = timed(9, 0.7, 2000, "fp32", sub)
+            literal[name] = {"ms_per_step": t, "targets_per_s": args.n / (t * 1e-3),
+                             "rel_l2_vs_fp64_direct": e}
+            log(f"[bench] literal point ({name}): {t:.1f} ms/step, err {e:.2e}")
 
     f_near, f_far = flops(st, args.nterms, screened, args.precision)
     kern = "far" if st["ms_far"] >= st["ms_near"] else "near"
@@ -231,6 +280,21 @@ def main():
             "other_kernel": {"name": "near" if kern == "far" else "far",
                              "tflops": (f_near / (st["ms_near"] * 1e-3) if kern == "far" else
                                         f_far / (st["ms_far"] * 1e-3)) / 1e12}}
+    # `frac` prices the far kernel at the sweep it runs (2 NQ(p) per pair); beside it the
+    # counter-measured executed FLOPs and the reference recurrence's 24 T(p) per pair,
+    # which the kernel does not execute (hence a fraction above 1)
+    key = (kern, args.n, args.seed, args.nterms, args.acc, args.maxmemb, args.kappa, args.delta,
+           args.precision)
+    ex = MEASURED_EXEC_FLOPS.get(key)
+    if ex is not None:
+        roof["executed"] = {"flops_per_launch": ex, "tflops": ex / (kms * 1e-3) / 1e12,
+                            "frac": ex / (kms * 1e-3) / peak,
+                            "source": "rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes"}
+    if kern == "far":
+        rec = roof["far_flops_per_pair"]["coefficient_recurrence"] * st["far_pairs"]
+        roof["recurrence_pricing"] = {"flops_per_launch": rec,
+                                      "tflops": rec / (kms * 1e-3) / 1e12,
+                                      "frac": rec / (kms * 1e-3) / peak}
 
     serving = None
     if args.serving_requests > 0:
@@ -278,6 +342,8 @@ def main():
                                             "near_items", "lane_far", "lane_near_pairs",
                                             "list_passes")},
         "fp64_ms_per_step": fp64_ms,
+        "fp64": fp64,
+        "literal": literal,
         "roofline": roof,
         "serving": serving,
         "cpu_baseline": cpu,
