@@ -291,136 +291,11 @@ bool finite_all(int n, const double* a) {
   return true;
 }
 
-// Split rounds of the tree build executed on the slot arrays in HBM.
-class DeviceEngine : public bit::PartitionEngine {
- public:
-  DeviceEngine(bit_ctx* c, const bit::dev::SlotArrays& a) : c_(c), a_(a) {}
-  int status = BIT_OK;
-
-  bool run(const std::vector<bit::PartOp>& ops, std::vector<bit::PartResult>& res,
-           std::string& err) override {
-    const int nops = (int)ops.size();
-    hops_.resize(nops);
-    c0_.resize(nops);
-    nc_.resize(nops);
-    chunks_.clear();
-    for (int o = 0; o < nops; o++) {
-      hops_[o] = bit::dev::PartOpDev{ops[o].b, ops[o].e, ops[o].dir, ops[o].par, ops[o].mid};
-      c0_[o] = (int)chunks_.size();
-      for (int s0 = ops[o].b; s0 < ops[o].e; s0 += bit::dev::kPartChunk)
-        chunks_.push_back(bit::dev::PartChunk{o, s0, std::min(ops[o].e, s0 + bit::dev::kPartChunk), 0});
-      nc_[o] = (int)chunks_.size() - c0_[o];
-    }
-    const int nch = (int)chunks_.size();
-    hres_.resize(nops);
-    status = step(nops, nch);
-    if (status != BIT_OK) {
-      err = bit_last_error();
-      return false;
-    }
-    res.resize(nops);
-    for (int o = 0; o < nops; o++) {
-      const bit::dev::ChunkResult& r = hres_[o];
-      res[o].nl = r.nl;
-      for (int sd = 0; sd < 2; sd++)
-        for (int d = 0; d < 3; d++) {
-          res[o].mn[sd][d] = r.v[3 * sd + d];
-          res[o].mx[sd][d] = r.v[6 + 3 * sd + d];
-        }
-    }
-    return true;
-  }
-
- private:
-  static size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
-
-  // One round: the op and chunk tables go up in one copy from pinned memory,
-  // the per-op results come back into pinned memory as soon as the counting
-  // phase is done, and the host works out the next round while the slots move
-  // (the next round's work is queued behind the move on the same stream).
-  int step(int nops, int nch) {
-    hipStream_t st = c_->stream;
-    const size_t o_ch = al(nops * sizeof(bit::dev::PartOpDev));
-    const size_t o_c0 = o_ch + al(nch * sizeof(bit::dev::PartChunk));
-    const size_t o_nc = o_c0 + al(nops * sizeof(int));
-    const size_t total = o_nc + al(nops * sizeof(int));
-    const size_t rbytes = nops * sizeof(bit::dev::ChunkResult);
-    BIT_TRY(up_.ensure(total));
-    BIT_TRY(down_.ensure(rbytes));
-    char* h = static_cast<char*>(up_.p);
-    std::memcpy(h, hops_.data(), nops * sizeof(bit::dev::PartOpDev));
-    std::memcpy(h + o_ch, chunks_.data(), nch * sizeof(bit::dev::PartChunk));
-    std::memcpy(h + o_c0, c0_.data(), nops * sizeof(int));
-    std::memcpy(h + o_nc, nc_.data(), nops * sizeof(int));
-    BIT_TRY(d_blob_.ensure(c_, total));
-    BIT_TRY(d_cres_.ensure(c_, (size_t)nch * sizeof(bit::dev::ChunkResult)));
-    BIT_TRY(d_cbase_.ensure(c_, (size_t)nch * sizeof(int2)));
-    BIT_TRY(d_ores_.ensure(c_, rbytes));
-    HIP_TRY(hipMemcpyAsync(d_blob_.p, h, total, hipMemcpyHostToDevice, st));
-    char* d = d_blob_.as<char>();
-    auto* dops = reinterpret_cast<bit::dev::PartOpDev*>(d);
-    auto* dch = reinterpret_cast<bit::dev::PartChunk*>(d + o_ch);
-    HIP_TRY(bit::dev::partition_count(a_, nops, dops, nch, dch, reinterpret_cast<int*>(d + o_c0),
-                                      reinterpret_cast<int*>(d + o_nc),
-                                      d_cres_.as<bit::dev::ChunkResult>(), d_cbase_.as<int2>(),
-                                      d_ores_.as<bit::dev::ChunkResult>(), st));
-    HIP_TRY(hipMemcpyAsync(down_.p, d_ores_.p, rbytes, hipMemcpyDeviceToHost, st));
-    if (!counted_) HIP_TRY(hipEventCreateWithFlags(&counted_, hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(counted_, st));
-    HIP_TRY(bit::dev::partition_move(a_, dops, nch, dch, d_cbase_.as<int2>(),
-                                     d_ores_.as<bit::dev::ChunkResult>(), st));
-    // The pinned tables are reused next round: the upload has run before the
-    // counting phase, and the results are read here.
-    HIP_TRY(hipEventSynchronize(counted_));
-    std::memcpy(hres_.data(), down_.p, rbytes);
-    return BIT_OK;
-  }
-
-  // Page-locked host staging, grown on demand.
-  struct Pinned {
-    void* p = nullptr;
-    size_t cap = 0;
-    Pinned() = default;
-    Pinned(const Pinned&) = delete;
-    Pinned& operator=(const Pinned&) = delete;
-    ~Pinned() {
-      if (p) (void)hipHostFree(p);
-    }
-    int ensure(size_t need) {
-      if (need <= cap && p) return BIT_OK;
-      if (p) (void)hipHostFree(p);
-      p = nullptr;
-      cap = 0;
-      if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess) {
-        p = nullptr;
-        (void)hipGetLastError();
-        return fail(BIT_E_OOM, "pinned host staging allocation failed");
-      }
-      cap = need;
-      return BIT_OK;
-    }
-  };
-
-  bit_ctx* c_;
-  bit::dev::SlotArrays a_;
-  std::vector<bit::dev::PartOpDev> hops_;
-  std::vector<bit::dev::PartChunk> chunks_;
-  std::vector<int> c0_, nc_;
-  std::vector<bit::dev::ChunkResult> hres_;
-  Pinned up_, down_;
-  DevBuf d_blob_, d_cres_, d_cbase_, d_ores_;
-  hipEvent_t counted_ = nullptr;
-
- public:
-  ~DeviceEngine() {
-    if (counted_) (void)hipEventDestroy(counted_);
-  }
-};
-
 // Builds the tree of n particles whose coordinates are in HBM (original order)
 // and leaves the system's sorted sources, permutation and node table on the
-// device.  The split logic runs on the host (bit-exact program of treemake);
-// the data never leaves the device.
+// device.  Every split decision is taken on the device (split programs, one
+// thread per node, kernels.hip); the host launches a generation's rounds and
+// reads back only the generation's size, then the finished node table.
 int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const double* dy,
                     const double* dz, const double* dq, int maxmemb) {
   hipStream_t st = ctx->stream;
@@ -441,29 +316,115 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
   bit::dev::SlotArrays a{s->perm.as<int>(), s->sx.as<double>(), s->sy.as<double>(),
                          s->sz.as<double>(), tP.as<int>(), tX.as<double>(), tY.as<double>(),
                          tZ.as<double>()};
-  // root bounding box (tree3d.cpp:1385-1403)
+  // node table, grown between generations (a generation of cnt nodes makes at
+  // most 8 cnt children)
+  int cap = 0;
+  DevBuf nbox, nchild, nbegin, nend, ndepth, npar;
+  auto node_view = [&]() {
+    return bit::dev::BuildNodes{nbox.as<double>(), nchild.as<int>(), nbegin.as<int>(),
+                                nend.as<int>(), ndepth.as<int>(), npar.as<int>()};
+  };
+  auto grow = [&](int need, int used) -> int {
+    if (need <= cap) return BIT_OK;
+    const int ncap = std::max(need, 2 * cap);
+    DevBuf b2, c2, g2, e2, d2, p2;
+    BIT_TRY(b2.ensure(ctx, (size_t)ncap * 13 * sizeof(double)));
+    BIT_TRY(c2.ensure(ctx, (size_t)ncap * 8 * sizeof(int)));
+    BIT_TRY(g2.ensure(ctx, (size_t)ncap * sizeof(int)));
+    BIT_TRY(e2.ensure(ctx, (size_t)ncap * sizeof(int)));
+    BIT_TRY(d2.ensure(ctx, (size_t)ncap * sizeof(int)));
+    BIT_TRY(p2.ensure(ctx, (size_t)ncap * sizeof(int)));
+    if (used > 0) {
+      HIP_TRY(hipMemcpyAsync(b2.p, nbox.p, (size_t)used * 13 * sizeof(double), hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(c2.p, nchild.p, (size_t)used * 8 * sizeof(int), hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(g2.p, nbegin.p, (size_t)used * sizeof(int), hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(e2.p, nend.p, (size_t)used * sizeof(int), hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(d2.p, ndepth.p, (size_t)used * sizeof(int), hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(p2.p, npar.p, (size_t)used * sizeof(int), hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));  // the old blocks go back to the cache
+    }
+    std::swap(nbox.p, b2.p); std::swap(nbox.bytes, b2.bytes);
+    std::swap(nchild.p, c2.p); std::swap(nchild.bytes, c2.bytes);
+    std::swap(nbegin.p, g2.p); std::swap(nbegin.bytes, g2.bytes);
+    std::swap(nend.p, e2.p); std::swap(nend.bytes, e2.bytes);
+    std::swap(ndepth.p, d2.p); std::swap(ndepth.bytes, d2.bytes);
+    std::swap(npar.p, p2.p); std::swap(npar.bytes, p2.bytes);
+    nbox.ctx = nchild.ctx = nbegin.ctx = nend.ctx = ndepth.ctx = npar.ctx = ctx;
+    cap = ncap;
+    return BIT_OK;
+  };
+  BIT_TRY(grow(std::max(1024, 8 * (n / std::max(1, maxmemb)) + 64), 0));
+  // root bounding box (tree3d.cpp:1385-1403) and root node, on the device
   const int mblocks = std::max(1, std::min(256, (n + 255) / 256));
   DevBuf part;
   BIT_TRY(part.ensure(ctx, (size_t)mblocks * 6 * sizeof(double)));
   HIP_TRY(bit::dev::minmax(n, s->sx.as<double>(), s->sy.as<double>(), s->sz.as<double>(),
                            mblocks, part.as<double>(), st));
-  std::vector<double> hp((size_t)mblocks * 6);
-  HIP_TRY(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(bit::dev::build_root(n, mblocks, part.as<double>(), node_view(), st));
+  // generations
+  const double lim = 1.0 / std::sqrt(2.0);  // aspectlimit, tree3d.cpp:1371
+  const long long node_cap = 4LL * n + 64;
+  DevBuf progs, ops, nch, chunk0, chunks, nchunks, cres, cbase, ores, nkid, start, result;
+  BIT_TRY(result.ensure(ctx, sizeof(bit::dev::GenResult)));
+  BIT_TRY(nchunks.ensure(ctx, sizeof(int)));
+  bit::dev::GenResult gr{};
+  int gen_lo = 0, gen_hi = 1, maxdepth = 0;
+  bool work = n > maxmemb;
+  while (work) {
+    const int cnt = gen_hi - gen_lo;
+    BIT_TRY(grow(gen_hi + 8 * cnt, gen_hi));
+    const int maxch = (n + bit::dev::kPartChunk - 1) / bit::dev::kPartChunk + cnt;
+    BIT_TRY(progs.ensure(ctx, (size_t)cnt * sizeof(bit::dev::DevProg)));
+    BIT_TRY(ops.ensure(ctx, (size_t)cnt * sizeof(bit::dev::PartOpDev)));
+    BIT_TRY(nch.ensure(ctx, (size_t)cnt * sizeof(int)));
+    BIT_TRY(chunk0.ensure(ctx, (size_t)cnt * sizeof(int)));
+    BIT_TRY(ores.ensure(ctx, (size_t)cnt * sizeof(bit::dev::ChunkResult)));
+    BIT_TRY(nkid.ensure(ctx, (size_t)cnt * sizeof(int)));
+    BIT_TRY(start.ensure(ctx, (size_t)cnt * sizeof(int)));
+    BIT_TRY(chunks.ensure(ctx, (size_t)maxch * sizeof(bit::dev::PartChunk)));
+    BIT_TRY(cres.ensure(ctx, (size_t)maxch * sizeof(bit::dev::ChunkResult)));
+    BIT_TRY(cbase.ensure(ctx, (size_t)maxch * sizeof(int2)));
+    bit::dev::BuildScratch w{progs.as<bit::dev::DevProg>(), ops.as<bit::dev::PartOpDev>(),
+                             nch.as<int>(), chunk0.as<int>(), chunks.as<bit::dev::PartChunk>(),
+                             nchunks.as<int>(), cres.as<bit::dev::ChunkResult>(),
+                             cbase.as<int2>(), ores.as<bit::dev::ChunkResult>(), nkid.as<int>(),
+                             start.as<int>(), result.as<bit::dev::GenResult>()};
+    HIP_TRY(bit::dev::build_generation(a, n, gen_lo, gen_hi, maxmemb, lim, node_view(), w, st));
+    HIP_TRY(hipMemcpyAsync(&gr, result.p, sizeof(gr), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (gr.failed) return fail(BIT_E_TREE, "tree split overflow (more than 8 children)");
+    if (gr.next > node_cap)
+      return fail(BIT_E_TREE,
+                  "tree exceeds 4n+64 nodes: particles cannot be separated into leaves of maxmemb");
+    maxdepth = std::max(maxdepth, gr.maxdepth);
+    gen_lo = gen_hi;
+    gen_hi = gr.next;
+    work = gr.splits > 0;
+  }
+  // the node table on the host
+  bit::HostTree& T = s->tree;
+  T = bit::HostTree();
+  T.n = n;
+  T.numtree = gen_hi;
+  T.maxdepth = maxdepth;
+  const int nt = gen_hi;
+  T.box.resize((size_t)nt * bit::BOX_FIELDS);
+  T.children.resize((size_t)nt * 8);
+  T.begin.resize(nt);
+  T.end.resize(nt);
+  T.depth.resize(nt);
+  std::vector<int> par(nt);
+  HIP_TRY(hipMemcpyAsync(T.box.data(), nbox.p, T.box.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(T.children.data(), nchild.p, T.children.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(T.begin.data(), nbegin.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(T.end.data(), nend.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(T.depth.data(), ndepth.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(par.data(), npar.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  double mn[3] = {hp[0], hp[1], hp[2]}, mx[3] = {hp[3], hp[4], hp[5]};
-  for (int b = 1; b < mblocks; b++)
-    for (int d = 0; d < 3; d++) {
-      mn[d] = std::min(mn[d], hp[6 * b + d]);
-      mx[d] = std::max(mx[d], hp[6 * b + 3 + d]);
-    }
-  DeviceEngine eng(ctx, a);
-  std::string err;
-  if (!bit::build_tree_generic(eng, n, mn, mx, maxmemb, s->tree, err))
-    return eng.status != BIT_OK ? eng.status : fail(BIT_E_TREE, err);
+  T.par.assign(par.begin(), par.end());
   // Morton order inside leaves (same keys as the host build), into the sorted
   // arrays from the buffer each leaf's slots are in
   std::vector<bit::dev::LeafBox> lb;
-  const bit::HostTree& T = s->tree;
   for (int i = 0; i < T.numtree; i++) {
     if (T.children[(size_t)i * 8] >= 0) continue;
     const double* f = &T.box[(size_t)i * bit::BOX_FIELDS];

@@ -93,16 +93,62 @@ hipError_t nonfinite(int n, const double* x, const double* y, const double* z, c
                      int* flag, hipStream_t s);
 hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int blocks,
                   double* part, hipStream_t s);
-// All split ops of one round, in two phases on one stream.  After
-// partition_count, ores[o] holds op o's count and boxes (what the host needs
-// for the next round); partition_move then moves the slots, and after it the
-// slot arrays hold the split ranges.
-hipError_t partition_count(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
-                           const PartChunk* chunks, const int* op_chunk0, const int* op_nchunk,
-                           ChunkResult* cres, int2* cbase, ChunkResult* ores, hipStream_t s);
-hipError_t partition_move(const SlotArrays& a, const PartOpDev* ops, int nchunks,
-                          const PartChunk* chunks, const int2* cbase, const ChunkResult* ores,
-                          hipStream_t s);
+// ---- tree build with the split decisions on the device ----
+// One slot of a node's children block and the node's split program (the
+// host's SplitProgram state, run by one device thread per node).
+struct DevSlot {
+  int b, e, par, used;
+  double f[13];  // TREE box fields
+};
+struct DevProg {
+  DevSlot s[16];
+  DevSlot parent, src;
+  int kids[8];
+  int nchild, pc, xc, yc, j, k, ind2, nchild2, made, c1, c2, failed, pending, maxmemb;
+  double lim;
+  PartOpDev op;
+};
+// Node table being built (capacity >= every node written).
+struct BuildNodes {
+  double* box;  // 13 per node
+  int* child;   // 8 per node
+  int* begin;
+  int* end;
+  int* depth;
+  int* par;
+};
+struct GenResult {
+  int next;      // end of the next generation (node count so far)
+  int failed;    // a program overflowed its children block or did not finish
+  int maxdepth;  // depth of the generation's children
+  int splits;    // children with more than maxmemb members (next generation's work)
+};
+// Per-generation work arrays, sized for cnt programs and
+// ceil(n / kPartChunk) + cnt chunks.
+struct BuildScratch {
+  DevProg* progs;
+  PartOpDev* ops;
+  int* nch;
+  int* chunk0;
+  PartChunk* chunks;
+  int* nchunks;
+  ChunkResult* cres;
+  int2* cbase;
+  ChunkResult* ores;
+  int* nkid;
+  int* start;
+  GenResult* result;
+};
+constexpr int kMaxSplitsPerNode = 7;  // x, then y and two z per x-child
+// Root node from k_minmax's mblocks partial boxes.
+hipError_t build_root(int n, int mblocks, const double* part, const BuildNodes& nd,
+                      hipStream_t s);
+// Nodes [gen_lo, gen_hi): split programs, kMaxSplitsPerNode partition rounds,
+// children written from gen_hi on; *w.result receives the next generation's end.
+hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int maxmemb,
+                            double lim, const BuildNodes& nd, const BuildScratch& w,
+                            hipStream_t s);
+
 // Morton order inside leaves, written to the first buffer (P..Z) from the
 // buffer each leaf's slots are in (every slot covered).
 hipError_t leaf_morton(const SlotArrays& a, int nleaves, const LeafBox* leaves, hipStream_t s);

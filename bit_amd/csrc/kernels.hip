@@ -611,9 +611,11 @@ hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int 
 // decides by select which box a slot widens.
 __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpDev* __restrict__ ops,
                                                     const PartChunk* __restrict__ chunks,
-                                                    ChunkResult* __restrict__ cres) {
+                                                    ChunkResult* __restrict__ cres,
+                                                    const int* __restrict__ nchunks) {
   __shared__ double red[12][256];
   __shared__ int cnt[256];
+  if (nchunks && (int)blockIdx.x >= *nchunks) return;  // grid sized for the most chunks
   const PartChunk ch = chunks[blockIdx.x];
   const PartOpDev op = ops[ch.op];
   const double* __restrict__ X = op.src ? sa.tX : sa.X;
@@ -741,8 +743,10 @@ __global__ __launch_bounds__(64) void k_part_offsets(const PartChunk* __restrict
 __global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartOpDev* __restrict__ ops,
                                                       const PartChunk* __restrict__ chunks,
                                                       const int2* __restrict__ cbase,
-                                                      const ChunkResult* __restrict__ ores) {
+                                                      const ChunkResult* __restrict__ ores,
+                                                      const int* __restrict__ nchunks) {
   __shared__ int wsum[4][4];  // [row][wave] counts of `<=` slots
+  if (nchunks && (int)blockIdx.x >= *nchunks) return;
   const PartChunk ch = chunks[blockIdx.x];
   const PartOpDev op = ops[ch.op];
   const SlotBuf a = slot_buf(sa, op.src), t = slot_buf(sa, op.src ^ 1);
@@ -802,21 +806,422 @@ __global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartO
   }
 }
 
-hipError_t partition_count(const SlotArrays& a, int nops, const PartOpDev* ops, int nchunks,
-                           const PartChunk* chunks, const int* op_chunk0, const int* op_nchunk,
-                           ChunkResult* cres, int2* cbase, ChunkResult* ores, hipStream_t s) {
-  if (nchunks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_part_count, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cres);
-  hipLaunchKernelGGL(k_part_offsets, dim3(nops), dim3(64), 0, s, chunks, op_chunk0, op_nchunk,
-                     cres, cbase, ores);
+// ---------------------------------------------------------------------------
+// Tree build with the split decisions on the device.  Every node of a
+// generation runs treemake's loop body (tree3d.cpp:1418-1545) as the same
+// resumable split program the host build uses, one thread per node: a round
+// applies each program's last split result, advances it to its next split
+// request (at most seven per node), and lays the requests out as chunks for
+// the partition kernels; after the last round the children are numbered in
+// node order (a prefix sum over the generation, as the reference numbers
+// them) and written to the node table.  The host only reads the generation's
+// size between generations.  The arithmetic that decides the tree (treeinit's
+// midpoints, aspect ratios and squared radius, the `<=` tests) rounds as the
+// reference's: no contraction.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kDone = -1;
+constexpr int kF = 13;  // box fields per node
+enum { FX0 = 0, FX1, FY0, FY1, FZ0, FZ1, FXMID, FYMID, FZMID, FAXY, FAXZ, FAYZ, FSQR };
+
+// treeinit (tree3d.cpp:1045-1073).
+__device__ void box_init_dev(double* f, double x0, double x1, double y0, double y1, double z0,
+                             double z1) {
+#pragma clang fp contract(off)
+  double x = 0.5 * (x1 + x0);
+  double y = 0.5 * (y1 + y0);
+  double z = 0.5 * (z1 + z0);
+  f[FX0] = x0; f[FX1] = x1; f[FY0] = y0; f[FY1] = y1; f[FZ0] = z0; f[FZ1] = z1;
+  f[FXMID] = x; f[FYMID] = y; f[FZMID] = z;
+  f[FAXY] = fabs((x1 - x0) / (y1 - y0));
+  f[FAXZ] = fabs((x1 - x0) / (z1 - z0));
+  f[FAYZ] = fabs((y1 - y0) / (z1 - z0));
+  x = x1 - x;
+  y = y1 - y;
+  z = z1 - z;
+  f[FSQR] = x * x + y * y + z * z;
+}
+
+__device__ bool want_y(const DevSlot& c, double lim, int maxmemb) {
+  return c.used && c.f[FAXY] < lim && c.f[FAYZ] > lim && (c.e - c.b) > maxmemb;
+}
+__device__ bool want_z(const DevSlot& c, double lim, int maxmemb) {
+  return c.used && c.f[FAXZ] < lim && c.f[FAYZ] < lim && (c.e - c.b) > maxmemb;
+}
+
+__device__ void request(DevProg& P, int dir, const DevSlot& src, int c1, int c2) {
+  P.src = src;
+  P.c1 = c1;
+  P.c2 = c2;
+  P.op = PartOpDev{src.b, src.e, dir, src.par, src.f[FXMID + dir]};
+}
+
+__device__ bool add_kid(DevProg& P, int slot) {
+  if (P.nchild >= 8) {
+    P.failed = 1;
+    P.pc = kDone;
+    return false;
+  }
+  P.kids[P.nchild++] = slot;
+  return true;
+}
+
+// Runs the program until it requests a split (returns) or ends (pc = kDone):
+// the same states and slot arithmetic as the host's SplitProgram::advance.
+__device__ void advance(DevProg& P) {
+  const double lim = P.lim;
+  const int mm = P.maxmemb;
+  for (;;) {
+    switch (P.pc) {
+      case 0:
+        if (P.parent.f[FAXY] > lim && P.parent.f[FAXZ] > lim) {
+          request(P, 0, P.parent, 0, 1);
+          P.pc = 1;
+        } else if (P.parent.f[FAXY] < lim && P.parent.f[FAYZ] > lim) {
+          request(P, 1, P.parent, 0, 1);
+          P.pc = 20;
+        } else {
+          request(P, 2, P.parent, 0, 1);
+          P.pc = 30;
+        }
+        return;
+      case 1:
+        P.xc = P.made;
+        P.ind2 = P.xc;
+        P.j = 0;
+        P.pc = 2;
+        break;
+      case 2:
+        if (P.j >= P.xc) {
+          P.pc = kDone;
+          return;
+        }
+        if (!add_kid(P, P.j)) return;
+        if (want_y(P.s[P.j], lim, mm)) {
+          request(P, 1, P.s[P.j], P.j, P.ind2);
+          P.pc = 3;
+          return;
+        }
+        if (want_z(P.s[P.j], lim, mm)) {
+          request(P, 2, P.s[P.j], P.j, P.ind2);
+          P.pc = 6;
+          return;
+        }
+        P.j++;
+        break;
+      case 3:
+        P.yc = P.made;
+        if (P.yc == 2) {
+          if (!add_kid(P, P.ind2)) return;
+          P.ind2++;
+        }
+        P.k = 0;
+        P.pc = 4;
+        break;
+      case 4: {
+        if (P.k >= P.yc) {
+          P.j++;
+          P.pc = 2;
+          break;
+        }
+        const int t = P.j + P.k * (P.nchild2 + 1);
+        if (t < 16 && want_z(P.s[t], lim, mm)) {
+          request(P, 2, P.s[t], t, P.ind2);
+          P.pc = 5;
+          return;
+        }
+        P.k++;
+        break;
+      }
+      case 5:
+        if (P.made == 2) {
+          if (!add_kid(P, P.ind2)) return;
+          P.nchild2++;
+          P.ind2++;
+        }
+        P.k++;
+        P.pc = 4;
+        break;
+      case 6:
+        if (P.made == 2) {
+          if (!add_kid(P, P.ind2)) return;
+          P.ind2++;
+        }
+        P.j++;
+        P.pc = 2;
+        break;
+      case 20:
+        P.yc = P.made;
+        P.ind2 = P.yc;
+        P.j = 0;
+        P.pc = 21;
+        break;
+      case 21:
+        if (P.j >= P.yc) {
+          P.pc = kDone;
+          return;
+        }
+        if (!add_kid(P, P.j)) return;
+        if (want_z(P.s[P.j], lim, mm)) {
+          request(P, 2, P.s[P.j], P.j, P.ind2);
+          P.pc = 22;
+          return;
+        }
+        P.j++;
+        break;
+      case 22:
+        if (P.made == 2) {
+          if (!add_kid(P, P.ind2)) return;
+          P.ind2++;
+        }
+        P.j++;
+        P.pc = 21;
+        break;
+      case 30:
+        for (int jj = 0; jj < P.made; jj++)
+          if (!add_kid(P, jj)) return;
+        P.pc = kDone;
+        return;
+      default:
+        return;
+    }
+    if (P.ind2 >= 15) {
+      P.failed = 1;
+      P.pc = kDone;
+      return;
+    }
+  }
+}
+
+// treedivide's outcome: the first non-empty side goes to c1, the second to c2.
+__device__ void apply(DevProg& P, const ChunkResult& r) {
+  const int lo[2] = {P.src.b, P.src.b + r.nl}, hi[2] = {P.src.b + r.nl, P.src.e};
+  int target = P.c1;
+  P.made = 0;
+  for (int sd = 0; sd < 2; sd++) {
+    if (hi[sd] > lo[sd]) {
+      DevSlot& t = P.s[target];
+      t.used = 1;
+      t.b = lo[sd];
+      t.e = hi[sd];
+      t.par = P.src.par ^ 1;
+      // r.v: min side0 xyz, min side1 xyz, max side0 xyz, max side1 xyz
+      box_init_dev(t.f, r.v[3 * sd], r.v[6 + 3 * sd], r.v[3 * sd + 1], r.v[6 + 3 * sd + 1],
+                   r.v[3 * sd + 2], r.v[6 + 3 * sd + 2]);
+      P.made++;
+      target = P.c2;
+    }
+  }
+  advance(P);
+}
+
+// Exclusive scan of v[0..cnt) by one 1024-thread work-group, in index order;
+// returns the total.  `red` is a 1024-int LDS array.
+__device__ int block_scan(int* v, int* out, int cnt, int* red) {
+  int carry = 0;
+  for (int base = 0; base < cnt; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int x = i < cnt ? v[i] : 0;
+    red[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int y = threadIdx.x >= (unsigned)off ? red[threadIdx.x - off] : 0;
+      __syncthreads();
+      red[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < cnt) out[i] = carry + red[threadIdx.x] - x;
+    const int tot = red[1023];
+    __syncthreads();
+    carry += tot;
+  }
+  return carry;
+}
+
+}  // namespace
+
+__global__ void k_root_init(int n, int mblocks, const double* __restrict__ part, BuildNodes nd) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double mn[3] = {part[0], part[1], part[2]}, mx[3] = {part[3], part[4], part[5]};
+  for (int b = 1; b < mblocks; b++)
+    for (int d = 0; d < 3; d++) {
+      mn[d] = fmin(mn[d], part[6 * b + d]);
+      mx[d] = fmax(mx[d], part[6 * b + 3 + d]);
+    }
+  box_init_dev(nd.box, mn[0], mx[0], mn[1], mx[1], mn[2], mx[2]);  // tree3d.cpp:1385-1403
+  for (int c = 0; c < 8; c++) nd.child[c] = -1;
+  nd.begin[0] = 0;
+  nd.end[0] = n;
+  nd.depth[0] = 0;
+  nd.par[0] = 0;
+}
+
+__global__ __launch_bounds__(64) void k_prog_init(int gen_lo, int cnt, BuildNodes nd,
+                                                  DevProg* __restrict__ progs, int maxmemb,
+                                                  double lim, GenResult* __restrict__ res) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx == 0) res->failed = 0;
+  if (idx >= cnt) return;
+  DevProg& P = progs[idx];
+  const int i = gen_lo + idx;
+  P.nchild = 0;
+  P.failed = 0;
+  P.pending = 0;
+  P.made = 0;
+  P.nchild2 = 0;
+  P.xc = P.yc = P.j = P.k = P.ind2 = 0;
+  P.maxmemb = maxmemb;
+  P.lim = lim;
+  for (int sl = 0; sl < 16; sl++) P.s[sl].used = 0;
+  if (nd.end[i] - nd.begin[i] <= maxmemb) {
+    P.pc = kDone;
+    return;
+  }
+  P.parent.used = 1;
+  P.parent.b = nd.begin[i];
+  P.parent.e = nd.end[i];
+  P.parent.par = nd.par[i];
+  for (int k = 0; k < kF; k++) P.parent.f[k] = nd.box[(size_t)i * kF + k];
+  P.pc = 0;
+  advance(P);
+  P.pending = 0;
+}
+
+// One round, per program: apply last round's result, advance to the next
+// request (op index = program index), and its chunk count.
+__global__ __launch_bounds__(64) void k_prog_advance(int cnt, DevProg* __restrict__ progs,
+                                                     const ChunkResult* __restrict__ ores,
+                                                     PartOpDev* __restrict__ ops,
+                                                     int* __restrict__ nch) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx >= cnt) return;
+  DevProg& P = progs[idx];
+  if (P.pending) apply(P, ores[idx]);
+  int c = 0;
+  if (P.pc != kDone) {
+    ops[idx] = P.op;
+    c = (P.op.e - P.op.b + kPartChunk - 1) / kPartChunk;
+    P.pending = 1;
+  } else {
+    P.pending = 0;
+  }
+  nch[idx] = c;
+}
+
+// The round's requests as chunks of at most kPartChunk slots, in program order.
+__global__ __launch_bounds__(1024) void k_prog_layout(int cnt, const PartOpDev* __restrict__ ops,
+                                                      int* __restrict__ nch,
+                                                      int* __restrict__ chunk0,
+                                                      PartChunk* __restrict__ chunks,
+                                                      int* __restrict__ nchunks) {
+  __shared__ int red[1024];
+  const int total = block_scan(nch, chunk0, cnt, red);
+  for (int idx = threadIdx.x; idx < cnt; idx += 1024) {
+    const int c = nch[idx];
+    if (!c) continue;
+    const PartOpDev o = ops[idx];
+    for (int k = 0; k < c; k++) {
+      const int s0 = o.b + k * kPartChunk;
+      chunks[chunk0[idx] + k] = PartChunk{idx, s0, min(o.e, s0 + kPartChunk), 0};
+    }
+  }
+  if (threadIdx.x == 0) *nchunks = total;
+}
+
+// After the generation's last round, per program: the last result applied;
+// the number of children (0 and a failure flag for a program that overflowed
+// its block or did not finish).
+__global__ __launch_bounds__(64) void k_prog_last(int cnt, DevProg* __restrict__ progs,
+                                                  const ChunkResult* __restrict__ ores,
+                                                  int* __restrict__ nkid,
+                                                  GenResult* __restrict__ res) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx >= cnt) return;
+  DevProg& P = progs[idx];
+  if (P.pending) apply(P, ores[idx]);
+  P.pending = 0;
+  const bool bad = P.failed || P.pc != kDone;
+  if (bad) atomicOr(&res->failed, 1);
+  nkid[idx] = bad ? 0 : P.nchild;
+}
+
+// Children numbered in node order (a prefix sum over the generation) and
+// written to the node table.
+__global__ __launch_bounds__(1024) void k_prog_number(int gen_lo, int gen_hi, int cnt,
+                                                      int maxmemb,
+                                                      const DevProg* __restrict__ progs,
+                                                      BuildNodes nd, int* __restrict__ nkid,
+                                                      int* __restrict__ start,
+                                                      GenResult* __restrict__ res) {
+  __shared__ int red[1024];
+  __shared__ int maxd, big;
+  if (threadIdx.x == 0) {
+    maxd = 0;
+    big = 0;
+  }
+  __syncthreads();
+  const int total = block_scan(nkid, start, cnt, red);
+  for (int idx = threadIdx.x; idx < cnt; idx += 1024) {
+    const DevProg& P = progs[idx];
+    const int nc = nkid[idx];
+    if (!nc) continue;
+    const int parent = gen_lo + idx;
+    const int st = gen_hi + start[idx];
+    const int d = nd.depth[parent] + 1;
+    for (int sl = 0; sl < nc; sl++) {
+      const int g = st + sl;
+      for (int k = 0; k < kF; k++) nd.box[(size_t)g * kF + k] = P.s[sl].f[k];
+      nd.begin[g] = P.s[sl].b;
+      nd.end[g] = P.s[sl].e;
+      nd.depth[g] = d;
+      nd.par[g] = P.s[sl].par;
+      for (int c = 0; c < 8; c++) nd.child[(size_t)g * 8 + c] = -1;
+      if (P.s[sl].e - P.s[sl].b > maxmemb) atomicAdd(&big, 1);
+    }
+    for (int c = 0; c < nc; c++) nd.child[(size_t)parent * 8 + c] = st + P.kids[c];
+    atomicMax(&maxd, d);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    res->next = gen_hi + total;
+    res->maxdepth = maxd;
+    res->splits = big;
+  }
+}
+
+hipError_t build_root(int n, int mblocks, const double* part, const BuildNodes& nd,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_root_init, dim3(1), dim3(64), 0, s, n, mblocks, part, nd);
   return hipGetLastError();
 }
 
-hipError_t partition_move(const SlotArrays& a, const PartOpDev* ops, int nchunks,
-                          const PartChunk* chunks, const int2* cbase, const ChunkResult* ores,
-                          hipStream_t s) {
-  if (nchunks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_part_scatter, dim3(nchunks), dim3(256), 0, s, a, ops, chunks, cbase, ores);
+hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int maxmemb,
+                            double lim, const BuildNodes& nd, const BuildScratch& w,
+                            hipStream_t s) {
+  const int cnt = gen_hi - gen_lo;
+  const int maxch = (n + kPartChunk - 1) / kPartChunk + cnt;
+  const int pblocks = (cnt + 63) / 64;
+  hipLaunchKernelGGL(k_prog_init, dim3(pblocks), dim3(64), 0, s, gen_lo, cnt, nd, w.progs,
+                     maxmemb, lim, w.result);
+  for (int r = 0; r < kMaxSplitsPerNode; r++) {
+    hipLaunchKernelGGL(k_prog_advance, dim3(pblocks), dim3(64), 0, s, cnt, w.progs,
+                       (const ChunkResult*)w.ores, w.ops, w.nch);
+    hipLaunchKernelGGL(k_prog_layout, dim3(1), dim3(1024), 0, s, cnt, (const PartOpDev*)w.ops,
+                       w.nch, w.chunk0, w.chunks, w.nchunks);
+    hipLaunchKernelGGL(k_part_count, dim3(maxch), dim3(256), 0, s, a, (const PartOpDev*)w.ops,
+                       (const PartChunk*)w.chunks, w.cres, (const int*)w.nchunks);
+    hipLaunchKernelGGL(k_part_offsets, dim3(cnt), dim3(64), 0, s, (const PartChunk*)w.chunks,
+                       (const int*)w.chunk0, (const int*)w.nch, (const ChunkResult*)w.cres,
+                       w.cbase, w.ores);
+    hipLaunchKernelGGL(k_part_scatter, dim3(maxch), dim3(256), 0, s, a, (const PartOpDev*)w.ops,
+                       (const PartChunk*)w.chunks, (const int2*)w.cbase,
+                       (const ChunkResult*)w.ores, (const int*)w.nchunks);
+  }
+  hipLaunchKernelGGL(k_prog_last, dim3(pblocks), dim3(64), 0, s, cnt, w.progs,
+                     (const ChunkResult*)w.ores, w.nkid, w.result);
+  hipLaunchKernelGGL(k_prog_number, dim3(1), dim3(1024), 0, s, gen_lo, gen_hi, cnt, maxmemb,
+                     (const DevProg*)w.progs, nd, w.nkid, w.start, w.result);
   return hipGetLastError();
 }
 
