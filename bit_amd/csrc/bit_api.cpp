@@ -368,7 +368,7 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
   BIT_TRY(result.ensure(ctx, sizeof(bit::dev::GenResult)));
   BIT_TRY(nchunks.ensure(ctx, sizeof(int)));
   bit::dev::GenResult gr{};
-  int gen_lo = 0, gen_hi = 1, maxdepth = 0;
+  int gen_lo = 0, gen_hi = 1, maxdepth = 0, maxsize = n;
   bool work = n > maxmemb;
   while (work) {
     const int cnt = gen_hi - gen_lo;
@@ -389,7 +389,8 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
                              nchunks.as<int>(), cres.as<bit::dev::ChunkResult>(),
                              cbase.as<int2>(), ores.as<bit::dev::ChunkResult>(), nkid.as<int>(),
                              start.as<int>(), result.as<bit::dev::GenResult>()};
-    HIP_TRY(bit::dev::build_generation(a, n, gen_lo, gen_hi, maxmemb, lim, node_view(), w, st));
+    HIP_TRY(bit::dev::build_generation(a, n, gen_lo, gen_hi, maxmemb, lim,
+                                       maxsize <= bit::dev::kNodeProgramMax, node_view(), w, st));
     HIP_TRY(hipMemcpyAsync(&gr, result.p, sizeof(gr), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (gr.failed) return fail(BIT_E_TREE, "tree split overflow (more than 8 children)");
@@ -400,6 +401,7 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
     gen_lo = gen_hi;
     gen_hi = gr.next;
     work = gr.splits > 0;
+    maxsize = gr.maxsize;
   }
   // the node table on the host
   bit::HostTree& T = s->tree;

@@ -122,7 +122,13 @@ struct GenResult {
   int failed;    // a program overflowed its children block or did not finish
   int maxdepth;  // depth of the generation's children
   int splits;    // children with more than maxmemb members (next generation's work)
+  int maxsize;   // the largest of them
 };
+// Generations whose nodes all hold at most this many slots run one work-group
+// per node (k_node_program) instead of rounds across the generation (4M-blob
+// tree: 6.50 ms with rounds only; 5.90 / 5.86 / 5.73 / 6.34 ms at 16K / 32K /
+// 64K / 128K).
+constexpr int kNodeProgramMax = 65536;
 // Per-generation work arrays, sized for cnt programs and
 // ceil(n / kPartChunk) + cnt chunks.
 struct BuildScratch {
@@ -143,11 +149,12 @@ constexpr int kMaxSplitsPerNode = 7;  // x, then y and two z per x-child
 // Root node from k_minmax's mblocks partial boxes.
 hipError_t build_root(int n, int mblocks, const double* part, const BuildNodes& nd,
                       hipStream_t s);
-// Nodes [gen_lo, gen_hi): split programs, kMaxSplitsPerNode partition rounds,
-// children written from gen_hi on; *w.result receives the next generation's end.
+// Nodes [gen_lo, gen_hi): split programs, run in kMaxSplitsPerNode partition
+// rounds across the generation or (per_node) one work-group per node; children
+// written from gen_hi on; *w.result receives the next generation's end.
 hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int maxmemb,
-                            double lim, const BuildNodes& nd, const BuildScratch& w,
-                            hipStream_t s);
+                            double lim, bool per_node, const BuildNodes& nd,
+                            const BuildScratch& w, hipStream_t s);
 
 // Morton order inside leaves, written to the first buffer (P..Z) from the
 // buffer each leaf's slots are in (every slot covered).

@@ -606,21 +606,13 @@ hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int 
   return hipGetLastError();
 }
 
-// Per chunk: how many slots go to the `<=` side, and both sides' boxes.
-// Four slots per thread and pass (their loads in flight together); the side
-// decides by select which box a slot widens.
-__global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpDev* __restrict__ ops,
-                                                    const PartChunk* __restrict__ chunks,
-                                                    ChunkResult* __restrict__ cres,
-                                                    const int* __restrict__ nchunks) {
-  __shared__ double red[12][256];
-  __shared__ int cnt[256];
-  if (nchunks && (int)blockIdx.x >= *nchunks) return;  // grid sized for the most chunks
-  const PartChunk ch = chunks[blockIdx.x];
-  const PartOpDev op = ops[ch.op];
-  const double* __restrict__ X = op.src ? sa.tX : sa.X;
-  const double* __restrict__ Y = op.src ? sa.tY : sa.Y;
-  const double* __restrict__ Z = op.src ? sa.tZ : sa.Z;
+// Slots [s0, s1) of one split by a 256-thread work-group: how many go to the
+// `<=` side, and both sides' boxes (into *out, written by thread 0; the block
+// is synchronised on return).  Four slots per thread and pass (their loads in
+// flight together); the side decides by select which box a slot widens.
+__device__ __forceinline__ void count_range(const double* __restrict__ X, const double* __restrict__ Y,
+                            const double* __restrict__ Z, int s0, int s1, int dir, double mid,
+                            double (*red)[256], int* cnt, ChunkResult* out) {
   int c = 0;
   // v[k]: min side 0 xyz, min side 1 xyz, max side 0 xyz, max side 1 xyz
   double v[12];
@@ -629,14 +621,14 @@ __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpD
     v[k] = INFINITY;
     v[6 + k] = -INFINITY;
   }
-  for (int i0 = ch.s0 + threadIdx.x; i0 < ch.s1; i0 += 4 * 256) {
+  for (int i0 = s0 + threadIdx.x; i0 < s1; i0 += 4 * 256) {
     double px[4], py[4], pz[4];
     bool in[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int i = i0 + u * 256;
-      in[u] = i < ch.s1;
-      const int j = in[u] ? i : ch.s0;
+      in[u] = i < s1;
+      const int j = in[u] ? i : s0;
       px[u] = X[j];
       py[u] = Y[j];
       pz[u] = Z[j];
@@ -645,8 +637,8 @@ __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpD
     for (int u = 0; u < 4; u++) {
       if (!in[u]) continue;
       const double p[3] = {px[u], py[u], pz[u]};
-      const double cv = op.dir == 0 ? p[0] : (op.dir == 1 ? p[1] : p[2]);
-      const bool left = cv <= op.mid;
+      const double cv = dir == 0 ? p[0] : (dir == 1 ? p[1] : p[2]);
+      const bool left = cv <= mid;
       c += left;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
@@ -676,8 +668,23 @@ __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpD
     r.nl = cnt[0];
     r.pad = 0;
     for (int k = 0; k < 12; k++) r.v[k] = red[k][0];
-    cres[blockIdx.x] = r;
+    *out = r;
   }
+  __syncthreads();
+}
+
+// Per chunk: count_range of the chunk's slots.
+__global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpDev* __restrict__ ops,
+                                                    const PartChunk* __restrict__ chunks,
+                                                    ChunkResult* __restrict__ cres,
+                                                    const int* __restrict__ nchunks) {
+  __shared__ double red[12][256];
+  __shared__ int cnt[256];
+  if (nchunks && (int)blockIdx.x >= *nchunks) return;  // grid sized for the most chunks
+  const PartChunk ch = chunks[blockIdx.x];
+  const PartOpDev op = ops[ch.op];
+  count_range(op.src ? sa.tX : sa.X, op.src ? sa.tY : sa.Y, op.src ? sa.tZ : sa.Z, ch.s0, ch.s1,
+              op.dir, op.mid, red, cnt, cres + blockIdx.x);
 }
 
 // Per op (one thread): chunk bases in chunk order, totals and boxes.
@@ -736,40 +743,32 @@ __global__ __launch_bounds__(64) void k_part_offsets(const PartChunk* __restrict
   }
 }
 
-// Stable scatter of each chunk into the other buffer.  Tiles of 1024 slots,
-// four rows of 256 (slot t0 + 256 u + thread); a slot's rank among its tile's
-// `<=` slots comes from the wave ballots of all four rows, one barrier per
-// tile, and every load of the tile is in flight before it.
-__global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartOpDev* __restrict__ ops,
-                                                      const PartChunk* __restrict__ chunks,
-                                                      const int2* __restrict__ cbase,
-                                                      const ChunkResult* __restrict__ ores,
-                                                      const int* __restrict__ nchunks) {
-  __shared__ int wsum[4][4];  // [row][wave] counts of `<=` slots
-  if (nchunks && (int)blockIdx.x >= *nchunks) return;
-  const PartChunk ch = chunks[blockIdx.x];
-  const PartOpDev op = ops[ch.op];
-  const SlotBuf a = slot_buf(sa, op.src), t = slot_buf(sa, op.src ^ 1);
-  const double* __restrict__ C = op.dir == 0 ? a.X : (op.dir == 1 ? a.Y : a.Z);
-  const int opnl = ores[ch.op].nl;
-  const int2 base = cbase[blockIdx.x];
-  int lrun = base.x, rrun = base.y;
+// Stable scatter of slots [s0, s1) of one split into the other buffer by a
+// 256-thread work-group: the `<=` slots to op_b + lrun + rank, the others to
+// op_b + opnl + rrun + rank.  Tiles of 1024 slots, four rows of 256 (slot
+// t0 + 256 u + thread); a slot's rank among its tile's `<=` slots comes from
+// the wave ballots of all four rows, one barrier per tile, and every load of
+// the tile is in flight before it.  The block is synchronised on return.
+__device__ __forceinline__ void scatter_range(const SlotBuf& a, const SlotBuf& t, int s0, int s1, int dir,
+                              double mid, int op_b, int opnl, int lrun, int rrun,
+                              int (*wsum)[4]) {
+  const double* __restrict__ C = dir == 0 ? a.X : (dir == 1 ? a.Y : a.Z);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (int t0 = ch.s0; t0 < ch.s1; t0 += 1024) {
+  for (int t0 = s0; t0 < s1; t0 += 1024) {
     bool in[4], left[4];
     int before[4], pv[4];
     double xv[4], yv[4], zv[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int i = t0 + u * 256 + threadIdx.x;
-      in[u] = i < ch.s1;
-      const int j = in[u] ? i : ch.s0;
+      in[u] = i < s1;
+      const int j = in[u] ? i : s0;
       pv[u] = a.P[j];
       xv[u] = a.X[j];
       yv[u] = a.Y[j];
       zv[u] = a.Z[j];
-      left[u] = in[u] && C[j] <= op.mid;
+      left[u] = in[u] && C[j] <= mid;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -791,7 +790,7 @@ __global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartO
       if (in[u]) {
         const int lrank = rowoff + woff + before[u];
         const int pos = u * 256 + threadIdx.x;  // slot's place in the tile
-        const int dst = left[u] ? op.b + lrun + lrank : op.b + opnl + rrun + (pos - lrank);
+        const int dst = left[u] ? op_b + lrun + lrank : op_b + opnl + rrun + (pos - lrank);
         t.P[dst] = pv[u];
         t.X[dst] = xv[u];
         t.Y[dst] = yv[u];
@@ -799,11 +798,26 @@ __global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartO
       }
       rowoff += tot;
     }
-    const int ntile = min(1024, ch.s1 - t0);
+    const int ntile = min(1024, s1 - t0);
     lrun += rowoff;
     rrun += ntile - rowoff;
     __syncthreads();  // wsum is rewritten by the next tile
   }
+}
+
+// Per chunk: scatter_range from the chunk's bases in its op's two runs.
+__global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartOpDev* __restrict__ ops,
+                                                      const PartChunk* __restrict__ chunks,
+                                                      const int2* __restrict__ cbase,
+                                                      const ChunkResult* __restrict__ ores,
+                                                      const int* __restrict__ nchunks) {
+  __shared__ int wsum[4][4];  // [row][wave] counts of `<=` slots
+  if (nchunks && (int)blockIdx.x >= *nchunks) return;
+  const PartChunk ch = chunks[blockIdx.x];
+  const PartOpDev op = ops[ch.op];
+  const int2 base = cbase[blockIdx.x];
+  scatter_range(slot_buf(sa, op.src), slot_buf(sa, op.src ^ 1), ch.s0, ch.s1, op.dir, op.mid,
+                op.b, ores[ch.op].nl, base.x, base.y, wsum);
 }
 
 // ---------------------------------------------------------------------------
@@ -1155,10 +1169,11 @@ __global__ __launch_bounds__(1024) void k_prog_number(int gen_lo, int gen_hi, in
                                                       int* __restrict__ start,
                                                       GenResult* __restrict__ res) {
   __shared__ int red[1024];
-  __shared__ int maxd, big;
+  __shared__ int maxd, big, bigsize;
   if (threadIdx.x == 0) {
     maxd = 0;
     big = 0;
+    bigsize = 0;
   }
   __syncthreads();
   const int total = block_scan(nkid, start, cnt, red);
@@ -1177,7 +1192,10 @@ __global__ __launch_bounds__(1024) void k_prog_number(int gen_lo, int gen_hi, in
       nd.depth[g] = d;
       nd.par[g] = P.s[sl].par;
       for (int c = 0; c < 8; c++) nd.child[(size_t)g * 8 + c] = -1;
-      if (P.s[sl].e - P.s[sl].b > maxmemb) atomicAdd(&big, 1);
+      if (P.s[sl].e - P.s[sl].b > maxmemb) {
+        atomicAdd(&big, 1);
+        atomicMax(&bigsize, P.s[sl].e - P.s[sl].b);
+      }
     }
     for (int c = 0; c < nc; c++) nd.child[(size_t)parent * 8 + c] = st + P.kids[c];
     atomicMax(&maxd, d);
@@ -1187,6 +1205,37 @@ __global__ __launch_bounds__(1024) void k_prog_number(int gen_lo, int gen_hi, in
     res->next = gen_hi + total;
     res->maxdepth = maxd;
     res->splits = big;
+    res->maxsize = bigsize;
+  }
+}
+
+// A generation of small nodes: one work-group runs one node's whole split
+// program, each requested split a count pass and a stable scatter over the
+// node's own range (no rounds across the generation).
+__global__ __launch_bounds__(256) void k_node_program(SlotArrays sa, int cnt,
+                                                      DevProg* __restrict__ progs) {
+  __shared__ double red[12][256];
+  __shared__ int cnts[256];
+  __shared__ int wsum[4][4];
+  __shared__ ChunkResult res;
+  __shared__ PartOpDev op;
+  __shared__ int done;
+  const int idx = blockIdx.x;
+  if (idx >= cnt) return;
+  DevProg& P = progs[idx];
+  for (;;) {
+    if (threadIdx.x == 0) {
+      done = P.pc == kDone;
+      op = P.op;
+    }
+    __syncthreads();
+    if (done) break;
+    const PartOpDev o = op;
+    const SlotBuf a = slot_buf(sa, o.src);
+    count_range(a.X, a.Y, a.Z, o.b, o.e, o.dir, o.mid, red, cnts, &res);
+    scatter_range(a, slot_buf(sa, o.src ^ 1), o.b, o.e, o.dir, o.mid, o.b, res.nl, 0, 0, wsum);
+    if (threadIdx.x == 0) apply(P, res);
+    __syncthreads();
   }
 }
 
@@ -1197,14 +1246,16 @@ hipError_t build_root(int n, int mblocks, const double* part, const BuildNodes& 
 }
 
 hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int maxmemb,
-                            double lim, const BuildNodes& nd, const BuildScratch& w,
-                            hipStream_t s) {
+                            double lim, bool per_node, const BuildNodes& nd,
+                            const BuildScratch& w, hipStream_t s) {
   const int cnt = gen_hi - gen_lo;
   const int maxch = (n + kPartChunk - 1) / kPartChunk + cnt;
   const int pblocks = (cnt + 63) / 64;
   hipLaunchKernelGGL(k_prog_init, dim3(pblocks), dim3(64), 0, s, gen_lo, cnt, nd, w.progs,
                      maxmemb, lim, w.result);
-  for (int r = 0; r < kMaxSplitsPerNode; r++) {
+  if (per_node)
+    hipLaunchKernelGGL(k_node_program, dim3(cnt), dim3(256), 0, s, a, cnt, w.progs);
+  for (int r = 0; r < (per_node ? 0 : kMaxSplitsPerNode); r++) {
     hipLaunchKernelGGL(k_prog_advance, dim3(pblocks), dim3(64), 0, s, cnt, w.progs,
                        (const ChunkResult*)w.ores, w.ops, w.nch);
     hipLaunchKernelGGL(k_prog_layout, dim3(1), dim3(1024), 0, s, cnt, (const PartOpDev*)w.ops,
