@@ -18,6 +18,8 @@
 //                                     charge sets (treemoments after each)
 //   dropin_treecode points_ds IN OUT  force_point_ds_noreg/_reg, shielded_pot_particle_ds,
 //                                     freespace_particles_ds_noreg/_reg
+//   dropin_treecode primitives IN OUT one-cluster Taylor sums and one-pair gradients
+//                                     (host only)
 // OUT receives raw doubles; "PASSED"/"FAILED" on stdout as the reference tests.
 #include <omp.h>
 
@@ -367,6 +369,44 @@ static int points_ds_mode(const char* in, const char* out) {
   return 0;
 }
 
+// The one-cluster and one-pair primitives (host functions of the drop-in; no
+// device).  IN: nterms, kappa, ncase, then per case dx, dy, dz, rs and the
+// moment cube [nterms]^3.  OUT per case: shielded_treetaylorpot's pot (from 0),
+// treetaylorpotential's pot (from 0), eval_grad_kernel_noreg (3) and
+// eval_grad_kernel_reg (3) at (dx, dy, dz, rs).
+static int primitives_mode(const char* in, const char* out) {
+  FILE* f = fopen(in, "rb");
+  double hdr[3];
+  if (fread(hdr, sizeof(double), 3, f) != 3) return 2;
+  const int nt = (int)hdr[0], ncase = (int)hdr[2];
+  const double kappa = hdr[1];
+  std::vector<double> res;
+  std::vector<double> cube((size_t)nt * nt * nt);
+  std::vector<double*> rows((size_t)nt * nt);
+  std::vector<double**> planes(nt);
+  for (int i = 0; i < nt; i++) {
+    planes[i] = &rows[(size_t)i * nt];
+    for (int j = 0; j < nt; j++) rows[(size_t)i * nt + j] = &cube[((size_t)i * nt + j) * nt];
+  }
+  for (int c = 0; c < ncase; c++) {
+    double d[4];
+    if (fread(d, sizeof(double), 4, f) != 4) return 2;
+    if (fread(cube.data(), sizeof(double), cube.size(), f) != cube.size()) return 2;
+    double p1 = 0.0, p2 = 0.0, g[6];
+    shielded_treetaylorpot(nt, d[0], d[1], d[2], d[3], &p1, planes.data(), kappa);
+    treetaylorpotential(nt, d[0], d[1], d[2], d[3], &p2, planes.data());
+    eval_grad_kernel_noreg(d[0], d[1], d[2], d[3], &g[0], &g[1], &g[2]);
+    eval_grad_kernel_reg(d[0], d[1], d[2], d[3], &g[3], &g[4], &g[5]);
+    res.push_back(p1);
+    res.push_back(p2);
+    res.insert(res.end(), g, g + 6);
+  }
+  fclose(f);
+  write_all(out, res);
+  printf("Test PASSED\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc >= 3 && std::string(argv[1]) == "field") return field_mode(argv[2]);
   if (argc >= 3 && std::string(argv[1]) == "conv") return conv_mode(argv[2]);
@@ -376,6 +416,7 @@ int main(int argc, char** argv) {
   if (argc >= 3 && std::string(argv[1]) == "treecode") return treecode_mode(argv[2]);
   if (argc >= 4 && std::string(argv[1]) == "recharge") return recharge_mode(argv[2], argv[3]);
   if (argc >= 4 && std::string(argv[1]) == "points_ds") return points_ds_mode(argv[2], argv[3]);
+  if (argc >= 4 && std::string(argv[1]) == "primitives") return primitives_mode(argv[2], argv[3]);
   fprintf(stderr,
           "usage: dropin_treecode field|conv|directsum|treecode OUT | "
           "screened|points|recharge|points_ds IN OUT\n");
