@@ -107,6 +107,47 @@ def cpu_baseline(P, nterms, acc, kappa, d2, maxmemb, sample: int, seed: int = 5)
     }
 
 
+def field_timings(ctx, log):
+    """The field path (treeforce, regularized Coulomb with DEL = 0.01 and the
+    MAC's DEL shift) timed as whole evaluations (tree, moments, lists, far,
+    near) from HBM-resident inputs: tests/treecode_mp.cpp's size (16,000
+    uniform particles, maxmemb 100, nterms 5) and 1M uniform particles
+    (maxmemb 1000, nterms 9), acc 0.5, FP64 and FP32; the FP32 field's
+    relative L2 distance to the FP64 one is reported."""
+    import bit_amd
+    from bit_amd import workloads
+    out = {}
+    for name, n, mm, nt in (("uniform16k", 16_000, 100, 5), ("uniform1m", 1_000_000, 1000, 9)):
+        P = workloads.uniform_cube(n, seed=21)
+        dev = [bit_amd.DeviceArray.from_numpy(ctx, a) for a in (P.x, P.y, P.z, P.q)]
+        rec = {"n": n, "maxmemb": mm, "nterms": nt, "acc": 0.5, "delta2": 0.01}
+        E = {}
+        for prec in ("fp64", "fp32"):
+            def one():
+                s = bit_amd.System(ctx, *dev, mm)
+                e = s.field(nt, 0.5, 0.01, 0.01, precision=prec)
+                st = s.stats()
+                s.close()
+                return e, st
+            one()
+            ctx.sync()
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                E[prec], st = one()
+            ms = 1e3 * (time.perf_counter() - t0) / reps
+            rec[prec] = {"ms_per_evaluation": ms, "targets_per_s": n / (ms * 1e-3),
+                         "ms_far": st["ms_far"], "ms_near": st["ms_near"]}
+        rec["fp32_rel_l2_vs_fp64"] = float(np.linalg.norm(E["fp32"] - E["fp64"]) /
+                                           np.linalg.norm(E["fp64"]))
+        for d in dev:
+            d.free()
+        log(f"[bench] field {name}: fp64 {rec['fp64']['ms_per_evaluation']:.2f} ms, fp32 "
+            f"{rec['fp32']['ms_per_evaluation']:.2f} ms (fp32 vs fp64 {rec['fp32_rel_l2_vs_fp64']:.1e})")
+        out[name] = rec
+    return out
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -140,6 +181,8 @@ def main():
                          "step (the autotuner's FP64 point at 1e-6)")
     ap.add_argument("--no-literal", action="store_true",
                     help="skip the literal-parameter runs (p 8, acc 0.7, N0 2000)")
+    ap.add_argument("--no-field", action="store_true",
+                    help="skip the field (treeforce) timings at 16k and 1M particles")
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=400.0)
@@ -296,6 +339,10 @@ def main():
                                       "tflops": rec / (kms * 1e-3) / 1e12,
                                       "frac": rec / (kms * 1e-3) / peak}
 
+    field = None
+    if not args.no_field:
+        field = field_timings(ctx, log)
+
     serving = None
     if args.serving_requests > 0:
         from bit_amd import serving as srv_mod
@@ -344,6 +391,7 @@ def main():
         "fp64_ms_per_step": fp64_ms,
         "fp64": fp64,
         "literal": literal,
+        "field": field,
         "roofline": roof,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -528,10 +528,11 @@ class System:
         self.ctx.sync()
         return out
 
-    def field(self, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0):
+    def field(self, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0,
+              precision="fp64"):
         """E = sum q (x_i - x_j)/R^3 (regularized Coulomb), [n, 3], original order;
         kernel_order >= 1 adds the regularized family's higher terms."""
-        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64", kernel_order)
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, precision, kernel_order)
         res = np.zeros(3 * self.n)
         _check(lib.bit_eval_field(self.h, C.byref(prm), _ptr(res), 0))
         return res.reshape(self.n, 3)
@@ -546,9 +547,10 @@ class System:
                                              _ptr(pz), _ptr(out)))
         return out
 
-    def field_at(self, px, py, pz, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0):
+    def field_at(self, px, py, pz, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0,
+                 precision="fp64"):
         """Regularized Coulomb field at arbitrary points, [m, 3]."""
-        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64", kernel_order)
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, precision, kernel_order)
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         out = np.zeros(3 * len(px))
         _check(lib.bit_eval_field_points(self.h, C.byref(prm), len(px), _ptr(px), _ptr(py),

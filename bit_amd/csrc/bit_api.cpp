@@ -230,6 +230,9 @@ struct bit_system {
   int nchunks = 0;
   DevBuf chunk_node, chunk_rng, node_chunk0, node_nchunk, partial, mom, mom_hat64, qc32, qc64;
   int qc64_nterms = 0;  // nterms of qc64, 0 = not computed for the current moments
+  // field coefficients Q^(e) (FP32, FP64) and the nterms they were made for
+  DevBuf qf32, qf64;
+  int qf_nterms[2] = {0, 0};
   // internal nodes grouped by height (leaves 0): up_off[h-1]..up_off[h] in up_nodes
   DevBuf up_nodes;
   std::vector<int> up_off;
@@ -949,6 +952,7 @@ static int moments_on(bit_system* s, int nterms, hipStream_t st) {
   // call (e.g. BIT_E_OOM) leaves the system as it was, never half-initialised.
   s->mom_nterms = 0;
   s->qc64_nterms = 0;
+  s->qf_nterms[0] = s->qf_nterms[1] = 0;
   if (s->nchunks == 0) {
     // leaves: sums over their particles in chunks; internal nodes: upward pass
     // from their children, height by height (a child's index exceeds its
@@ -1028,6 +1032,27 @@ static int ensure_qc64(bit_system* s, int nterms, hipStream_t st = nullptr) {
   HIP_TRY(bit::dev::far_coeffs64(nt, nterms, s->mom_hat64.as<double>(), s->qc64.as<double>(),
                                  st ? st : c->stream));
   s->qc64_nterms = nterms;
+  return BIT_OK;
+}
+
+// Field coefficients Q^(e) for the current moments, computed on first use
+// (index 0: FP32, 1: FP64).
+static int ensure_qf(bit_system* s, int nterms, bool fp32) {
+  const int k = fp32 ? 0 : 1;
+  if (s->qf_nterms[k] == nterms) return BIT_OK;
+  bit_ctx* c = s->ctx;
+  const int nt = s->tree.numtree;
+  const size_t per = (size_t)3 * bit::dev::nqpad(nterms + 1);
+  if (fp32) {
+    BIT_TRY(s->qf32.ensure(c, (size_t)nt * per * sizeof(float)));
+    HIP_TRY(bit::dev::field_coeffs32(nt, nterms, s->mom_hat64.as<double>(), s->qf32.as<float>(),
+                                     c->stream));
+  } else {
+    BIT_TRY(s->qf64.ensure(c, (size_t)nt * per * sizeof(double)));
+    HIP_TRY(bit::dev::field_coeffs64(nt, nterms, s->mom_hat64.as<double>(), s->qf64.as<double>(),
+                                     c->stream));
+  }
+  s->qf_nterms[k] = nterms;
   return BIT_OK;
 }
 
@@ -1252,14 +1277,17 @@ int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_d
     BIT_TRY(s->out.ensure(c, (size_t)3 * n * sizeof(double)));
     dst = s->out.as<double>();
   }
+  const bool fp32 = p->precision == BIT_FP32;
+  BIT_TRY(ensure_qf(s, p->nterms, fp32));
   s->t_far.start(c->stream);
-  HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, p->kernel_order,
-                                   s->mom_hat64.as<double>(),
+  HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, p->kernel_order, fp32,
+                                   fp32 ? (const void*)s->qf32.p : (const void*)s->qf64.p,
                                    s->pl.far_off.as<int>(), s->pl.far_items.as<bit::dev::Item>(),
                                    s->phi_far.as<double>(), c->stream));
   s->t_far.stop(c->stream);
   s->t_near.start(c->stream);
-  HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, p->kernel_order, s->pl.near_off.as<int>(),
+  HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, p->kernel_order, fp32,
+                                    s->pl.near_off.as<int>(),
                                     s->pl.near_items.as<bit::dev::Item>(), s->phi_far.as<double>(),
                                     dst, c->stream));
   s->t_near.stop(c->stream);
@@ -1366,10 +1394,14 @@ int bit_eval_field_points(bit_system* s, const bit_params* p, int m, const doubl
   BIT_TRY(build_lists(s, s->ql, tv, p));
   BIT_TRY(s->qphi.ensure(c, (size_t)3 * m * sizeof(double)));
   BIT_TRY(s->qout.ensure(c, (size_t)3 * m * sizeof(double)));
-  HIP_TRY(bit::dev::far_field_grad(tv, p->nterms, p->delta2, p->kernel_order, s->mom_hat64.as<double>(),
+  const bool fp32 = p->precision == BIT_FP32;
+  BIT_TRY(ensure_qf(s, p->nterms, fp32));
+  HIP_TRY(bit::dev::far_field_grad(tv, p->nterms, p->delta2, p->kernel_order, fp32,
+                                   fp32 ? (const void*)s->qf32.p : (const void*)s->qf64.p,
                                    s->ql.far_off.as<int>(), s->ql.far_items.as<bit::dev::Item>(),
                                    s->qphi.as<double>(), c->stream));
-  HIP_TRY(bit::dev::near_field_grad(tv, p->delta2, p->kernel_order, s->ql.near_off.as<int>(),
+  HIP_TRY(bit::dev::near_field_grad(tv, p->delta2, p->kernel_order, fp32,
+                                    s->ql.near_off.as<int>(),
                                     s->ql.near_items.as<bit::dev::Item>(), s->qphi.as<double>(),
                                     s->qout.as<double>(), c->stream));
   HIP_TRY(hipMemcpyAsync(out, s->qout.p, (size_t)3 * m * sizeof(double), hipMemcpyDeviceToHost,

@@ -278,14 +278,20 @@ hipError_t vec_multi(int count, int maxn, const VecJob* jobs, hipStream_t s);
 hipError_t vec_op(int n, int op, double a, double b, const double* u, const double* v,
                   const double* z, double* y, hipStream_t s);
 
-// field (Coulomb, regularized; korder >= 1 adds the terms l = 1..korder of the
-// regularized family): E_far sorted [3n], then near + scatter [3n]
-hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, int korder,
-                          const double* mom_hat64, const int* far_off, const Item* far_items,
+// Field (Coulomb, regularized; korder >= 1 adds the terms l = 1..korder of the
+// regularized family): per-node coefficients Q^(e) of the radial-derivative form
+// at degree nterms (3 x nqpad(nterms + 1) values per node, FP32 or FP64), E_far
+// sorted [3n], then near + scatter [3n].
+hipError_t field_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qf,
+                          hipStream_t s);
+hipError_t field_coeffs64(int numtree, int nterms, const double* mom_hat64, double* qf,
+                          hipStream_t s);
+hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, int korder, bool fp32,
+                          const void* qf, const int* far_off, const Item* far_items,
                           double* e_far, hipStream_t s);
-hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, const int* near_off,
-                           const Item* near_items, const double* e_far, double* out,
-                           hipStream_t s);
+hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, bool fp32,
+                           const int* near_off, const Item* near_items, const double* e_far,
+                           double* out, hipStream_t s);
 
 // FP64 direct sum at ntargets sorted target slots; partial sums over
 // nsplit source chunks, then reduced in order.

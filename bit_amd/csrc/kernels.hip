@@ -27,154 +27,16 @@ namespace bit {
 namespace dev {
 
 // ---------------------------------------------------------------------------
-// Coefficient tables.  Index of multi-index (i,j,k), n = i+j+k:
+// Packed moments.  Index of multi-index (i,j,k), n = i+j+k:
 //   n(n+1)(n+2)/6 + (j+k)(j+k+1)/2 + k
 // i.e. shells of increasing degree, inside a shell i descending, then k
-// ascending.  For every coefficient the table holds the direction d of the
-// one-direction recurrence (first index equal to 1, else first nonzero index:
-// the choice of the reference's index families, tree3d.cpp:3075-3214) and the
-// positions of k-e_e and k-2e_e.
+// ascending.
 // ---------------------------------------------------------------------------
 constexpr int cidx(int i, int j, int k) {
   int n = i + j + k, m = j + k;
   return n * (n + 1) * (n + 2) / 6 + m * (m + 1) / 2 + k;
 }
 constexpr int ncoef(int nt) { return nt * (nt + 1) * (nt + 2) / 6; }
-
-template <int NT>
-struct Tab {
-  static constexpr int NC = ncoef(NT);
-  int k[NC][3];
-  int d[NC];
-  int m1[NC][3];
-  int m2[NC][3];
-  constexpr Tab() : k(), d(), m1(), m2() {
-    for (int n = 0; n < NT; n++)
-      for (int m = 0; m <= n; m++)
-        for (int kk = 0; kk <= m; kk++) {
-          int i = n - m, j = m - kk;
-          int c = cidx(i, j, kk);
-          int kv[3] = {i, j, kk};
-          for (int e = 0; e < 3; e++) k[c][e] = kv[e];
-          int dd = -1;
-          for (int e = 0; e < 3 && dd < 0; e++)
-            if (kv[e] == 1) dd = e;
-          for (int e = 0; e < 3 && dd < 0; e++)
-            if (kv[e] > 0) dd = e;
-          d[c] = dd < 0 ? 0 : dd;
-          for (int e = 0; e < 3; e++) {
-            int a1[3] = {i, j, kk}, a2[3] = {i, j, kk};
-            a1[e] -= 1;
-            a2[e] -= 2;
-            m1[c][e] = kv[e] >= 1 ? cidx(a1[0], a1[1], a1[2]) : -1;
-            m2[c][e] = kv[e] >= 2 ? cidx(a2[0], a2[1], a2[2]) : -1;
-          }
-        }
-  }
-};
-template <int NT>
-inline constexpr Tab<NT> kTab{};
-
-// One coefficient of the recurrence in normalised form.  With R the target's
-// (regularized) distance to the cluster centre, u = (x - centre)/R and
-// lam = kappa R, the Taylor coefficients of exp(-kappa R)/R in the source
-// offset (in cluster units s) are a_k = rho^{|k|+1} c_k, rho = s/R, and those of
-// exp(-kappa R) are b_k = rho^{|k|} beta_k, where
-//   V   = u_d c_{k-e_d} - c_{k-2e_d}
-//   beta_k = (lam/k_d) V
-//   c_k = sum_e (2 u_e c_{k-e} - c_{k-2e}) - V/k_d
-//         + (lam/k_d)(u_d beta_{k-e_d} - beta_{k-2e_d}),
-// the recurrence of the unnormalised a, b (x -> u, kappa s -> lam, no 1/R^2
-// factor per coefficient).  The direction-d term of the sum is
-// 2 u_d c_{k-e_d} - c_{k-2e_d} = 2V + c_{k-2e_d}, so the bracket is evaluated as
-//   S' + c_{k-2e_d} + (2 - 1/k_d) V + (lam/k_d) W,   S' = sum over e != d.
-// Each coefficient is folded into a per-degree-shell sum as soon as it
-// exists, and each finished shell n into the total with rho^{n+1}:
-//   GRAD = false: sh0 += M_k c_k                      (potential)
-//   GRAD = true:  sh_e += k_e M_{k-e} c_k, e = x,y,z   (field, M of degree < NT-1)
-template <typename T, int NT, int C, int E, int D>
-__device__ __forceinline__ void add_other(const T* a, const T (&x2)[3], T& S, bool& has) {
-  constexpr const Tab<NT>& tb = kTab<NT>;
-  if constexpr (E != D) {
-    constexpr int m1 = tb.m1[C][E], m2 = tb.m2[C][E];
-    if constexpr (m1 >= 0) {
-      S = has ? fma(x2[E], a[m1], S) : x2[E] * a[m1];
-      has = true;
-    }
-    if constexpr (m2 >= 0) {
-      S = has ? S - a[m2] : -a[m2];
-      has = true;
-    }
-  }
-}
-
-template <typename T>
-struct Shells {
-  T sh[3];   // current shell's partial sums
-  T tot[3];  // folded shells
-  T pw;      // rho^{n+1} of the last folded shell n
-  T rho;
-};
-
-template <typename T, int NT, bool SCR, bool GRAD, int C, typename MP>
-__device__ __forceinline__ void coef_step(T* a, T* b, const T (&x)[3], const T (&x2)[3],
-                                          const T* kinv, MP M, Shells<T>& z) {
-  constexpr const Tab<NT>& tb = kTab<NT>;
-  constexpr int d = tb.d[C];
-  constexpr int kd = tb.k[C][d];
-  constexpr int e0 = tb.m1[C][0], e1 = tb.m1[C][1], e2 = tb.m1[C][2];
-  constexpr int md1 = tb.m1[C][d], md2 = tb.m2[C][d];
-  constexpr int deg = tb.k[C][0] + tb.k[C][1] + tb.k[C][2];
-  constexpr bool first = C == ncoef(deg), last = C + 1 == ncoef(deg + 1);
-  T S = T(0);
-  bool has = false;  // resolved at compile time
-  add_other<T, NT, C, 0, d>(a, x2, S, has);
-  add_other<T, NT, C, 1, d>(a, x2, S, has);
-  add_other<T, NT, C, 2, d>(a, x2, S, has);
-  if constexpr (md2 >= 0) {
-    S = has ? S + a[md2] : a[md2];
-    has = true;
-  }
-  T V;
-  if constexpr (md2 >= 0) V = fma(x[d], a[md1], -a[md2]);
-  else V = x[d] * a[md1];
-  constexpr double ck = 2.0 - 1.0 / kd;
-  T t = has ? fma(T(ck), V, S) : T(ck) * V;
-  if constexpr (SCR) {
-    T W;
-    if constexpr (md2 >= 0) W = fma(x[d], b[md1], -b[md2]);
-    else W = x[d] * b[md1];
-    t = fma(kinv[kd], W, t);
-    b[C] = kinv[kd] * V;
-  }
-  a[C] = t;
-  if constexpr (!GRAD) {
-    z.sh[0] = first ? M[C] * t : fma(M[C], t, z.sh[0]);
-  } else {
-    // the integer factor multiplies the per-lane coefficient, so the moment
-    // stays a scalar operand of the FMA (no per-lane copies of M)
-    constexpr int k0 = tb.k[C][0], k1 = tb.k[C][1], k2 = tb.k[C][2];
-    if constexpr (first) z.sh[0] = z.sh[1] = z.sh[2] = T(0);
-    if constexpr (e0 >= 0) z.sh[0] = fma(M[e0], k0 == 1 ? t : T(k0) * t, z.sh[0]);
-    if constexpr (e1 >= 0) z.sh[1] = fma(M[e1], k1 == 1 ? t : T(k1) * t, z.sh[1]);
-    if constexpr (e2 >= 0) z.sh[2] = fma(M[e2], k2 == 1 ? t : T(k2) * t, z.sh[2]);
-  }
-  if constexpr (last) {
-    z.pw *= z.rho;
-    z.tot[0] = fma(z.sh[0], z.pw, z.tot[0]);
-    if constexpr (GRAD) {
-      z.tot[1] = fma(z.sh[1], z.pw, z.tot[1]);
-      z.tot[2] = fma(z.sh[2], z.pw, z.tot[2]);
-    }
-  }
-}
-
-template <typename T, int NT, bool SCR, bool GRAD, typename MP, int... C>
-__device__ __forceinline__ void coef_all(std::integer_sequence<int, C...>, T* a, T* b,
-                                         const T (&x)[3], const T (&x2)[3], const T* kinv, MP M,
-                                         Shells<T>& z) {
-  (coef_step<T, NT, SCR, GRAD, C + 1, MP>(a, b, x, x2, kinv, M, z), ...);
-}
 
 // Pointers read from a descriptor are generic (flat) to the compiler; loads
 // through a global-address-space pointer with a wave-uniform address can be
@@ -190,76 +52,6 @@ __device__ __forceinline__ gptr<T> as_global(const T* p) {
   return (gptr<T>)(p);
 }
 
-// Field: E_e = sum_k (k_e+1) M_k a_{k+e}, coefficients to degree NT (Coulomb),
-// in the normalised variables of far_eval (c0 = 1).
-template <typename T, int NT, typename MP>
-__device__ __forceinline__ void far_eval_grad(T u0, T u1, T u2, T rho, MP M, T& ex, T& ey,
-                                              T& ez) {
-  constexpr int NC = ncoef(NT + 1);
-  T a[NC];
-  T b[1];
-  a[0] = T(1);
-  const T x[3] = {u0, u1, u2};
-  const T xx[3] = {u0 + u0, u1 + u1, u2 + u2};
-  T kinv[2] = {T(0), T(0)};
-  Shells<T> z{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, rho, rho};
-  coef_all<T, NT + 1, false, true, MP>(std::make_integer_sequence<int, NC - 1>{}, a, b, x, xx,
-                                       kinv, M, z);
-  ex = z.tot[0];
-  ey = z.tot[1];
-  ez = z.tot[2];
-}
-
-// Term l >= 1 of the regularized field family, R^-(2l+1) (tree3d.cpp:614-791):
-// with S1 = sum_e 2 x_e a_{k-e}, S2 = sum_e a_{k-2e} and n = |k|,
-//   a_k = fac [ (1 + (l - 1/2)/n) S1 - (1 + (2l - 1)/n) S2 ],
-// which is |k| R^2 a_k = (2|k|+2l-1) sum x_e a_{k-e} - (|k|+2l-1) sum a_{k-2e}
-// in scaled variables.  lm = l - 1/2, l2 = 2l - 1; folded into the field
-// sums as far_eval_grad does.
-template <typename T, int NT, int C, typename MP>
-__device__ __forceinline__ void pow_step(T* a, const T (&x2)[3], T fac, T lm, T l2, MP M,
-                                         T (&acc)[3]) {
-  constexpr const Tab<NT>& tb = kTab<NT>;
-  constexpr int e0 = tb.m1[C][0], e1 = tb.m1[C][1], e2 = tb.m1[C][2];
-  constexpr int f0 = tb.m2[C][0], f1 = tb.m2[C][1], f2 = tb.m2[C][2];
-  constexpr int n = tb.k[C][0] + tb.k[C][1] + tb.k[C][2];
-  constexpr double invn = 1.0 / n;
-  T S1 = T(0), S2 = T(0);
-  if constexpr (e0 >= 0) S1 = fma(x2[0], a[e0], S1);
-  if constexpr (e1 >= 0) S1 = fma(x2[1], a[e1], S1);
-  if constexpr (e2 >= 0) S1 = fma(x2[2], a[e2], S1);
-  if constexpr (f0 >= 0) S2 += a[f0];
-  if constexpr (f1 >= 0) S2 += a[f1];
-  if constexpr (f2 >= 0) S2 += a[f2];
-  const T c1 = fma(lm, T(invn), T(1)), c2 = fma(l2, T(invn), T(1));
-  a[C] = fac * fma(c1, S1, -c2 * S2);
-  if constexpr (e0 >= 0) acc[0] = fma(M[e0], T(tb.k[C][0]) * a[C], acc[0]);
-  if constexpr (e1 >= 0) acc[1] = fma(M[e1], T(tb.k[C][1]) * a[C], acc[1]);
-  if constexpr (e2 >= 0) acc[2] = fma(M[e2], T(tb.k[C][2]) * a[C], acc[2]);
-}
-
-template <typename T, int NT, typename MP, int... C>
-__device__ __forceinline__ void pow_all(std::integer_sequence<int, C...>, T* a, const T (&x2)[3],
-                                        T fac, T lm, T l2, MP M, T (&acc)[3]) {
-  (pow_step<T, NT, C + 1, MP>(a, x2, fac, lm, l2, M, acc), ...);
-}
-
-// Field of a0 * R^-(2l+1)/R^-(2l+1)(0) in scaled variables, coefficients to
-// degree NT (a0 carries the term's weight and R^-(2l+1)).
-template <typename T, int NT, typename MP>
-__device__ __forceinline__ void far_eval_grad_pow(T x0, T x1, T x2_, T fac, T a0, int l, MP M,
-                                                  T& ex, T& ey, T& ez) {
-  constexpr int NC = ncoef(NT + 1);
-  T a[NC];
-  a[0] = a0;
-  const T xx[3] = {x0 + x0, x1 + x1, x2_ + x2_};
-  T acc[3] = {T(0), T(0), T(0)};
-  pow_all<T, NT + 1, MP>(std::make_integer_sequence<int, NC - 1>{}, a, xx, fac, T(l - 0.5),
-                         T(2 * l - 1), M, acc);
-  ex = acc[0];
-  ey = acc[1];
-  ez = acc[2];
-}
 
 // ---------------------------------------------------------------------------
 // Radial-derivative form of the far field (FP32 kernel).  The same truncated
@@ -1867,8 +1659,8 @@ hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp
 // Far field.  Per item the wave's lanes in the mask evaluate the cluster's
 // Taylor sum in the radial-derivative form (qall): FP32 (k_farq, per-cluster
 // values added with Kahan compensation) and FP64 (k_farq64).  Both stage the
-// cluster's coefficients in LDS.  The coefficient recurrence (coef_step) is
-// used by the field kernel.
+// cluster's coefficients in LDS.  The field kernels (k_farq_field) use the
+// same form with three coefficient sets per node.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Acc;
@@ -2497,101 +2289,295 @@ hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool 
 }
 
 // ---------------------------------------------------------------------------
-// Field of the regularized Coulomb kernel (treeforce, tree3d.cpp:851-971), FP64.
+// Field of the regularized Coulomb kernel (treeforce, tree3d.cpp:851-971, with
+// treetaylorforce 456-810 and the KERNEL_ORDER family 614-791), FP64 or FP32.
+// The far term of one cluster, E_e = sum_{|k|<=p} (k_e+1) M_k a_{k+e}, is a
+// Taylor sum of degree p+1 with moments M^(e)_k' = k'_e M_{k'-e}: the same
+// radial-derivative form as the potential, over three per-node coefficient
+// sets Q^(e) (k_qcoef_field) and with the radial derivatives of the
+// regularized family G_L = sum_{l<=L} w_l delta^2l R^-(2l+1):
+//   ghat_m = R^(2m+1) (R^-1 d/dR)^m G_L = (-1)^m sum_l w_l (delta^2/R^2)^l
+//            prod_{i<m} (2l + 2i + 1),
+// the sweep's ratios r_m = ghat_{m+1} / ghat_m (Coulomb: -(2m+1)).  Since
+// Mhat^(e)_k' = k'_e Mhat_{k'-e} / s, the Q^(e) omit the 1/s and each pair's
+// sums are multiplied by 1/(R s).
 // ---------------------------------------------------------------------------
-// FAM: regularized family of order korder >= 1 (terms l = 1..korder added to
-// the Coulomb term, each weighted w_l delta2^l, w_l = prod_{j<=l} (j-1/2)/j).
-template <int NT, bool FAM>
-__global__ __launch_bounds__(64) void k_far_grad(TreeView tv, double delta2, int korder,
-                                                 const double* __restrict__ mom_hat,
-                                                 const int* __restrict__ far_off,
-                                                 const Item* __restrict__ items,
-                                                 double* __restrict__ e_far) {
-  constexpr int NC = ncoef(NT);
-  const int b = blockIdx.x;
-  const int lane = threadIdx.x;
+// One thread per (node, component e, Q entry); NT = nterms + 1 (degree p + 1).
+template <int NT, typename T>
+__global__ __launch_bounds__(256) void k_qcoef_field(int numtree, const double* __restrict__ mom_hat,
+                                                     T* __restrict__ qf) {
+  constexpr int NQ = nqcoef(NT), NQP = nqpad(NT), NCM = ncoef(NT - 1);
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)numtree * 3 * NQP) return;
+  const int node = (int)(g / (3 * NQP));
+  const int rem = (int)(g - (long long)node * 3 * NQP);
+  const int e = rem / NQP, q = rem - e * NQP;
+  if (q >= NQ) {
+    qf[g] = T(0);
+    return;
+  }
+  const int l[3] = {kQTabDev<NT>.l[q][0], kQTabDev<NT>.l[q][1], kQTabDev<NT>.l[q][2]};
+  const int t = kQTabDev<NT>.t[q];
+  const double* M = mom_hat + (size_t)node * NCM;
+  double sum = 0.0;
+  for (int j0 = 0; j0 <= t; j0++)
+    for (int j1 = 0; j0 + j1 <= t; j1++) {
+      const int j2 = t - j0 - j1;
+      int k[3] = {l[0] + 2 * j0, l[1] + 2 * j1, l[2] + 2 * j2};
+      const int ke = k[e];
+      if (ke == 0) continue;  // M^(e)_k' = k'_e M_{k'-e}
+      k[e] -= 1;
+      sum = fma((double)ke * M[cidx(k[0], k[1], k[2])],
+                kInvFact[j0] * kInvFact[j1] * kInvFact[j2], sum);
+    }
+  const double w = ldexp(kInvFact[l[0]] * kInvFact[l[1]] * kInvFact[l[2]], -t);
+  qf[g] = (T)(((l[0] + l[1] + l[2]) & 1 ? -w : w) * sum);
+}
+
+template <int NT, typename T>
+static void launch_qcoef_field(int numtree, const double* mom_hat, T* qf, hipStream_t s) {
+  const long long total = (long long)numtree * 3 * nqpad(NT);
+  hipLaunchKernelGGL((k_qcoef_field<NT, T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     s, numtree, mom_hat, qf);
+}
+
+template <typename T, int... NTs>
+static bool dispatch_qcoef_field(int nterms, std::integer_sequence<int, NTs...>, int numtree,
+                                 const double* mom_hat, T* qf, hipStream_t s) {
+  bool done = false;
+  ((nterms + 1 == NTs + 2
+        ? (launch_qcoef_field<NTs + 2, T>(numtree, mom_hat, qf, s), done = true)
+        : false),
+   ...);
+  return done;
+}
+
+template <typename T>
+static hipError_t field_coeffs(int numtree, int nterms, const double* mom_hat64, T* qf,
+                               hipStream_t s) {
+  if (numtree == 0) return hipSuccess;
+  if (!dispatch_qcoef_field(nterms, std::make_integer_sequence<int, kMaxNterms - 1>{}, numtree,
+                            mom_hat64, qf, s))
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t field_coeffs32(int numtree, int nterms, const double* mom_hat64, float* qf,
+                          hipStream_t s) {
+  return field_coeffs(numtree, nterms, mom_hat64, qf, s);
+}
+hipError_t field_coeffs64(int numtree, int nterms, const double* mom_hat64, double* qf,
+                          hipStream_t s) {
+  return field_coeffs(numtree, nterms, mom_hat64, qf, s);
+}
+
+__device__ __forceinline__ float rsqrtT(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrtT(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpT(double x) { return 1.0 / x; }
+
+// Per-pair values of the field sweep from the target's offset d from the
+// centre (rounded to T), the cluster scale s and the family; returns 1/(R s).
+template <int NT, bool FAM, typename T>
+__device__ __forceinline__ T field_setup(QPair<NT, T>& v, T dx, T dy, T dz, T s, T delta2,
+                                         int korder) {
+  constexpr int P = NT - 1;
+  const T r2 = fmaT(dx, dx, fmaT(dy, dy, fmaT(dz, dz, delta2)));
+  const T ri = rsqrtT(r2);
+  const T rho = s * ri, f = rho * ri;
+  T r[P > 0 ? P : 1];
+  T g0 = T(1);
+  if constexpr (FAM) {
+    // ghat_m = (-1)^m sum_l t_l prod_{i<m} (2l + 2i + 1), t_l = w_l (delta^2/R^2)^l
+    constexpr int LM = 8;  // BIT_MAX_KERNEL_ORDER
+    T tl[LM + 1];
+    const T x = delta2 * ri * ri;
+    tl[0] = T(1);
+#pragma unroll
+    for (int l = 1; l <= LM; l++) tl[l] = l <= korder ? tl[l - 1] * (T(l - 0.5) / T(l)) * x : T(0);
+    T gh[P + 1];
+#pragma unroll
+    for (int m = 0; m <= P; m++) {
+      T sum = T(0);
+#pragma unroll
+      for (int l = 0; l <= LM; l++) {
+        sum += tl[l];
+        tl[l] *= T(2 * l + 2 * m + 1);
+      }
+      gh[m] = (m & 1) ? -sum : sum;
+    }
+#pragma unroll
+    for (int m = 0; m < P; m++) r[m] = gh[m + 1] * rcpT(gh[m]);
+    g0 = gh[0];
+  } else {
+#pragma unroll
+    for (int m = 0; m < P; m++) r[m] = -T(2 * m + 1);
+  }
+  v.g[0] = g0;
+#pragma unroll
+  for (int t = 1; t <= P / 2; t++) v.g[t] = v.g[t - 1] * r[t - 1];
+  const T w0 = dx * f, w1 = dy * f, w2 = dz * f;
+#pragma unroll
+  for (int m = 0; m < P; m++) {
+    v.mult[0][m] = r[m] * w0;
+    v.mult[1][m] = r[m] * w1;
+    v.mult[2][m] = r[m] * w2;
+  }
+  v.z = rho * rho;
+  return ri * rcpT(s);
+}
+
+// One wavefront per batch, one lane per target; a far item's three Q^(e) tiles
+// are staged in LDS and read as broadcasts.  FP32: the offset from the centre
+// is formed in FP64 and rounded, per-cluster sums Kahan-compensated.
+template <int NT, bool FAM, typename T>
+__global__ __launch_bounds__(64) void k_farq_field(TreeView tv, double delta2, int korder,
+                                                   const T* __restrict__ qf,
+                                                   const int* __restrict__ far_off,
+                                                   const Item* __restrict__ items,
+                                                   double* __restrict__ e_far) {
+  constexpr int NQ = nqcoef(NT), NQP = nqpad(NT), PF = 3 * NQP / kBatch;
+  __shared__ __align__(16) T tile[3 * NQP];
+  const int b = blockIdx.x, lane = threadIdx.x;
   const int t = b * kBatch + lane;
   const bool valid = t < tv.nt;
   const int tt = valid ? t : tv.nt - 1;
   const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
-  double ex = 0, ey = 0, ez = 0;
+  const T d2 = (T)delta2;
+  Acc<T> ax, ay, az;
   const int i0 = uniform(far_off[b]), i1 = uniform(far_off[b + 1]);
   for (int it = i0; it < i1; it++) {
     const unsigned long long mask = uniform64(items[it].mask);
     const int node = uniform(items[it].node);
+    const T* src = qf + (size_t)node * 3 * NQP;
+#pragma unroll
+    for (int j = 0; j < PF; j++) tile[j * kBatch + lane] = src[j * kBatch + lane];
+    __syncthreads();
     if ((mask >> lane) & 1ull) {
       const double4 c = tv.center[node];
-      const double s = tv.scale[node];
-      const double inv = 1.0 / s;
-      const double dx = px - c.x, dy = py - c.y, dz = pz - c.z;
-      const double R2h = (dx * dx + dy * dy + dz * dz + delta2) * inv * inv;
-      const double a0 = 1.0 / sqrt(R2h);  // s/R
-      double gx, gy, gz;
-      far_eval_grad<double, NT, gptr<double>>(dx * inv * a0, dy * inv * a0, dz * inv * a0, a0,
-                                               as_global(mom_hat) + (size_t)node * NC, gx, gy,
-                                               gz);
-      if constexpr (FAM) {
-        // a0 of term l: w_l (delta2/s^2)^l Rh^-(2l+1), built up from term l-1
-        const double fac = 1.0 / R2h, dh = delta2 * inv * inv;
-        double al = a0;
-        for (int l = 1; l <= korder; l++) {
-          al = al * ((l - 0.5) / l) * dh * fac;
-          double hx, hy, hz;
-          far_eval_grad_pow<double, NT, gptr<double>>(dx * inv, dy * inv, dz * inv, fac, al, l,
-                                                      as_global(mom_hat) + (size_t)node * NC,
-                                                      hx, hy, hz);
-          gx += hx;
-          gy += hy;
-          gz += hz;
-        }
-      }
-      // d/dx of (1/s) f(x/s) -> 1/s^2
-      ex += gx * inv * inv;
-      ey += gy * inv * inv;
-      ez += gz * inv * inv;
+      QPair<NT, T> v;
+      const T sc = field_setup<NT, FAM, T>(v, (T)(px - c.x), (T)(py - c.y), (T)(pz - c.z),
+                                           (T)tv.scale[node], d2, korder);
+      ax.add(qall<NT, T>(std::make_integer_sequence<int, NQ>{}, v, tile) * sc);
+      ay.add(qall<NT, T>(std::make_integer_sequence<int, NQ>{}, v, tile + NQP) * sc);
+      az.add(qall<NT, T>(std::make_integer_sequence<int, NQ>{}, v, tile + 2 * NQP) * sc);
     }
+    __syncthreads();  // the tile is overwritten by the next item
   }
   if (valid) {
-    e_far[3 * (size_t)t + 0] = ex;
-    e_far[3 * (size_t)t + 1] = ey;
-    e_far[3 * (size_t)t + 2] = ez;
+    e_far[3 * (size_t)t + 0] = (double)ax.value();
+    e_far[3 * (size_t)t + 1] = (double)ay.value();
+    e_far[3 * (size_t)t + 2] = (double)az.value();
   }
 }
 
-template <int NT>
-static void launch_far_grad(const TreeView& tv, double delta2, int korder, const double* mom,
-                            const int* far_off, const Item* items, double* e_far, hipStream_t s) {
-  int nb = (tv.nt + kBatch - 1) / kBatch;
-  if (korder > 0)
-    hipLaunchKernelGGL((k_far_grad<NT, true>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, korder,
-                       mom, far_off, items, e_far);
-  else
-    hipLaunchKernelGGL((k_far_grad<NT, false>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, 0, mom,
-                       far_off, items, e_far);
-}
-
-template <int... NTs>
-static bool dispatch_far_grad(int nterms, std::integer_sequence<int, NTs...>,
-                              const TreeView& tv, double delta2, int korder, const double* mom,
+template <int NT, typename T>
+static void launch_farq_field(const TreeView& tv, double delta2, int korder, const T* qf,
                               const int* far_off, const Item* items, double* e_far,
                               hipStream_t s) {
+  const int nb = (tv.nt + kBatch - 1) / kBatch;
+  if (korder > 0)
+    hipLaunchKernelGGL((k_farq_field<NT, true, T>), dim3(nb), dim3(kBatch), 0, s, tv, delta2,
+                       korder, qf, far_off, items, e_far);
+  else
+    hipLaunchKernelGGL((k_farq_field<NT, false, T>), dim3(nb), dim3(kBatch), 0, s, tv, delta2, 0,
+                       qf, far_off, items, e_far);
+}
+
+template <typename T, int... NTs>
+static bool dispatch_farq_field(int nterms, std::integer_sequence<int, NTs...>,
+                                const TreeView& tv, double delta2, int korder, const T* qf,
+                                const int* far_off, const Item* items, double* e_far,
+                                hipStream_t s) {
   bool done = false;
-  ((nterms == NTs + 1
-        ? (launch_far_grad<NTs + 1>(tv, delta2, korder, mom, far_off, items, e_far, s),
+  ((nterms + 1 == NTs + 2
+        ? (launch_farq_field<NTs + 2, T>(tv, delta2, korder, qf, far_off, items, e_far, s),
            done = true)
         : false),
    ...);
   return done;
 }
 
-hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, int korder,
-                          const double* mom_hat64, const int* far_off, const Item* far_items,
+hipError_t far_field_grad(const TreeView& tv, int nterms, double delta2, int korder, bool fp32,
+                          const void* qf, const int* far_off, const Item* far_items,
                           double* e_far, hipStream_t s) {
-  if (korder < 0) return hipErrorInvalidValue;
-  if (!dispatch_far_grad(nterms, std::make_integer_sequence<int, kMaxNterms - 1>{}, tv, delta2,
-                         korder, mom_hat64, far_off, far_items, e_far, s))
-    return hipErrorInvalidValue;
+  if (korder < 0 || korder > 8) return hipErrorInvalidValue;
+  if (tv.nt == 0) return hipSuccess;
+  const bool ok =
+      fp32 ? dispatch_farq_field(nterms, std::make_integer_sequence<int, kMaxNterms - 1>{}, tv,
+                                 delta2, korder, (const float*)qf, far_off, far_items, e_far, s)
+           : dispatch_farq_field(nterms, std::make_integer_sequence<int, kMaxNterms - 1>{}, tv,
+                                 delta2, korder, (const double*)qf, far_off, far_items, e_far, s);
+  if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+// FP32 near field of the field: sources in leaf-centre coordinates (sloc), the
+// target's offset from the leaf centre formed in FP64 and rounded, groups of 16
+// pairs summed plainly and added with Kahan compensation; the family's factor
+// chain as eval_grad_kernel_reg (tree3d.cpp:80-91).
+__global__ __launch_bounds__(64) void k_near_grad32(TreeView tv, double d2, int korder,
+                                                    const int* __restrict__ near_off,
+                                                    const Item* __restrict__ items,
+                                                    const double* __restrict__ e_far,
+                                                    double* __restrict__ out) {
+  __shared__ float4 tile[64];
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int t = b * kBatch + lane;
+  const bool valid = t < tv.nt;
+  const int tt = valid ? t : tv.nt - 1;
+  const double px = tv.tx[tt], py = tv.ty[tt], pz = tv.tz[tt];
+  const int t0 = b * kBatch, t1 = min(tv.nt, t0 + kBatch);
+  const float dd = (float)d2;
+  Acc<float> ax, ay, az;
+  const int i0 = uniform(near_off[b]), i1 = uniform(near_off[b + 1]);
+  for (int it = i0; it < i1; it++) {
+    const unsigned long long mask = uniform64(items[it].mask);
+    const int node = uniform(items[it].node);
+    const int2 r = tv.range[node];
+    const double4 c = tv.center[node];
+    const bool act = (mask >> lane) & 1ull;
+    const float ox = (float)(px - c.x), oy = (float)(py - c.y), oz = (float)(pz - c.z);
+    const bool selfp = tv.self ? (r.x < t1 && r.y > t0) : true;
+    for (int base = r.x; base < r.y; base += 64) {
+      const int cnt = min(64, r.y - base);
+      if (lane < cnt) tile[lane] = tv.sloc[base + lane];
+      __syncthreads();
+      if (act) {
+        const int me = tv.self && selfp ? t - base : -1;
+        for (int j0 = 0; j0 < cnt; j0 += 16) {
+          float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+          const int j1 = min(cnt, j0 + 16);
+          for (int j = j0; j < j1; j++) {
+            if (j == me) continue;
+            const float4 q = tile[j];
+            const float dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
+            const float rs = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, dd)));
+            const float ri = __builtin_amdgcn_rsqf(rs);
+            float temp = ri * ri * ri, f = temp;
+            for (int l = 1; l <= korder; l++) {
+              f = f * ((l + 0.5f) / l) * (dd * ri * ri);
+              temp += f;
+            }
+            const float wq = q.w * temp;
+            gx = fmaf(dx, wq, gx);
+            gy = fmaf(dy, wq, gy);
+            gz = fmaf(dz, wq, gz);
+          }
+          ax.add(gx);
+          ay.add(gy);
+          az.add(gz);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (valid) {
+    const size_t o = 3 * (size_t)tv.tperm[t];
+    out[o + 0] = e_far[3 * (size_t)t + 0] + ax.value();
+    out[o + 1] = e_far[3 * (size_t)t + 1] + ay.value();
+    out[o + 2] = e_far[3 * (size_t)t + 2] + az.value();
+  }
 }
 
 __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2, int korder,
@@ -2659,12 +2645,17 @@ __global__ __launch_bounds__(64) void k_near_grad(TreeView tv, double d2, int ko
   }
 }
 
-hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, const int* near_off,
-                           const Item* near_items, const double* e_far, double* out,
-                           hipStream_t s) {
+hipError_t near_field_grad(const TreeView& tv, double delta2, int korder, bool fp32,
+                           const int* near_off, const Item* near_items, const double* e_far,
+                           double* out, hipStream_t s) {
+  if (tv.nt == 0) return hipSuccess;
   int nb = (tv.nt + kBatch - 1) / kBatch;
-  hipLaunchKernelGGL(k_near_grad, dim3(nb), dim3(kBatch), 0, s, tv, delta2, korder, near_off,
-                     near_items, e_far, out);
+  if (fp32)
+    hipLaunchKernelGGL(k_near_grad32, dim3(nb), dim3(kBatch), 0, s, tv, delta2, korder,
+                       near_off, near_items, e_far, out);
+  else
+    hipLaunchKernelGGL(k_near_grad, dim3(nb), dim3(kBatch), 0, s, tv, delta2, korder, near_off,
+                       near_items, e_far, out);
   return hipGetLastError();
 }
 

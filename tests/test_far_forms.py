@@ -147,3 +147,103 @@ def test_sweep_takes_one_fma_per_coefficient():
             fma += t != p // 2                  # S * z + g_t phi
         assert fma == bench.nqcoef(nt) - 1, nt
         assert mul == p // 2 + 1
+
+
+def family_weight(l):
+    """w_l = prod_{j<=l} (j - 1/2) / j (the regularized family's weights)."""
+    w = 1.0
+    for j in range(1, l + 1):
+        w *= (j - 0.5) / j
+    return w
+
+
+def field_qcoef(mhat, p):
+    """The field kernels' per-node coefficients: E_e = sum_k (k_e + 1) M_k a_{k+e}
+    is a Taylor sum of degree p + 1 with moments M^(e)_k' = k'_e M_{k'-e}, so
+    Q^(e) is qcoef of those, scaled by s (Mhat^(e)_k' = k'_e Mhat_{k'-e} / s; the
+    kernel applies the 1/s per pair)."""
+    out = []
+    for e in range(3):
+        me = {}
+        for k in monos(p + 1):
+            if k[e] >= 1:
+                km = list(k)
+                km[e] -= 1
+                me[k] = k[e] * mhat[tuple(km)]
+            else:
+                me[k] = 0.0
+        out.append(qcoef(me, p + 1))
+    return out
+
+
+def field_q(qs, p, d, s, delta2, korder):
+    """The field kernels' form: for each component the ratio-Horner sweep of
+    far_q at degree p + 1, with ghat_m = R^(2m+1) (R^-1 d/dR)^m G_L of the
+    regularized family G_L = sum_{l<=L} w_l delta^2l R^-(2l+1):
+        ghat_m = (-1)^m sum_l w_l (delta^2/R^2)^l prod_{i<m} (2l + 2i + 1),
+    times 1/(R s)."""
+    P = p + 1
+    R = math.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + delta2)
+    rho = s / R
+    w = [rho * d[e] / R for e in range(3)]
+    z = rho * rho
+    t_l = [family_weight(l) * (delta2 / (R * R)) ** l for l in range(korder + 1)]
+    gh = []
+    for m in range(P + 1):
+        gh.append((-1) ** m * sum(t_l))
+        t_l = [t * (2 * l + 2 * m + 1) for l, t in enumerate(t_l)]
+    r = [gh[m + 1] / gh[m] for m in range(P)]
+    E = []
+    for q in qs:
+        S = 0.0
+        for t in range(P // 2, -1, -1):
+            L = P - 2 * t
+            phi = 0.0
+            for l0 in range(L, -1, -1):
+                a = 0.0
+                for l1 in range(L - l0, -1, -1):
+                    b = 0.0
+                    for l2 in range(L - l0 - l1, -1, -1):
+                        qv = q[(l0 + l1 + l2 + t, (l0, l1, l2))]
+                        b = qv if l2 == L - l0 - l1 else b * r[l0 + l1 + l2 + t] * w[2] + qv
+                    a = b if l1 == L - l0 else a * r[l0 + l1 + t] * w[1] + b
+                phi = a if l0 == L else phi * r[l0 + t] * w[0] + a
+            S = gh[t] * phi if t == P // 2 else S * z + gh[t] * phi
+        E.append(S / (R * s))
+    return E
+
+
+@pytest.mark.parametrize("p,delta2,korder", [(0, 0.0, 0), (4, 1e-4, 0), (7, 0.01, 0),
+                                             (8, 0.01, 1), (6, 0.02, 3), (11, 1e-4, 2)])
+def test_field_radial_form_equals_taylor_sum(ora, p, delta2, korder):
+    """The field form against E_e = sum_l w_l delta^2l sum_k (k_e+1) M_k A^(l)_{k+e}
+    with the oracle's coefficients of R^-(2l+1) (ora_taylor_coeffs for l = 0,
+    ora_powinv_coeffs above), the oracle's treeforce far term."""
+    rng = np.random.default_rng(200 + p + korder)
+    worst = 0.0
+    for _ in range(10):
+        h = 10 ** rng.uniform(-2, -0.5)
+        ys = rng.uniform(-h, h, (40, 3))
+        qsrc = rng.uniform(-1, 1, 40)
+        s = math.sqrt(3.0) * h
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        d = u * s * (2.0 + 10 ** rng.uniform(-1, 1.5))
+        M = {k: float(np.sum(qsrc * ys[:, 0] ** k[0] * ys[:, 1] ** k[1] * ys[:, 2] ** k[2]))
+             for k in monos(p)}
+        mhat = {k: v / s ** sum(k) for k, v in M.items()}
+        R2 = float(d @ d) + delta2
+        ref = np.zeros(3)
+        for l in range(korder + 1):
+            a = (ora.taylor_coeffs(p + 2, d[0], d[1], d[2], R2, 0.0) if l == 0 else
+                 ora.powinv_coeffs(p + 2, l, d[0], d[1], d[2], R2))
+            c = family_weight(l) * delta2 ** l
+            for k in monos(p):
+                for e in range(3):
+                    ke = list(k)
+                    ke[e] += 1
+                    ref[e] += c * (k[e] + 1) * M[k] * a[tuple(ke)]
+        got = np.array(field_q(field_qcoef(mhat, p), p, d, s, delta2, korder))
+        scale = float(np.sum(np.abs(qsrc))) / R2
+        worst = max(worst, float(np.max(np.abs(got - ref))) / scale)
+    assert worst < 1e-12

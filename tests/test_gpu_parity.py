@@ -517,3 +517,33 @@ def test_full_size_blobs4m(ctx):
     s.close()
     for a in dev:
         a.free()
+
+
+@pytest.mark.parametrize("L,nterms", [(0, 5), (0, 12), (2, 9), (1, 12)])
+def test_field_fp32_and_high_order(ctx, oracle, L, nterms):
+    """The field in the radial-derivative form: FP64 within 1e-10 of the oracle's
+    treeforce up to nterms 12 (with and without the family), FP32 within
+    FP32_TOL (relative L2) of FP64, at particles and at points."""
+    P = workloads.gaussian_blobs(30000, seed=47)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 200)
+    o = oracle.tree(P.x, P.y, P.z, 200)
+    m = o.moments(P.q, nterms)
+    tidx = workloads.sample_targets(P.n, 1500, seed=19)
+    ref = o.treeforce(P.q, m, nterms, 0.5, 0.01, 0.01, tidx, kernel_order=L)
+    E64 = s.field(nterms, 0.5, 0.01, 0.01, kernel_order=L)
+    assert rel_max(E64[tidx], ref) < 1e-10
+    E32 = s.field(nterms, 0.5, 0.01, 0.01, kernel_order=L, precision="fp32")
+    assert rel_l2(E32, E64) < FP32_TOL
+    px, py, pz = _points(P, 500, seed=20)
+    p64 = s.field_at(px, py, pz, nterms, 0.5, 0.01, 0.01, kernel_order=L)
+    p32 = s.field_at(px, py, pz, nterms, 0.5, 0.01, 0.01, kernel_order=L, precision="fp32")
+    assert rel_l2(p32, p64) < FP32_TOL
+
+
+def test_field_fp32_reference_fixture(ctx):
+    """tests/treecode_mp.cpp's set (boxparticles(16000), maxmemb 100, nterms 5):
+    the FP32 field within FP32_TOL of the reference's treeforce values."""
+    fx = load_fixture("box16k")
+    s = _system(ctx, fx)
+    E = s.field(5, 0.5, 0.01, 0.01, precision="fp32")[fx["tidx"]]
+    assert rel_l2(E, fx["field_tree"]) < FP32_TOL
