@@ -81,6 +81,10 @@ struct bit_ctx {
   // this device's scratch of bit_eval_potential_noalloc (a static device
   // array shared by every context on the device)
   void* scratch = nullptr;
+  // device tree builds: each generation's result, written by the device into
+  // pinned coherent host memory and waited for by polling (guarded by gen_mu)
+  std::mutex gen_mu;
+  bit::dev::GenResult* gen_host = nullptr;
 };
 
 namespace {
@@ -363,39 +367,89 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
   BIT_TRY(part.ensure(ctx, (size_t)mblocks * 6 * sizeof(double)));
   HIP_TRY(bit::dev::minmax(n, s->sx.as<double>(), s->sy.as<double>(), s->sz.as<double>(),
                            mblocks, part.as<double>(), st));
-  HIP_TRY(bit::dev::build_root(n, mblocks, part.as<double>(), node_view(), st));
-  // generations
+  // generations: the programs of one, and those of the next (started as its
+  // children are written)
   const double lim = 1.0 / std::sqrt(2.0);  // aspectlimit, tree3d.cpp:1371
   const long long node_cap = 4LL * n + 64;
-  DevBuf progs, ops, nch, chunk0, chunks, nchunks, cres, cbase, ores, nkid, start, result;
-  BIT_TRY(result.ensure(ctx, sizeof(bit::dev::GenResult)));
-  BIT_TRY(nchunks.ensure(ctx, sizeof(int)));
+  DevBuf progs, next, cres, nkid, start;
+  // round layouts, alternating between generations: generation g runs from
+  // lay[g & 1], written while generation g - 1 ends
+  struct LayoutBufs {
+    DevBuf ops, nch, cbeg, cprog, ntot;
+    // sized for at most cnt programs
+    int ensure(bit_ctx* c, int n, int cnt) {
+      BIT_TRY(ops.ensure(c, (size_t)2 * cnt * sizeof(bit::dev::PartOpDev)));
+      BIT_TRY(nch.ensure(c, (size_t)2 * cnt * sizeof(int)));
+      BIT_TRY(cbeg.ensure(c, (size_t)(cnt + 1) * sizeof(int)));
+      BIT_TRY(cprog.ensure(
+          c, ((size_t)(n + bit::dev::kPartChunk - 1) / bit::dev::kPartChunk + cnt) * sizeof(int)));
+      BIT_TRY(ntot.ensure(c, 2 * sizeof(int)));
+      return BIT_OK;
+    }
+    bit::dev::RoundLayout view() {
+      return bit::dev::RoundLayout{ops.as<bit::dev::PartOpDev>(), nch.as<int>(), cbeg.as<int>(),
+                                   cprog.as<int>(), ntot.as<int>()};
+    }
+  } lay[2];
+  BIT_TRY(progs.ensure(ctx, sizeof(bit::dev::DevProg)));
+  BIT_TRY(lay[0].ensure(ctx, n, 1));
+  HIP_TRY(bit::dev::build_root(n, mblocks, part.as<double>(), node_view(),
+                               progs.as<bit::dev::DevProg>(), maxmemb, lim, lay[0].view(), st));
+  std::lock_guard<std::mutex> gen_lock(ctx->gen_mu);
+  if (!ctx->gen_host)
+    HIP_TRY(hipHostMalloc((void**)&ctx->gen_host, sizeof(bit::dev::GenResult),
+                          hipHostMallocCoherent));
+  bit::dev::GenResult* hr = ctx->gen_host;
+  // Waits for generation `token`'s result: polls the pinned word, and the
+  // stream now and then (a finished stream without the token is an error).
+  auto wait_gen = [&](int token) -> int {
+    for (long it = 0;; it++) {
+      if (__atomic_load_n(&hr->seq, __ATOMIC_ACQUIRE) == token) return BIT_OK;
+      if ((it & 1023) == 1023) {
+        const hipError_t q = hipStreamQuery(st);
+        if (q == hipSuccess) {
+          if (__atomic_load_n(&hr->seq, __ATOMIC_ACQUIRE) == token) return BIT_OK;
+          return fail(BIT_E_HIP, "tree generation finished without its result");
+        }
+        if (q != hipErrorNotReady) return fail(BIT_E_HIP, hipGetErrorString(q));
+      }
+    }
+  };
   bit::dev::GenResult gr{};
-  int gen_lo = 0, gen_hi = 1, maxdepth = 0, maxsize = n;
+  int gen_lo = 0, gen_hi = 1, depth = 0, maxdepth = 0, maxsize = n;
   bool work = n > maxmemb;
   while (work) {
     const int cnt = gen_hi - gen_lo;
+    depth++;
     BIT_TRY(grow(gen_hi + 8 * cnt, gen_hi));
     const int maxch = (n + bit::dev::kPartChunk - 1) / bit::dev::kPartChunk + cnt;
-    BIT_TRY(progs.ensure(ctx, (size_t)cnt * sizeof(bit::dev::DevProg)));
-    BIT_TRY(ops.ensure(ctx, (size_t)cnt * sizeof(bit::dev::PartOpDev)));
-    BIT_TRY(nch.ensure(ctx, (size_t)cnt * sizeof(int)));
-    BIT_TRY(chunk0.ensure(ctx, (size_t)cnt * sizeof(int)));
-    BIT_TRY(ores.ensure(ctx, (size_t)cnt * sizeof(bit::dev::ChunkResult)));
+    BIT_TRY(next.ensure(ctx, (size_t)8 * cnt * sizeof(bit::dev::DevProg)));
+    LayoutBufs& cur = lay[(depth - 1) & 1];
+    LayoutBufs& nxt = lay[depth & 1];
+    BIT_TRY(nxt.ensure(ctx, n, 8 * cnt));  // free: generation g - 1 has ended
     BIT_TRY(nkid.ensure(ctx, (size_t)cnt * sizeof(int)));
     BIT_TRY(start.ensure(ctx, (size_t)cnt * sizeof(int)));
-    BIT_TRY(chunks.ensure(ctx, (size_t)maxch * sizeof(bit::dev::PartChunk)));
     BIT_TRY(cres.ensure(ctx, (size_t)maxch * sizeof(bit::dev::ChunkResult)));
-    BIT_TRY(cbase.ensure(ctx, (size_t)maxch * sizeof(int2)));
-    bit::dev::BuildScratch w{progs.as<bit::dev::DevProg>(), ops.as<bit::dev::PartOpDev>(),
-                             nch.as<int>(), chunk0.as<int>(), chunks.as<bit::dev::PartChunk>(),
-                             nchunks.as<int>(), cres.as<bit::dev::ChunkResult>(),
-                             cbase.as<int2>(), ores.as<bit::dev::ChunkResult>(), nkid.as<int>(),
-                             start.as<int>(), result.as<bit::dev::GenResult>()};
-    HIP_TRY(bit::dev::build_generation(a, n, gen_lo, gen_hi, maxmemb, lim,
-                                       maxsize <= bit::dev::kNodeProgramMax, node_view(), w, st));
-    HIP_TRY(hipMemcpyAsync(&gr, result.p, sizeof(gr), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    bit::dev::BuildScratch w{progs.as<bit::dev::DevProg>(), next.as<bit::dev::DevProg>(),
+                             cur.view(),
+                             nxt.view(),
+                             cres.as<bit::dev::ChunkResult>(),
+                             nkid.as<int>(),
+                             start.as<int>(),
+                             hr};
+    __atomic_store_n(&hr->seq, 0, __ATOMIC_RELEASE);
+    HIP_TRY(bit::dev::build_generation(a, n, gen_lo, gen_hi, depth, maxmemb, lim, maxsize,
+                                       node_view(), w, st));
+    // The generation's last kernel (k_prog_write) may still run: buffers
+    // resized below are only rewritten by kernels queued after it.
+    BIT_TRY(wait_gen(depth));
+    gr.next = hr->next;
+    gr.failed = hr->failed;
+    gr.maxdepth = hr->maxdepth;
+    gr.splits = hr->splits;
+    gr.maxsize = hr->maxsize;
+    std::swap(progs.p, next.p);
+    std::swap(progs.bytes, next.bytes);
     if (gr.failed) return fail(BIT_E_TREE, "tree split overflow (more than 8 children)");
     if (gr.next > node_cap)
       return fail(BIT_E_TREE,
@@ -406,13 +460,16 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
     work = gr.splits > 0;
     maxsize = gr.maxsize;
   }
-  // the node table on the host
+  // Morton order inside leaves (same keys as the host build), into the sorted
+  // arrays from the buffer each leaf's slots are in, straight from the node table
+  const int nt = gen_hi;
+  HIP_TRY(bit::dev::leaf_morton(a, nt, node_view(), st));
+  // the node table on the host (downloaded behind the device work)
   bit::HostTree& T = s->tree;
   T = bit::HostTree();
   T.n = n;
-  T.numtree = gen_hi;
+  T.numtree = nt;
   T.maxdepth = maxdepth;
-  const int nt = gen_hi;
   T.box.resize((size_t)nt * bit::BOX_FIELDS);
   T.children.resize((size_t)nt * 8);
   T.begin.resize(nt);
@@ -425,26 +482,12 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
   HIP_TRY(hipMemcpyAsync(T.end.data(), nend.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(T.depth.data(), ndepth.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(par.data(), npar.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  T.par.assign(par.begin(), par.end());
-  // Morton order inside leaves (same keys as the host build), into the sorted
-  // arrays from the buffer each leaf's slots are in
-  std::vector<bit::dev::LeafBox> lb;
-  for (int i = 0; i < T.numtree; i++) {
-    if (T.children[(size_t)i * 8] >= 0) continue;
-    const double* f = &T.box[(size_t)i * bit::BOX_FIELDS];
-    lb.push_back(bit::dev::LeafBox{T.begin[i], T.end[i], (int)T.par[i], 0, f[bit::BX0],
-                                   f[bit::BX1], f[bit::BY0],
-                                   f[bit::BY1], f[bit::BZ0], f[bit::BZ1]});
-  }
-  DevBuf d_lb;
-  BIT_TRY(upload(ctx, d_lb, lb.data(), lb.size() * sizeof(bit::dev::LeafBox)));
-  HIP_TRY(bit::dev::leaf_morton(a, (int)lb.size(), d_lb.as<bit::dev::LeafBox>(), st));
   // charges in slot order (the FP32 copy is completed by leaf_local)
   BIT_TRY(s->sq.ensure(ctx, nb));
   BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
   HIP_TRY(bit::dev::set_charges(s->view(), dq, s->sq.as<double>(), s->sloc.as<float4>(), st));
-  HIP_TRY(hipStreamSynchronize(st));  // leaf table and scratch die here
+  HIP_TRY(hipStreamSynchronize(st));  // the node table and scratch die here
+  T.par.assign(par.begin(), par.end());
   return BIT_OK;
 }
 
@@ -564,7 +607,7 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   BIT_TRY(L.far_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
   BIT_TRY(L.near_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
   BIT_TRY(s->lstats.ensure(c, sizeof(bit::dev::ListStats)));
-  BIT_TRY(s->pstat.ensure(c, (size_t)nb * 3 * sizeof(unsigned long long)));
+  BIT_TRY(s->pstat.ensure(c, (size_t)nb * 4 * sizeof(unsigned long long)));
   BIT_TRY(s->err_flag.ensure(c, sizeof(int)));
   HIP_TRY(hipMemsetAsync(L.far_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
   HIP_TRY(hipMemsetAsync(L.near_cnt.p, 0, (size_t)(nb + 1) * sizeof(int), c->stream));
@@ -649,7 +692,7 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   s->stats.near_items = tot[1];
   s->stats.far_pairs = (int64_t)s->hstats.far_pairs;
   s->stats.near_pairs = (int64_t)s->hstats.near_pairs;
-  s->stats.lane_far = 64 * (int64_t)tot[0];
+  s->stats.lane_far = 64 * (int64_t)s->hstats.far_sweeps;
   s->stats.lane_near_pairs = 64 * (int64_t)s->hstats.near_slots;
   s->stats.ms_lists = -1.0;  // filled after the next synchronisation
   s->stats.list_passes = L.passes;
@@ -718,6 +761,7 @@ int bit_ctx_destroy(bit_ctx* c) {
     pool_trim(c, 0);
   }
   if (c->own) (void)hipStreamDestroy(c->own);
+  if (c->gen_host) (void)hipHostFree(c->gen_host);
   delete c;
   return BIT_OK;
 }

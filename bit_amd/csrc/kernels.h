@@ -51,7 +51,7 @@ struct MacParams {
 
 // counters accumulated by the list-count pass
 struct ListStats {
-  unsigned long long far_items, near_items, far_pairs, near_pairs, near_slots;
+  unsigned long long far_items, near_items, far_pairs, near_pairs, near_slots, far_sweeps;
   unsigned long long far_max, near_max;  // largest per-batch entry counts
 };
 
@@ -73,9 +73,6 @@ struct PartOpDev {
   int b, e, dir, src;  // src: buffer holding the range (0: P..Z, 1: tP..tZ)
   double mid;
 };
-struct PartChunk {
-  int op, s0, s1, pad;
-};
 struct ChunkResult {
   int nl, pad;
   double v[12];  // min side0 xyz, min side1 xyz, max side0 xyz, max side1 xyz
@@ -84,7 +81,10 @@ struct LeafBox {
   int b, e, src, pad;  // src: buffer holding the leaf's slots
   double x0, x1, y0, y1, z0, z1;
 };
-constexpr int kPartChunk = 4096;  // slots per work-group of a split
+#ifndef BIT_PART_CHUNK
+#define BIT_PART_CHUNK 4096
+#endif
+constexpr int kPartChunk = BIT_PART_CHUNK;  // slots per work-group of a split
 constexpr int kMortonMax = 4096;  // largest leaf sorted along the Morton curve
 
 hipError_t iota(int n, int* p, hipStream_t s);
@@ -123,42 +123,61 @@ struct GenResult {
   int maxdepth;  // depth of the generation's children
   int splits;    // children with more than maxmemb members (next generation's work)
   int maxsize;   // the largest of them
+  int seq;       // written last (system scope, release): the generation's token
 };
 // Generations whose nodes all hold at most this many slots run one work-group
 // per node (k_node_program) instead of rounds across the generation (4M-blob
-// tree: 6.50 ms with rounds only; 5.90 / 5.86 / 5.73 / 6.34 ms at 16K / 32K /
-// 64K / 128K).
-constexpr int kNodeProgramMax = 65536;
-// Per-generation work arrays, sized for cnt programs and
-// ceil(n / kPartChunk) + cnt chunks.
+// tree, whole build: 4.20 / 4.18 / 4.80 / 5.30 ms at 32K / 64K / 128K / 256K;
+// 256 threads up to 8K / 16K / 32K: 4.25 / 4.18 / 4.24 ms).
+#ifndef BIT_NODE_PROGRAM_MAX
+#define BIT_NODE_PROGRAM_MAX 65536
+#endif
+#ifndef BIT_NODE_PROGRAM_WIDE
+#define BIT_NODE_PROGRAM_WIDE 16384
+#endif
+constexpr int kNodeProgramMax = BIT_NODE_PROGRAM_MAX;
+// ... with 256 threads per node up to this size, 1024 above it.
+constexpr int kNodeProgramWide = BIT_NODE_PROGRAM_WIDE;
+// A generation's rounds, for cnt programs: ops, nch 2 cnt (two buffers);
+// cbeg cnt + 1; cprog ceil(n / kPartChunk) + cnt; ntot 2.
+struct RoundLayout {
+  PartOpDev* ops;  // a round's request per program
+  int* nch;        // its chunks, tagged with the round
+  int* cbeg;       // first chunk of each program's budget
+  int* cprog;      // program of each budget chunk
+  int* ntot;       // [0] budget chunks in all, [1] programs
+};
+// Per-generation work arrays, sized for cnt programs (next: 8 cnt) and
+// ceil(n / kPartChunk) + cnt chunks (cres).
 struct BuildScratch {
-  DevProg* progs;
-  PartOpDev* ops;
-  int* nch;
-  int* chunk0;
-  PartChunk* chunks;
-  int* nchunks;
-  ChunkResult* cres;
-  int2* cbase;
-  ChunkResult* ores;
+  DevProg* progs;     // the generation's programs
+  DevProg* next;      // the next generation's, started by k_prog_write
+  RoundLayout cur;    // this generation's rounds (written by the last generation)
+  RoundLayout nxt;    // the next generation's, written by k_prog_number / k_prog_write
+  ChunkResult* cres;  // per chunk: count results
   int* nkid;
   int* start;
   GenResult* result;
 };
 constexpr int kMaxSplitsPerNode = 7;  // x, then y and two z per x-child
-// Root node from k_minmax's mblocks partial boxes.
+// Root node from k_minmax's mblocks partial boxes, its program and the layout
+// of its rounds.
 hipError_t build_root(int n, int mblocks, const double* part, const BuildNodes& nd,
+                      DevProg* progs, int maxmemb, double lim, const RoundLayout& L,
                       hipStream_t s);
-// Nodes [gen_lo, gen_hi): split programs, run in kMaxSplitsPerNode partition
-// rounds across the generation or (per_node) one work-group per node; children
-// written from gen_hi on; *w.result receives the next generation's end.
-hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int maxmemb,
-                            double lim, bool per_node, const BuildNodes& nd,
+// Nodes [gen_lo, gen_hi) at depth depth - 1, the largest that splits of
+// maxsize slots: split programs, run in kMaxSplitsPerNode partition rounds
+// across the generation or (maxsize <= kNodeProgramMax) one work-group per
+// node; children written from gen_hi on with their programs started in
+// w.next; *w.result receives the next generation's end.
+hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int depth,
+                            int maxmemb, double lim, int maxsize, const BuildNodes& nd,
                             const BuildScratch& w, hipStream_t s);
 
-// Morton order inside leaves, written to the first buffer (P..Z) from the
-// buffer each leaf's slots are in (every slot covered).
-hipError_t leaf_morton(const SlotArrays& a, int nleaves, const LeafBox* leaves, hipStream_t s);
+// Morton order inside the leaves of a node table of numtree nodes, written to
+// the first buffer (P..Z) from the buffer each leaf's slots are in (every
+// slot covered).
+hipError_t leaf_morton(const SlotArrays& a, int numtree, const BuildNodes& nd, hipStream_t s);
 
 hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,
                           const double* z, const double* q, double* sx, double* sy,
@@ -195,8 +214,8 @@ hipError_t far_coeffs64(int numtree, int nterms, const double* mom_hat64, double
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
                        int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s);
-// Totals over nb batches of the counts and of pstat (3 per batch: far pairs,
-// near pairs, near slots) written by lists_count.
+// Totals over nb batches of the counts and of pstat (4 per batch: far pairs,
+// near pairs, k_near32's pair slots per lane, k_farq's sweeps) written by lists_count.
 hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
                       const unsigned long long* pstat, ListStats* out, hipStream_t s);
 hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,

@@ -398,13 +398,17 @@ hipError_t minmax(int n, const double* X, const double* Y, const double* Z, int 
   return hipGetLastError();
 }
 
-// Slots [s0, s1) of one split by a 256-thread work-group: how many go to the
+// Slots [s0, s1) of one split by an NB-thread work-group: how many go to the
 // `<=` side, and both sides' boxes (into *out, written by thread 0; the block
 // is synchronised on return).  Four slots per thread and pass (their loads in
 // flight together); the side decides by select which box a slot widens.
-__device__ __forceinline__ void count_range(const double* __restrict__ X, const double* __restrict__ Y,
-                            const double* __restrict__ Z, int s0, int s1, int dir, double mid,
-                            double (*red)[256], int* cnt, ChunkResult* out) {
+// red / cnt: one entry per wave.
+template <int NB>
+__device__ __forceinline__ void count_range(const double* __restrict__ X,
+                                            const double* __restrict__ Y,
+                                            const double* __restrict__ Z, int s0, int s1, int dir,
+                                            double mid, double (*red)[NB / 64], int* cnt,
+                                            ChunkResult* out) {
   int c = 0;
   // v[k]: min side 0 xyz, min side 1 xyz, max side 0 xyz, max side 1 xyz
   double v[12];
@@ -413,12 +417,12 @@ __device__ __forceinline__ void count_range(const double* __restrict__ X, const 
     v[k] = INFINITY;
     v[6 + k] = -INFINITY;
   }
-  for (int i0 = s0 + threadIdx.x; i0 < s1; i0 += 4 * 256) {
+  for (int i0 = s0 + threadIdx.x; i0 < s1; i0 += 4 * NB) {
     double px[4], py[4], pz[4];
     bool in[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int i = i0 + u * 256;
+      const int i = i0 + u * NB;
       in[u] = i < s1;
       const int j = in[u] ? i : s0;
       px[u] = X[j];
@@ -441,119 +445,95 @@ __device__ __forceinline__ void count_range(const double* __restrict__ X, const 
       }
     }
   }
-  cnt[threadIdx.x] = c;
 #pragma unroll
-  for (int k = 0; k < 12; k++) red[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      cnt[threadIdx.x] += cnt[threadIdx.x + w];
+  for (int o = 32; o > 0; o >>= 1) {
+    c += __shfl_xor(c, o);
 #pragma unroll
-      for (int k = 0; k < 12; k++)
-        red[k][threadIdx.x] = k < 6 ? fmin(red[k][threadIdx.x], red[k][threadIdx.x + w])
-                                    : fmax(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+    for (int k = 0; k < 12; k++) {
+      const double w = __shfl_xor(v[k], o);
+      v[k] = k < 6 ? fmin(v[k], w) : fmax(v[k], w);
     }
-    __syncthreads();
   }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    cnt[wave] = c;
+#pragma unroll
+    for (int k = 0; k < 12; k++) red[k][wave] = v[k];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     ChunkResult r;
-    r.nl = cnt[0];
+    r.nl = 0;
     r.pad = 0;
-    for (int k = 0; k < 12; k++) r.v[k] = red[k][0];
+    for (int w = 0; w < NB / 64; w++) r.nl += cnt[w];
+    for (int k = 0; k < 12; k++) {
+      double x = red[k][0];
+      for (int w = 1; w < NB / 64; w++) x = k < 6 ? fmin(x, red[k][w]) : fmax(x, red[k][w]);
+      r.v[k] = x;
+    }
     *out = r;
   }
   __syncthreads();
 }
 
-// Per chunk: count_range of the chunk's slots.
+// A round's chunks: the programs' fixed budgets of kPartChunk-slot chunks,
+// laid out once per generation (cbeg[idx] .. cbeg[idx + 1], chunk c's program
+// cprog[c]).  Round r's request of program idx is ops[idx] over the first
+// nch[idx] chunks of its budget, where nch holds (r << kTagShift) | chunks and
+// an entry tagged with another round counts as no request: rounds alternate
+// between two (ops, nch) buffers, and a program writes the next round's entry
+// only while it has requests.  Returns false for a chunk with no work.
+constexpr int kTagShift = 20;
+__device__ __forceinline__ bool round_chunk(const int* __restrict__ cprog,
+                                            const int* __restrict__ cbeg,
+                                            const int* __restrict__ nch,
+                                            const int* __restrict__ ntot, int r, int c, int& idx,
+                                            int& k, int& nc) {
+  if (c >= *ntot) return false;  // grid sized for the most chunks
+  idx = cprog[c];
+  k = c - cbeg[idx];
+  const int v = nch[idx];
+  nc = (v >> kTagShift) == r ? v & ((1 << kTagShift) - 1) : 0;
+  return k < nc;
+}
+
+// Per chunk of a round: count_range of the chunk's slots.
 __global__ __launch_bounds__(256) void k_part_count(SlotArrays sa, const PartOpDev* __restrict__ ops,
-                                                    const PartChunk* __restrict__ chunks,
-                                                    ChunkResult* __restrict__ cres,
-                                                    const int* __restrict__ nchunks) {
-  __shared__ double red[12][256];
-  __shared__ int cnt[256];
-  if (nchunks && (int)blockIdx.x >= *nchunks) return;  // grid sized for the most chunks
-  const PartChunk ch = chunks[blockIdx.x];
-  const PartOpDev op = ops[ch.op];
-  count_range(op.src ? sa.tX : sa.X, op.src ? sa.tY : sa.Y, op.src ? sa.tZ : sa.Z, ch.s0, ch.s1,
+                                                    const int* __restrict__ cprog,
+                                                    const int* __restrict__ cbeg,
+                                                    const int* __restrict__ nch,
+                                                    const int* __restrict__ ntot, int r,
+                                                    ChunkResult* __restrict__ cres) {
+  __shared__ double red[12][4];
+  __shared__ int cnt[4];
+  int idx, k, nc;
+  if (!round_chunk(cprog, cbeg, nch, ntot, r, blockIdx.x, idx, k, nc)) return;
+  const PartOpDev op = ops[idx];
+  const int s0 = op.b + k * kPartChunk, s1 = min(op.e, s0 + kPartChunk);
+  count_range<256>(op.src ? sa.tX : sa.X, op.src ? sa.tY : sa.Y, op.src ? sa.tZ : sa.Z, s0, s1,
               op.dir, op.mid, red, cnt, cres + blockIdx.x);
 }
 
-// Per op (one thread): chunk bases in chunk order, totals and boxes.
-// One wavefront per op: exclusive scan of the chunks' left/right counts (the
-// chunks' bases in the op's left and right runs) and the op's child boxes.
-__global__ __launch_bounds__(64) void k_part_offsets(const PartChunk* __restrict__ chunks,
-                                                     const int* __restrict__ op_chunk0,
-                                                     const int* __restrict__ op_nchunk,
-                                                     const ChunkResult* __restrict__ cres,
-                                                     int2* __restrict__ cbase,
-                                                     ChunkResult* __restrict__ ores) {
-  const int o = blockIdx.x, lane = threadIdx.x;
-  const int c0 = op_chunk0[o], nc = op_nchunk[o];
-  int nl_run = 0, nr_run = 0;
-  double v[12];
-  for (int k = 0; k < 6; k++) {
-    v[k] = INFINITY;
-    v[6 + k] = -INFINITY;
-  }
-  for (int g = 0; g < nc; g += 64) {
-    const int c = c0 + g + lane;
-    const bool in = g + lane < nc;
-    int l = 0, r = 0;
-    if (in) {
-      const ChunkResult& x = cres[c];
-      l = x.nl;
-      r = (chunks[c].s1 - chunks[c].s0) - l;
-      for (int k = 0; k < 6; k++) {
-        v[k] = fmin(v[k], x.v[k]);
-        v[6 + k] = fmax(v[6 + k], x.v[6 + k]);
-      }
-    }
-    int il = l, ir = r;  // inclusive scan across the wave
-    for (int off = 1; off < 64; off <<= 1) {
-      const int tl = __shfl_up(il, off), tr = __shfl_up(ir, off);
-      if (lane >= off) {
-        il += tl;
-        ir += tr;
-      }
-    }
-    if (in) cbase[c] = make_int2(nl_run + il - l, nr_run + ir - r);
-    nl_run += __shfl(il, 63);
-    nr_run += __shfl(ir, 63);
-  }
-  for (int off = 32; off > 0; off >>= 1)
-    for (int k = 0; k < 12; k++) {
-      const double w = __shfl_down(v[k], off);
-      v[k] = k < 6 ? fmin(v[k], w) : fmax(v[k], w);
-    }
-  if (lane == 0) {
-    ChunkResult r;
-    r.nl = nl_run;
-    r.pad = 0;
-    for (int k = 0; k < 12; k++) r.v[k] = v[k];
-    ores[o] = r;
-  }
-}
-
-// Stable scatter of slots [s0, s1) of one split into the other buffer by a
-// 256-thread work-group: the `<=` slots to op_b + lrun + rank, the others to
-// op_b + opnl + rrun + rank.  Tiles of 1024 slots, four rows of 256 (slot
-// t0 + 256 u + thread); a slot's rank among its tile's `<=` slots comes from
+// Stable scatter of slots [s0, s1) of one split into the other buffer by an
+// NB-thread work-group: the `<=` slots to op_b + lrun + rank, the others to
+// op_b + opnl + rrun + rank.  Tiles of 4 NB slots, four rows of NB (slot
+// t0 + NB u + thread); a slot's rank among its tile's `<=` slots comes from
 // the wave ballots of all four rows, one barrier per tile, and every load of
 // the tile is in flight before it.  The block is synchronised on return.
-__device__ __forceinline__ void scatter_range(const SlotBuf& a, const SlotBuf& t, int s0, int s1, int dir,
-                              double mid, int op_b, int opnl, int lrun, int rrun,
-                              int (*wsum)[4]) {
+template <int NB>
+__device__ __forceinline__ void scatter_range(const SlotBuf& a, const SlotBuf& t, int s0, int s1,
+                                              int dir, double mid, int op_b, int opnl, int lrun,
+                                              int rrun, int (*wsum)[NB / 64]) {
   const double* __restrict__ C = dir == 0 ? a.X : (dir == 1 ? a.Y : a.Z);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (int t0 = s0; t0 < s1; t0 += 1024) {
+  for (int t0 = s0; t0 < s1; t0 += 4 * NB) {
     bool in[4], left[4];
     int before[4], pv[4];
     double xv[4], yv[4], zv[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int i = t0 + u * 256 + threadIdx.x;
+      const int i = t0 + u * NB + threadIdx.x;
       in[u] = i < s1;
       const int j = in[u] ? i : s0;
       pv[u] = a.P[j];
@@ -574,14 +554,14 @@ __device__ __forceinline__ void scatter_range(const SlotBuf& a, const SlotBuf& t
     for (int u = 0; u < 4; u++) {
       int woff = 0, tot = 0;
 #pragma unroll
-      for (int w = 0; w < 4; w++) {
+      for (int w = 0; w < NB / 64; w++) {
         const int c = wsum[u][w];
         woff += w < wave ? c : 0;
         tot += c;
       }
       if (in[u]) {
         const int lrank = rowoff + woff + before[u];
-        const int pos = u * 256 + threadIdx.x;  // slot's place in the tile
+        const int pos = u * NB + threadIdx.x;  // slot's place in the tile
         const int dst = left[u] ? op_b + lrun + lrank : op_b + opnl + rrun + (pos - lrank);
         t.P[dst] = pv[u];
         t.X[dst] = xv[u];
@@ -590,26 +570,11 @@ __device__ __forceinline__ void scatter_range(const SlotBuf& a, const SlotBuf& t
       }
       rowoff += tot;
     }
-    const int ntile = min(1024, s1 - t0);
+    const int ntile = min(4 * NB, s1 - t0);
     lrun += rowoff;
     rrun += ntile - rowoff;
     __syncthreads();  // wsum is rewritten by the next tile
   }
-}
-
-// Per chunk: scatter_range from the chunk's bases in its op's two runs.
-__global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartOpDev* __restrict__ ops,
-                                                      const PartChunk* __restrict__ chunks,
-                                                      const int2* __restrict__ cbase,
-                                                      const ChunkResult* __restrict__ ores,
-                                                      const int* __restrict__ nchunks) {
-  __shared__ int wsum[4][4];  // [row][wave] counts of `<=` slots
-  if (nchunks && (int)blockIdx.x >= *nchunks) return;
-  const PartChunk ch = chunks[blockIdx.x];
-  const PartOpDev op = ops[ch.op];
-  const int2 base = cbase[blockIdx.x];
-  scatter_range(slot_buf(sa, op.src), slot_buf(sa, op.src ^ 1), ch.s0, ch.s1, op.dir, op.mid,
-                op.b, ores[ch.op].nl, base.x, base.y, wsum);
 }
 
 // ---------------------------------------------------------------------------
@@ -822,29 +787,6 @@ __device__ void apply(DevProg& P, const ChunkResult& r) {
   advance(P);
 }
 
-// Exclusive scan of v[0..cnt) by one 1024-thread work-group, in index order;
-// returns the total.  `red` is a 1024-int LDS array.
-__device__ int block_scan(int* v, int* out, int cnt, int* red) {
-  int carry = 0;
-  for (int base = 0; base < cnt; base += 1024) {
-    const int i = base + threadIdx.x;
-    const int x = i < cnt ? v[i] : 0;
-    red[threadIdx.x] = x;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int y = threadIdx.x >= (unsigned)off ? red[threadIdx.x - off] : 0;
-      __syncthreads();
-      red[threadIdx.x] += y;
-      __syncthreads();
-    }
-    if (i < cnt) out[i] = carry + red[threadIdx.x] - x;
-    const int tot = red[1023];
-    __syncthreads();
-    carry += tot;
-  }
-  return carry;
-}
-
 }  // namespace
 
 __global__ void k_root_init(int n, int mblocks, const double* __restrict__ part, BuildNodes nd) {
@@ -863,14 +805,10 @@ __global__ void k_root_init(int n, int mblocks, const double* __restrict__ part,
   nd.par[0] = 0;
 }
 
-__global__ __launch_bounds__(64) void k_prog_init(int gen_lo, int cnt, BuildNodes nd,
-                                                  DevProg* __restrict__ progs, int maxmemb,
-                                                  double lim, GenResult* __restrict__ res) {
-  const int idx = blockIdx.x * 64 + threadIdx.x;
-  if (idx == 0) res->failed = 0;
-  if (idx >= cnt) return;
-  DevProg& P = progs[idx];
-  const int i = gen_lo + idx;
+// A node's program: treemake's loop body run to its first split request
+// (pc = kDone at once for a node of at most maxmemb members).
+__device__ void prog_start(DevProg& P, int b, int e, int par, const double* box, int maxmemb,
+                           double lim) {
   P.nchild = 0;
   P.failed = 0;
   P.pending = 0;
@@ -880,135 +818,291 @@ __global__ __launch_bounds__(64) void k_prog_init(int gen_lo, int cnt, BuildNode
   P.maxmemb = maxmemb;
   P.lim = lim;
   for (int sl = 0; sl < 16; sl++) P.s[sl].used = 0;
-  if (nd.end[i] - nd.begin[i] <= maxmemb) {
+  if (e - b <= maxmemb) {
     P.pc = kDone;
     return;
   }
   P.parent.used = 1;
-  P.parent.b = nd.begin[i];
-  P.parent.e = nd.end[i];
-  P.parent.par = nd.par[i];
-  for (int k = 0; k < kF; k++) P.parent.f[k] = nd.box[(size_t)i * kF + k];
+  P.parent.b = b;
+  P.parent.e = e;
+  P.parent.par = par;
+  for (int k = 0; k < kF; k++) P.parent.f[k] = box[k];
   P.pc = 0;
   advance(P);
-  P.pending = 0;
 }
 
-// One round, per program: apply last round's result, advance to the next
-// request (op index = program index), and its chunk count.
-__global__ __launch_bounds__(64) void k_prog_advance(int cnt, DevProg* __restrict__ progs,
-                                                     const ChunkResult* __restrict__ ores,
-                                                     PartOpDev* __restrict__ ops,
-                                                     int* __restrict__ nch) {
-  const int idx = blockIdx.x * 64 + threadIdx.x;
-  if (idx >= cnt) return;
-  DevProg& P = progs[idx];
-  if (P.pending) apply(P, ores[idx]);
-  int c = 0;
+// Chunks of kPartChunk slots a node of this size is given for its requests
+// (none for a node of at most maxmemb members, which does not split).
+__device__ __forceinline__ int chunk_budget(int size, int maxmemb) {
+  return size > maxmemb ? (size + kPartChunk - 1) / kPartChunk : 0;
+}
+
+// Round 0's entry of a started program: its first request, or none.
+__device__ __forceinline__ void first_request(const DevProg& P, int j, int cnt,
+                                              const RoundLayout& L) {
   if (P.pc != kDone) {
-    ops[idx] = P.op;
-    c = (P.op.e - P.op.b + kPartChunk - 1) / kPartChunk;
-    P.pending = 1;
+    L.ops[j] = P.op;
+    L.nch[j] = max(1, (P.op.e - P.op.b + kPartChunk - 1) / kPartChunk);  // tag 0
   } else {
-    P.pending = 0;
+    L.nch[j] = 0;
   }
-  nch[idx] = c;
+  L.nch[cnt + j] = -1;  // no round's tag
 }
 
-// The round's requests as chunks of at most kPartChunk slots, in program order.
-__global__ __launch_bounds__(1024) void k_prog_layout(int cnt, const PartOpDev* __restrict__ ops,
-                                                      int* __restrict__ nch,
-                                                      int* __restrict__ chunk0,
-                                                      PartChunk* __restrict__ chunks,
-                                                      int* __restrict__ nchunks) {
-  __shared__ int red[1024];
-  const int total = block_scan(nch, chunk0, cnt, red);
-  for (int idx = threadIdx.x; idx < cnt; idx += 1024) {
-    const int c = nch[idx];
-    if (!c) continue;
-    const PartOpDev o = ops[idx];
-    for (int k = 0; k < c; k++) {
-      const int s0 = o.b + k * kPartChunk;
-      chunks[chunk0[idx] + k] = PartChunk{idx, s0, min(o.e, s0 + kPartChunk), 0};
-    }
-  }
-  if (threadIdx.x == 0) *nchunks = total;
-}
-
-// After the generation's last round, per program: the last result applied;
-// the number of children (0 and a failure flag for a program that overflowed
-// its block or did not finish).
-__global__ __launch_bounds__(64) void k_prog_last(int cnt, DevProg* __restrict__ progs,
-                                                  const ChunkResult* __restrict__ ores,
-                                                  int* __restrict__ nkid,
-                                                  GenResult* __restrict__ res) {
-  const int idx = blockIdx.x * 64 + threadIdx.x;
-  if (idx >= cnt) return;
-  DevProg& P = progs[idx];
-  if (P.pending) apply(P, ores[idx]);
-  P.pending = 0;
-  const bool bad = P.failed || P.pc != kDone;
-  if (bad) atomicOr(&res->failed, 1);
-  nkid[idx] = bad ? 0 : P.nchild;
-}
-
-// Children numbered in node order (a prefix sum over the generation) and
-// written to the node table.
-__global__ __launch_bounds__(1024) void k_prog_number(int gen_lo, int gen_hi, int cnt,
-                                                      int maxmemb,
-                                                      const DevProg* __restrict__ progs,
-                                                      BuildNodes nd, int* __restrict__ nkid,
-                                                      int* __restrict__ start,
-                                                      GenResult* __restrict__ res) {
-  __shared__ int red[1024];
-  __shared__ int maxd, big, bigsize;
+// The root's program (generation 0) and its rounds' layout.
+__global__ __launch_bounds__(256) void k_prog_root(BuildNodes nd, DevProg* __restrict__ progs,
+                                                   int maxmemb, double lim, RoundLayout L) {
+  __shared__ int bud;
   if (threadIdx.x == 0) {
-    maxd = 0;
-    big = 0;
-    bigsize = 0;
+    prog_start(progs[0], nd.begin[0], nd.end[0], nd.par[0], nd.box, maxmemb, lim);
+    bud = chunk_budget(nd.end[0] - nd.begin[0], maxmemb);
+    L.cbeg[0] = 0;
+    L.cbeg[1] = bud;
+    L.ntot[0] = bud;
+    L.ntot[1] = 1;
+    first_request(progs[0], 0, 1, L);
   }
   __syncthreads();
-  const int total = block_scan(nkid, start, cnt, red);
-  for (int idx = threadIdx.x; idx < cnt; idx += 1024) {
-    const DevProg& P = progs[idx];
-    const int nc = nkid[idx];
-    if (!nc) continue;
-    const int parent = gen_lo + idx;
-    const int st = gen_hi + start[idx];
-    const int d = nd.depth[parent] + 1;
-    for (int sl = 0; sl < nc; sl++) {
-      const int g = st + sl;
-      for (int k = 0; k < kF; k++) nd.box[(size_t)g * kF + k] = P.s[sl].f[k];
-      nd.begin[g] = P.s[sl].b;
-      nd.end[g] = P.s[sl].e;
-      nd.depth[g] = d;
-      nd.par[g] = P.s[sl].par;
-      for (int c = 0; c < 8; c++) nd.child[(size_t)g * 8 + c] = -1;
-      if (P.s[sl].e - P.s[sl].b > maxmemb) {
-        atomicAdd(&big, 1);
-        atomicMax(&bigsize, P.s[sl].e - P.s[sl].b);
+  for (int k = threadIdx.x; k < bud; k += 256) L.cprog[k] = 0;
+}
+
+// Exclusive scan by one 1024-thread work-group of val(i), i in [0, cnt), in
+// index order; emit(i, x, offset) is called for every i.  Returns the total.
+template <typename Val, typename Emit>
+__device__ int block_scan_f(int cnt, int* red, Val val, Emit emit) {
+  int carry = 0;
+  for (int base = 0; base < cnt; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int x = i < cnt ? val(i) : 0;
+    red[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int y = threadIdx.x >= (unsigned)off ? red[threadIdx.x - off] : 0;
+      __syncthreads();
+      red[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < cnt) emit(i, x, carry + red[threadIdx.x] - x);
+    const int tot = red[1023];
+    __syncthreads();
+    carry += tot;
+  }
+  return carry;
+}
+
+// Per chunk of round r: the chunk's bases in its request's two runs (the
+// `<=` counts of the request's earlier chunks, all full) and the request's
+// `<=` total from the round's count results, then scatter_range.  The
+// request's first chunk also forms the request's result (total and both
+// sides' boxes), applies it to the program and writes the program's next
+// request into the other buffers (ops_n / nch_n, tagged r + 1); a request
+// larger than the program's budget fails the program.
+__global__ __launch_bounds__(256) void k_part_scatter(SlotArrays sa, const PartOpDev* __restrict__ ops,
+                                                      const int* __restrict__ cprog,
+                                                      const int* __restrict__ cbeg,
+                                                      const int* __restrict__ nch,
+                                                      const int* __restrict__ ntot, int r,
+                                                      const ChunkResult* __restrict__ cres,
+                                                      DevProg* __restrict__ progs,
+                                                      PartOpDev* __restrict__ ops_n,
+                                                      int* __restrict__ nch_n) {
+  __shared__ int wsum[4][4];  // [row][wave] counts of `<=` slots
+  __shared__ int part[2][4];
+  __shared__ double vred[12][4];
+  int idx, k, nc;
+  if (!round_chunk(cprog, cbeg, nch, ntot, r, blockIdx.x, idx, k, nc)) return;
+  const PartOpDev op = ops[idx];
+  const int c0 = cbeg[idx];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int before = 0, total = 0;
+  for (int i = threadIdx.x; i < nc; i += 256) {
+    const int v = cres[c0 + i].nl;
+    total += v;
+    before += i < k ? v : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    before += __shfl_xor(before, o);
+    total += __shfl_xor(total, o);
+  }
+  if (lane == 0) {
+    part[0][wave] = before;
+    part[1][wave] = total;
+  }
+  if (k == 0) {  // the request's boxes: min / max over its chunks
+    double v[12];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      v[q] = INFINITY;
+      v[6 + q] = -INFINITY;
+    }
+    for (int i = threadIdx.x; i < nc; i += 256) {
+      const ChunkResult& x = cres[c0 + i];
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        v[q] = fmin(v[q], x.v[q]);
+        v[6 + q] = fmax(v[6 + q], x.v[6 + q]);
       }
     }
-    for (int c = 0; c < nc; c++) nd.child[(size_t)parent * 8 + c] = st + P.kids[c];
-    atomicMax(&maxd, d);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int q = 0; q < 12; q++) {
+        const double w = __shfl_xor(v[q], o);
+        v[q] = q < 6 ? fmin(v[q], w) : fmax(v[q], w);
+      }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 12; q++) vred[q][wave] = v[q];
   }
   __syncthreads();
+  const int lrun = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+  const int opnl = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+  const int s0 = op.b + k * kPartChunk, s1 = min(op.e, s0 + kPartChunk);
+  scatter_range<256>(slot_buf(sa, op.src), slot_buf(sa, op.src ^ 1), s0, s1, op.dir, op.mid,
+                     op.b, opnl, lrun, k * kPartChunk - lrun, wsum);
+  if (k == 0 && threadIdx.x == 0) {
+    ChunkResult res;
+    res.nl = opnl;
+    res.pad = 0;
+    for (int q = 0; q < 12; q++) {
+      double x = vred[q][0];
+      for (int w = 1; w < 4; w++) x = q < 6 ? fmin(x, vred[q][w]) : fmax(x, vred[q][w]);
+      res.v[q] = x;
+    }
+    DevProg& P = progs[idx];
+    apply(P, res);
+    if (P.pc != kDone) {
+      const int c = max(1, (P.op.e - P.op.b + kPartChunk - 1) / kPartChunk);
+      if (c > cbeg[idx + 1] - c0) {
+        P.failed = 1;
+        P.pc = kDone;
+      } else {
+        ops_n[idx] = P.op;
+        nch_n[idx] = ((r + 1) << kTagShift) | c;
+      }
+    }
+  }
+}
+
+// After a generation: each program's children (none, and the failure flag,
+// for a program that overflowed its block or did not finish), numbered in
+// node order by a prefix sum over the generation, as the reference numbers
+// them; the next generation's end and work; when the next generation runs in
+// rounds, its chunk budgets laid out in program (child) order in L.
+// res may be host memory: its fields, then seq = depth (system scope).
+__global__ __launch_bounds__(1024) void k_prog_number(int gen_hi, int depth, int cnt, int maxmemb,
+                                                      const DevProg* __restrict__ progs,
+                                                      int* __restrict__ nkid,
+                                                      int* __restrict__ start,
+                                                      GenResult* __restrict__ res,
+                                                      RoundLayout L) {
+  __shared__ int red[1024];
+  __shared__ int big, bigsize, failed;
   if (threadIdx.x == 0) {
+    big = 0;
+    bigsize = 0;
+    failed = 0;
+  }
+  __syncthreads();
+  const int total = block_scan_f(
+      cnt, red,
+      [&](int i) {
+        const DevProg& P = progs[i];
+        if (P.failed || P.pc != kDone) {
+          atomicOr(&failed, 1);
+          return 0;
+        }
+        for (int sl = 0; sl < P.nchild; sl++) {
+          const int sz = P.s[sl].e - P.s[sl].b;
+          if (sz > maxmemb) {
+            atomicAdd(&big, 1);
+            atomicMax(&bigsize, sz);
+          }
+        }
+        return P.nchild;
+      },
+      [&](int i, int x, int off) {
+        nkid[i] = x;
+        start[i] = off;
+      });
+  __syncthreads();
+  if (bigsize > kNodeProgramMax) {  // the next generation runs in rounds
+    // (thread t handles the same programs as in the scan above: it reads
+    // the nkid / start it wrote)
+    const int chunks = block_scan_f(
+        cnt, red,
+        [&](int i) {
+          const DevProg& P = progs[i];
+          int b = 0;
+          for (int sl = 0; sl < nkid[i]; sl++) b += chunk_budget(P.s[sl].e - P.s[sl].b, maxmemb);
+          return b;
+        },
+        [&](int i, int x, int off) {
+          const DevProg& P = progs[i];
+          const int j0 = start[i];
+          for (int sl = 0; sl < nkid[i]; sl++) {
+            const int b = chunk_budget(P.s[sl].e - P.s[sl].b, maxmemb);
+            L.cbeg[j0 + sl] = off;
+            for (int k = 0; k < b; k++) L.cprog[off + k] = j0 + sl;
+            off += b;
+          }
+        });
+    if (threadIdx.x == 0) {
+      L.cbeg[total] = chunks;
+      L.ntot[0] = chunks;
+    }
+  }
+  if (threadIdx.x == 0) {
+    L.ntot[1] = total;
     res->next = gen_hi + total;
-    res->maxdepth = maxd;
+    res->failed = failed;
+    res->maxdepth = total > 0 ? depth : 0;
     res->splits = big;
     res->maxsize = bigsize;
+    __hip_atomic_store(&res->seq, depth, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+}
+
+// The generation's children: thread (program, slot) writes node gen_hi +
+// start + slot of the node table and the parent's child `slot`, and starts
+// the node's program for the next generation (with its round-0 entry in L).
+__global__ __launch_bounds__(256) void k_prog_write(int gen_lo, int gen_hi, int depth, int cnt,
+                                                    int maxmemb, double lim,
+                                                    const DevProg* __restrict__ progs,
+                                                    const int* __restrict__ nkid,
+                                                    const int* __restrict__ start, BuildNodes nd,
+                                                    DevProg* __restrict__ next, RoundLayout L) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int idx = t >> 3, sl = t & 7;
+  if (idx >= cnt || sl >= nkid[idx]) return;
+  const DevProg& P = progs[idx];
+  const int st = start[idx];
+  const int g = gen_hi + st + sl;
+  const DevSlot& S = P.s[sl];
+  for (int k = 0; k < kF; k++) nd.box[(size_t)g * kF + k] = S.f[k];
+  nd.begin[g] = S.b;
+  nd.end[g] = S.e;
+  nd.depth[g] = depth;
+  nd.par[g] = S.par;
+  for (int c = 0; c < 8; c++) nd.child[(size_t)g * 8 + c] = -1;
+  nd.child[(size_t)(gen_lo + idx) * 8 + sl] = gen_hi + st + P.kids[sl];
+  DevProg& Q = next[st + sl];
+  prog_start(Q, S.b, S.e, S.par, S.f, maxmemb, lim);
+  first_request(Q, st + sl, L.ntot[1], L);
 }
 
 // A generation of small nodes: one work-group runs one node's whole split
 // program, each requested split a count pass and a stable scatter over the
-// node's own range (no rounds across the generation).
-__global__ __launch_bounds__(256) void k_node_program(SlotArrays sa, int cnt,
-                                                      DevProg* __restrict__ progs) {
-  __shared__ double red[12][256];
-  __shared__ int cnts[256];
-  __shared__ int wsum[4][4];
+// node's own range (no rounds across the generation).  The generation's
+// time is its largest node's chain of splits: NB = 1024 threads for
+// generations with large nodes, 256 otherwise.
+template <int NB>
+__global__ __launch_bounds__(NB) void k_node_program(SlotArrays sa, int cnt,
+                                                     DevProg* __restrict__ progs) {
+  constexpr int NW = NB / 64;
+  __shared__ double red[12][NW];
+  __shared__ int cnts[NW];
+  __shared__ int wsum[4][NW];
   __shared__ ChunkResult res;
   __shared__ PartOpDev op;
   __shared__ int done;
@@ -1024,47 +1118,49 @@ __global__ __launch_bounds__(256) void k_node_program(SlotArrays sa, int cnt,
     if (done) break;
     const PartOpDev o = op;
     const SlotBuf a = slot_buf(sa, o.src);
-    count_range(a.X, a.Y, a.Z, o.b, o.e, o.dir, o.mid, red, cnts, &res);
-    scatter_range(a, slot_buf(sa, o.src ^ 1), o.b, o.e, o.dir, o.mid, o.b, res.nl, 0, 0, wsum);
+    count_range<NB>(a.X, a.Y, a.Z, o.b, o.e, o.dir, o.mid, red, cnts, &res);
+    scatter_range<NB>(a, slot_buf(sa, o.src ^ 1), o.b, o.e, o.dir, o.mid, o.b, res.nl, 0, 0,
+                      wsum);
     if (threadIdx.x == 0) apply(P, res);
     __syncthreads();
   }
 }
 
 hipError_t build_root(int n, int mblocks, const double* part, const BuildNodes& nd,
+                      DevProg* progs, int maxmemb, double lim, const RoundLayout& L,
                       hipStream_t s) {
   hipLaunchKernelGGL(k_root_init, dim3(1), dim3(64), 0, s, n, mblocks, part, nd);
+  hipLaunchKernelGGL(k_prog_root, dim3(1), dim3(256), 0, s, nd, progs, maxmemb, lim, L);
   return hipGetLastError();
 }
 
-hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int maxmemb,
-                            double lim, bool per_node, const BuildNodes& nd,
+hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, int depth,
+                            int maxmemb, double lim, int maxsize, const BuildNodes& nd,
                             const BuildScratch& w, hipStream_t s) {
   const int cnt = gen_hi - gen_lo;
   const int maxch = (n + kPartChunk - 1) / kPartChunk + cnt;
-  const int pblocks = (cnt + 63) / 64;
-  hipLaunchKernelGGL(k_prog_init, dim3(pblocks), dim3(64), 0, s, gen_lo, cnt, nd, w.progs,
-                     maxmemb, lim, w.result);
-  if (per_node)
-    hipLaunchKernelGGL(k_node_program, dim3(cnt), dim3(256), 0, s, a, cnt, w.progs);
-  for (int r = 0; r < (per_node ? 0 : kMaxSplitsPerNode); r++) {
-    hipLaunchKernelGGL(k_prog_advance, dim3(pblocks), dim3(64), 0, s, cnt, w.progs,
-                       (const ChunkResult*)w.ores, w.ops, w.nch);
-    hipLaunchKernelGGL(k_prog_layout, dim3(1), dim3(1024), 0, s, cnt, (const PartOpDev*)w.ops,
-                       w.nch, w.chunk0, w.chunks, w.nchunks);
-    hipLaunchKernelGGL(k_part_count, dim3(maxch), dim3(256), 0, s, a, (const PartOpDev*)w.ops,
-                       (const PartChunk*)w.chunks, w.cres, (const int*)w.nchunks);
-    hipLaunchKernelGGL(k_part_offsets, dim3(cnt), dim3(64), 0, s, (const PartChunk*)w.chunks,
-                       (const int*)w.chunk0, (const int*)w.nch, (const ChunkResult*)w.cres,
-                       w.cbase, w.ores);
-    hipLaunchKernelGGL(k_part_scatter, dim3(maxch), dim3(256), 0, s, a, (const PartOpDev*)w.ops,
-                       (const PartChunk*)w.chunks, (const int2*)w.cbase,
-                       (const ChunkResult*)w.ores, (const int*)w.nchunks);
+  if (maxsize <= kNodeProgramWide) {
+    hipLaunchKernelGGL(k_node_program<256>, dim3(cnt), dim3(256), 0, s, a, cnt, w.progs);
+  } else if (maxsize <= kNodeProgramMax) {
+    hipLaunchKernelGGL(k_node_program<1024>, dim3(cnt), dim3(1024), 0, s, a, cnt, w.progs);
+  } else {
+    const RoundLayout& L = w.cur;
+    for (int r = 0; r < kMaxSplitsPerNode; r++) {
+      const int cur = (r & 1) * cnt, nxt = ((r + 1) & 1) * cnt;
+      hipLaunchKernelGGL(k_part_count, dim3(maxch), dim3(256), 0, s, a,
+                         (const PartOpDev*)L.ops + cur, (const int*)L.cprog, (const int*)L.cbeg,
+                         (const int*)L.nch + cur, (const int*)L.ntot, r, w.cres);
+      hipLaunchKernelGGL(k_part_scatter, dim3(maxch), dim3(256), 0, s, a,
+                         (const PartOpDev*)L.ops + cur, (const int*)L.cprog, (const int*)L.cbeg,
+                         (const int*)L.nch + cur, (const int*)L.ntot, r,
+                         (const ChunkResult*)w.cres, w.progs, L.ops + nxt, L.nch + nxt);
+    }
   }
-  hipLaunchKernelGGL(k_prog_last, dim3(pblocks), dim3(64), 0, s, cnt, w.progs,
-                     (const ChunkResult*)w.ores, w.nkid, w.result);
-  hipLaunchKernelGGL(k_prog_number, dim3(1), dim3(1024), 0, s, gen_lo, gen_hi, cnt, maxmemb,
-                     (const DevProg*)w.progs, nd, w.nkid, w.start, w.result);
+  hipLaunchKernelGGL(k_prog_number, dim3(1), dim3(1024), 0, s, gen_hi, depth, cnt, maxmemb,
+                     (const DevProg*)w.progs, w.nkid, w.start, w.result, w.nxt);
+  hipLaunchKernelGGL(k_prog_write, dim3((cnt * 8 + 255) / 256), dim3(256), 0, s, gen_lo, gen_hi,
+                     depth, cnt, maxmemb, lim, (const DevProg*)w.progs, (const int*)w.nkid,
+                     (const int*)w.start, nd, w.next, w.nxt);
   return hipGetLastError();
 }
 
@@ -1089,10 +1185,15 @@ __device__ __forceinline__ unsigned quant_dev(double v, double lo, double hi) {
   return (unsigned)t;
 }
 
-__global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays sa,
-                                                     const LeafBox* __restrict__ leaves) {
+// One block per node of the table; internal nodes return at once.  A leaf's
+// slots are in the buffer par names; its box is the node's (treeinit fields).
+__global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays sa, BuildNodes nd) {
   __shared__ unsigned long long key[kMortonMax];
-  const LeafBox L = leaves[blockIdx.x];
+  const int node = blockIdx.x;
+  if (nd.child[(size_t)node * 8] >= 0) return;
+  const double* f = nd.box + (size_t)node * kF;
+  const LeafBox L{nd.begin[node], nd.end[node], nd.par[node], 0,
+                  f[FX0],         f[FX1],       f[FY0],       f[FY1], f[FZ0], f[FZ1]};
   const int n = L.e - L.b;
   const SlotBuf a = slot_buf(sa, L.src), out = slot_buf(sa, 0);
   if (n <= 1 || n > kMortonMax) {
@@ -1155,9 +1256,9 @@ __global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays sa,
   }
 }
 
-hipError_t leaf_morton(const SlotArrays& a, int nleaves, const LeafBox* leaves, hipStream_t s) {
-  if (nleaves == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_leaf_morton, dim3(nleaves), dim3(256), 0, s, a, leaves);
+hipError_t leaf_morton(const SlotArrays& a, int numtree, const BuildNodes& nd, hipStream_t s) {
+  if (numtree == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_leaf_morton, dim3(numtree), dim3(256), 0, s, a, nd);
   return hipGetLastError();
 }
 
@@ -1418,6 +1519,14 @@ __device__ __forceinline__ bool mac_accept(double px, double py, double pz, doub
 // (entries beyond a region's capacity are counted, not written).
 enum { kListCount = 0, kListFill = 1, kListStage = 2 };
 
+// Near items with at most this many targets are evaluated source-parallel
+// (FP32: measured best among 24, 32, 40 and 48 at the 4M-blob point; FP64: 32).
+#ifndef BIT_NEAR_SPARSE32
+#define BIT_NEAR_SPARSE32 40
+#endif
+constexpr int kNearSparse32 = BIT_NEAR_SPARSE32;
+constexpr int kNearSparse = 32;
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int stack_cap,
                                               int* __restrict__ far_cnt,
@@ -1449,7 +1558,8 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   }
   int sp = 1;
   int nf = 0, nn = 0;
-  unsigned long long far_pairs = 0, near_pairs = 0, near_slots = 0;
+  unsigned long long far_pairs = 0, near_pairs = 0, near_slots = 0, far_sweeps = 0;
+  int fpend = -1;  // targets of an item k_farq may still pair with the next one
   constexpr bool kWrite = MODE != kListCount;
   const size_t fbase = MODE == kListFill    ? (size_t)far_off[b]
                        : MODE == kListStage ? (size_t)b * (size_t)cap_far
@@ -1487,7 +1597,16 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
           far_items[fbase + (nf - 63 + lane)] = Item{fbm, fbn, 0};
       }
       nf++;
-      far_pairs += __popcll(am);
+      const int pc = __popcll(am);
+      far_pairs += pc;
+      // k_farq's units: an item joins the previous unpaired one when the two
+      // have at most 64 targets together, else it starts a sweep of its own
+      if (fpend >= 0 && fpend + pc <= kBatch) {
+        fpend = -1;
+      } else {
+        far_sweeps++;
+        fpend = pc;
+      }
     }
     const unsigned long long rm = m & ~am;
     if (rm) {
@@ -1502,8 +1621,13 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
         }
         nn++;
         const int2 r = make_int2(grange[2 * node], grange[2 * node + 1]);
-        near_pairs += (unsigned long long)__popcll(rm) * (unsigned long long)(r.y - r.x);
-        near_slots += (unsigned long long)(r.y - r.x);
+        const int pc = __popcll(rm);
+        const unsigned long long len = (unsigned long long)(r.y - r.x);
+        near_pairs += (unsigned long long)pc * len;
+        // pair slots per lane of k_near32: the leaf's length for a
+        // target-parallel item, 4 per 64 sources per four targets otherwise
+        near_slots += pc <= kNearSparse32 ? (unsigned long long)((pc + 3) / 4) * ((len + 63) / 64) * 4
+                                          : len;
       } else {
         int nch = 0;  // children fill the first slots, -1 after
 #pragma unroll
@@ -1533,9 +1657,10 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   if (MODE != kListFill && lane == 0) {
     far_cnt[b] = nf;
     near_cnt[b] = nn;
-    pstat[3 * b] = far_pairs;
-    pstat[3 * b + 1] = near_pairs;
-    pstat[3 * b + 2] = near_slots;
+    pstat[4 * b] = far_pairs;
+    pstat[4 * b + 1] = near_pairs;
+    pstat[4 * b + 2] = near_slots;
+    pstat[4 * b + 3] = far_sweeps;
   }
 }
 
@@ -1544,25 +1669,26 @@ __global__ __launch_bounds__(1024) void k_list_stats(int nb, const int* __restri
                                                      const int* __restrict__ near_cnt,
                                                      const unsigned long long* __restrict__ pstat,
                                                      ListStats* __restrict__ out) {
-  __shared__ unsigned long long red[7][1024];
-  unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};  // five sums, two maxima
+  __shared__ unsigned long long red[8][1024];
+  unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // six sums, two maxima
   for (int b = threadIdx.x; b < nb; b += 1024) {
     const unsigned long long f = (unsigned long long)far_cnt[b];
     const unsigned long long n = (unsigned long long)near_cnt[b];
     v[0] += f;
     v[1] += n;
-    v[2] += pstat[3 * b];
-    v[3] += pstat[3 * b + 1];
-    v[4] += pstat[3 * b + 2];
-    v[5] = f > v[5] ? f : v[5];
-    v[6] = n > v[6] ? n : v[6];
+    v[2] += pstat[4 * b];
+    v[3] += pstat[4 * b + 1];
+    v[4] += pstat[4 * b + 2];
+    v[5] += pstat[4 * b + 3];
+    v[6] = f > v[6] ? f : v[6];
+    v[7] = n > v[7] ? n : v[7];
   }
-  for (int k = 0; k < 7; k++) red[k][threadIdx.x] = v[k];
+  for (int k = 0; k < 8; k++) red[k][threadIdx.x] = v[k];
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
-      for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-      for (int k = 5; k < 7; k++) {
+      for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+      for (int k = 6; k < 8; k++) {
         const unsigned long long o = red[k][threadIdx.x + w];
         red[k][threadIdx.x] = o > red[k][threadIdx.x] ? o : red[k][threadIdx.x];
       }
@@ -1571,7 +1697,7 @@ __global__ __launch_bounds__(1024) void k_list_stats(int nb, const int* __restri
   }
   if (threadIdx.x == 0)
     *out = ListStats{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0], red[5][0],
-                     red[6][0]};
+                     red[6][0], red[7][0]};
 }
 
 hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
@@ -1966,13 +2092,6 @@ __device__ __forceinline__ double pair64(double px, double py, double pz, double
   else return s.w * ri;
 }
 
-// Near items with at most this many targets are evaluated source-parallel
-// (FP32: measured best among 24, 32, 40 and 48 at the 4M-blob point; FP64: 32).
-#ifndef BIT_NEAR_SPARSE32
-#define BIT_NEAR_SPARSE32 40
-#endif
-constexpr int kNearSparse32 = BIT_NEAR_SPARSE32;
-constexpr int kNearSparse = 32;
 
 __device__ __forceinline__ float bcast_lane(float x, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));

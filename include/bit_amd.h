@@ -68,8 +68,11 @@ typedef struct bit_stats {
   int64_t far_pairs;   /* cluster-target Taylor evaluations */
   int64_t far_items;   /* batch list entries (far), incl. masked lanes */
   int64_t near_items;  /* batch list entries (near) */
-  int64_t lane_far;    /* 64 x far_items: lane slots the far kernel ran */
-  int64_t lane_near_pairs; /* 64 x sum over near items of leaf size */
+  int64_t lane_far;    /* lane slots of the FP32 far kernel: 64 x its sweeps (two
+                        * consecutive items of at most 64 targets share one) */
+  int64_t lane_near_pairs; /* lane-pair slots of the FP32 near kernel: 64 x the
+                            * leaf size per target-parallel item, 4 per 64 sources
+                            * per four targets for items of at most 40 targets */
   double ms_tree, ms_upload, ms_moments, ms_lists, ms_far, ms_near; /* last timings */
   /* tree traversals of the last list build: 1 when every batch fit the
    * per-batch regions sized from the previous build, else 2 (count, fill) */

@@ -493,6 +493,40 @@ def _popcount64(m):
     return ((m * np.uint64(0x0101010101010101)) >> np.uint64(56)).astype(np.int64)
 
 
+def test_lane_stats_count_the_kernels_slots(ctx):
+    """lane_far and lane_near_pairs count what the FP32 kernels run: k_farq's
+    sweeps (an item shares one with the next when the two have at most 64
+    targets together) and k_near32's pair slots (the leaf's length per lane
+    for a target-parallel item; 4 per 64 sources per four targets for items
+    of at most 40 targets), restated here from the lists."""
+    P = workloads.gaussian_blobs(60000, seed=13)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 200)
+    s.potential(9, 0.4, 0.5, 1e-5, None, "fp32")
+    st = s.stats()
+    lst = s.lists(9, 0.4, 1e-5, None)
+    t = s.export_tree()
+    size = (t.end - t.begin).astype(np.int64)
+    fo, fpc = lst["far_off"], _popcount64(lst["far_mask"])
+    sweeps = 0
+    for b in range(len(fo) - 1):
+        pend = -1
+        for pc in fpc[fo[b]:fo[b + 1]]:
+            if pend >= 0 and pend + pc <= 64:
+                pend = -1
+            else:
+                sweeps += 1
+                pend = pc
+    npc, nlen = _popcount64(lst["near_mask"]), size[lst["near_node"]]
+    sparse = npc <= 40
+    slots = int(np.sum(nlen[~sparse]))
+    slots += int(np.sum((npc[sparse] + 3) // 4 * ((nlen[sparse] + 63) // 64) * 4))
+    assert st["lane_far"] == 64 * sweeps
+    assert st["lane_near_pairs"] == 64 * slots
+    assert st["near_pairs"] == int(np.sum(npc * nlen))
+    assert sparse.any() and (~sparse).any() and sweeps < len(fpc)
+    s.close()
+
+
 def test_full_size_blobs4m(ctx):
     """The 4M-particle, 16-blob workload at its tuned point (kappa 0.5, delta
     0.005, p 11, acc 0.25, N0 500): the FP32 evaluation stays within 1e-6
