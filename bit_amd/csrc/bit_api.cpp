@@ -85,6 +85,12 @@ struct bit_ctx {
   // pinned coherent host memory and waited for by polling (guarded by gen_mu)
   std::mutex gen_mu;
   bit::dev::GenResult* gen_host = nullptr;
+  // ... and the node table's download: pinned staging, a side stream (default
+  // priority) and the event it waits for
+  void* pin = nullptr;
+  size_t pin_bytes = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t built = nullptr;
 };
 
 namespace {
@@ -227,6 +233,7 @@ struct bit_system {
   bit::HostTree tree;
   int stack_cap = 0;
   int nleaves = 0;
+  bool view_ready = false;  // node arrays and sloc written by the device build
   // tree + sources
   DevBuf center, child, range, scale, perm, leaves, sx, sy, sz, sq, sloc;
   // moments
@@ -460,34 +467,69 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
     work = gr.splits > 0;
     maxsize = gr.maxsize;
   }
-  // Morton order inside leaves (same keys as the host build), into the sorted
-  // arrays from the buffer each leaf's slots are in, straight from the node table
+  // The node table goes to the host on a side stream (pinned staging) while
+  // the device sorts the leaves (Morton order, same keys as the host build,
+  // straight from the node table), gathers the charges and writes the
+  // evaluation's node arrays.
   const int nt = gen_hi;
+  const size_t off_child = (size_t)nt * bit::BOX_FIELDS * sizeof(double);
+  const size_t off_int = off_child + (size_t)nt * 8 * sizeof(int);  // begin, end, depth, par
+  const size_t pin_need = off_int + (size_t)4 * nt * sizeof(int);
+  if (!ctx->side) HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  if (!ctx->built) HIP_TRY(hipEventCreateWithFlags(&ctx->built, hipEventDisableTiming));
+  if (ctx->pin_bytes < pin_need) {
+    if (ctx->pin) HIP_TRY(hipHostFree(ctx->pin));
+    ctx->pin = nullptr;
+    ctx->pin_bytes = 0;
+    HIP_TRY(hipHostMalloc(&ctx->pin, pin_need, hipHostMallocDefault));
+    ctx->pin_bytes = pin_need;
+  }
+  char* pin = static_cast<char*>(ctx->pin);
+  int* pint = reinterpret_cast<int*>(pin + off_int);
+  HIP_TRY(hipEventRecord(ctx->built, st));
+  HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->built, 0));
+  HIP_TRY(hipMemcpyAsync(pin, nbox.p, off_child, hipMemcpyDeviceToHost, ctx->side));
+  HIP_TRY(hipMemcpyAsync(pin + off_child, nchild.p, (size_t)nt * 8 * sizeof(int),
+                         hipMemcpyDeviceToHost, ctx->side));
+  HIP_TRY(hipMemcpyAsync(pint, nbegin.p, nt * sizeof(int), hipMemcpyDeviceToHost, ctx->side));
+  HIP_TRY(hipMemcpyAsync(pint + nt, nend.p, nt * sizeof(int), hipMemcpyDeviceToHost, ctx->side));
+  HIP_TRY(hipMemcpyAsync(pint + 2 * nt, ndepth.p, nt * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->side));
+  HIP_TRY(hipMemcpyAsync(pint + 3 * nt, npar.p, nt * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->side));
   HIP_TRY(bit::dev::leaf_morton(a, nt, node_view(), st));
-  // the node table on the host (downloaded behind the device work)
+  // charges in slot order, the node arrays and the FP32 leaf-local copy
+  BIT_TRY(s->sq.ensure(ctx, nb));
+  BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
+  BIT_TRY(s->center.ensure(ctx, (size_t)nt * 4 * sizeof(double)));
+  BIT_TRY(s->scale.ensure(ctx, (size_t)nt * sizeof(double)));
+  BIT_TRY(s->range.ensure(ctx, (size_t)nt * 2 * sizeof(int)));
+  BIT_TRY(s->child.ensure(ctx, (size_t)nt * 8 * sizeof(int)));
+  s->tree.n = n;
+  s->tree.numtree = nt;
+  HIP_TRY(bit::dev::set_charges(s->view(), dq, s->sq.as<double>(), s->sloc.as<float4>(), st));
+  HIP_TRY(bit::dev::tree_view(nt, node_view(), s->view(), s->sloc.as<float4>(), st));
+  // the host's copy of the node table, from the staging, behind the device work
+  HIP_TRY(hipStreamSynchronize(ctx->side));
   bit::HostTree& T = s->tree;
   T = bit::HostTree();
   T.n = n;
   T.numtree = nt;
   T.maxdepth = maxdepth;
-  T.box.resize((size_t)nt * bit::BOX_FIELDS);
-  T.children.resize((size_t)nt * 8);
-  T.begin.resize(nt);
-  T.end.resize(nt);
-  T.depth.resize(nt);
-  std::vector<int> par(nt);
-  HIP_TRY(hipMemcpyAsync(T.box.data(), nbox.p, T.box.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(T.children.data(), nchild.p, T.children.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(T.begin.data(), nbegin.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(T.end.data(), nend.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(T.depth.data(), ndepth.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(par.data(), npar.p, nt * sizeof(int), hipMemcpyDeviceToHost, st));
-  // charges in slot order (the FP32 copy is completed by leaf_local)
-  BIT_TRY(s->sq.ensure(ctx, nb));
-  BIT_TRY(s->sloc.ensure(ctx, (size_t)n * sizeof(float4)));
-  HIP_TRY(bit::dev::set_charges(s->view(), dq, s->sq.as<double>(), s->sloc.as<float4>(), st));
+  T.box.assign(reinterpret_cast<const double*>(pin),
+               reinterpret_cast<const double*>(pin) + (size_t)nt * bit::BOX_FIELDS);
+  const int* pch = reinterpret_cast<const int*>(pin + off_child);
+  T.children.assign(pch, pch + (size_t)nt * 8);
+  T.begin.assign(pint, pint + nt);
+  T.end.assign(pint + nt, pint + 2 * nt);
+  T.depth.assign(pint + 2 * nt, pint + 3 * nt);
+  T.par.assign(pint + 3 * nt, pint + 4 * nt);
+  int nleaves = 0;
+  for (int i = 0; i < nt; i++) nleaves += T.children[(size_t)i * 8] < 0;
+  s->nleaves = nleaves;
+  s->stack_cap = 8 * (T.maxdepth + 2) + 8;
+  s->view_ready = true;
   HIP_TRY(hipStreamSynchronize(st));  // the node table and scratch die here
-  T.par.assign(par.begin(), par.end());
   return BIT_OK;
 }
 
@@ -537,6 +579,7 @@ int ensure_host_perm(const bit_system* cs) {
 // Upload the node table and create the system; sorted sources come from
 // `sorted` (host, 4 arrays) or are gathered on the device from d_src.
 int finish_system(bit_ctx* ctx, bit_system* s) {
+  if (s->view_ready) return BIT_OK;  // written by the device build
   const bit::HostTree& T = s->tree;
   const int nt = T.numtree;
   std::vector<double> cen(4 * (size_t)nt), scl(nt);
@@ -762,6 +805,9 @@ int bit_ctx_destroy(bit_ctx* c) {
   }
   if (c->own) (void)hipStreamDestroy(c->own);
   if (c->gen_host) (void)hipHostFree(c->gen_host);
+  if (c->pin) (void)hipHostFree(c->pin);
+  if (c->built) (void)hipEventDestroy(c->built);
+  if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
   return BIT_OK;
 }

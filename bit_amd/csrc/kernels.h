@@ -178,6 +178,11 @@ hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, 
 // the first buffer (P..Z) from the buffer each leaf's slots are in (every
 // slot covered).
 hipError_t leaf_morton(const SlotArrays& a, int numtree, const BuildNodes& nd, hipStream_t s);
+// tv's center / scale / range / child (writable device arrays of numtree
+// nodes) from a device-built node table, then sloc's positions relative to
+// each leaf's midpoint (tv's sorted sources and charges in place).
+hipError_t tree_view(int numtree, const BuildNodes& nd, const TreeView& tv, float4* sloc,
+                     hipStream_t s);
 
 hipError_t gather_sources(const TreeView& tv, const double* x, const double* y,
                           const double* z, const double* q, double* sx, double* sy,

@@ -1256,6 +1256,47 @@ __global__ __launch_bounds__(256) void k_leaf_morton(SlotArrays sa, BuildNodes n
   }
 }
 
+// The evaluation's node arrays from a device-built node table: one thread
+// per node.
+__global__ void k_tree_view(int numtree, BuildNodes nd, double4* __restrict__ center,
+                            double* __restrict__ scale, int2* __restrict__ range,
+                            int* __restrict__ child) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numtree) return;
+  const double* f = nd.box + (size_t)i * kF;
+  center[i] = make_double4(f[FXMID], f[FYMID], f[FZMID], f[FSQR]);
+  scale[i] = f[FSQR] > 0 ? sqrt(f[FSQR]) : 1.0;
+  range[i] = make_int2(nd.begin[i], nd.end[i]);
+  for (int c = 0; c < 8; c++) child[(size_t)i * 8 + c] = nd.child[(size_t)i * 8 + c];
+}
+
+// One block per node, leaves only: FP32 source copy relative to the leaf's
+// box midpoint (as k_leaf_local).
+__global__ void k_leaf_local_nodes(const int* __restrict__ child,
+                                   const double4* __restrict__ center,
+                                   const int2* __restrict__ range, const double* __restrict__ sx,
+                                   const double* __restrict__ sy, const double* __restrict__ sz,
+                                   const double* __restrict__ sq, float4* sloc) {
+  const int leaf = blockIdx.x;
+  if (child[(size_t)leaf * 8] >= 0) return;
+  const double4 c = center[leaf];
+  const int2 r = range[leaf];
+  for (int t = r.x + threadIdx.x; t < r.y; t += blockDim.x)
+    sloc[t] = make_float4((float)(sx[t] - c.x), (float)(sy[t] - c.y), (float)(sz[t] - c.z),
+                          (float)sq[t]);
+}
+
+hipError_t tree_view(int numtree, const BuildNodes& nd, const TreeView& tv, float4* sloc,
+                     hipStream_t s) {
+  if (numtree == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tree_view, dim3((numtree + 255) / 256), dim3(256), 0, s, numtree, nd,
+                     const_cast<double4*>(tv.center), const_cast<double*>(tv.scale),
+                     const_cast<int2*>(tv.range), const_cast<int*>(tv.child));
+  hipLaunchKernelGGL(k_leaf_local_nodes, dim3(numtree), dim3(256), 0, s, tv.child, tv.center,
+                     tv.range, tv.sx, tv.sy, tv.sz, tv.sq, sloc);
+  return hipGetLastError();
+}
+
 hipError_t leaf_morton(const SlotArrays& a, int numtree, const BuildNodes& nd, hipStream_t s) {
   if (numtree == 0) return hipSuccess;
   hipLaunchKernelGGL(k_leaf_morton, dim3(numtree), dim3(256), 0, s, a, nd);
