@@ -1599,7 +1599,9 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   }
   int sp = 1;
   int nf = 0, nn = 0;
-  unsigned long long far_pairs = 0, near_pairs = 0, near_slots = 0, far_sweeps = 0;
+  // per-batch counts: far pairs <= 64 numtree, slots <= 4 n, sweeps <= numtree
+  unsigned far_pairs = 0, far_sweeps = 0;
+  unsigned long long near_pairs = 0, near_slots = 0;
   int fpend = -1;  // targets of an item k_farq may still pair with the next one
   constexpr bool kWrite = MODE != kListCount;
   const size_t fbase = MODE == kListFill    ? (size_t)far_off[b]
@@ -1670,17 +1672,16 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
         near_slots += pc <= kNearSparse32 ? (unsigned long long)((pc + 3) / 4) * ((len + 63) / 64) * 4
                                           : len;
       } else {
-        int nch = 0;  // children fill the first slots, -1 after
+        int kid = kids[0];  // child `lane` (lanes 0..7)
 #pragma unroll
-        for (int q = 0; q < 8; q++) nch += (kids[q] >= 0 && nch == q) ? 1 : 0;
+        for (int q = 1; q < 8; q++) kid = (lane == q) ? kids[q] : kid;
+        // children fill the first slots, -1 after: the leading valid ones
+        const int nch = __builtin_ctzll(~ballot(lane < 8 && kid >= 0));
         if (sp + nch > stack_cap) {
           if (lane == 0) atomicOr(err_flag, 1);
           break;
         }
         if (lane < nch) {  // child `lane` goes to slot sp + nch - 1 - lane
-          int kid = kids[0];
-#pragma unroll
-          for (int q = 1; q < 8; q++) kid = (lane == q) ? kids[q] : kid;
           st_node[sp + (nch - 1 - lane)] = kid;
           st_mask[sp + (nch - 1 - lane)] = rm;
         }
