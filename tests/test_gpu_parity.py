@@ -195,6 +195,30 @@ def test_potential_fp32_near_scaled_and_unscaled(ctx, kappa):
     assert rel_l2(p32, p64) < FP32_TOL
 
 
+def test_points_fp32_on_particles_scaled_domain(ctx):
+    """Points exactly on particles in a domain 20 times the blobs' extent,
+    screened (kappa 0.5, so the FP32 near field runs in scaled coordinates)
+    and unregularized (delta2 0): targets and sources round their offsets
+    from the leaf centre by the same operations, so a coincident pair has a
+    zero offset and is skipped as in FP64 (a missed pair would add about
+    q / 1e-7).  FP32 stays within the FP32 tolerance of FP64, and every point
+    within 1e-5 of the largest |phi| (cancelling charges leave some points
+    with small potentials)."""
+    P = workloads.gaussian_blobs(30000, seed=43)
+    sc = 20.0
+    x, y, z = P.x * sc, P.y * sc, P.z * sc
+    s = bit_amd.System(ctx, x, y, z, P.q, 300)
+    rng = np.random.default_rng(8)
+    k = rng.choice(P.n, 600, replace=False)
+    px, py, pz = x[k].copy(), y[k].copy(), z[k].copy()
+    p64 = s.potential_at(px, py, pz, 9, 0.5, 0.5, 0.0, 0.01, "fp64")
+    p32 = s.potential_at(px, py, pz, 9, 0.5, 0.5, 0.0, 0.01, "fp32")
+    assert np.all(np.isfinite(p32)) and np.all(np.isfinite(p64))
+    assert rel_l2(p32, p64) < FP32_TOL
+    assert np.max(np.abs(p32 - p64)) < 1e-5 * np.max(np.abs(p64))
+    s.close()
+
+
 def test_field_fp64_matches_treeforce(ctx):
     fx = load_fixture("box16k")
     s = _system(ctx, fx)
