@@ -86,7 +86,7 @@ EXPORTS = [
     "bit_copy_on_device", "bit_eval_potential_points", "bit_eval_field_points",
     "bit_direct_field", "bit_direct_field_points", "bit_eval_potential_noalloc",
     "bit_krylov_create", "bit_krylov_destroy", "bit_krylov_status", "bit_krylov_solution",
-    "bit_serve_round",
+    "bit_serve_round", "bit_valu_ceiling",
 ]
 
 
@@ -150,6 +150,7 @@ def _load():
     L.bit_krylov_status.argtypes = [vp, vp, vp, vp]
     L.bit_krylov_solution.argtypes = [vp, vp]
     L.bit_serve_round.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp, vp]
+    L.bit_valu_ceiling.argtypes = [vp, dp, dp]
     for f in ("bit_eval_potential_points", "bit_eval_field_points"):
         getattr(L, f).argtypes = [vp, C.POINTER(Params), C.c_int, vp, vp, vp, vp]
     L.bit_tree_build.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.POINTER(vp)]
@@ -239,6 +240,13 @@ class Context:
 
     def sync(self):
         _check(lib.bit_ctx_sync(self.h))
+
+    def valu_ceiling(self) -> tuple[float, float]:
+        """(FP32, FP64) lane TFLOP/s of independent FMA chains on this device:
+        the issue ceiling of the library's instruction mix (no packed FP32)."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib.bit_valu_ceiling(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     @property
     def bytes_in_use(self) -> int:

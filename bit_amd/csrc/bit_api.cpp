@@ -824,6 +824,38 @@ int bit_ctx_sync(bit_ctx* c) {
   return BIT_OK;
 }
 
+int bit_valu_ceiling(bit_ctx* c, double* fp32_tflops, double* fp64_tflops) {
+  if (!c || !fp32_tflops || !fp64_tflops) return fail(BIT_E_INVALID, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  int ncu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int blocks = 8 * ncu, iters = 20000;  // eight waves per SIMD
+  DevBuf out;
+  BIT_TRY(out.ensure(c, (size_t)blocks * 256 * sizeof(double)));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double res[2] = {0.0, 0.0};
+  hipError_t err = hipSuccess;
+  for (int fp64 = 0; fp64 < 2 && err == hipSuccess; fp64++)
+    for (int rep = 0; rep < 2 && err == hipSuccess; rep++) {  // the first run warms up
+      err = hipEventRecord(e0, c->stream);
+      if (err == hipSuccess) err = bit::dev::fma_ceiling(blocks, iters, fp64, out.p, c->stream);
+      if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (err == hipSuccess && ms > 0)  // 8 FMAs = 16 flops per thread and step
+        res[fp64] = 16.0 * iters * (double)blocks * 256 / (ms * 1e-3) / 1e12;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != hipSuccess) return fail(BIT_E_HIP, hipGetErrorString(err));
+  *fp32_tflops = res[0];
+  *fp64_tflops = res[1];
+  return BIT_OK;
+}
+
 size_t bit_ctx_bytes_in_use(const bit_ctx* c) { return c ? c->in_use.load() : 0; }
 
 int bit_ctx_set_host_threads(bit_ctx* c, int threads) {

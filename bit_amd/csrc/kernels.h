@@ -178,6 +178,9 @@ hipError_t build_generation(const SlotArrays& a, int n, int gen_lo, int gen_hi, 
 // the first buffer (P..Z) from the buffer each leaf's slots are in (every
 // slot covered).
 hipError_t leaf_morton(const SlotArrays& a, int numtree, const BuildNodes& nd, hipStream_t s);
+// blocks x 256 threads of eight independent FMA chains of iters steps each
+// (FP64 or FP32), one result per thread into out.
+hipError_t fma_ceiling(int blocks, int iters, bool fp64, void* out, hipStream_t s);
 // tv's center / scale / range / child (writable device arrays of numtree
 // nodes) from a device-built node table, then sloc's positions relative to
 // each leaf's midpoint (tv's sorted sources and charges in place).

@@ -3168,5 +3168,35 @@ hipError_t put_desc(const EvalDesc& d, EvalDesc* dst, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// The FMA issue ceiling of this build's instruction mix (no packed FP32): eight
+// independent FMA chains per thread, eight waves per SIMD.  The kernels' VALU
+// throughput is compared against it.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_fma_ceiling(T* __restrict__ out, int iters, T b, T c) {
+  T a[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) a[i] = T(threadIdx.x) * T(1e-3) + T(i);
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) a[i] = fmaT(a[i], b, c);
+  }
+  T sum = T(0);
+#pragma unroll
+  for (int i = 0; i < 8; i++) sum += a[i];
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
+hipError_t fma_ceiling(int blocks, int iters, bool fp64, void* out, hipStream_t s) {
+  if (fp64)
+    hipLaunchKernelGGL(k_fma_ceiling<double>, dim3(blocks), dim3(256), 0, s, (double*)out, iters,
+                       0.999999, 1e-6);
+  else
+    hipLaunchKernelGGL(k_fma_ceiling<float>, dim3(blocks), dim3(256), 0, s, (float*)out, iters,
+                       0.999999f, 1e-6f);
+  return hipGetLastError();
+}
+
 }  // namespace dev
 }  // namespace bit
