@@ -3170,17 +3170,22 @@ hipError_t put_desc(const EvalDesc& d, EvalDesc* dst, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // The FMA issue ceiling of this build's instruction mix (no packed FP32): eight
-// independent FMA chains per thread, eight waves per SIMD.  The kernels' VALU
-// throughput is compared against it.
+// independent FMA chains per thread, eight waves per SIMD, in the fastest FMA
+// form, v_fmac (the accumulator tied to the destination, both multiplicands in
+// VGPRs).  The kernels' VALU throughput is compared against it.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_fma_ceiling(T* __restrict__ out, int iters, T b, T c) {
-  T a[8];
+  T a[8], q[8];
 #pragma unroll
-  for (int i = 0; i < 8; i++) a[i] = T(threadIdx.x) * T(1e-3) + T(i);
+  for (int i = 0; i < 8; i++) {
+    a[i] = T(threadIdx.x) * T(1e-3) + T(i);
+    q[i] = c * T(i + 1) + T(threadIdx.x) * T(1e-6);
+  }
+  const T m = b + T(threadIdx.x) * T(1e-9);  // per-lane values: VGPR operands
   for (int it = 0; it < iters; it++) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) a[i] = fmaT(a[i], b, c);
+    for (int i = 0; i < 8; i++) a[i] = fmaT(q[i], m, a[i]);
   }
   T sum = T(0);
 #pragma unroll

@@ -287,8 +287,10 @@ int bit_gmres(int n, bit_matvec_fn matvec, void *user, const double *b, double *
 /* Diagnostic, with no counterpart in the reference: the FMA issue ceiling of
  * this library's instruction mix on the context's device (no packed FP32),
  * measured with eight independent FMA chains per thread at eight waves per
- * SIMD; lane TFLOP/s (an FMA is two flops) for FP32 and FP64.  The kernels'
- * measured VALU throughput is quoted against it beside the published peaks. */
+ * SIMD in the fastest FMA form (v_fmac: accumulator tied to the destination,
+ * VGPR multiplicands); lane TFLOP/s (an FMA is two flops) for FP32 and FP64.
+ * The kernels' measured VALU throughput is quoted against it beside the
+ * published peaks. */
 int bit_valu_ceiling(bit_ctx *ctx, double *fp32_tflops, double *fp64_tflops);
 
 int bit_direct_potential(bit_system *sys, double kappa, double delta2, int ntargets,
