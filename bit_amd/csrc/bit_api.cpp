@@ -252,6 +252,10 @@ struct bit_system {
   ListSet pl, ql;
   DevBuf scan_tmp, lstats, pstat, err_flag, stage_far, stage_near;
   bit::dev::ListStats hstats{};
+  // the last list build's pair statistics (4 totals on the device), copied
+  // into stats when they are asked for
+  DevBuf pair_dev;
+  mutable bool pair_pending = false;
   // external point targets (batch order) and their results
   DevBuf qx, qy, qz, qperm, qphi, qout, qdesc;
   bit::dev::EvalDesc qhdesc{};
@@ -681,7 +685,6 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
                                   L.near_cnt.as<int>(), s->pstat.as<unsigned long long>(),
                                   s->err_flag.as<int>(), c->stream));
   HIP_TRY(bit::dev::list_stats(nb, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
-                               s->pstat.as<unsigned long long>(),
                                s->lstats.as<bit::dev::ListStats>(), c->stream));
   size_t tmp = 0;
   HIP_TRY(bit::dev::exclusive_scan(L.far_cnt.as<int>(), L.far_off.as<int>(), nb + 1, nullptr, tmp,
@@ -716,13 +719,26 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
                                     cap_near, s->stage_far.as<bit::dev::Item>(),
                                     s->stage_near.as<bit::dev::Item>(),
                                     L.far_items.as<bit::dev::Item>(),
-                                    L.near_items.as<bit::dev::Item>(), c->stream));
+                                    L.near_items.as<bit::dev::Item>(), tv.range,
+                                    s->pstat.as<unsigned long long>(), c->stream));
   else
     HIP_TRY(bit::dev::lists_fill(tv, mp, s->stack_cap, L.far_off.as<int>(),
                                  L.near_off.as<int>(), L.far_items.as<bit::dev::Item>(),
                                  L.near_items.as<bit::dev::Item>(), s->err_flag.as<int>(),
                                  c->stream));
   s->t_lists.stop(c->stream);
+  // pair statistics: counted by the count traversal, else by the compaction;
+  // a staged build that fell back to the fill takes them from the final lists
+  if (staged && !fits)
+    HIP_TRY(bit::dev::lists_compact(nb, nullptr, nullptr, L.far_off.as<int>(),
+                                    L.near_off.as<int>(), 0, 0, nullptr, nullptr,
+                                    L.far_items.as<bit::dev::Item>(),
+                                    L.near_items.as<bit::dev::Item>(), tv.range,
+                                    s->pstat.as<unsigned long long>(), c->stream));
+  BIT_TRY(s->pair_dev.ensure(c, 4 * sizeof(unsigned long long)));
+  HIP_TRY(bit::dev::pair_stats(nb, s->pstat.as<unsigned long long>(),
+                               s->pair_dev.as<unsigned long long>(), c->stream));
+  s->pair_pending = true;
   L.passes = fits ? 1 : 2;
   const long long total = (long long)tot[0] + tot[1];
   c->list_cap[kind][0].store(stage_cap(s->hstats.far_max, nb, total));
@@ -733,10 +749,6 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   L.sub = p->mac_sub;
   s->stats.far_items = tot[0];
   s->stats.near_items = tot[1];
-  s->stats.far_pairs = (int64_t)s->hstats.far_pairs;
-  s->stats.near_pairs = (int64_t)s->hstats.near_pairs;
-  s->stats.lane_far = 64 * (int64_t)s->hstats.far_sweeps;
-  s->stats.lane_near_pairs = 64 * (int64_t)s->hstats.near_slots;
   s->stats.ms_lists = -1.0;  // filled after the next synchronisation
   s->stats.list_passes = L.passes;
   return BIT_OK;
@@ -1534,6 +1546,18 @@ int bit_eval_field_points(bit_system* s, const bit_params* p, int m, const doubl
 
 int bit_system_stats(const bit_system* s, bit_stats* out) {
   if (!s || !out) return fail(BIT_E_INVALID, "null system or output");
+  if (s->pair_pending) {  // the last list build's pair totals
+    unsigned long long v[4];
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipMemcpyAsync(v, s->pair_dev.p, sizeof(v), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    bit_stats& st = const_cast<bit_system*>(s)->stats;
+    st.far_pairs = (int64_t)v[0];
+    st.near_pairs = (int64_t)v[1];
+    st.lane_near_pairs = 64 * (int64_t)v[2];
+    st.lane_far = 64 * (int64_t)v[3];
+    s->pair_pending = false;
+  }
   *out = s->stats;
   return BIT_OK;
 }

@@ -51,7 +51,7 @@ struct MacParams {
 
 // counters accumulated by the list-count pass
 struct ListStats {
-  unsigned long long far_items, near_items, far_pairs, near_pairs, near_slots, far_sweeps;
+  unsigned long long far_items, near_items;
   unsigned long long far_max, near_max;  // largest per-batch entry counts
 };
 
@@ -222,10 +222,14 @@ hipError_t far_coeffs64(int numtree, int nterms, const double* mom_hat64, double
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
                        int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s);
-// Totals over nb batches of the counts and of pstat (4 per batch: far pairs,
-// near pairs, k_near32's pair slots per lane, k_farq's sweeps) written by lists_count.
-hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
-                      const unsigned long long* pstat, ListStats* out, hipStream_t s);
+// Totals and maxima over nb batches of the counts.
+hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt, ListStats* out,
+                      hipStream_t s);
+// Totals over nb batches of pstat (4 per batch: far pairs, near pairs,
+// k_near32's pair slots per lane, k_farq's sweeps), written by lists_count or
+// lists_compact, into out[4].
+hipError_t pair_stats(int nb, const unsigned long long* pstat, unsigned long long* out,
+                      hipStream_t s);
 hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,
                       const int* near_off, Item* far_items, Item* near_items, int* err_flag,
                       hipStream_t s);
@@ -234,11 +238,13 @@ hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int
 hipError_t lists_stage(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
                        int* near_cnt, unsigned long long* pstat, int cap_far, int cap_near,
                        Item* far_stage, Item* near_stage, int* err_flag, hipStream_t s);
-// Staged entries to the scanned offsets (valid when no count exceeded its cap).
+// Staged entries to the scanned offsets (valid when no count exceeded its cap),
+// and each batch's pair statistics into pstat (range: the nodes' slot ranges);
+// with null stages the statistics of the final lists only.
 hipError_t lists_compact(int nb, const int* far_cnt, const int* near_cnt, const int* far_off,
                          const int* near_off, int cap_far, int cap_near, const Item* far_stage,
                          const Item* near_stage, Item* far_items, Item* near_items,
-                         hipStream_t s);
+                         const int2* range, unsigned long long* pstat, hipStream_t s);
 hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp_bytes,
                           hipStream_t s);
 

@@ -1568,6 +1568,13 @@ enum { kListCount = 0, kListFill = 1, kListStage = 2 };
 constexpr int kNearSparse32 = BIT_NEAR_SPARSE32;
 constexpr int kNearSparse = 32;
 
+// Pair slots per lane of k_near32 for a near item of pc targets over a leaf of
+// len sources: the leaf's length for a target-parallel item, four per 64
+// sources per four targets for a source-parallel one.
+__device__ __forceinline__ unsigned long long near_item_slots(int pc, unsigned long long len) {
+  return pc <= kNearSparse32 ? (unsigned long long)((pc + 3) / 4) * ((len + 63) / 64) * 4 : len;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int stack_cap,
                                               int* __restrict__ far_cnt,
@@ -1604,6 +1611,9 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   unsigned long long near_pairs = 0, near_slots = 0;
   int fpend = -1;  // targets of an item k_farq may still pair with the next one
   constexpr bool kWrite = MODE != kListCount;
+  // the per-batch pair statistics: counted here in the two-traversal path,
+  // from the items by the compaction after a staged traversal
+  constexpr bool kStats = MODE == kListCount;
   const size_t fbase = MODE == kListFill    ? (size_t)far_off[b]
                        : MODE == kListStage ? (size_t)b * (size_t)cap_far
                                             : 0;
@@ -1640,15 +1650,17 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
           far_items[fbase + (nf - 63 + lane)] = Item{fbm, fbn, 0};
       }
       nf++;
-      const int pc = __popcll(am);
-      far_pairs += pc;
-      // k_farq's units: an item joins the previous unpaired one when the two
-      // have at most 64 targets together, else it starts a sweep of its own
-      if (fpend >= 0 && fpend + pc <= kBatch) {
-        fpend = -1;
-      } else {
-        far_sweeps++;
-        fpend = pc;
+      if constexpr (kStats) {
+        const int pc = __popcll(am);
+        far_pairs += pc;
+        // k_farq's units: an item joins the previous unpaired one when the
+        // two have at most 64 targets together, else it starts a sweep of its own
+        if (fpend >= 0 && fpend + pc <= kBatch) {
+          fpend = -1;
+        } else {
+          far_sweeps++;
+          fpend = pc;
+        }
       }
     }
     const unsigned long long rm = m & ~am;
@@ -1663,14 +1675,13 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
             near_items[nbase + (nn - 63 + lane)] = Item{nbm, nbn, 0};
         }
         nn++;
-        const int2 r = make_int2(grange[2 * node], grange[2 * node + 1]);
-        const int pc = __popcll(rm);
-        const unsigned long long len = (unsigned long long)(r.y - r.x);
-        near_pairs += (unsigned long long)pc * len;
-        // pair slots per lane of k_near32: the leaf's length for a
-        // target-parallel item, 4 per 64 sources per four targets otherwise
-        near_slots += pc <= kNearSparse32 ? (unsigned long long)((pc + 3) / 4) * ((len + 63) / 64) * 4
-                                          : len;
+        if constexpr (kStats) {
+          const int2 r = make_int2(grange[2 * node], grange[2 * node + 1]);
+          const int pc = __popcll(rm);
+          const unsigned long long len = (unsigned long long)(r.y - r.x);
+          near_pairs += (unsigned long long)pc * len;
+          near_slots += near_item_slots(pc, len);
+        }
       } else {
         int kid = kids[0];  // child `lane` (lanes 0..7)
 #pragma unroll
@@ -1699,6 +1710,8 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   if (MODE != kListFill && lane == 0) {
     far_cnt[b] = nf;
     near_cnt[b] = nn;
+  }
+  if (kStats && lane == 0) {
     pstat[4 * b] = far_pairs;
     pstat[4 * b + 1] = near_pairs;
     pstat[4 * b + 2] = near_slots;
@@ -1706,45 +1719,38 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   }
 }
 
-// Totals of the per-batch list counts (one work-group, fixed order).
+// Totals and maxima of the per-batch list counts (one work-group, fixed order).
 __global__ __launch_bounds__(1024) void k_list_stats(int nb, const int* __restrict__ far_cnt,
                                                      const int* __restrict__ near_cnt,
-                                                     const unsigned long long* __restrict__ pstat,
                                                      ListStats* __restrict__ out) {
-  __shared__ unsigned long long red[8][1024];
-  unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // six sums, two maxima
+  __shared__ unsigned long long red[4][1024];
+  unsigned long long v[4] = {0, 0, 0, 0};  // two sums, two maxima
   for (int b = threadIdx.x; b < nb; b += 1024) {
     const unsigned long long f = (unsigned long long)far_cnt[b];
     const unsigned long long n = (unsigned long long)near_cnt[b];
     v[0] += f;
     v[1] += n;
-    v[2] += pstat[4 * b];
-    v[3] += pstat[4 * b + 1];
-    v[4] += pstat[4 * b + 2];
-    v[5] += pstat[4 * b + 3];
-    v[6] = f > v[6] ? f : v[6];
-    v[7] = n > v[7] ? n : v[7];
+    v[2] = f > v[2] ? f : v[2];
+    v[3] = n > v[3] ? n : v[3];
   }
-  for (int k = 0; k < 8; k++) red[k][threadIdx.x] = v[k];
+  for (int k = 0; k < 4; k++) red[k][threadIdx.x] = v[k];
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
-      for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-      for (int k = 6; k < 8; k++) {
+      for (int k = 0; k < 2; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+      for (int k = 2; k < 4; k++) {
         const unsigned long long o = red[k][threadIdx.x + w];
         red[k][threadIdx.x] = o > red[k][threadIdx.x] ? o : red[k][threadIdx.x];
       }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    *out = ListStats{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0], red[5][0],
-                     red[6][0], red[7][0]};
+  if (threadIdx.x == 0) *out = ListStats{red[0][0], red[1][0], red[2][0], red[3][0]};
 }
 
-hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt,
-                      const unsigned long long* pstat, ListStats* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(1024), 0, s, nb, far_cnt, near_cnt, pstat, out);
+hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt, ListStats* out,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(1024), 0, s, nb, far_cnt, near_cnt, out);
   return hipGetLastError();
 }
 
@@ -1782,7 +1788,10 @@ hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int
 }
 
 // Staged items to their scanned offsets: one wavefront per batch and list,
-// 16-byte items copied as uint4 (far lists in blocks [0, nb), near after).
+// 16-byte items copied as uint4 (far lists in blocks [0, nb), near after).  The
+// same pass writes the batch's pair statistics (pstat, 4 per batch): far pairs
+// and k_farq's sweeps, near pairs and k_near32's pair slots.  With no staging
+// (far_stage null) it reads the final lists and writes the statistics only.
 __global__ __launch_bounds__(256) void k_list_compact(int nb, const int* __restrict__ far_cnt,
                                                       const int* __restrict__ near_cnt,
                                                       const int* __restrict__ far_off,
@@ -1791,30 +1800,107 @@ __global__ __launch_bounds__(256) void k_list_compact(int nb, const int* __restr
                                                       const uint4* __restrict__ far_stage,
                                                       const uint4* __restrict__ near_stage,
                                                       uint4* __restrict__ far_items,
-                                                      uint4* __restrict__ near_items) {
+                                                      uint4* __restrict__ near_items,
+                                                      const int2* __restrict__ range,
+                                                      unsigned long long* __restrict__ pstat) {
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= 2 * nb) return;
   const bool isfar = w < nb;
   const int b = isfar ? w : w - nb;
-  const int n = isfar ? far_cnt[b] : near_cnt[b];
-  const uint4* src = isfar ? far_stage + (size_t)b * (size_t)cap_far
-                           : near_stage + (size_t)b * (size_t)cap_near;
+  const bool copy = far_stage != nullptr;
+  const int n = copy ? (isfar ? far_cnt[b] : near_cnt[b])
+                     : (isfar ? far_off[b + 1] - far_off[b] : near_off[b + 1] - near_off[b]);
   uint4* dst = isfar ? far_items + far_off[b] : near_items + near_off[b];
-  for (int i = lane; i < n; i += 64) dst[i] = src[i];
+  const uint4* src = !copy ? dst
+                     : isfar ? far_stage + (size_t)b * (size_t)cap_far
+                             : near_stage + (size_t)b * (size_t)cap_near;
+  unsigned long long s0 = 0, s1 = 0;  // far: pairs, sweeps; near: pairs, slots
+  // far sweeps: item i starts one unless it is the second of a pair; pairs are
+  // taken greedily from the start of each run of consecutive items whose
+  // neighbours fit together (target counts summing to at most 64), so an item
+  // starts a sweep iff its distance from its run's start is even
+  int prev_pc = 0, run0 = 0;  // carried across 64-item passes
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
+    const bool in = i < n;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (in) {
+      v = src[i];
+      if (copy) dst[i] = v;
+    }
+    const int pc = __popcll(((unsigned long long)v.y << 32) | v.x);
+    if (isfar) {
+      const int left = __shfl_up(pc, 1);
+      const int lpc = lane == 0 ? prev_pc : left;
+      const bool start = in && (i == 0 || lpc + pc > kBatch);  // a run starts here
+      const unsigned long long st = ballot(start);
+      const unsigned long long sb = st & (below | (1ull << lane));
+      const int rs = sb ? i0 + 63 - __builtin_clzll(sb) : run0;  // this item's run start
+      s0 += in ? pc : 0;
+      s1 += (in && ((i - rs) & 1) == 0) ? 1 : 0;
+      prev_pc = __shfl(pc, 63);
+      run0 = st ? i0 + 63 - __builtin_clzll(st) : run0;
+    } else if (in) {
+      const int2 r = range[(int)v.z];
+      const unsigned long long len = (unsigned long long)(r.y - r.x);
+      s0 += (unsigned long long)pc * len;
+      s1 += near_item_slots(pc, len);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_xor(s0, o);
+    s1 += __shfl_xor(s1, o);
+  }
+  if (lane == 0) {
+    if (isfar) {
+      pstat[4 * b] = s0;
+      pstat[4 * b + 3] = s1;
+    } else {
+      pstat[4 * b + 1] = s0;
+      pstat[4 * b + 2] = s1;
+    }
+  }
+}
+
+// Totals of the per-batch pair statistics (one work-group, fixed order).
+__global__ __launch_bounds__(1024) void k_pair_stats(int nb,
+                                                     const unsigned long long* __restrict__ pstat,
+                                                     unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[4][1024];
+  unsigned long long v[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += 1024)
+    for (int k = 0; k < 4; k++) v[k] += pstat[4 * b + k];
+  for (int k = 0; k < 4; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t pair_stats(int nb, const unsigned long long* pstat, unsigned long long* out,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_stats, dim3(1), dim3(1024), 0, s, nb, pstat, out);
+  return hipGetLastError();
 }
 
 hipError_t lists_compact(int nb, const int* far_cnt, const int* near_cnt, const int* far_off,
                          const int* near_off, int cap_far, int cap_near, const Item* far_stage,
                          const Item* near_stage, Item* far_items, Item* near_items,
-                         hipStream_t s) {
+                         const int2* range, unsigned long long* pstat, hipStream_t s) {
   static_assert(sizeof(Item) == sizeof(uint4), "items are copied as 16-byte words");
   if (nb == 0) return hipSuccess;
+  if ((far_stage == nullptr) != (near_stage == nullptr)) return hipErrorInvalidValue;
   const int nblk = (2 * nb + 3) / 4;
   hipLaunchKernelGGL(k_list_compact, dim3(nblk), dim3(256), 0, s, nb, far_cnt, near_cnt, far_off,
                      near_off, cap_far, cap_near, reinterpret_cast<const uint4*>(far_stage),
                      reinterpret_cast<const uint4*>(near_stage),
-                     reinterpret_cast<uint4*>(far_items), reinterpret_cast<uint4*>(near_items));
+                     reinterpret_cast<uint4*>(far_items), reinterpret_cast<uint4*>(near_items),
+                     range, pstat);
   return hipGetLastError();
 }
 

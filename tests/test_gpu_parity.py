@@ -127,25 +127,32 @@ def test_lists_single_traversal_equals_two(ctx):
     """A rebuild stages items into per-batch regions sized by the context's
     previous build (one traversal) and falls back to count-then-fill when a
     batch outgrows them; either way the lists equal a first, two-traversal
-    build on a fresh context."""
+    build on a fresh context.  The pair statistics (counted by the count
+    traversal, or from the items by the compaction or after a fallback fill)
+    agree too."""
     fx = load_fixture("blobs12k_screened")
     own = bit_amd.Context(0)  # no hint from earlier builds
+    keys = ("far_pairs", "near_pairs", "lane_far", "lane_near_pairs")
     try:
         s = _system(own, fx)
         seen = []
         for acc in (0.5, 0.7, 0.3, 0.3001, 0.9, 0.5):
             got = s.lists(4, acc, 2.5e-5, 2.5e-5)
-            passes = s.stats()["list_passes"]
+            st = s.stats()
+            passes = st["list_passes"]
             seen.append(passes)
             fresh = bit_amd.Context(0)
             try:
                 r = _system(fresh, fx)
                 ref = r.lists(4, acc, 2.5e-5, 2.5e-5)
-                assert r.stats()["list_passes"] == 2
+                st_ref = r.stats()
+                assert st_ref["list_passes"] == 2
             finally:
                 fresh.close()
             for k in ref:
                 assert np.array_equal(got[k], ref[k]), (acc, k, passes)
+            for k in keys:
+                assert st[k] == st_ref[k], (acc, k, passes)
     finally:
         own.close()
     assert seen[0] == 2 and set(seen) == {1, 2}, seen
