@@ -1465,8 +1465,11 @@ __device__ __forceinline__ double shift_1d(const double* src, int e, int i, int 
 
 // One work-group per parent; the eight children are shifted together, one
 // axis per pass over all (child, coefficient) pairs, then summed per
-// coefficient in child order.
-__global__ __launch_bounds__(256) void k_m2m(int nterms, const int* __restrict__ nodes,
+// coefficient in child order.  NB threads: 1024 for the upper heights, whose
+// few parents leave the GPU idle, so each parent's passes finish sooner; the
+// arithmetic per coefficient is the same either way.
+template <int NB>
+__global__ __launch_bounds__(NB) void k_m2m(int nterms, const int* __restrict__ nodes,
                                              const int* __restrict__ child,
                                              const double4* __restrict__ center,
                                              const double* __restrict__ scale,
@@ -1490,31 +1493,31 @@ __global__ __launch_bounds__(256) void k_m2m(int nterms, const int* __restrict__
   }
   __syncthreads();
   const int tot = 8 * nc;
-  for (int x = tid; x < tot; x += 256) {
+  for (int x = tid; x < tot; x += NB) {
     const int q = x / nc, c = x - q * nc;
     A[x] = K[q] >= 0 ? mom[(size_t)K[q] * nc + c] : 0.0;
   }
   __syncthreads();
-  for (int x = tid; x < tot; x += 256) {
+  for (int x = tid; x < tot; x += NB) {
     const int q = x / nc, c = x - q * nc;
     if (K[q] >= 0)
       B[x] = shift_1d(A + q * nc, 0, kM2M.ijk[c][0], kM2M.ijk[c][1], kM2M.ijk[c][2], H[q][0]);
   }
   __syncthreads();
-  for (int x = tid; x < tot; x += 256) {
+  for (int x = tid; x < tot; x += NB) {
     const int q = x / nc, c = x - q * nc;
     if (K[q] >= 0)
       A[x] = shift_1d(B + q * nc, 1, kM2M.ijk[c][0], kM2M.ijk[c][1], kM2M.ijk[c][2], H[q][1]);
   }
   __syncthreads();
-  for (int x = tid; x < tot; x += 256) {
+  for (int x = tid; x < tot; x += NB) {
     const int q = x / nc, c = x - q * nc;
     if (K[q] >= 0)
       B[x] = shift_1d(A + q * nc, 2, kM2M.ijk[c][0], kM2M.ijk[c][1], kM2M.ijk[c][2], H[q][2]);
   }
   __syncthreads();
   const double inv = 1.0 / scale[P];
-  for (int c = tid; c < nc; c += 256) {
+  for (int c = tid; c < nc; c += NB) {
     double acc = 0.0;
     for (int q = 0; q < 8; q++)
       if (K[q] >= 0) acc += B[q * nc + c];
@@ -1527,12 +1530,19 @@ __global__ __launch_bounds__(256) void k_m2m(int nterms, const int* __restrict__
   }
 }
 
+// Heights with fewer parents than this run 1024 threads per parent.
+constexpr int kM2MWide = 512;
+
 hipError_t moments_upward(const TreeView& tv, int nterms, int nnodes, const int* nodes,
                           double* mom, double* mom_hat64, hipStream_t s) {
   if (nnodes == 0) return hipSuccess;
   if (nterms < 1 || nterms > kMaxNterms) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_m2m, dim3(nnodes), dim3(256), 0, s, nterms, nodes, tv.child, tv.center,
-                     tv.scale, mom, mom_hat64);
+  if (nnodes < kM2MWide)
+    hipLaunchKernelGGL(k_m2m<1024>, dim3(nnodes), dim3(1024), 0, s, nterms, nodes, tv.child,
+                       tv.center, tv.scale, mom, mom_hat64);
+  else
+    hipLaunchKernelGGL(k_m2m<256>, dim3(nnodes), dim3(256), 0, s, nterms, nodes, tv.child,
+                       tv.center, tv.scale, mom, mom_hat64);
   return hipGetLastError();
 }
 
@@ -1633,11 +1643,12 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
     const int node = uniform(st_node[sp]);
     const unsigned long long m = uniform64(st_mask[sp]);
     __syncthreads();
-    const double4 c = make_double4(gcenter[4 * node], gcenter[4 * node + 1],
-                                   gcenter[4 * node + 2], gcenter[4 * node + 3]);
+    const unsigned un = (unsigned)node;  // 32-bit offsets: no sign extension
+    const double4 c = make_double4(gcenter[4u * un], gcenter[4u * un + 1u],
+                                   gcenter[4u * un + 2u], gcenter[4u * un + 3u]);
     int kids[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) kids[q] = gchild[8 * node + q];
+    for (int q = 0; q < 8; q++) kids[q] = gchild[8u * un + (unsigned)q];
     const bool acc = mac_accept(px, py, pz, c, mp);
     const unsigned long long am = ballot(acc) & m;
     if (am) {
