@@ -356,11 +356,16 @@ def test_invalid_params_rejected(ctx):
 @pytest.mark.parametrize("kind,n,seed,mm", [("blobs", 300_000, 31, 2000), ("aniso", 60_000, 32, 25),
                                             ("uniform", 100_000, 33, 7)])
 def test_device_tree_equals_host_and_oracle(ctx, oracle, kind, n, seed, mm):
-    """The device tree build (split rounds on HBM-resident slots) gives the host
-    build's tree and slot order, and the oracle's (= reference's) tree."""
+    """The device tree build (inputs in HBM: split rounds across a generation,
+    then per-node programs of 1024 or 256 threads) gives the host build's tree
+    and slot order, and the oracle's (= reference's) tree."""
     P = workloads.make(kind, n, seed)
-    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, mm)
+    dev = [bit_amd.DeviceArray.from_numpy(ctx, a) for a in (P.x, P.y, P.z, P.q)]
+    s = bit_amd.System(ctx, *dev, mm)
     d = s.export_tree()
+    s.close()
+    for a in dev:
+        a.free()
     h = bit_amd.build_tree(P.x, P.y, P.z, mm)
     assert d.numtree == h.numtree
     assert np.array_equal(d.children, h.children)
