@@ -309,6 +309,52 @@ bool finite_all(int n, const double* a) {
   return true;
 }
 
+// The moments' work layout of a tree: leaves in chunks of kMomChunk slots
+// (chunk node, slot range; per node its first chunk and count) and the internal
+// nodes grouped by height (leaves 0): up_off[h-1]..up_off[h] in up.  A child's
+// index exceeds its parent's, so one backward sweep gives every height.
+struct MomLayout {
+  std::vector<int> cn, cr, c0, ck, up, off;
+};
+
+void mom_layout(const bit::HostTree& T, MomLayout& M) {
+  const int nt = T.numtree;
+  std::vector<int> height(nt, 0);
+  M.c0.assign(nt, 0);
+  M.ck.assign(nt, 0);
+  int hmax = 0;
+  for (int i = nt - 1; i >= 0; i--)
+    for (int q = 0; q < 8; q++) {
+      const int ch = T.children[(size_t)i * 8 + q];
+      if (ch >= 0) height[i] = std::max(height[i], height[ch] + 1);
+    }
+  for (int i = 0; i < nt; i++) {
+    M.c0[i] = (int)M.cn.size();
+    int k = 0;
+    if (height[i] == 0) {
+      const int b = T.begin[i], e = T.end[i];
+      for (int x = b; x < e; x += kMomChunk, k++) {
+        M.cn.push_back(i);
+        M.cr.push_back(x);
+        M.cr.push_back(std::min(e, x + kMomChunk));
+      }
+    }
+    M.ck[i] = k;
+    hmax = std::max(hmax, height[i]);
+  }
+  M.off.assign(1, 0);
+  std::vector<int> count(hmax + 1, 0);
+  for (int i = 0; i < nt; i++) count[height[i]]++;
+  std::vector<int> at(hmax + 1, 0);
+  for (int h = 1; h <= hmax; h++) {
+    at[h] = M.off.back();
+    M.off.push_back(M.off.back() + count[h]);
+  }
+  M.up.assign(M.off.back(), 0);
+  for (int i = 0; i < nt; i++)
+    if (height[i] > 0) M.up[at[height[i]]++] = i;
+}
+
 // Builds the tree of n particles whose coordinates are in HBM (original order)
 // and leaves the system's sorted sources, permutation and node table on the
 // device.  Every split decision is taken on the device (split programs, one
@@ -533,7 +579,35 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
   s->nleaves = nleaves;
   s->stack_cap = 8 * (T.maxdepth + 2) + 8;
   s->view_ready = true;
-  HIP_TRY(hipStreamSynchronize(st));  // the node table and scratch die here
+  // the moments' work layout, also behind the device work: the host tables
+  // staged in the pinned buffer (free again) and copied on the stream
+  MomLayout M;
+  mom_layout(T, M);
+  const std::vector<int>* tab[5] = {&M.cn, &M.cr, &M.c0, &M.ck, &M.up};
+  DevBuf* dst[5] = {&s->chunk_node, &s->chunk_rng, &s->node_chunk0, &s->node_nchunk,
+                    &s->up_nodes};
+  size_t need = 0;
+  for (const auto* v : tab) need += std::max<size_t>(1, v->size()) * sizeof(int);
+  if (ctx->pin_bytes < need) {
+    if (ctx->pin) HIP_TRY(hipHostFree(ctx->pin));
+    ctx->pin = nullptr;
+    ctx->pin_bytes = 0;
+    HIP_TRY(hipHostMalloc(&ctx->pin, need, hipHostMallocDefault));
+    ctx->pin_bytes = need;
+  }
+  char* at = static_cast<char*>(ctx->pin);
+  for (int k = 0; k < 5; k++) {
+    const size_t bytes = tab[k]->size() * sizeof(int);
+    BIT_TRY(dst[k]->ensure(ctx, std::max<size_t>(1, tab[k]->size()) * sizeof(int)));
+    if (bytes) {
+      std::memcpy(at, tab[k]->data(), bytes);
+      HIP_TRY(hipMemcpyAsync(dst[k]->p, at, bytes, hipMemcpyHostToDevice, st));
+    }
+    at += std::max<size_t>(1, tab[k]->size()) * sizeof(int);
+  }
+  s->up_off = M.off;
+  s->nchunks = (int)M.cn.size();
+  HIP_TRY(hipStreamSynchronize(st));  // the node table, staging and scratch die here
   return BIT_OK;
 }
 
@@ -1088,44 +1162,16 @@ static int moments_on(bit_system* s, int nterms, hipStream_t st) {
   s->qc64_nterms = 0;
   s->qf_nterms[0] = s->qf_nterms[1] = 0;
   if (s->nchunks == 0) {
-    // leaves: sums over their particles in chunks; internal nodes: upward pass
-    // from their children, height by height (a child's index exceeds its
-    // parent's, so one backward sweep gives every height)
-    std::vector<int> cn, c0(nt), ck(nt), cr, height(nt, 0);
-    int hmax = 0;
-    for (int i = nt - 1; i >= 0; i--)
-      for (int q = 0; q < 8; q++) {
-        const int ch = T.children[(size_t)i * 8 + q];
-        if (ch >= 0) height[i] = std::max(height[i], height[ch] + 1);
-      }
-    for (int i = 0; i < nt; i++) {
-      c0[i] = (int)cn.size();
-      int k = 0;
-      if (height[i] == 0) {
-        int b = T.begin[i], e = T.end[i];
-        for (int x = b; x < e; x += kMomChunk, k++) {
-          cn.push_back(i);
-          cr.push_back(x);
-          cr.push_back(std::min(e, x + kMomChunk));
-        }
-      }
-      ck[i] = k;
-      hmax = std::max(hmax, height[i]);
-    }
-    std::vector<int> up, off(1, 0);
-    for (int h = 1; h <= hmax; h++) {
-      for (int i = 0; i < nt; i++)
-        if (height[i] == h) up.push_back(i);
-      off.push_back((int)up.size());
-    }
-    BIT_TRY(upload(c, s->chunk_node, cn.data(), cn.size() * sizeof(int)));
-    BIT_TRY(upload(c, s->chunk_rng, cr.data(), cr.size() * sizeof(int)));
-    BIT_TRY(upload(c, s->node_chunk0, c0.data(), c0.size() * sizeof(int)));
-    BIT_TRY(upload(c, s->node_nchunk, ck.data(), ck.size() * sizeof(int)));
-    if (!up.empty()) BIT_TRY(upload(c, s->up_nodes, up.data(), up.size() * sizeof(int)));
+    MomLayout M;
+    mom_layout(T, M);
+    BIT_TRY(upload(c, s->chunk_node, M.cn.data(), M.cn.size() * sizeof(int)));
+    BIT_TRY(upload(c, s->chunk_rng, M.cr.data(), M.cr.size() * sizeof(int)));
+    BIT_TRY(upload(c, s->node_chunk0, M.c0.data(), M.c0.size() * sizeof(int)));
+    BIT_TRY(upload(c, s->node_nchunk, M.ck.data(), M.ck.size() * sizeof(int)));
+    if (!M.up.empty()) BIT_TRY(upload(c, s->up_nodes, M.up.data(), M.up.size() * sizeof(int)));
     HIP_TRY(hipStreamSynchronize(c->stream));  // host tables die at scope end
-    s->up_off = off;
-    s->nchunks = (int)cn.size();
+    s->up_off = M.off;
+    s->nchunks = (int)M.cn.size();
   }
   BIT_TRY(s->partial.ensure(c, (size_t)s->nchunks * nc * sizeof(double)));
   BIT_TRY(s->mom.ensure(c, (size_t)nt * nc * sizeof(double)));
