@@ -91,6 +91,10 @@ struct bit_ctx {
   size_t pin_bytes = 0;
   hipStream_t side = nullptr;
   hipEvent_t built = nullptr;
+  // an evaluation's moments run on the side stream, concurrently with its list
+  // build (fork: the tree is done; join: the moments are)
+  std::mutex fork_mu;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
@@ -893,6 +897,8 @@ int bit_ctx_destroy(bit_ctx* c) {
   if (c->gen_host) (void)hipHostFree(c->gen_host);
   if (c->pin) (void)hipHostFree(c->pin);
   if (c->built) (void)hipEventDestroy(c->built);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
   if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
   return BIT_OK;
@@ -1276,8 +1282,27 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   const int n = s->tree.n;
-  BIT_TRY(bit_moments(s, p->nterms));
-  BIT_TRY(ensure_lists(s, p));
+  const ListSet& L = s->pl;
+  const bool lists_ok =
+      L.valid && L.acc == p->acc && L.delta2 == p->delta2 && L.sub == p->mac_sub;
+  if (s->mom_nterms != p->nterms && !lists_ok && s->nchunks > 0) {
+    // the moments (latency-bound upward pass) on the side stream while the
+    // list traversal (bound by scalar issue) runs on the context's stream
+    std::lock_guard<std::mutex> lk(c->fork_mu);
+    if (!c->side) HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    if (!c->fork) HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+    if (!c->join) HIP_TRY(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(c->fork, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->fork, 0));
+    int rc = moments_on(s, p->nterms, c->side);
+    HIP_TRY(hipEventRecord(c->join, c->side));
+    if (rc == BIT_OK) rc = ensure_lists(s, p);
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
+    BIT_TRY(rc);
+  } else {
+    BIT_TRY(bit_moments(s, p->nterms));
+    BIT_TRY(ensure_lists(s, p));
+  }
   BIT_TRY(s->phi_far.ensure(c, (size_t)n * sizeof(double)));
   if (!dst) {
     BIT_TRY(s->out.ensure(c, (size_t)n * sizeof(double)));
