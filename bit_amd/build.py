@@ -57,6 +57,8 @@ def build(incremental: bool = False) -> str:
     if need(tree_o, src):
         _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
               *inc, "-c", src, "-o", tree_o])
+    # the far kernel's asm sweep is generated (far_sweep_asm.h, rewritten only if it changed)
+    _run([sys.executable, os.path.join(CSRC, "gen_far_sweep.py")])
     src = os.path.join(CSRC, "kernels.hip")
     if need(ker_o, src):
         _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=on",

@@ -264,7 +264,10 @@ struct bit_system {
   DevBuf qx, qy, qz, qperm, qphi, qout, qdesc;
   bit::dev::EvalDesc qhdesc{};
   // outputs
-  DevBuf phi_far, out;
+  // per-target partial results (potential: the near field's; field: the far field's)
+  DevBuf part, out;
+  // a deferred prepare_eval left the moments on the side stream (join_moments)
+  bool join_pending = false;
   // direct sum (particle targets tslot; point targets dpx, dpy, dpz)
   DevBuf tslot, dpart, dout, dpx, dpy, dpz;
   // charge updates
@@ -1277,7 +1280,8 @@ static void collect_times(bit_system* s) {
 // Moments and lists for params, the far-field buffer, and the evaluation
 // descriptor (result to `dst`, a device pointer); nothing is launched for the
 // far/near kernels.
-static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
+static int prepare_eval(bit_system* s, const bit_params* p, double* dst,
+                        bool defer_join = false) {
   BIT_TRY(check_params(s, p));
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
@@ -1297,13 +1301,17 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
     int rc = moments_on(s, p->nterms, c->side);
     HIP_TRY(hipEventRecord(c->join, c->side));
     if (rc == BIT_OK) rc = ensure_lists(s, p);
-    HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
+    // deferred: the caller launches the near field (lists only) first and
+    // joins before the far field (join_moments).  Not for FP64, whose far
+    // coefficients are computed below on the context's stream from the moments.
+    if (defer_join && rc == BIT_OK && p->precision == BIT_FP32) s->join_pending = true;
+    else HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
     BIT_TRY(rc);
   } else {
     BIT_TRY(bit_moments(s, p->nterms));
     BIT_TRY(ensure_lists(s, p));
   }
-  BIT_TRY(s->phi_far.ensure(c, (size_t)n * sizeof(double)));
+  BIT_TRY(s->part.ensure(c, (size_t)n * sizeof(double)));
   if (!dst) {
     BIT_TRY(s->out.ensure(c, (size_t)n * sizeof(double)));
     dst = s->out.as<double>();
@@ -1319,26 +1327,42 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst) {
   d.far_items = s->pl.far_items.as<bit::dev::Item>();
   d.near_off = s->pl.near_off.as<int>();
   d.near_items = s->pl.near_items.as<bit::dev::Item>();
-  d.phi_far = s->phi_far.as<double>();
+  d.phi_near = s->part.as<double>();
   d.out = dst;
+  return BIT_OK;
+}
+
+// The context's stream waits for the moments a deferred prepare_eval left on
+// the side stream.  The join event is only ever recorded on the side stream,
+// so waiting on a later record of it covers this system's moments too.
+static int join_moments(bit_system* s) {
+  if (!s->join_pending) return BIT_OK;
+  s->join_pending = false;
+  HIP_TRY(hipStreamWaitEvent(s->ctx->stream, s->ctx->join, 0));
   return BIT_OK;
 }
 
 int bit_eval_potential(bit_system* s, const bit_params* p, double* out, int out_on_device) {
   if (!out) return fail(BIT_E_INVALID, "null output");
-  BIT_TRY(prepare_eval(s, p, out_on_device ? out : nullptr));
+  const int rc = prepare_eval(s, p, out_on_device ? out : nullptr, true);
+  struct Join {  // every exit leaves the stream joined with the moments
+    bit_system* s;
+    ~Join() { (void)join_moments(s); }
+  } join{s};
+  BIT_TRY(rc);
   bit_ctx* c = s->ctx;
   const int n = s->tree.n;
   const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
   BIT_TRY(upload(c, s->desc, &s->hdesc, sizeof(bit::dev::EvalDesc)));
   const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
   const bit::dev::EvalDesc* dd = s->desc.as<bit::dev::EvalDesc>();
-  s->t_far.start(c->stream);
-  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
-  s->t_far.stop(c->stream);
   s->t_near.start(c->stream);
   HIP_TRY(bit::dev::near_field(dd, nullptr, nb, fp32, scr, c->stream));
   s->t_near.stop(c->stream);
+  BIT_TRY(join_moments(s));  // the far field needs the moments from the side stream
+  s->t_far.start(c->stream);
+  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
+  s->t_far.stop(c->stream);
   if (!out_on_device)
     HIP_TRY(hipMemcpyAsync(out, s->hdesc.out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
@@ -1374,13 +1398,13 @@ int bit_eval_potential_noalloc(bit_system* s, const bit_params* p, double* d_out
   d.far_items = L.far_items.as<bit::dev::Item>();
   d.near_off = L.near_off.as<int>();
   d.near_items = L.near_items.as<bit::dev::Item>();
-  d.phi_far = reinterpret_cast<double*>(base + bit::dev::kNoAllocDescBytes);
+  d.phi_near = reinterpret_cast<double*>(base + bit::dev::kNoAllocDescBytes);
   d.out = d_out;
   const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
   const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
   HIP_TRY(bit::dev::put_desc(d, dd, c->stream));
-  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
   HIP_TRY(bit::dev::near_field(dd, nullptr, nb, fp32, scr, c->stream));
+  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
   return BIT_OK;
 }
 
@@ -1444,15 +1468,15 @@ int bit_eval_potential_multi(bit_ctx* ctx, int count, bit_system* const* systems
     const bit_params& p = params[std::get<2>(g)];
     const int2* m = d_map.as<int2>() + std::get<0>(g);
     const bool fp32 = p.precision == BIT_FP32, scr = p.kappa != 0.0;
-    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), p.nterms,
-                                fp32, scr, ctx->stream));
+    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), fp32, scr,
+                                 ctx->stream));
   }
   for (auto& g : groups) {
     const bit_params& p = params[std::get<2>(g)];
     const int2* m = d_map.as<int2>() + std::get<0>(g);
     const bool fp32 = p.precision == BIT_FP32, scr = p.kappa != 0.0;
-    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), fp32, scr,
-                                 ctx->stream));
+    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), p.nterms,
+                                fp32, scr, ctx->stream));
   }
   tm.stop(ctx->stream);
   if (!out_on_device)
@@ -1476,7 +1500,7 @@ int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_d
   const int n = s->tree.n;
   BIT_TRY(bit_moments(s, p->nterms));
   BIT_TRY(ensure_lists(s, p));
-  BIT_TRY(s->phi_far.ensure(c, (size_t)3 * n * sizeof(double)));
+  BIT_TRY(s->part.ensure(c, (size_t)3 * n * sizeof(double)));
   double* dst = out;
   if (!out_on_device) {
     BIT_TRY(s->out.ensure(c, (size_t)3 * n * sizeof(double)));
@@ -1488,12 +1512,12 @@ int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_d
   HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, p->kernel_order, fp32,
                                    fp32 ? (const void*)s->qf32.p : (const void*)s->qf64.p,
                                    s->pl.far_off.as<int>(), s->pl.far_items.as<bit::dev::Item>(),
-                                   s->phi_far.as<double>(), c->stream));
+                                   s->part.as<double>(), c->stream));
   s->t_far.stop(c->stream);
   s->t_near.start(c->stream);
   HIP_TRY(bit::dev::near_field_grad(s->view(), p->delta2, p->kernel_order, fp32,
                                     s->pl.near_off.as<int>(),
-                                    s->pl.near_items.as<bit::dev::Item>(), s->phi_far.as<double>(),
+                                    s->pl.near_items.as<bit::dev::Item>(), s->part.as<double>(),
                                     dst, c->stream));
   s->t_near.stop(c->stream);
   if (!out_on_device)
@@ -1568,14 +1592,14 @@ int bit_eval_potential_points(bit_system* s, const bit_params* p, int m, const d
   d.far_items = s->ql.far_items.as<bit::dev::Item>();
   d.near_off = s->ql.near_off.as<int>();
   d.near_items = s->ql.near_items.as<bit::dev::Item>();
-  d.phi_far = s->qphi.as<double>();
+  d.phi_near = s->qphi.as<double>();
   d.out = s->qout.as<double>();
   BIT_TRY(upload(c, s->qdesc, &d, sizeof(bit::dev::EvalDesc)));
   const int nb = (m + bit::dev::kBatch - 1) / bit::dev::kBatch;
   const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
   const bit::dev::EvalDesc* dd = s->qdesc.as<bit::dev::EvalDesc>();
-  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
   HIP_TRY(bit::dev::near_field(dd, nullptr, nb, fp32, scr, c->stream));
+  HIP_TRY(bit::dev::far_field(dd, nullptr, nb, p->nterms, fp32, scr, c->stream));
   HIP_TRY(hipMemcpyAsync(out, s->qout.p, (size_t)m * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1761,14 +1785,14 @@ int mixed_launch(bit_ctx* ctx, const std::vector<bit_system*>& sys,
   BIT_TRY(upload_vec(ctx, d_map, map, st));
   for (auto& gr : groups) {
     const bit_params& p = prm[std::get<2>(gr)];
-    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
-                                std::get<1>(gr), p.nterms, p.precision == BIT_FP32,
-                                p.kappa != 0.0, st));
+    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
+                                 std::get<1>(gr), p.precision == BIT_FP32, p.kappa != 0.0, st));
   }
   for (auto& gr : groups) {
     const bit_params& p = prm[std::get<2>(gr)];
-    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
-                                 std::get<1>(gr), p.precision == BIT_FP32, p.kappa != 0.0, st));
+    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
+                                std::get<1>(gr), p.nterms, p.precision == BIT_FP32,
+                                p.kappa != 0.0, st));
   }
   return BIT_OK;
 }

@@ -260,18 +260,18 @@ struct EvalDesc {
   const Item* far_items;
   const int* near_off;
   const Item* near_items;
-  double* phi_far;       // far-field result, sorted order
+  double* phi_near;      // near-field result, sorted order (near kernel -> far kernel)
   double* out;           // potential, caller's original order
 };
 
-// far field -> d.phi_far; map == nullptr: one system, work-group b = batch b;
-// else map[g] = (system, batch).  One launch serves one (nterms, precision,
-// screened) group.
-hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nterms, bool fp32,
-                     bool screened, hipStream_t s);
-// near field + combine: d.out[perm[t]] = d.phi_far[t] + near(t)
+// near field -> d.phi_near; map == nullptr: one system, work-group b = batch b;
+// else map[g] = (system, batch).  Runs first: it needs the lists, not the moments.
 hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool fp32,
                       bool screened, hipStream_t s);
+// far field + combine: d.out[perm[t]] = far(t) + d.phi_near[t], after near_field.
+// One launch serves one (nterms, precision, screened) group.
+hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nterms, bool fp32,
+                     bool screened, hipStream_t s);
 
 // Krylov vectors (FP64): out[j] = V_j . w for j < ncols (deterministic order;
 // partial holds kDotBlocks * ncols doubles), y += sign * sum_j c[j] V_j, and the
