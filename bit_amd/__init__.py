@@ -384,8 +384,9 @@ def gmres(n, matvec, b, x0=None, restart=50, tol=1e-6, maxit=500):
 
 
 def eval_multi(ctx: "Context", systems, params, precision_out=None):
-    """Potentials of several systems in mixed launches (one per parameter
-    group); returns (list of numpy arrays, GPU ms of the launches)."""
+    """Potentials of several systems in one persistent launch per precision
+    (bit_eval_potential_multi); returns (list of numpy arrays, GPU ms of the
+    launches)."""
     k = len(systems)
     outs = [np.zeros(s.n) for s in systems]
     sys_arr = (C.c_void_p * k)(*[s.h.value for s in systems])

@@ -95,6 +95,11 @@ struct bit_ctx {
   // build (fork: the tree is done; join: the moments are)
   std::mutex fork_mu;
   hipEvent_t fork = nullptr, join = nullptr;
+  // persistent launches of mixed evaluations: their work counters (two, used
+  // in turn; serve_mu orders the launches on the context's stream)
+  std::mutex serve_mu;
+  void* serve_ctr = nullptr;
+  int serve_parity = 0;
 };
 
 namespace {
@@ -903,6 +908,7 @@ int bit_ctx_destroy(bit_ctx* c) {
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
   if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->serve_ctr) (void)hipFree(c->serve_ctr);
   delete c;
   return BIT_OK;
 }
@@ -1329,6 +1335,7 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst,
   d.near_items = s->pl.near_items.as<bit::dev::Item>();
   d.phi_near = s->part.as<double>();
   d.out = dst;
+  d.nterms = p->nterms;
   return BIT_OK;
 }
 
@@ -1400,6 +1407,7 @@ int bit_eval_potential_noalloc(bit_system* s, const bit_params* p, double* d_out
   d.near_items = L.near_items.as<bit::dev::Item>();
   d.phi_near = reinterpret_cast<double*>(base + bit::dev::kNoAllocDescBytes);
   d.out = d_out;
+  d.nterms = p->nterms;
   const int nb = (n + bit::dev::kBatch - 1) / bit::dev::kBatch;
   const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
   HIP_TRY(bit::dev::put_desc(d, dd, c->stream));
@@ -1414,6 +1422,12 @@ int bit_system_prepare(bit_system* s, const bit_params* p) {
   collect_times(s);
   return BIT_OK;
 }
+
+namespace {
+int mixed_launch(bit_ctx* ctx, const std::vector<bit_system*>& sys,
+                 const std::vector<bit_params>& prm, std::vector<bit::dev::EvalDesc>& descs,
+                 std::vector<int2>& map, DevBuf& d_desc, DevBuf& d_map, hipStream_t st);
+}  // namespace
 
 int bit_eval_potential_multi(bit_ctx* ctx, int count, bit_system* const* systems,
                              const bit_params* params, double* const* outs, int out_on_device,
@@ -1436,48 +1450,14 @@ int bit_eval_potential_multi(bit_ctx* ctx, int count, bit_system* const* systems
     BIT_TRY(prepare_eval(systems[i], &params[i], out_on_device ? outs[i] : nullptr));
     HIP_TRY(hipStreamSynchronize(systems[i]->ctx->stream));  // lists/moments ready
   }
-  // group by the kernel instantiation: (nterms, precision, screened)
-  std::vector<int> order(count);
-  for (int i = 0; i < count; i++) order[i] = i;
-  auto key = [&](int i) {
-    return std::make_tuple(params[i].nterms, params[i].precision, params[i].kappa != 0.0);
-  };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
-  std::vector<bit::dev::EvalDesc> descs(count);
+  std::vector<bit_system*> sys(systems, systems + count);
+  std::vector<bit_params> prm(params, params + count);
+  std::vector<bit::dev::EvalDesc> descs;
   std::vector<int2> map;
-  std::vector<std::tuple<int, int, int, int>> groups;  // (first map entry, blocks, i0 in order)
-  for (int k = 0; k < count;) {
-    int e = k;
-    while (e < count && key(order[e]) == key(order[k])) e++;
-    int first = (int)map.size();
-    for (int j = k; j < e; j++) {
-      bit_system* s = systems[order[j]];
-      descs[j] = s->hdesc;
-      int nb = (s->tree.n + bit::dev::kBatch - 1) / bit::dev::kBatch;
-      for (int b = 0; b < nb; b++) map.push_back(make_int2(j, b));
-    }
-    groups.emplace_back(first, (int)map.size() - first, order[k], 0);
-    k = e;
-  }
   DevBuf d_desc, d_map;
-  BIT_TRY(upload(ctx, d_desc, descs.data(), descs.size() * sizeof(bit::dev::EvalDesc)));
-  BIT_TRY(upload(ctx, d_map, map.data(), map.size() * sizeof(int2)));
   Timer tm;
   tm.start(ctx->stream);
-  for (auto& g : groups) {
-    const bit_params& p = params[std::get<2>(g)];
-    const int2* m = d_map.as<int2>() + std::get<0>(g);
-    const bool fp32 = p.precision == BIT_FP32, scr = p.kappa != 0.0;
-    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), fp32, scr,
-                                 ctx->stream));
-  }
-  for (auto& g : groups) {
-    const bit_params& p = params[std::get<2>(g)];
-    const int2* m = d_map.as<int2>() + std::get<0>(g);
-    const bool fp32 = p.precision == BIT_FP32, scr = p.kappa != 0.0;
-    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), m, std::get<1>(g), p.nterms,
-                                fp32, scr, ctx->stream));
-  }
+  BIT_TRY(mixed_launch(ctx, sys, prm, descs, map, d_desc, d_map, ctx->stream));
   tm.stop(ctx->stream);
   if (!out_on_device)
     for (int i = 0; i < count; i++)
@@ -1594,6 +1574,7 @@ int bit_eval_potential_points(bit_system* s, const bit_params* p, int m, const d
   d.near_items = s->ql.near_items.as<bit::dev::Item>();
   d.phi_near = s->qphi.as<double>();
   d.out = s->qout.as<double>();
+  d.nterms = p->nterms;
   BIT_TRY(upload(c, s->qdesc, &d, sizeof(bit::dev::EvalDesc)));
   const int nb = (m + bit::dev::kBatch - 1) / bit::dev::kBatch;
   const bool fp32 = p->precision == BIT_FP32, scr = p->kappa != 0.0;
@@ -1751,9 +1732,12 @@ int upload_vec(bit_ctx* c, DevBuf& d, const std::vector<T>& v, hipStream_t st) {
   return BIT_OK;
 }
 
-// Mixed far/near launches on `st` over systems whose descriptors (hdesc) were
-// set by prepare_eval: one launch per (nterms, precision, screened) group.
-// The host tables stay alive in `descs`/`map` until the caller synchronises.
+// One persistent launch per precision on `st` (the context's stream) over
+// systems whose descriptors (hdesc) were set by prepare_eval: every batch of
+// every system is one work item, systems taken in order of decreasing nterms,
+// then size, so the costliest batches start first and the cheapest fill the
+// tail.  The host tables stay alive in `descs`/`map` until the caller
+// synchronises.
 int mixed_launch(bit_ctx* ctx, const std::vector<bit_system*>& sys,
                  const std::vector<bit_params>& prm, std::vector<bit::dev::EvalDesc>& descs,
                  std::vector<int2>& map, DevBuf& d_desc, DevBuf& d_map, hipStream_t st) {
@@ -1762,15 +1746,15 @@ int mixed_launch(bit_ctx* ctx, const std::vector<bit_system*>& sys,
   std::vector<int> order(count);
   for (int i = 0; i < count; i++) order[i] = i;
   auto key = [&](int i) {
-    return std::make_tuple(prm[i].nterms, prm[i].precision, prm[i].kappa != 0.0);
+    return std::make_tuple(prm[i].precision, -prm[i].nterms, -sys[i]->tree.n);
   };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
   descs.assign(count, bit::dev::EvalDesc{});
   map.clear();
-  std::vector<std::tuple<int, int, int>> groups;  // (first map entry, blocks, params index)
+  std::vector<std::tuple<int, int, bool>> groups;  // (first item, items, FP32) per precision
   for (int k = 0; k < count;) {
     int e = k;
-    while (e < count && key(order[e]) == key(order[k])) e++;
+    while (e < count && prm[order[e]].precision == prm[order[k]].precision) e++;
     const int first = (int)map.size();
     for (int j = k; j < e; j++) {
       bit_system* s = sys[order[j]];
@@ -1778,21 +1762,22 @@ int mixed_launch(bit_ctx* ctx, const std::vector<bit_system*>& sys,
       const int nb = (s->tree.n + bit::dev::kBatch - 1) / bit::dev::kBatch;
       for (int b = 0; b < nb; b++) map.push_back(make_int2(j, b));
     }
-    groups.emplace_back(first, (int)map.size() - first, order[k]);
+    groups.emplace_back(first, (int)map.size() - first, prm[order[k]].precision == BIT_FP32);
     k = e;
   }
   BIT_TRY(upload_vec(ctx, d_desc, descs, st));
   BIT_TRY(upload_vec(ctx, d_map, map, st));
-  for (auto& gr : groups) {
-    const bit_params& p = prm[std::get<2>(gr)];
-    HIP_TRY(bit::dev::near_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
-                                 std::get<1>(gr), p.precision == BIT_FP32, p.kappa != 0.0, st));
+  std::lock_guard<std::mutex> lk(ctx->serve_mu);
+  if (!ctx->serve_ctr) {
+    HIP_TRY(hipMalloc(&ctx->serve_ctr, 2 * sizeof(int)));
+    HIP_TRY(hipMemsetAsync(ctx->serve_ctr, 0, 2 * sizeof(int), st));  // once, at creation
   }
-  for (auto& gr : groups) {
-    const bit_params& p = prm[std::get<2>(gr)];
-    HIP_TRY(bit::dev::far_field(d_desc.as<bit::dev::EvalDesc>(), d_map.as<int2>() + std::get<0>(gr),
-                                std::get<1>(gr), p.nterms, p.precision == BIT_FP32,
-                                p.kappa != 0.0, st));
+  for (const auto& g : groups) {
+    HIP_TRY(bit::dev::serve_field(d_desc.as<bit::dev::EvalDesc>(),
+                                  d_map.as<int2>() + std::get<0>(g), std::get<1>(g),
+                                  std::get<2>(g), static_cast<int*>(ctx->serve_ctr),
+                                  ctx->serve_parity, st));
+    ctx->serve_parity ^= 1;
   }
   return BIT_OK;
 }

@@ -262,6 +262,7 @@ struct EvalDesc {
   const Item* near_items;
   double* phi_near;      // near-field result, sorted order (near kernel -> far kernel)
   double* out;           // potential, caller's original order
+  int nterms;            // Taylor terms (read per item by the persistent launch)
 };
 
 // near field -> d.phi_near; map == nullptr: one system, work-group b = batch b;
@@ -272,6 +273,16 @@ hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool 
 // One launch serves one (nterms, precision, screened) group.
 hipError_t far_field(const EvalDesc* descs, const int2* map, int nblocks, int nterms, bool fp32,
                      bool screened, hipStream_t s);
+
+// Persistent evaluation of many systems in one launch (all nterms and both
+// kernels; one precision): work-groups sized to what the device holds at once
+// take (system, batch) items from a device counter, in the order of `items`,
+// and run each batch's near field, then its far field and the combine, until
+// the counter passes nitems.  Results equal those of near_field + far_field.
+// ctr: two ints with ctr[parity] zero at the launch; the launch zeroes
+// ctr[parity ^ 1] for the next one (launches that share ctr share a stream).
+hipError_t serve_field(const EvalDesc* descs, const int2* items, int nitems, bool fp32,
+                       int* ctr, int parity, hipStream_t s);
 
 // Krylov vectors (FP64): out[j] = V_j . w for j < ncols (deterministic order;
 // partial holds kDotBlocks * ncols doubles), y += sign * sum_j c[j] V_j, and the

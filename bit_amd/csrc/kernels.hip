@@ -16,6 +16,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <climits>
 #include <mutex>
 #include <type_traits>
@@ -2027,28 +2028,31 @@ __device__ __forceinline__ double qsetup64(QPair<NT, double>& v, double px, doub
   return rinv;
 }
 
-// FP32 far field in the radial-derivative form: the cluster's Q is staged in
-// LDS like the FP64 kernel's moments (the next tiles are loaded into registers
-// while the current ones are evaluated) and read as broadcasts.  Two
-// consecutive items with at most 64 targets together share one sweep: lanes
-// are remapped to the first item's targets, then the second's, each reading
-// its item's tile, and the results return to the targets' lanes, which add
-// them in list order, so the results are those of one item at a time.
-// Four waves per SIMD: the sweep is chains of dependent FMAs, and the issue
-// rate of one wave is well below the SIMD's.
+// LDS of the FP32 far sweep: two tiles (the second offset by 16 floats onto
+// other banks), the packed unit's lane map and per-item results.
+constexpr int far32_tiles(int nt) { return 2 * nqpad(nt) + 16; }
+struct FarLds32 {
+  int tslot[kBatch];      // packed unit: lane j's target lane
+  float res[2][kBatch];   // packed unit: per-item results by target lane
+};
+
+// One batch's far field (FP32) in the radial-derivative form: per lane, the
+// Kahan sum over the batch's cluster items of its target (unspecified on lanes
+// past the last target).  The cluster's Q is staged in LDS (the next tiles are
+// loaded into registers while the current ones are evaluated) and read as
+// broadcasts.  Two consecutive items with at most 64 targets together share
+// one sweep: lanes are remapped to the first item's targets, then the
+// second's, each reading its item's tile, and the results return to the
+// targets' lanes, which add them in list order, so the results are those of
+// one item at a time.
 template <int NT, bool SCR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_farq(
-    const EvalDesc* __restrict__ descs, const int2* __restrict__ map) {
-  constexpr int NQ = nqcoef(NT);
+__device__ __forceinline__ double far32_batch(const EvalDesc& d, int b, float* mbuf,
+                                              FarLds32& fl) {
   constexpr int NQP = nqpad(NT);
   constexpr int PF = NQP / kBatch;   // Q values per lane per tile
   constexpr int T2 = NQP + 16;       // second tile's offset: other LDS banks
-  __shared__ __align__(16) float mbuf[T2 + NQP];
-  __shared__ int tslot[kBatch];      // packed unit: lane j's target lane
-  __shared__ float res[2][kBatch];   // packed unit: per-item results by target lane
-  int sys, b;
-  which(map, sys, b);
-  const EvalDesc& d = descs[sys];
+  int* tslot = fl.tslot;
+  float (*res)[kBatch] = fl.res;
   const TreeView& tv = d.tv;
   const float kappa32 = (float)d.kappa, delta2_32 = (float)d.delta2;
   const gptr<float> qc = as_global(d.qc32);
@@ -2142,21 +2146,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     u = un;
     n = nn;
   }
-  if (valid) d.out[tv.tperm[t]] = acc.value() + d.phi_near[t];
+  (void)valid;
+  return acc.value();
+}
+
+// FP32 far field of one batch per work-group, plus the combine with the near
+// field's results: out[perm[t]] = far(t) + phi_near[t].  Four waves per SIMD:
+// the sweep is chains of dependent FMAs, and one wave's issue rate is well
+// below the SIMD's.
+template <int NT, bool SCR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_farq(
+    const EvalDesc* __restrict__ descs, const int2* __restrict__ map) {
+  __shared__ __align__(16) float mbuf[far32_tiles(NT)];
+  __shared__ FarLds32 fl;
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const double f = far32_batch<NT, SCR>(d, b, mbuf, fl);
+  const int t = b * kBatch + threadIdx.x;
+  if (t < d.tv.nt) d.out[d.tv.tperm[t]] = f + d.phi_near[t];
 }
 
 // FP64 far field: the same radial-derivative sweep in FP64 over FP64 cluster
 // coefficients, one item at a time (a tile is 6.5 KB at p = 11).
 template <int NT, bool SCR>
-__global__ __launch_bounds__(64) void k_farq64(const EvalDesc* __restrict__ descs,
-                                               const int2* __restrict__ map) {
+__device__ __forceinline__ double far64_batch(const EvalDesc& d, int b, double* mbuf) {
   constexpr int NQ = nqcoef(NT);
   constexpr int NQP = nqpad(NT);
   constexpr int PF = NQP / kBatch;  // Q values per lane per tile
-  __shared__ __align__(16) double mbuf[NQP];
-  int sys, b;
-  which(map, sys, b);
-  const EvalDesc& d = descs[sys];
   const TreeView& tv = d.tv;
   const double kappa = d.kappa, delta2 = d.delta2;
   const gptr<double> qc = as_global(d.qc64);
@@ -2193,7 +2210,20 @@ __global__ __launch_bounds__(64) void k_farq64(const EvalDesc* __restrict__ desc
     }
     __syncthreads();  // the tile is overwritten next item
   }
-  if (valid) d.out[tv.tperm[t]] = acc.value() + d.phi_near[t];
+  (void)valid;
+  return acc.value();
+}
+
+template <int NT, bool SCR>
+__global__ __launch_bounds__(64) void k_farq64(const EvalDesc* __restrict__ descs,
+                                               const int2* __restrict__ map) {
+  __shared__ __align__(16) double mbuf[nqpad(NT)];
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const double f = far64_batch<NT, SCR>(d, b, mbuf);
+  const int t = b * kBatch + threadIdx.x;
+  if (t < d.tv.nt) d.out[d.tv.tperm[t]] = f + d.phi_near[t];
 }
 
 template <typename T, bool SCR, int NT>
@@ -2325,7 +2355,7 @@ __device__ __forceinline__ void near_sparse_sweep(gptr<float4> sloc, int j0, int
 // one multiplication by cs per target.  Otherwise cs = 1 and the exponent is
 // kneg2 R.
 template <bool SCR, bool SCALED>
-__device__ __forceinline__ void near32_batch(const EvalDesc& d, int b, float4* tile, double cs) {
+__device__ __forceinline__ double near32_batch(const EvalDesc& d, int b, float4* tile, double cs) {
   const TreeView& tv = d.tv;
   const float csf = (float)cs;
   const float d2 = (float)(d.delta2 * cs * cs);
@@ -2444,8 +2474,20 @@ __device__ __forceinline__ void near32_batch(const EvalDesc& d, int b, float4* t
       __syncthreads();
     }
   }
-  if (valid)
-    d.phi_near[t] = SCALED ? (double)acc.value() * cs : acc.value();
+  (void)valid;
+  return SCALED ? (double)acc.value() * cs : acc.value();
+}
+
+// One batch's near field (FP32), per lane: the screened kernel in coordinates
+// scaled by cs = kappa log2(e) while cs keeps leaf-size offsets and their
+// squares far from FP32's range limits.
+template <bool SCR>
+__device__ __forceinline__ double near32_value(const EvalDesc& d, int b, float4* tile) {
+  if constexpr (SCR) {
+    const double cs = d.kappa * 1.4426950408889634;
+    if (cs >= 0x1p-20 && cs <= 0x1p20) return near32_batch<true, true>(d, b, tile, cs);
+  }
+  return near32_batch<SCR, false>(d, b, tile, 1.0);
 }
 
 // At least six waves per SIMD: left alone the compiler keeps a whole group of
@@ -2458,25 +2500,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   int sys, b;
   which(map, sys, b);
   const EvalDesc& d = descs[sys];
-  if constexpr (SCR) {
-    // scaled coordinates while cs keeps leaf-size offsets and their squares
-    // far from FP32's range limits
-    const double cs = d.kappa * 1.4426950408889634;
-    if (cs >= 0x1p-20 && cs <= 0x1p20) {
-      near32_batch<true, true>(d, b, tile, cs);
-      return;
-    }
-  }
-  near32_batch<SCR, false>(d, b, tile, 1.0);
+  const double v = near32_value<SCR>(d, b, tile);
+  const int t = b * kBatch + threadIdx.x;
+  if (t < d.tv.nt) d.phi_near[t] = v;
 }
 
 template <bool SCR>
-__global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ descs,
-                                               const int2* __restrict__ map) {
-  __shared__ double4 tile[64];
-  int sys, b;
-  which(map, sys, b);
-  const EvalDesc& d = descs[sys];
+__device__ __forceinline__ double near64_batch(const EvalDesc& d, int b, double4* tile) {
   const TreeView& tv = d.tv;
   const double d2 = d.delta2, kappa = d.kappa;
   const int lane = threadIdx.x;
@@ -2557,7 +2587,20 @@ __global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ desc
       __syncthreads();
     }
   }
-  if (valid) d.phi_near[t] = acc;
+  (void)valid;
+  return acc;
+}
+
+template <bool SCR>
+__global__ __launch_bounds__(64) void k_near64(const EvalDesc* __restrict__ descs,
+                                               const int2* __restrict__ map) {
+  __shared__ double4 tile[64];
+  int sys, b;
+  which(map, sys, b);
+  const EvalDesc& d = descs[sys];
+  const double v = near64_batch<SCR>(d, b, tile);
+  const int t = b * kBatch + threadIdx.x;
+  if (t < d.tv.nt) d.phi_near[t] = v;
 }
 
 hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool fp32,
@@ -2574,6 +2617,125 @@ hipError_t near_field(const EvalDesc* descs, const int2* map, int nblocks, bool 
     else
       hipLaunchKernelGGL(k_near64<false>, dim3(nblocks), dim3(kBatch), 0, s, descs, map);
   }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Persistent launch of a serving round or a mixed evaluation: one wave per
+// work-group, as many work-groups as the device holds at once, each taking
+// (system, batch) items from a device counter until the counter passes
+// nitems (every wave reaches that exit).  An item runs the batch's near field
+// and then its far field in the same work-group (their LDS is shared, so a
+// barrier separates them) and writes far + near, the combine of the two-kernel
+// path, so a target's result is the same either way.  The far sweep is chosen
+// per item from the system's nterms and kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int next_item(int* ctr) {
+  int v = 0;
+  if (threadIdx.x == 0) v = atomicAdd(ctr, 1);
+  return uniform(__shfl(v, 0));
+}
+
+template <bool SCR, int... NTs>
+__device__ __forceinline__ double far32_any(std::integer_sequence<int, NTs...>, int nt,
+                                            const EvalDesc& d, int b, float* mbuf,
+                                            FarLds32& fl) {
+  double r = 0.0;
+  (void)((nt == NTs + 1 ? (r = far32_batch<NTs + 1, SCR>(d, b, mbuf, fl), true) : false) || ...);
+  return r;
+}
+
+template <bool SCR, int... NTs>
+__device__ __forceinline__ double far64_any(std::integer_sequence<int, NTs...>, int nt,
+                                            const EvalDesc& d, int b, double* mbuf) {
+  double r = 0.0;
+  (void)((nt == NTs + 1 ? (r = far64_batch<NTs + 1, SCR>(d, b, mbuf), true) : false) || ...);
+  return r;
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_serve32(
+    const EvalDesc* __restrict__ descs, const int2* __restrict__ items, int nitems,
+    int* __restrict__ ctr, int parity) {
+  __shared__ __align__(16) float mbuf[far32_tiles(kMaxNterms)];  // near tile aliases it
+  __shared__ FarLds32 fl;
+  using Seq = std::make_integer_sequence<int, kMaxNterms>;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[parity ^ 1] = 0;
+  for (;;) {
+    const int i = next_item(ctr + parity);
+    if (i >= nitems) break;
+    const int2 m = items[i];
+    const int sys = uniform(m.x), b = uniform(m.y);
+    const EvalDesc& d = descs[sys];
+    const bool scr = d.kappa != 0.0;
+    float4* tile = reinterpret_cast<float4*>(mbuf);
+    const double nv = scr ? near32_value<true>(d, b, tile) : near32_value<false>(d, b, tile);
+    __syncthreads();
+    const double fv = scr ? far32_any<true>(Seq{}, d.nterms, d, b, mbuf, fl)
+                          : far32_any<false>(Seq{}, d.nterms, d, b, mbuf, fl);
+    const int t = b * kBatch + threadIdx.x;
+    if (t < d.tv.nt) d.out[d.tv.tperm[t]] = fv + nv;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void k_serve64(const EvalDesc* __restrict__ descs,
+                                                const int2* __restrict__ items, int nitems,
+                                                int* __restrict__ ctr, int parity) {
+  __shared__ __align__(16) double mbuf[nqpad(kMaxNterms)];  // near tile aliases it
+  using Seq = std::make_integer_sequence<int, kMaxNterms>;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[parity ^ 1] = 0;
+  for (;;) {
+    const int i = next_item(ctr + parity);
+    if (i >= nitems) break;
+    const int2 m = items[i];
+    const int sys = uniform(m.x), b = uniform(m.y);
+    const EvalDesc& d = descs[sys];
+    const bool scr = d.kappa != 0.0;
+    double4* tile = reinterpret_cast<double4*>(mbuf);
+    const double nv = scr ? near64_batch<true>(d, b, tile) : near64_batch<false>(d, b, tile);
+    __syncthreads();
+    const double fv = scr ? far64_any<true>(Seq{}, d.nterms, d, b, mbuf)
+                          : far64_any<false>(Seq{}, d.nterms, d, b, mbuf);
+    const int t = b * kBatch + threadIdx.x;
+    if (t < d.tv.nt) d.out[d.tv.tperm[t]] = fv + nv;
+    __syncthreads();
+  }
+}
+static_assert(nqpad(kMaxNterms) * sizeof(double) >= 64 * sizeof(double4), "near tile fits");
+
+// Work-groups of one persistent launch on the current device (what its CUs
+// hold at once), computed once per device.
+static int serve_groups(bool fp32) {
+  static std::mutex mu;
+  static int cached[2][64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  int& g = cached[fp32 ? 0 : 1][dev];
+  if (g == 0) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    const hipError_t e =
+        fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_serve32, kBatch, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_serve64, kBatch, 0);
+    if (e != hipSuccess || per <= 0 || cus <= 0) return 0;
+    g = cus * per;
+  }
+  return g;
+}
+
+hipError_t serve_field(const EvalDesc* descs, const int2* items, int nitems, bool fp32,
+                       int* ctr, int parity, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  const int groups = serve_groups(fp32);
+  if (groups <= 0) return hipErrorInvalidDevice;
+  const int nb = std::min(groups, nitems);
+  parity &= 1;
+  if (fp32)
+    hipLaunchKernelGGL(k_serve32, dim3(nb), dim3(kBatch), 0, s, descs, items, nitems, ctr, parity);
+  else
+    hipLaunchKernelGGL(k_serve64, dim3(nb), dim3(kBatch), 0, s, descs, items, nitems, ctr, parity);
   return hipGetLastError();
 }
 

@@ -10,11 +10,12 @@ boundary-integral GMRES solves) arrive over time.  The server:
   the tree);
 * runs a GPU loop of rounds (bit_serve_round): each round takes every ready
   evaluation and one step of every running GMRES solve -- its matrix-vector
-  product goes into the round's mixed far/near launches (one per parameter
-  group) with the evaluations, and its Arnoldi step (two Gram-Schmidt passes,
-  the norm, the updates) is batched with the other solves' -- so requests
-  arriving while a round runs join the next one, and a solve occupies a slot
-  for as many rounds as it needs products;
+  product goes into the round's persistent launch with the evaluations (one
+  launch per precision: device-sized work-groups take (system, batch) items
+  from a counter, all nterms and kernels in the same launch), and its Arnoldi
+  step (two Gram-Schmidt passes, the norm, the updates) is batched with the
+  other solves' -- so requests arriving while a round runs join the next one,
+  and a solve occupies a slot for as many rounds as it needs products;
 * keeps each solve's tree, lists and Krylov state (basis, residual, iterate)
   resident in HBM from admission to completion; only the Hessenberg column and
   the Givens rotations are on the host.

@@ -179,14 +179,16 @@ int bit_system_prepare(bit_system *sys, const bit_params *params);
 int bit_eval_potential_noalloc(bit_system *sys, const bit_params *params, double *d_out);
 
 /* Potentials of `count` independent systems (any sizes, per-system params) in
- * mixed launches on ctx's stream: one far-field and one near-field launch per
- * (nterms, precision, screened) group, each work-group serving one 64-target
- * batch of one system.  outs[i] receives system i's +phi (original order);
- * every target's sums are formed exactly as in bit_eval_potential, so results
- * are bit-identical alone and in any mix.  Systems may belong to other
- * contexts on the same device.  kernel_ms (may be NULL) gets the GPU time of
- * the launches.  No reference counterpart: batches the reference's per-system
- * drivers (tree3d.cpp:2368-2436) across requests. */
+ * one persistent launch per precision on ctx's stream: as many work-groups as
+ * the device holds at once take (system, 64-target batch) items from a device
+ * counter -- systems of larger nterms, then larger size, first -- and run each
+ * batch's near field and then its far field (the sweep chosen per item from
+ * the system's nterms and kernel).  outs[i] receives system i's +phi
+ * (original order); every target's sums are formed exactly as in
+ * bit_eval_potential, so results are bit-identical alone and in any mix.
+ * Systems may belong to other contexts on the same device.  kernel_ms (may be
+ * NULL) gets the GPU time of the launches.  No reference counterpart: batches
+ * the reference's per-system drivers (tree3d.cpp:2368-2436) across requests. */
 int bit_eval_potential_multi(bit_ctx *ctx, int count, bit_system *const *systems,
                              const bit_params *params, double *const *outs, int out_on_device,
                              double *kernel_ms);
@@ -259,7 +261,8 @@ int bit_krylov_solution(const bit_krylov *k, double *sigma);
 
 /* One serving round on ctx's stream: the potentials of n_eval systems (host
  * outputs, as bit_eval_potential_multi) and one step of each running solve,
- * all products in shared mixed far/near launches; each Gram-Schmidt pass, the
+ * all products in the same persistent launch (one per precision) as
+ * bit_eval_potential_multi's; each Gram-Schmidt pass, the
  * norms and the vector updates of all solves are one launch each, with one
  * read-back per pass.  Every result is bit-identical to what the request gets
  * alone (bit_eval_potential, bit_solve_second_kind).  A system may appear once

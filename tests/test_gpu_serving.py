@@ -1,5 +1,6 @@
-"""Serving: mixed multi-system launches and the continuous-batching server give
-each request the same result it gets alone."""
+"""Serving: mixed multi-system evaluations (one persistent launch per precision)
+and the continuous-batching server give each request the same result it gets
+alone."""
 import numpy as np
 import pytest
 
@@ -31,6 +32,33 @@ def test_mixed_launch_bit_identical(ctx):
     outs2, _ = bit_amd.eval_multi(ctx, [systems[i] for i in idx], [params[i] for i in idx])
     for i, b in zip(idx, outs2):
         assert np.array_equal(alone[i], b)
+    for s in systems:
+        s.close()
+
+
+def test_persistent_launch_every_nterms(ctx):
+    """One persistent launch per precision serves systems of every nterms (the
+    far sweep chosen per item), sizes from one particle up, both kernels; the
+    counters alternate between launches, so repeated launches must agree."""
+    rng = np.random.default_rng(8)
+    systems, params, alone = [], [], []
+    sizes = [1, 5, 63, 64, 65, 700, 3000, 9000]
+    for k, nt in enumerate(range(1, 14)):
+        n = sizes[k % len(sizes)]
+        P = workloads.make("blobs" if k % 2 else "uniform", n, 300 + k)
+        kap = 0.0 if k % 3 == 0 else float(rng.uniform(0.2, 2.0))
+        d2 = float(rng.uniform(0, 0.02)) ** 2
+        for prec in ("fp32", "fp64"):
+            s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, int(rng.integers(8, 120)))
+            alone.append(s.potential(nt, 0.6, kap, d2, None, prec))
+            systems.append(s)
+            params.append(bit_amd.make_params(nt, 0.6, kap, d2, None, prec))
+    for rep in range(3):
+        order = rng.permutation(len(systems))
+        outs, _ = bit_amd.eval_multi(ctx, [systems[i] for i in order],
+                                     [params[i] for i in order])
+        for i, b in zip(order, outs):
+            assert np.array_equal(alone[i], b), (rep, int(i))
     for s in systems:
         s.close()
 
