@@ -1969,10 +1969,12 @@ struct Acc<float> {
 // and each XCD has its own L2.  Consecutive batches share most of their
 // clusters and leaves, so work-group i takes unit xcd_unit(i, n): runs of
 // kXcdRun consecutive units per XCD (for i below the last whole round of
-// 8 kXcdRun, the identity after it).  A bijection on [0, n).  At 4M blobs the
-// far kernel's L2 misses fall by a fifth and the near kernel's by a third;
-// neither kernel's time changes (both are bound by issue, not memory).
-constexpr int kXcdRun = 16;
+// 8 kXcdRun, the identity after it).  A bijection on [0, n).  At 4M blobs,
+// against the identity, runs of 16 / 64 / 128 cut the far kernel's L2 misses
+// by about 19 / 47 / 59% and the near kernel's by 34 / 67 / 80% at unchanged times
+// (both kernels are bound by issue, not memory); runs of 256 and more slow
+// the step (the XCDs' shares of dense regions grow uneven).
+constexpr int kXcdRun = 64;
 __device__ __forceinline__ int xcd_unit(int i, int n) {
   const int full = n / (8 * kXcdRun) * (8 * kXcdRun);
   if (i >= full) return i;
