@@ -41,8 +41,8 @@ PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 MEASURED_TRAFFIC = {
     # (kernel, n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
     # (profiles/pmc_traffic_blobs4m_all_kernels.md)
-    ("far", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 3.79212e9,
-    ("near", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 4.94937e9,
+    ("far", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 3.121765e9,
+    ("near", 4_000_000, 3, 12, 0.25, 500, 0.5, 0.005, "fp32"): 1.408129e9,
 }
 # Executed FP32 lane FLOPs per launch: 64 x rocprofv3's SQ_INSTS_VALU_FLOPS_FP32 (a
 # per-wave count, 2 FMA + ADD + MUL + TRANS) on this exact configuration

@@ -352,7 +352,7 @@ hipError_t direct_field(const TreeView& tv, double delta2, int korder, int ntarg
 
 // ---- allocation-free evaluation ----
 // Scratch of bit_eval_potential_noalloc: the evaluation descriptor, then the
-// far-field results of up to kNoAllocMaxN targets.
+// near-field results of up to kNoAllocMaxN targets.
 constexpr size_t kNoAllocDescBytes = 1024;
 constexpr int kNoAllocMaxN = 1 << 22;
 constexpr size_t kNoAllocScratchBytes = kNoAllocDescBytes + (size_t)kNoAllocMaxN * sizeof(double);

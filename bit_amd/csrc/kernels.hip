@@ -3422,7 +3422,7 @@ hipError_t direct_field(const TreeView& tv, double delta2, int korder, int ntarg
 
 // ---------------------------------------------------------------------------
 // Allocation-free evaluation: its scratch (the evaluation descriptor and the
-// far-field results of up to kNoAllocMaxN targets) lives in this file-scope
+// near-field results of up to kNoAllocMaxN targets) lives in this file-scope
 // device array, one copy per GPU, held until the process ends.  Host code never
 // uses the array's name as an address: noalloc_scratch() resolves each device's
 // copy once with hipGetSymbolAddress and hands out that pointer.

@@ -166,7 +166,7 @@ int bit_system_prepare(bit_system *sys, const bit_params *params);
 
 /* No counterpart in the reference: bit_eval_potential for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
- * harnesses that forbid hipMalloc).  It allocates nothing: the far-field
+ * harnesses that forbid hipMalloc).  It allocates nothing: the near-field
  * results and the evaluation descriptor go to a static device array of
  * 32 MiB + 1 KiB per GPU (held until the process ends), so the system may hold
  * at most 4,194,304 particles (BIT_E_INVALID beyond).  The system must have
