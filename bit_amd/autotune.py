@@ -2,8 +2,9 @@
 
 For a given particle set and kernel (kappa, delta) it measures, on the GPU,
 both the error (relative L2 against an FP64 direct sum on a seeded sample of
-targets) and the cost (device time of one whole evaluation: tree, moments,
-lists, far and near kernels, from the library's own event timers) of candidate
+targets) and the cost (the wall time of one whole evaluation from inputs
+resident in HBM, as the benchmark's step runs it: device tree build, moments on
+a side stream beside the lists and the near field, far field) of candidate
 parameter sets, and returns the cheapest one whose error stays below the
 target with a safety margin.  For each (N0, nterms) the MAC parameter is
 scanned from the largest (cheapest) value down and the scan stops at the
@@ -40,18 +41,21 @@ def autotune(ctx, P, kappa: float, delta: float, target: float = 1e-6, margin: f
     ref = None
     try:
         for n0 in n0s:
-            s = bit_amd.System(ctx, *dev, n0)
-            tree_ms = s.stats()["ms_tree"]
             if ref is None:
+                s = bit_amd.System(ctx, *dev, n0)
                 ref = s.direct(kappa, d2, tidx)
+                s.close()
             for nt in nterms_list:
                 for acc in accs:  # largest (cheapest) first
+                    ctx.sync()
+                    t0 = time.perf_counter()
+                    s = bit_amd.System(ctx, *dev, n0)
                     s.potential(nt, acc, kappa, d2, None, precision, out=out)
+                    ms = 1e3 * (time.perf_counter() - t0)
                     st = s.stats()
+                    s.close()
                     phi = out.numpy()[tidx]
                     err = float(np.linalg.norm(phi - ref) / np.linalg.norm(ref))
-                    ms = (tree_ms + st["ms_moments"] + st["ms_lists"] + st["ms_far"] +
-                          st["ms_near"])
                     row = dict(maxmemb=n0, nterms=nt, acc=acc, err=err, ms=ms,
                                far_ms=st["ms_far"], near_ms=st["ms_near"])
                     table.append(row)
@@ -61,7 +65,6 @@ def autotune(ctx, P, kappa: float, delta: float, target: float = 1e-6, margin: f
                         if best is None or ms < best["ms"]:
                             best = row
                         break
-            s.close()
     finally:
         for d in dev + [out]:
             d.free()

@@ -1,7 +1,7 @@
 """Parity at the benchmark configurations' full sizes.
 
 * 4M-particle 16-blob screened workload at its tuned point (kappa 0.5, delta
-  0.005, p 11, acc 0.25, N0 500): tree (device build) and deep moments equal the
+  0.005, p 11, acc 0.255, N0 500): tree (device build) and deep moments equal the
   oracle's, the FP64 potential is within 1e-10 of the oracle's traversal on
   2,000 sampled targets, FP32 within FP32_TOL of the same.
 * 1M uniform, regularized Coulomb (delta 0.01, p 8, acc 0.5, N0 1000), FP64
@@ -69,7 +69,7 @@ def _full(ctx, oracle, P, maxmemb, nterms, acc, kappa, d2, ntargets, seed, tol32
 
 def test_blobs4m_fp64_and_fp32_equal_oracle(ctx, oracle):
     P = workloads.gaussian_blobs(4_000_000, seed=3)
-    s, dev, p64, p32 = _full(ctx, oracle, P, 500, 12, 0.25, 0.5, 0.005 ** 2, 2000, seed=17)
+    s, dev, p64, p32 = _full(ctx, oracle, P, 500, 12, 0.255, 0.5, 0.005 ** 2, 2000, seed=17)
     s.close()
     for a in dev:
         a.free()
