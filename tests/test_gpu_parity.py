@@ -379,6 +379,26 @@ def test_device_tree_equals_host_and_oracle(ctx, oracle, kind, n, seed, mm):
         assert np.array_equal(d.member_list(i), e.node_members(i))
 
 
+def test_device_build_rejects_inseparable_particles(ctx):
+    """Host inputs above 8,192 particles are built on the device.  Two points
+    holding 10,000 coincident particles each can never be split into leaves of
+    100: the build stops with BIT_E_TREE at its generation cap (the reference
+    grows its tree without bound), and the context stays usable."""
+    n = 20_000
+    x = np.where(np.arange(n) % 2 == 0, 0.25, -0.5)
+    y = np.full(n, 0.125)
+    z = np.where(np.arange(n) % 2 == 0, -0.375, 0.75)
+    q = np.ones(n)
+    with pytest.raises(bit_amd.BitError) as e:
+        bit_amd.System(ctx, x, y, z, q, 100)
+    assert e.value.code == bit_amd.BIT_E_TREE
+    P = workloads.uniform_cube(n, seed=5)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 100)
+    phi = s.potential(5, 0.5, 0.0, 0.0, None, "fp64")
+    assert s.numtree > 1 and np.all(np.isfinite(phi))
+    s.close()
+
+
 def test_device_inputs_rejected_when_nonfinite(ctx):
     x = np.random.default_rng(1).uniform(size=1000)
     x[17] = np.nan
