@@ -372,8 +372,9 @@ def summarize(reqs, makespan, rounds):
 def benchmark(requests: int = 2000, rate: float = 400.0, seed: int = 2024, builders: int = 8,
               hbm_budget: int = 16 << 30, warmup: int = 64, baseline: bool = True,
               sweep=(), sweep_requests: int = 1000, log=None):
-    """The serving stream at `rate` (continuous batching, then the same arrivals
-    one at a time), and the server alone at each rate of `sweep`."""
+    """The serving stream at `rate` and `sweep_requests` more at each rate of
+    `sweep`, each served with continuous batching and (baseline) the same
+    arrivals one request at a time."""
     srv = Server(hbm_budget=hbm_budget, builders=builders)
     try:
         if warmup > 0:  # first use of kernel instantiations and builder contexts
@@ -398,10 +399,20 @@ def benchmark(requests: int = 2000, rate: float = 400.0, seed: int = 2024, build
                 log(f"[serving] sweep {rt}/s: p50 {mm['p50_ms']:.1f} ms, p99 {mm['p99_ms']:.1f} ms,"
                     f" {mm['mean_systems_per_round']:.2f} systems/round, GPU busy "
                     f"{mm['gpu_busy_frac']:.2f} (rounds)")
-        if rows:
-            out["rate_sweep"] = rows
     finally:
         srv.close()
+    if baseline:
+        # the sweep's arrivals served one request at a time as well
+        for row in rows:
+            rq = materialize(make_requests(sweep_requests, row["rate_per_s"], seed + 7))
+            b = run_one_at_a_time(rq)
+            row["one_at_a_time"] = {k: b[k] for k in ("p50_ms", "p99_ms", "targets_per_s",
+                                                      "makespan_s")}
+            if log:
+                log(f"[serving] one at a time at {row['rate_per_s']}/s: p50 {b['p50_ms']:.1f} ms,"
+                    f" p99 {b['p99_ms']:.1f} ms, {b['targets_per_s'] / 1e6:.1f}M targets/s")
+    if rows:
+        out["rate_sweep"] = rows
     if baseline:
         base = materialize(make_requests(requests, rate, seed))
         out["one_at_a_time"] = run_one_at_a_time(base)
