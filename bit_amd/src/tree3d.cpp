@@ -712,7 +712,70 @@ double cluster_sum(int nterms, double dx, double dy, double dz, double dist, dou
   return sum;
 }
 
+// Taylor coefficients T_k(d) = (1/k!) d^k G of the regularized family
+// G_L(R) = sum_{l <= L} w_l DEL^l R^-(2l+1), R^2 = rs, w_l = prod_{j<=l} (j-1/2)/j
+// (L = KERNEL_ORDER; tree3d.cpp:614-791), for |k| <= deg, in t[i][j][k]
+// (i + j + k <= deg).  Radial derivatives of each term:
+//   (R^-1 d/dR)^m R^-(2l+1) = (-1)^m prod_{i<m} (2l+2i+1) R^-(2l+2m+1).
+void family_coeffs(int deg, double dx, double dy, double dz, double rs,
+                   std::vector<double>& t) {
+  const double R = sqrt(rs);
+  std::vector<double> g(deg + 1, 0.0);
+  double w = 1.0;  // w_l DEL^l
+  for (int l = 0; l <= KERNEL_ORDER; l++) {
+    if (l > 0) w *= (l - 0.5) / l * DEL;
+    double term = w / pow(R, 2 * l + 1);  // (R^-1 d/dR)^m R^-(2l+1) for m = 0, 1, ...
+    for (int m = 0; m <= deg; m++) {
+      g[m] += term;
+      term *= -(2.0 * l + 2.0 * m + 1.0) / rs;
+    }
+  }
+  std::vector<double> px(deg + 1), py(deg + 1), pz(deg + 1), h(deg + 1);
+  px[0] = py[0] = pz[0] = h[0] = 1.0;
+  for (int a = 1; a <= deg; a++) {
+    px[a] = px[a - 1] * dx / a;
+    py[a] = py[a - 1] * dy / a;
+    pz[a] = pz[a - 1] * dz / a;
+    h[a] = h[a - 1] / (2.0 * a);
+  }
+  const int n1 = deg + 1;
+  t.assign((size_t)n1 * n1 * n1, 0.0);
+  for (int i = 0; i <= deg; i++)
+    for (int j = 0; i + j <= deg; j++)
+      for (int k = 0; i + j + k <= deg; k++) {
+        double a = 0.0;
+        for (int l0 = 0; 2 * l0 <= i; l0++)
+          for (int l1 = 0; 2 * l1 <= j; l1++)
+            for (int l2 = 0; 2 * l2 <= k; l2++)
+              a += px[i - 2 * l0] * py[j - 2 * l1] * pz[k - 2 * l2] * h[l0] * h[l1] * h[l2] *
+                   g[i + j + k - l0 - l1 - l2];
+        t[((size_t)i * n1 + j) * n1 + k] = a;
+      }
+}
+
 }  // namespace
+
+// One cluster's field at offset (dx, dy, dz) = target - centre, rs = |d|^2 + DEL
+// (tree3d.cpp:456-810): E_e += sum_{|k| < nterms} (-1)^(|k|+1) (k_e + 1) M_k
+// T_{k+e}(d), the same sum as the reference's, with its coefficients from the
+// family's radial derivatives instead of the reference's recurrences.
+void treetaylorforce(int nterms, double dx, double dy, double dz, double rs, double* xfor,
+                     double* yfor, double* zfor, double*** moment) {
+  if (nterms < 1) return;
+  std::vector<double> t;
+  family_coeffs(nterms, dx, dy, dz, rs, t);
+  const int n1 = nterms + 1;
+  auto T = [&](int i, int j, int k) { return t[((size_t)i * n1 + j) * n1 + k]; };
+  for (int m = 0; m < nterms; m++)
+    for (int k = 0; k < nterms - m; k++)
+      for (int i = 0; i < nterms - m - k; i++) {
+        const double s = ((i + k + m + 1) % 2 == 0) ? 1.0 : -1.0;
+        const double M = moment[i][k][m];
+        *xfor += s * M * T(i + 1, k, m) * (i + 1);
+        *yfor += s * M * T(i, k + 1, m) * (k + 1);
+        *zfor += s * M * T(i, k, m + 1) * (m + 1);
+      }
+}
 
 void shielded_treetaylorpot(int nterms, double dx, double dy, double dz, double dist,
                             double* pot, double*** moment, double kappa) {

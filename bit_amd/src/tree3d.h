@@ -21,7 +21,7 @@
 //                       GPU FP64 direct sum over all particles; per-particle
 //                       calls are served from one evaluation of all particles
 //                       while the array holds the same positions and charges
-//   eval_grad_kernel_*, treetaylorpotential, shielded_treetaylorpot
+//   eval_grad_kernel_*, treetaylorpotential, shielded_treetaylorpot, treetaylorforce
 //                       one pair / one cluster on the host (primitives)
 #ifndef tree3D_h
 #define tree3D_h 1
@@ -109,6 +109,9 @@ void shielded_treetaylorpot(int nterms, double dx, double dy, double dz, double 
                             double* pot, double*** moment, double kappa);
 void treetaylorpotential(int nterms, double dx, double dy, double dz, double rs, double* pot,
                          double*** moment);
+// the field of one cluster (rs = |d|^2 + DEL; KERNEL_ORDER family; += into the outputs)
+void treetaylorforce(int nterms, double dx, double dy, double dz, double rs, double* xfor,
+                     double* yfor, double* zfor, double*** moment);
 
 // ---- per-target evaluation (accumulate into the outputs) ----
 double treeforce(int ind, int p, int nterms, double x, double y, double z, double acc,

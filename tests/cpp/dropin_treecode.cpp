@@ -373,7 +373,8 @@ static int points_ds_mode(const char* in, const char* out) {
 // device).  IN: nterms, kappa, ncase, then per case dx, dy, dz, rs and the
 // moment cube [nterms]^3.  OUT per case: shielded_treetaylorpot's pot (from 0),
 // treetaylorpotential's pot (from 0), eval_grad_kernel_noreg (3) and
-// eval_grad_kernel_reg (3) at (dx, dy, dz, rs).
+// eval_grad_kernel_reg (3) at (dx, dy, dz, rs), treetaylorforce's field (3,
+// from 0).
 static int primitives_mode(const char* in, const char* out) {
   FILE* f = fopen(in, "rb");
   double hdr[3];
@@ -397,9 +398,12 @@ static int primitives_mode(const char* in, const char* out) {
     treetaylorpotential(nt, d[0], d[1], d[2], d[3], &p2, planes.data());
     eval_grad_kernel_noreg(d[0], d[1], d[2], d[3], &g[0], &g[1], &g[2]);
     eval_grad_kernel_reg(d[0], d[1], d[2], d[3], &g[3], &g[4], &g[5]);
+    double e[3] = {0.0, 0.0, 0.0};
+    treetaylorforce(nt, d[0], d[1], d[2], d[3], &e[0], &e[1], &e[2], planes.data());
     res.push_back(p1);
     res.push_back(p2);
     res.insert(res.end(), g, g + 6);
+    res.insert(res.end(), e, e + 3);
   }
   fclose(f);
   write_all(out, res);
