@@ -11,7 +11,8 @@ interaction lists, far field and near field, result left in HBM.
 Prints ONE JSON line.  Extra keys: `accuracy` (relative L2 error of the last
 step against an FP64 GPU direct sum on 10,000 seeded targets), `breakdown`
 (ms per stage of the last step), `fp64_ms_per_step` (one FP64 step, untimed
-for the metric), `roofline` (dominant kernel), `serving` (p50/p99 request
+for the metric), `configs` (the workload set's other configurations, timed),
+`roofline` (dominant kernel), `serving` (p50/p99 request
 latency and throughput of a seeded Poisson request stream served with
 continuous batching) and `cpu_baseline` (the oracle's OpenMP treecode on the
 host cores, extrapolated from a bounded target sample).
@@ -148,6 +149,70 @@ def field_timings(ctx, log):
     return out
 
 
+def config_timings(ctx, log):
+    """The other configurations of the workload set, each timed as whole
+    evaluations from inputs already in HBM (tree, moments, lists, near and far
+    field; one untimed evaluation first, then the mean of three):
+    * uniform10k_coulomb: 10,000 uniform particles in [-0.5, 0.5]^3, 1/r,
+      theta 0.5, p 6, N0 500, FP64; error against the FP64 direct sum at every
+      particle;
+    * uniform1m_regularized: 1,000,000 uniform particles, 1/sqrt(r^2 + delta^2),
+      delta 0.01, theta 0.5, p 8, N0 1000, FP64 and FP32; error against the FP64
+      direct sum on 10,000 seeded targets;
+    * sphere_bie_level7: the second-kind screened-Poisson solve on the level-7
+      icosphere (327,680 unknowns, kappa 1, delta = mean edge, theta 0.6, p 6,
+      GMRES(50) to 1e-6) with FP64 and FP32 products; setup (tree build) and solve
+      wall times, iterations, residual and distance to the modal solution."""
+    import bit_amd
+    from bit_amd import bie, workloads
+    out = {}
+    for name, n, seed, mm, nt, d2, precs, ntgt in (
+            ("uniform10k_coulomb", 10_000, 1, 500, 7, 0.0, ("fp64",), None),
+            ("uniform1m_regularized", 1_000_000, 2, 1000, 9, 1e-4, ("fp64", "fp32"), 10_000)):
+        P = workloads.uniform_cube(n, seed=seed)
+        dev = [bit_amd.DeviceArray.from_numpy(ctx, a) for a in (P.x, P.y, P.z, P.q)]
+        res = bit_amd.DeviceArray(ctx, n)
+        tidx = (np.arange(n, dtype=np.int32) if ntgt is None else
+                workloads.sample_targets(n, ntgt, seed=7))
+        s = bit_amd.System(ctx, *dev, mm)
+        ref = s.direct(0.0, d2, tidx)
+        s.close()
+        rec = {"n": n, "maxmemb": mm, "nterms": nt, "p": nt - 1, "acc": 0.5, "delta2": d2,
+               "targets_checked": int(len(tidx))}
+        for prec in precs:
+            def one():
+                s = bit_amd.System(ctx, *dev, mm)
+                s.potential(nt, 0.5, 0.0, d2, None, prec, out=res)
+                s.close()
+            one()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                one()
+            ctx.sync()
+            ms = 1e3 * (time.perf_counter() - t0) / 3
+            err = float(np.linalg.norm(res.numpy()[tidx] - ref) / np.linalg.norm(ref))
+            rec[prec] = {"ms_per_evaluation": ms, "targets_per_s": n / (ms * 1e-3),
+                         "rel_l2_vs_fp64_direct": err}
+            log(f"[bench] {name} {prec}: {ms:.2f} ms per evaluation, err {err:.2e}")
+        for d in dev + [res]:
+            d.free()
+        out[name] = rec
+    rec = {"level": 7, "kappa": 1.0, "acc": 0.6, "nterms": 7, "p": 6, "restart": 50,
+           "tol": 1e-6}
+    for prec in ("fp64", "fp32"):
+        r = bie.solve_sphere(ctx, level=7, kappa=1.0, nterms=7, acc=0.6, restart=50, tol=1e-6,
+                             lmax=8, seed=17, maxmemb=500, precision=prec)
+        rec["unknowns"] = r["n"]
+        rec[prec] = {"setup_ms": 1e3 * r["setup_s"], "solve_ms": 1e3 * r["solve_s"],
+                     "iterations": r["iters"], "relres": r["relres"],
+                     "rel_l2_vs_modal_solution": r["err_vs_modal"]}
+        log(f"[bench] sphere BIE level 7 {prec}: setup {1e3 * r['setup_s']:.1f} ms, solve "
+            f"{1e3 * r['solve_s']:.1f} ms, {r['iters']} iterations, relres {r['relres']:.2e}")
+    out["sphere_bie_level7"] = rec
+    return out
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -183,6 +248,9 @@ def main():
                     help="skip the literal-parameter runs (p 8, acc 0.7, N0 2000)")
     ap.add_argument("--no-field", action="store_true",
                     help="skip the field (treeforce) timings at 16k and 1M particles")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (10k Coulomb, 1M regularized, sphere "
+                         "BIE at level 7)")
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=400.0)
@@ -351,6 +419,9 @@ def main():
     field = None
     if not args.no_field:
         field = field_timings(ctx, log)
+    configs = None
+    if not args.no_configs:
+        configs = config_timings(ctx, log)
 
     serving = None
     if args.serving_requests > 0:
@@ -401,6 +472,7 @@ def main():
         "fp64": fp64,
         "literal": literal,
         "field": field,
+        "configs": configs,
         "roofline": roof,
         "serving": serving,
         "cpu_baseline": cpu,
