@@ -95,8 +95,29 @@ def cpu_baseline(P, nterms, acc, kappa, d2, maxmemb, sample: int, seed: int = 5)
     t3 = time.perf_counter()
     per_target = (t3 - t2) / len(tidx)
     total = (t1 - t0) + (t2 - t1) + P.n * per_target
+    # the workload set's other potential configurations on the same host cores:
+    # 10,000 uniform Coulomb in full, 1M uniform regularized with the traversal
+    # timed on 5,000 seeded targets (tree and moments in full)
+    configs = {}
+    for name, n, seed, mm, nt, d2, samp in (("uniform10k_coulomb", 10_000, 1, 500, 7, 0.0, None),
+                                           ("uniform1m_regularized", 1_000_000, 2, 1000, 9,
+                                            1e-4, 5000)):
+        Q = workloads.uniform_cube(n, seed=seed)
+        a0 = time.perf_counter()
+        tq = O.tree(Q.x, Q.y, Q.z, mm)
+        mq = tq.moments(Q.q, nt)
+        a1 = time.perf_counter()
+        ti = (np.arange(n, dtype=np.int32) if samp is None else
+              workloads.sample_targets(n, samp, seed=seed + 10))
+        tq.potential(Q.q, mq, nt, 0.5, 0.0, d2, d2, ti, nthreads=threads)
+        a2 = time.perf_counter()
+        tot = (a1 - a0) + n * (a2 - a1) / len(ti)
+        configs[name] = {"value": n / tot, "unit": "targets/s", "ms_per_evaluation": 1e3 * tot,
+                         "sample": ("all targets" if samp is None else
+                                    f"traversal on {samp} seeded targets, extrapolated")}
     return {
         "value": P.n / total, "unit": "targets/s", "cores": threads, "kind": "port",
+        "configs": configs,
         "cpu_model": cpu_model(), "nproc": os.cpu_count(),
         "sample": f"tree+moments of all {P.n} particles timed in full ({t1 - t0:.2f}s + "
                   f"{t2 - t1:.2f}s); traversal timed on {len(tidx)} seeded targets "
@@ -444,6 +465,11 @@ def main():
         cpu = cpu_baseline(P, args.nterms, args.acc, args.kappa, d2, args.maxmemb,
                            args.cpu_sample)
         log(f"[bench] CPU baseline {cpu['value']:.3e} targets/s on {cpu['cores']} threads")
+        if configs:
+            for name, c in cpu["configs"].items():
+                if name in configs:
+                    configs[name]["cpu_port_targets_per_s"] = c["value"]
+                    log(f"[bench] {name} on the host: {c['value']:.3e} targets/s")
 
     line = {
         "metric": "targets/sec at N=4M, rel L2 err<=1e-6 vs FP64; p50/p99 request latency",
