@@ -452,7 +452,7 @@ def main():
             d.free()
         sweep = [float(v) for v in args.serving_sweep.split(",") if v]
         serving = srv_mod.benchmark(args.serving_requests, args.serving_rate, seed=2024,
-                                    builders=8, hbm_budget=16 << 30, warmup=64,
+                                    builders=4, hbm_budget=16 << 30, warmup=64,
                                     baseline=not args.no_serving_baseline, sweep=sweep,
                                     sweep_requests=1000, log=log)
         serving["workload"] = ("Poisson arrivals; N log-uniform [1e3,2e5]; uniform/blobs; "
