@@ -169,10 +169,9 @@ class Server:
     request's system (tree, moments, lists; for a solve also its resident
     Krylov state), and the GPU loop serves rounds (bit_serve_round): every
     ready evaluation and one step of every running solve in shared launches.
-    Four builders: each builder's context adds streams, which share the
-    process's few hardware queues with the rounds' stream; at 6,400
-    requests/s four builders serve 88-90M targets/s against 82M with eight,
-    and at 400/s the latencies are the same."""
+    Four builders by default: at 6,400 requests/s four serve 88-90M targets/s
+    against 82M with eight (every builder's device work competes with the
+    rounds), and at 400/s the latencies are the same."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 16 << 30, builders: int = 4,
                  host_threads: int = 2, precision: str = "fp32", max_round_systems: int = 256):
