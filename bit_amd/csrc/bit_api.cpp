@@ -639,7 +639,6 @@ int build_on_device(bit_ctx* ctx, bit_system* s, int n, const double* dx, const 
 // 0.48 / 0.57 ms, 20,000 1.7 / 0.95 ms, 200,000 15.1 / 2.0 ms, 500,000 41 / 3.1 ms.
 constexpr int kHostBuildMax = 8192;
 
-
 int build_on_host(bit_ctx* ctx, bit_system* s, int n, const double* x, const double* y,
                   const double* z, const double* dq, int maxmemb) {
   std::string err;
@@ -926,9 +925,11 @@ int bit_ctx_destroy(bit_ctx* c) {
 int bit_ctx_set_stream(bit_ctx* c, void* stream) {
   if (!c) return fail(BIT_E_INVALID, "null context");
   // the persistent launches' counters are handed from launch to launch in
-  // stream order: the old stream's launches finish before the new one's
+  // stream order: the old stream's launches finish before the new one's (every
+  // call already ends synchronised; this covers calls from other threads, and a
+  // caller's stream destroyed before the switch is not an error)
   std::lock_guard<std::mutex> lk(c->serve_mu);
-  if (c->serve_ctr) HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->serve_ctr) (void)hipStreamSynchronize(c->stream);
   c->stream = stream ? static_cast<hipStream_t>(stream) : c->own;
   return BIT_OK;
 }
@@ -1297,9 +1298,10 @@ static void collect_times(bit_system* s) {
   s->stats.ms_near = s->t_near.ms();
 }
 
-// Moments and lists for params, the far-field buffer, and the evaluation
-// descriptor (result to `dst`, a device pointer); nothing is launched for the
-// far/near kernels.
+// Moments and lists for params, the near field's per-target buffer, and the
+// evaluation descriptor (result to `dst`, a device pointer); nothing is
+// launched for the near/far kernels.  defer_join (FP32): the moments may still
+// run on the side stream when this returns (join_moments before the far field).
 static int prepare_eval(bit_system* s, const bit_params* p, double* dst,
                         bool defer_join = false) {
   BIT_TRY(check_params(s, p));

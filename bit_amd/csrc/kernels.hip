@@ -2148,7 +2148,6 @@ __device__ __forceinline__ double far32_batch(const EvalDesc& d, int b, float* m
     u = un;
     n = nn;
   }
-  (void)valid;
   return acc.value();
 }
 
@@ -2212,7 +2211,6 @@ __device__ __forceinline__ double far64_batch(const EvalDesc& d, int b, double* 
     }
     __syncthreads();  // the tile is overwritten next item
   }
-  (void)valid;
   return acc.value();
 }
 
@@ -2476,7 +2474,6 @@ __device__ __forceinline__ double near32_batch(const EvalDesc& d, int b, float4*
       __syncthreads();
     }
   }
-  (void)valid;
   return SCALED ? (double)acc.value() * cs : acc.value();
 }
 
@@ -2589,7 +2586,6 @@ __device__ __forceinline__ double near64_batch(const EvalDesc& d, int b, double4
       __syncthreads();
     }
   }
-  (void)valid;
   return acc;
 }
 
