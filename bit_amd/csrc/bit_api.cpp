@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <climits>
 #include <mutex>
+#include <functional>
 #include <map>
 #include <atomic>
 #include <chrono>
@@ -1247,20 +1248,21 @@ static int ensure_qc64(bit_system* s, int nterms, hipStream_t st = nullptr) {
 
 // Field coefficients Q^(e) for the current moments, computed on first use
 // (index 0: FP32, 1: FP64).
-static int ensure_qf(bit_system* s, int nterms, bool fp32) {
+static int ensure_qf(bit_system* s, int nterms, bool fp32, hipStream_t st = nullptr) {
   const int k = fp32 ? 0 : 1;
   if (s->qf_nterms[k] == nterms) return BIT_OK;
   bit_ctx* c = s->ctx;
+  if (!st) st = c->stream;
   const int nt = s->tree.numtree;
   const size_t per = (size_t)3 * bit::dev::nqpad(nterms + 1);
   if (fp32) {
     BIT_TRY(s->qf32.ensure(c, (size_t)nt * per * sizeof(float)));
     HIP_TRY(bit::dev::field_coeffs32(nt, nterms, s->mom_hat64.as<double>(), s->qf32.as<float>(),
-                                     c->stream));
+                                     st));
   } else {
     BIT_TRY(s->qf64.ensure(c, (size_t)nt * per * sizeof(double)));
     HIP_TRY(bit::dev::field_coeffs64(nt, nterms, s->mom_hat64.as<double>(), s->qf64.as<double>(),
-                                     c->stream));
+                                     st));
   }
   s->qf_nterms[k] = nterms;
   return BIT_OK;
@@ -1298,22 +1300,19 @@ static void collect_times(bit_system* s) {
   s->stats.ms_near = s->t_near.ms();
 }
 
-// Moments and lists for params, the near field's per-target buffer, and the
-// evaluation descriptor (result to `dst`, a device pointer); nothing is
-// launched for the near/far kernels.  defer_join (FP32): the moments may still
-// run on the side stream when this returns (join_moments before the far field).
-static int prepare_eval(bit_system* s, const bit_params* p, double* dst,
-                        bool defer_join = false) {
-  BIT_TRY(check_params(s, p));
+// The moments for params (and, after them on the same stream, the cluster
+// coefficients `coeffs` launches) and the lists.  When both have to be made,
+// the moments run on the side stream while the lists (bound by scalar issue)
+// build on the context's stream; the context's stream then waits for them, or,
+// with defer_join, leaves that to join_moments (the caller runs work that needs
+// the lists only in between).
+static int moments_and_lists(bit_system* s, const bit_params* p, bool defer_join,
+                             const std::function<int(hipStream_t)>& coeffs) {
   bit_ctx* c = s->ctx;
-  HIP_TRY(hipSetDevice(c->device));
-  const int n = s->tree.n;
   const ListSet& L = s->pl;
   const bool lists_ok =
       L.valid && L.acc == p->acc && L.delta2 == p->delta2 && L.sub == p->mac_sub;
   if (s->mom_nterms != p->nterms && !lists_ok && s->nchunks > 0) {
-    // the moments (latency-bound upward pass) on the side stream while the
-    // list traversal (bound by scalar issue) runs on the context's stream
     std::lock_guard<std::mutex> lk(c->fork_mu);
     if (!c->side) HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     if (!c->fork) HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
@@ -1321,18 +1320,33 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst,
     HIP_TRY(hipEventRecord(c->fork, c->stream));
     HIP_TRY(hipStreamWaitEvent(c->side, c->fork, 0));
     int rc = moments_on(s, p->nterms, c->side);
+    if (rc == BIT_OK) rc = coeffs(c->side);
     HIP_TRY(hipEventRecord(c->join, c->side));
     if (rc == BIT_OK) rc = ensure_lists(s, p);
-    // deferred: the caller launches the near field (lists only) first and
-    // joins before the far field (join_moments).  Not for FP64, whose far
-    // coefficients are computed below on the context's stream from the moments.
-    if (defer_join && rc == BIT_OK && p->precision == BIT_FP32) s->join_pending = true;
+    if (defer_join && rc == BIT_OK) s->join_pending = true;
     else HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
-    BIT_TRY(rc);
-  } else {
-    BIT_TRY(bit_moments(s, p->nterms));
-    BIT_TRY(ensure_lists(s, p));
+    return rc;
   }
+  BIT_TRY(bit_moments(s, p->nterms));
+  BIT_TRY(coeffs(c->stream));
+  return ensure_lists(s, p);
+}
+
+// Moments and lists for params, the near field's per-target buffer, and the
+// evaluation descriptor (result to `dst`, a device pointer); nothing is
+// launched for the near/far kernels.  defer_join: the moments may still run on
+// the side stream when this returns (join_moments before the far field).
+static int prepare_eval(bit_system* s, const bit_params* p, double* dst,
+                        bool defer_join = false) {
+  BIT_TRY(check_params(s, p));
+  bit_ctx* c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = s->tree.n;
+  // FP64's cluster coefficients follow the moments on their stream (FP32's are
+  // made with the moments)
+  BIT_TRY(moments_and_lists(s, p, defer_join, [&](hipStream_t st) {
+    return p->precision == BIT_FP64 ? ensure_qc64(s, p->nterms, st) : BIT_OK;
+  }));
   BIT_TRY(s->part.ensure(c, (size_t)n * sizeof(double)));
   if (!dst) {
     BIT_TRY(s->out.ensure(c, (size_t)n * sizeof(double)));
@@ -1343,7 +1357,6 @@ static int prepare_eval(bit_system* s, const bit_params* p, double* dst,
   d.kappa = p->kappa;
   d.delta2 = p->delta2;
   d.qc32 = s->qc32.as<float>();
-  if (p->precision == BIT_FP64) BIT_TRY(ensure_qc64(s, p->nterms));
   d.qc64 = s->qc64.as<double>();
   d.far_off = s->pl.far_off.as<int>();
   d.far_items = s->pl.far_items.as<bit::dev::Item>();
@@ -1494,15 +1507,16 @@ int bit_eval_field(bit_system* s, const bit_params* p, double* out, int out_on_d
   bit_ctx* c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   const int n = s->tree.n;
-  BIT_TRY(bit_moments(s, p->nterms));
-  BIT_TRY(ensure_lists(s, p));
+  const bool fp32 = p->precision == BIT_FP32;
+  // the field's coefficients follow the moments on their stream, beside the lists
+  BIT_TRY(moments_and_lists(s, p, false,
+                            [&](hipStream_t st) { return ensure_qf(s, p->nterms, fp32, st); }));
   BIT_TRY(s->part.ensure(c, (size_t)3 * n * sizeof(double)));
   double* dst = out;
   if (!out_on_device) {
     BIT_TRY(s->out.ensure(c, (size_t)3 * n * sizeof(double)));
     dst = s->out.as<double>();
   }
-  const bool fp32 = p->precision == BIT_FP32;
   BIT_TRY(ensure_qf(s, p->nterms, fp32));
   s->t_far.start(c->stream);
   HIP_TRY(bit::dev::far_field_grad(s->view(), p->nterms, p->delta2, p->kernel_order, fp32,
