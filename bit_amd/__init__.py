@@ -57,7 +57,7 @@ class BitError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("nterms", C.c_int), ("acc", C.c_double), ("kappa", C.c_double),
                 ("delta2", C.c_double), ("mac_sub", C.c_double), ("precision", C.c_int),
-                ("kernel_order", C.c_int)]
+                ("kernel_order", C.c_int), ("root", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -194,9 +194,11 @@ def _f64(a):
 
 
 def make_params(nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None, precision="fp64",
-                kernel_order=0):
+                kernel_order=0, root=0):
     """Evaluation parameters; mac_sub defaults to delta2 (the MAC on the true
-    squared distance), the reference's treepotential/treeforce convention."""
+    squared distance), the reference's treepotential/treeforce convention.
+    root: the node the traversal starts at (the reference's `ind`); 0 = the
+    whole tree, another node gives that subtree's contribution."""
     p = Params()
     p.nterms = int(nterms)
     p.acc = float(acc)
@@ -205,6 +207,7 @@ def make_params(nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None, precisio
     p.mac_sub = float(delta2 if mac_sub is None else mac_sub)
     p.precision = BIT_FP32 if precision in ("fp32", BIT_FP32) else BIT_FP64
     p.kernel_order = int(kernel_order)
+    p.root = int(root)
     return p
 
 
@@ -512,11 +515,12 @@ class System:
         return out.reshape(self.numtree, p, p, p)
 
     def potential(self, nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None,
-                  precision="fp64", out=None):
+                  precision="fp64", out=None, root=0):
         """+phi at every particle (self excluded), original order.  `out` may be
         a float64 DeviceArray / torch device tensor of n elements (the result
-        then stays in HBM)."""
-        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+        then stays in HBM).  root: start the traversal at that node (the
+        subtree's contribution, the reference's `ind`)."""
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision, 0, root)
         if out is not None and _dev_ptr(out) is not None:
             _check(lib.bit_eval_potential(self.h, C.byref(prm), C.c_void_p(_dev_ptr(out)), 1))
             return out
@@ -538,18 +542,18 @@ class System:
         return out
 
     def field(self, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0,
-              precision="fp64"):
+              precision="fp64", root=0):
         """E = sum q (x_i - x_j)/R^3 (regularized Coulomb), [n, 3], original order;
         kernel_order >= 1 adds the regularized family's higher terms."""
-        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, precision, kernel_order)
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, precision, kernel_order, root)
         res = np.zeros(3 * self.n)
         _check(lib.bit_eval_field(self.h, C.byref(prm), _ptr(res), 0))
         return res.reshape(self.n, 3)
 
     def potential_at(self, px, py, pz, nterms=6, acc=ACC, kappa=0.0, delta2=0.0, mac_sub=None,
-                     precision="fp64"):
-        """+phi at arbitrary points (all sources contribute)."""
-        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision)
+                     precision="fp64", root=0):
+        """+phi at arbitrary points (all sources of the subtree at `root`)."""
+        prm = make_params(nterms, acc, kappa, delta2, mac_sub, precision, 0, root)
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         out = np.zeros(len(px))
         _check(lib.bit_eval_potential_points(self.h, C.byref(prm), len(px), _ptr(px), _ptr(py),
@@ -557,9 +561,9 @@ class System:
         return out
 
     def field_at(self, px, py, pz, nterms=6, acc=ACC, delta2=DEL, mac_sub=None, kernel_order=0,
-                 precision="fp64"):
+                 precision="fp64", root=0):
         """Regularized Coulomb field at arbitrary points, [m, 3]."""
-        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, precision, kernel_order)
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, precision, kernel_order, root)
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         out = np.zeros(3 * len(px))
         _check(lib.bit_eval_field_points(self.h, C.byref(prm), len(px), _ptr(px), _ptr(py),
@@ -591,10 +595,10 @@ class System:
                                            _ptr(px), _ptr(py), _ptr(pz), _ptr(out)))
         return out.reshape(-1, 3)
 
-    def lists(self, nterms=6, acc=ACC, delta2=0.0, mac_sub=None):
+    def lists(self, nterms=6, acc=ACC, delta2=0.0, mac_sub=None, root=0):
         """Batch interaction lists for the MAC parameters: dict of far/near
         offsets [nbatch+1], masks (uint64) and nodes."""
-        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64")
+        prm = make_params(nterms, acc, 0.0, delta2, mac_sub, "fp64", 0, root)
         nb, nf, nn = C.c_int(), C.c_int(), C.c_int()
         _check(lib.bit_system_export_lists(self.h, C.byref(prm), C.byref(nb), C.byref(nf),
                                            C.byref(nn), None, None, None, None, None, None))

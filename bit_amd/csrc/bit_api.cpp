@@ -233,6 +233,7 @@ struct Timer {
 struct ListSet {
   bool valid = false;
   double acc = 0, delta2 = 0, sub = 0;
+  int root = 0;
   int nfar = 0, nnear = 0;
   int passes = 0;  // traversals of the last build
   DevBuf far_cnt, near_cnt, far_off, near_off, far_items, near_items;
@@ -725,7 +726,16 @@ int check_params(const bit_system* s, const bit_params* p, bool field = false) {
     return fail(BIT_E_INVALID, "acc/kappa/delta2/mac_sub must be finite, kappa,delta2 >= 0");
   if (p->precision != BIT_FP64 && p->precision != BIT_FP32)
     return fail(BIT_E_INVALID, "precision must be BIT_FP64 or BIT_FP32");
+  if (p->root < 0 || p->root >= s->tree.numtree)
+    return fail(BIT_E_INVALID, "root must be a node of the tree: [0, " +
+                                   std::to_string(s->tree.numtree) + ")");
   return BIT_OK;
+}
+
+// Lists built for these MAC parameters and start node.
+bool lists_match(const ListSet& L, const bit_params* p) {
+  return L.valid && L.acc == p->acc && L.delta2 == p->delta2 && L.sub == p->mac_sub &&
+         L.root == p->root;
 }
 
 // Staging capacity for a batch's next build: the largest count seen plus an
@@ -744,7 +754,7 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   L.valid = false;  // set again only when both passes were issued
   bit_ctx* c = s->ctx;
   const int nb = (tv.nt + bit::dev::kBatch - 1) / bit::dev::kBatch;
-  bit::dev::MacParams mp{p->acc, p->delta2, p->mac_sub};
+  bit::dev::MacParams mp{p->acc, p->delta2, p->mac_sub, p->root};
   BIT_TRY(L.far_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
   BIT_TRY(L.near_cnt.ensure(c, (size_t)(nb + 1) * sizeof(int)));
   BIT_TRY(L.far_off.ensure(c, (size_t)(nb + 1) * sizeof(int)));
@@ -843,6 +853,7 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   L.acc = p->acc;
   L.delta2 = p->delta2;
   L.sub = p->mac_sub;
+  L.root = p->root;
   s->stats.far_items = tot[0];
   s->stats.near_items = tot[1];
   s->stats.ms_lists = -1.0;  // filled after the next synchronisation
@@ -853,7 +864,7 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
 // Lists of the particles as targets, cached per MAC parameters.
 int ensure_lists(bit_system* s, const bit_params* p) {
   ListSet& L = s->pl;
-  if (L.valid && L.acc == p->acc && L.delta2 == p->delta2 && L.sub == p->mac_sub) return BIT_OK;
+  if (lists_match(L, p)) return BIT_OK;
   return build_lists(s, L, s->view(), p);
 }
 
@@ -869,6 +880,7 @@ void bit_params_default(bit_params* p) {
   p->mac_sub = 0.0;
   p->precision = BIT_FP64;
   p->kernel_order = 0;
+  p->root = 0;
 }
 
 const char* bit_last_error(void) { return g_err.c_str(); }
@@ -1310,8 +1322,7 @@ static int moments_and_lists(bit_system* s, const bit_params* p, bool defer_join
                              const std::function<int(hipStream_t)>& coeffs) {
   bit_ctx* c = s->ctx;
   const ListSet& L = s->pl;
-  const bool lists_ok =
-      L.valid && L.acc == p->acc && L.delta2 == p->delta2 && L.sub == p->mac_sub;
+  const bool lists_ok = lists_match(L, p);
   if (s->mom_nterms != p->nterms && !lists_ok && s->nchunks > 0) {
     std::lock_guard<std::mutex> lk(c->fork_mu);
     if (!c->side) HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
@@ -1416,8 +1427,8 @@ int bit_eval_potential_noalloc(bit_system* s, const bit_params* p, double* d_out
                                    std::to_string(bit::dev::kNoAllocMaxN) + " particles (system has " +
                                    std::to_string(n) + ")");
   const ListSet& L = s->pl;
-  if (s->mom_nterms != p->nterms || !L.valid || L.acc != p->acc || L.delta2 != p->delta2 ||
-      L.sub != p->mac_sub || (p->precision == BIT_FP64 && s->qc64_nterms != p->nterms))
+  if (s->mom_nterms != p->nterms || !lists_match(L, p) ||
+      (p->precision == BIT_FP64 && s->qc64_nterms != p->nterms))
     return fail(BIT_E_STATE, "system not prepared for these parameters: call bit_system_prepare");
   bit_ctx* c = s->ctx;
   if (!c->scratch) return fail(BIT_E_STATE, "context has no allocation-free scratch");

@@ -47,6 +47,7 @@ struct MacParams {
   double acc;
   double delta2;
   double mac_sub;
+  int root;  // node the traversal starts at (0: the whole tree)
 };
 
 // counters accumulated by the list-count pass

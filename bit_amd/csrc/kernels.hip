@@ -1567,7 +1567,8 @@ hipError_t moments_upward(const TreeView& tv, int nterms, int nnodes, const int*
 // ---------------------------------------------------------------------------
 // Batch interaction lists.  One wavefront walks the tree for its 64 targets
 // with a shared depth-first stack of (node, mask) entries, children pushed in
-// reverse so they are visited in children[0..7] order.  At each node every
+// reverse so they are visited in children[0..7] order, from the node mp.root
+// (0, or a subtree as the reference's `ind`).  At each node every
 // lane evaluates the reference MAC (tree3d.cpp:3469-3473 / 4687-4691) in FP64
 // without contraction, so the decisions are bit-identical to the reference:
 //   rs = dx*dx + dy*dy + dz*dz + delta2;  accept = sqradius < acc*(rs - mac_sub)
@@ -1629,7 +1630,7 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   const cptr<int> gchild = (cptr<int>)(tv.child);
   const cptr<int> grange = (cptr<int>)(tv.range);
   if (lane == 0) {
-    st_node[0] = 0;
+    st_node[0] = mp.root;
     st_mask[0] = active;
   }
   int sp = 1;

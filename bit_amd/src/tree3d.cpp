@@ -4,7 +4,8 @@
 // A tree made by treemake is bound to its PARTICLE array; the first evaluation
 // that needs the device creates a bit_system for it (same tree: both are built
 // by the same bit-exact host code) and every batched result is cached per
-// (tree, quantity, nterms, acc, alpha) until treeempty releases the tree.
+// (tree, start node, quantity, nterms, acc, alpha) until treeempty releases the
+// tree.
 #include "tree3d.h"
 
 #include <algorithm>
@@ -36,7 +37,7 @@ struct Bound {
   int numtree = 0;
   int maxmemb = 0;
   bit_system* sys = nullptr;
-  std::map<std::tuple<int, int, double, double>, std::vector<double>> cache;
+  std::map<std::tuple<int, int, int, double, double>, std::vector<double>> cache;
 };
 std::map<TREE*, Bound> g_bound;
 std::set<double***> g_own_moments;  // moment blocks allocated by treemoments
@@ -82,9 +83,11 @@ void release(Bound& b) {
   b.cache.clear();
 }
 
-bit_params params(int nterms, double acc, double kappa, double delta2, double mac_sub) {
+bit_params params(int nterms, double acc, double kappa, double delta2, double mac_sub,
+                  int root) {
   bit_params p;
   bit_params_default(&p);
+  p.root = root;
   p.nterms = nterms;
   p.acc = acc;
   p.kappa = kappa;
@@ -95,27 +98,34 @@ bit_params params(int nterms, double acc, double kappa, double delta2, double ma
 }
 
 // Field (3 per particle) or potential of every particle, in the reference
-// semantics of the per-target function the quantity stands for.
+// semantics of the per-target function the quantity stands for, from the
+// traversal that starts at node `root` (the reference's `ind`).  `node` is that
+// node (nullptr: the whole particle set): shielded_treepot adds its self term
+// when the target's own leaf is summed directly, that is for the node's members.
 bool evaluate(bit_system* s, int n, const PARTICLE* part, int what, int nterms, double acc,
-              double alpha, std::vector<double>& out) {
+              double alpha, int root, const TREE* node, std::vector<double>& out) {
   if (what == kField) {
     out.resize(3 * (size_t)n);
-    bit_params p = params(nterms, acc, 0.0, DEL, DEL);  // treeforce: rs = r^2 + DEL
-    p.kernel_order = KERNEL_ORDER;                      // regularized family (problem.h)
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL, root);  // treeforce: rs = r^2 + DEL
+    p.kernel_order = KERNEL_ORDER;  // regularized family (problem.h)
     if (bit_eval_field(s, &p, out.data(), 0) != BIT_OK) return false;
     return true;
   }
   out.resize(n);
   if (what == kPot) {
-    bit_params p = params(nterms, acc, 0.0, DEL, DEL);  // treepotential: rs = r^2 + DEL
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL, root);  // treepotential: rs = r^2 + DEL
     return bit_eval_potential(s, &p, out.data(), 0) == BIT_OK;
   }
   // shielded_treepot: unregularized kernel, MAC shifted by DEL, result -phi plus
   // the target's self term.
-  bit_params p = params(nterms, acc, alpha, 0.0, DEL);
+  bit_params p = params(nterms, acc, alpha, 0.0, DEL, root);
   if (bit_eval_potential(s, &p, out.data(), 0) != BIT_OK) return false;
-  for (int i = 0; i < n; i++)
-    out[i] = -out[i] + part[i].tot_charge * kSelfIntG / part[i].weight;
+  for (int i = 0; i < n; i++) out[i] = -out[i];
+  auto self = [&](int i) { out[i] += part[i].tot_charge * kSelfIntG / part[i].weight; };
+  if (!node)
+    for (int i = 0; i < n; i++) self(i);
+  else
+    for (int m = 0; m < node->nummemb; m++) self(node->members[m]);
   return true;
 }
 
@@ -129,18 +139,23 @@ const double* lookup(const char* fn, TREE* tree, PARTICLE* part, int ind, int p,
     return nullptr;
   }
   Bound& b = it->second;
-  if (ind != 0 || p < 0 || p >= b.npart || part[p].x != x || part[p].y != y || part[p].z != z) {
-    cout << "[" << fn << "] served for whole-tree calls (ind == 0) at particle targets "
-         << "(0 <= p < npart, x,y,z = particle p) or at points (p == -1) only" << endl;
+  if (ind < 0 || ind >= b.numtree) {
+    cout << "[" << fn << "] node " << ind << " is not in the tree (numtree " << b.numtree
+         << ")" << endl;
     return nullptr;
   }
-  auto key = std::make_tuple(what, nterms, acc, alpha);
+  if (p < 0 || p >= b.npart || part[p].x != x || part[p].y != y || part[p].z != z) {
+    cout << "[" << fn << "] served at particle targets (0 <= p < npart, x,y,z = particle p) "
+         << "or at points (p == -1) only" << endl;
+    return nullptr;
+  }
+  auto key = std::make_tuple(ind, what, nterms, acc, alpha);
   auto c = b.cache.find(key);
   if (c == b.cache.end()) {
     if (!b.sys) b.sys = make_system(part, b.npart, b.maxmemb);
     if (!b.sys) return nullptr;
     std::vector<double> v;
-    if (!evaluate(b.sys, b.npart, part, what, nterms, acc, alpha, v)) {
+    if (!evaluate(b.sys, b.npart, part, what, nterms, acc, alpha, ind, &tree[ind], v)) {
       cout << "[" << fn << "] " << bit_last_error() << endl;
       return nullptr;
     }
@@ -149,11 +164,11 @@ const double* lookup(const char* fn, TREE* tree, PARTICLE* part, int ind, int p,
   return c->second.data();
 }
 
-// One arbitrary point (the reference's p = -1 calls): every source
-// contributes, pairs closer than eps skipped.  Returns false (message printed)
-// when the tree is not bound or the evaluation fails.
-bool point_eval(const char* fn, TREE* tree, PARTICLE* part, int what, int nterms, double acc,
-                double alpha, double x, double y, double z, double* out) {
+// One arbitrary point (the reference's p = -1 calls): every source of the
+// subtree at node ind contributes, pairs closer than eps skipped.  Returns
+// false (message printed) when the tree is not bound or the evaluation fails.
+bool point_eval(const char* fn, TREE* tree, PARTICLE* part, int ind, int what, int nterms,
+                double acc, double alpha, double x, double y, double z, double* out) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_bound.find(tree);
   if (it == g_bound.end() || it->second.part != part) {
@@ -161,18 +176,23 @@ bool point_eval(const char* fn, TREE* tree, PARTICLE* part, int what, int nterms
     return false;
   }
   Bound& b = it->second;
+  if (ind < 0 || ind >= b.numtree) {
+    cout << "[" << fn << "] node " << ind << " is not in the tree (numtree " << b.numtree
+         << ")" << endl;
+    return false;
+  }
   if (!b.sys) b.sys = make_system(part, b.npart, b.maxmemb);
   if (!b.sys) return false;
   int rc;
   if (what == kField) {
-    bit_params p = params(nterms, acc, 0.0, DEL, DEL);
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL, ind);
     p.kernel_order = KERNEL_ORDER;
     rc = bit_eval_field_points(b.sys, &p, 1, &x, &y, &z, out);
   } else if (what == kPot) {
-    bit_params p = params(nterms, acc, 0.0, DEL, DEL);
+    bit_params p = params(nterms, acc, 0.0, DEL, DEL, ind);
     rc = bit_eval_potential_points(b.sys, &p, 1, &x, &y, &z, out);
   } else {
-    bit_params p = params(nterms, acc, alpha, 0.0, DEL);
+    bit_params p = params(nterms, acc, alpha, 0.0, DEL, ind);
     rc = bit_eval_potential_points(b.sys, &p, 1, &x, &y, &z, out);
     out[0] = -out[0];  // shielded_treepot's sign, near and far alike
   }
@@ -198,7 +218,7 @@ void drive(const char* fn, int npart, PARTICLE* part, int what, int nterms, doub
   if (!s) return;
   if (print_numtree) cout << "numtree = " << bit_system_numtree(s) << endl;
   std::vector<double> v;
-  if (!evaluate(s, npart, part, what, nterms, acc, alpha, v)) {
+  if (!evaluate(s, npart, part, what, nterms, acc, alpha, 0, nullptr, v)) {
     cout << "[" << fn << "] " << bit_last_error() << endl;
     bit_system_destroy(s);
     return;
@@ -382,9 +402,9 @@ void treemoments(int numtree, int nterms, TREE* tree, PARTICLE* part) {
 // ---------------------------------------------------------------------------
 double treeforce(int ind, int p, int nterms, double x, double y, double z, double acc,
                  double* xfor, double* yfor, double* zfor, TREE* tree, PARTICLE* part) {
-  if (ind == 0 && p == -1) {
+  if (p == -1) {
     double E[3];
-    if (point_eval("treeforce", tree, part, kField, nterms, acc, 0.0, x, y, z, E)) {
+    if (point_eval("treeforce", tree, part, ind, kField, nterms, acc, 0.0, x, y, z, E)) {
       *xfor += E[0];
       *yfor += E[1];
       *zfor += E[2];
@@ -402,9 +422,10 @@ double treeforce(int ind, int p, int nterms, double x, double y, double z, doubl
 
 double treepotential(int ind, int p, int nterms, double x, double y, double z, double acc,
                      double* pot, TREE* tree, PARTICLE* part) {
-  if (ind == 0 && p == -1) {
+  if (p == -1) {
     double v;
-    if (point_eval("treepotential", tree, part, kPot, nterms, acc, 0.0, x, y, z, &v)) *pot += v;
+    if (point_eval("treepotential", tree, part, ind, kPot, nterms, acc, 0.0, x, y, z, &v))
+      *pot += v;
     return 0;
   }
   const double* v = lookup("treepotential", tree, part, ind, p, x, y, z, kPot, nterms, acc, 0.0);
@@ -415,9 +436,10 @@ double treepotential(int ind, int p, int nterms, double x, double y, double z, d
 double shielded_treepot(int ind, int p, int nterms, double x, double y, double z, double alpha,
                         double acc, double* pot, TREE* tree, PARTICLE* part, int, int, int,
                         double, double, double) {
-  if (ind == 0 && p == -1) {
+  if (p == -1) {
     double v;
-    if (point_eval("shielded_treepot", tree, part, kShielded, nterms, acc, alpha, x, y, z, &v))
+    if (point_eval("shielded_treepot", tree, part, ind, kShielded, nterms, acc, alpha, x, y, z,
+                   &v))
       *pot += v;
     return 0;
   }

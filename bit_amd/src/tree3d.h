@@ -10,10 +10,13 @@
 // Where the work goes:
 //   treemake            host, bit-exact bisection tree (bit_tree_build)
 //   treemoments         GPU FP64 moments, copied into TREE::moments
-//   treeforce, treepotential, shielded_treepot  (ind == 0, particle targets p >= 0)
+//   treeforce, treepotential, shielded_treepot  (particle targets p >= 0)
 //                       one batched GPU evaluation of all particles the first
-//                       time a (tree, parameters) pair is asked for; later
-//                       calls, from any thread, read that result
+//                       time a (tree, start node ind, parameters) triple is
+//                       asked for; later calls, from any thread, read that
+//                       result.  ind != 0 starts the traversal at that node
+//                       (the subtree's contribution); p == -1 is one GPU
+//                       evaluation at the point.
 //   freespace_particles_tree, freespace_pot_tree, shielded_freespace_pot_tree
 //                       batched GPU evaluation
 //   direct sums (force_*_ds_*, freespace_particles_ds_*, shielded_pot_particle_ds,

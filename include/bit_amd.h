@@ -60,6 +60,11 @@ typedef struct bit_params {
                        R^-(2l+1), w_l = prod_{j<=l} (j-1/2)/j (reference KERNEL_ORDER,
                        problem.h:13; tree3d.cpp:88, 614-791); 0 = 1/R.  Potential
                        entry points require 0. */
+  int root;         /* node the traversal starts at (reference `ind` of treeforce,
+                       shielded_treepot and treepotential, tree3d.cpp:851, 3448,
+                       4679): the result is the contribution of that node's subtree,
+                       the node's own MAC tested first.  0 = the whole tree; must be
+                       in [0, numtree). */
 } bit_params;
 
 /* Interaction statistics of the last evaluation of a system (exact counts). */

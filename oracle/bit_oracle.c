@@ -404,7 +404,7 @@ void ora_potential(const ora_tree *T, const double *x, const double *y, const do
     for (int t = 0; t < nt; t++) {
       pot_acc acc = {0.0, 0, 0};
       int p = tidx[t];
-      pot_rec(T, 0, p, x[p], y[p], z[p], x, y, z, q, moments, prm, coef, &acc);
+      pot_rec(T, prm.root, p, x[p], y[p], z[p], x, y, z, q, moments, prm, coef, &acc);
       pot[t] = acc.pot;
       if (n_near) n_near[t] = acc.nnear;
       if (n_far) n_far[t] = acc.nfar;
@@ -424,7 +424,7 @@ void ora_potential_points(const ora_tree *T, const double *x, const double *y,
 #pragma omp for schedule(dynamic, 16)
     for (int t = 0; t < m; t++) {
       pot_acc acc = {0.0, 0, 0};
-      pot_rec(T, 0, -1, px[t], py[t], pz[t], x, y, z, q, moments, prm, coef, &acc);
+      pot_rec(T, prm.root, -1, px[t], py[t], pz[t], x, y, z, q, moments, prm, coef, &acc);
       pot[t] = acc.pot;
     }
     free(coef);
@@ -457,7 +457,7 @@ void ora_lists(const ora_tree *T, const double *x, const double *y, const double
                ora_params prm, int ti, int *far, int *nfar, int *near, int *nnear, int cap) {
   *nfar = 0;
   *nnear = 0;
-  lists_rec(T, 0, ti, x, y, z, prm, far, nfar, near, nnear, cap);
+  lists_rec(T, prm.root, ti, x, y, z, prm, far, nfar, near, nnear, cap);
 }
 
 /* shielded_pot_particle_ds, tree3d.cpp:1966-2016 (sign: +phi), generalised to
@@ -603,7 +603,7 @@ void ora_treeforce(const ora_tree *T, const double *x, const double *y, const do
     for (int t = 0; t < nt; t++) {
       double E[3] = {0, 0, 0};
       int p = tidx[t];
-      force_rec(T, 0, p, x[p], y[p], z[p], x, y, z, q, moments, prm, coef, E);
+      force_rec(T, prm.root, p, x[p], y[p], z[p], x, y, z, q, moments, prm, coef, E);
       ex[t] = E[0];
       ey[t] = E[1];
       ez[t] = E[2];
@@ -623,7 +623,7 @@ void ora_treeforce_points(const ora_tree *T, const double *x, const double *y,
 #pragma omp for schedule(dynamic, 16)
     for (int t = 0; t < m; t++) {
       double E[3] = {0, 0, 0};
-      force_rec(T, 0, -1, px[t], py[t], pz[t], x, y, z, q, moments, prm, coef, E);
+      force_rec(T, prm.root, -1, px[t], py[t], pz[t], x, y, z, q, moments, prm, coef, E);
       ex[t] = E[0];
       ey[t] = E[1];
       ez[t] = E[2];

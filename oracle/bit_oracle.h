@@ -58,6 +58,8 @@ typedef struct ora_params {
   double delta2;  /* regularization delta^2 */
   double mac_sub; /* value subtracted from (r^2+delta2) in the MAC */
   int kernel_order; /* field only: regularized family G_L (problem.h KERNEL_ORDER) */
+  int root;         /* node the traversals start at: the reference's `ind`
+                       (tree3d.cpp:851, 3448, 4679); 0 = the whole tree */
 } ora_params;
 
 /* ---- tree (tree3d.cpp:990-1551) ---- */

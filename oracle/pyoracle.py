@@ -26,7 +26,8 @@ _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 class OraParams(C.Structure):
     _fields_ = [("nterms", C.c_int), ("acc", C.c_double), ("kappa", C.c_double),
-                ("delta2", C.c_double), ("mac_sub", C.c_double), ("kernel_order", C.c_int)]
+                ("delta2", C.c_double), ("mac_sub", C.c_double), ("kernel_order", C.c_int),
+                ("root", C.c_int)]
 
 
 @dataclass
@@ -155,54 +156,60 @@ class OracleTree:
         self.ora.L.ora_moments(self.h, self.x, self.y, self.z, _f64(q), nterms, out)
         return out.reshape(self.numtree, nterms, nterms, nterms)
 
+    def _root(self, root):
+        if not 0 <= int(root) < self.numtree:
+            raise ValueError(f"root {root} is not a node (numtree {self.numtree})")
+        return int(root)
+
+    # root: the node the traversal starts at (the reference's `ind`), 0 = whole tree
     def potential(self, q, moments, nterms, acc, kappa, delta2, mac_sub, tidx=None,
-                  nthreads=0, counts=False):
+                  nthreads=0, counts=False, root=0):
         q = _f64(q)
         tidx = _i32(np.arange(len(q)) if tidx is None else tidx)
         nt = len(tidx)
         pot = np.zeros(nt)
         nn = np.zeros(nt, dtype=np.int64)
         nf = np.zeros(nt, dtype=np.int64)
-        prm = OraParams(nterms, acc, kappa, delta2, mac_sub)
+        prm = OraParams(nterms, acc, kappa, delta2, mac_sub, 0, self._root(root))
         self.ora.L.ora_potential(self.h, self.x, self.y, self.z, q, _f64(moments).ravel(),
                                  prm, nt, tidx, pot, nn, nf, nthreads)
         return (pot, nn, nf) if counts else pot
 
     def potential_points(self, q, moments, nterms, acc, kappa, delta2, mac_sub, px, py, pz,
-                         nthreads=0):
+                         nthreads=0, root=0):
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         out = np.zeros(len(px))
-        prm = OraParams(nterms, acc, kappa, delta2, mac_sub)
+        prm = OraParams(nterms, acc, kappa, delta2, mac_sub, 0, self._root(root))
         self.ora.L.ora_potential_points(self.h, self.x, self.y, self.z, _f64(q),
                                         _f64(moments).ravel(), prm, len(px), px, py, pz, out,
                                         nthreads)
         return out
 
     def treeforce_points(self, q, moments, nterms, acc, delta2, mac_sub, px, py, pz, nthreads=0,
-                         kernel_order=0):
+                         kernel_order=0, root=0):
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         e = [np.zeros(len(px)) for _ in range(3)]
-        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub, kernel_order)
+        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub, kernel_order, self._root(root))
         self.ora.L.ora_treeforce_points(self.h, self.x, self.y, self.z, _f64(q),
                                         _f64(moments).ravel(), prm, len(px), px, py, pz, *e,
                                         nthreads)
         return np.stack(e, axis=1)
 
-    def lists(self, ti, nterms, acc, delta2, mac_sub, cap=4096):
+    def lists(self, ti, nterms, acc, delta2, mac_sub, cap=4096, root=0):
         far = np.zeros(cap, dtype=np.int32)
         near = np.zeros(cap, dtype=np.int32)
         nf, nn = C.c_int(0), C.c_int(0)
-        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub)
+        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub, 0, self._root(root))
         self.ora.L.ora_lists(self.h, self.x, self.y, self.z, prm, int(ti), far, C.byref(nf),
                              near, C.byref(nn), cap)
         return far[:nf.value].copy(), near[:nn.value].copy()
 
     def treeforce(self, q, moments, nterms, acc, delta2, mac_sub, tidx=None, nthreads=0,
-                  kernel_order=0):
+                  kernel_order=0, root=0):
         q = _f64(q)
         tidx = _i32(np.arange(len(q)) if tidx is None else tidx)
         e = [np.zeros(len(tidx)) for _ in range(3)]
-        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub, kernel_order)
+        prm = OraParams(nterms, acc, 0.0, delta2, mac_sub, kernel_order, self._root(root))
         self.ora.L.ora_treeforce(self.h, self.x, self.y, self.z, q, _f64(moments).ravel(), prm,
                                  len(tidx), tidx, *e, nthreads)
         return np.stack(e, axis=1)
@@ -230,6 +237,9 @@ class Reference:
                                           _ip, _dp]
         L.ref_force_ds_reg.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp, _dp,
                                        _dp]
+        L.ref_subtree_calls.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_int, _ip, _ip, _dp, _dp,
+                                        _dp, _dp, _dp]
         L.ref_boxparticles.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.ref_self_intG.restype = C.c_double
         self.L = L
@@ -285,6 +295,18 @@ class Reference:
         self.L.ref_treeforce(len(x), _f64(x), _f64(y), _f64(z), _f64(q), maxmemb, nterms, acc,
                              len(tidx), tidx, *e)
         return np.stack(e, axis=1)
+
+    def subtree_calls(self, x, y, z, q, maxmemb, nterms, acc, kappa, inds, tidx):
+        """Raw reference values of the calls that start at node inds[t] for
+        particle tidx[t]: shielded_treepot(ind, i), treepotential(ind, -1, x_i)
+        and treeforce(ind, i) -> (shielded, pot_point, field [npairs, 3])."""
+        inds, tidx = _i32(inds), _i32(tidx)
+        m = len(tidx)
+        sh, pp = np.zeros(m), np.zeros(m)
+        e = [np.zeros(m) for _ in range(3)]
+        self.L.ref_subtree_calls(len(x), _f64(x), _f64(y), _f64(z), _f64(q), maxmemb, nterms,
+                                 acc, kappa, m, inds, tidx, sh, pp, *e)
+        return sh, pp, np.stack(e, axis=1)
 
     def shielded_phi_ds(self, x, y, z, q, kappa, tidx):
         tidx = _i32(tidx)

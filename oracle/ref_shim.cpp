@@ -184,6 +184,41 @@ void ref_treeforce(int n, const double *x, const double *y, const double *z, con
   destroy(s, nterms);
 }
 
+// Calls that start the traversal at node inds[t] (the reference's `ind`) for
+// particle tidx[t]: shielded_treepot(ind, i, ...) raw (tree3d.cpp:3448),
+// treepotential(ind, -1, x_i, ...) (tree3d.cpp:4679, moments padded as in
+// ref_treepotential_point) and treeforce(ind, i, ...) (tree3d.cpp:851).
+void ref_subtree_calls(int n, const double *x, const double *y, const double *z,
+                       const double *q, int maxmemb, int nterms, double acc, double kappa,
+                       int npairs, const int *inds, const int *tidx, double *shielded,
+                       double *pot_point, double *ex, double *ey, double *ez) {
+  RefSystem s, sp;
+  fill_particles(s, n, x, y, z, q);
+  build(s, maxmemb, nterms, false);
+  fill_particles(sp, n, x, y, z, q);
+  build(sp, maxmemb, nterms, true);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int t = 0; t < npairs; t++) {
+    const int ind = inds[t], i = tidx[t];
+    double pot = 0.0;
+    shielded_treepot(ind, i, nterms, s.part[i].x, s.part[i].y, s.part[i].z, kappa, acc, &pot,
+                     s.tree, s.part, 100, 100, 100, 1.0, 1.0, 1.0);
+    shielded[t] = pot;
+    pot = 0.0;
+    treepotential(ind, -1, nterms, sp.part[i].x, sp.part[i].y, sp.part[i].z, acc, &pot,
+                  sp.tree, sp.part);
+    pot_point[t] = pot;
+    double fx = 0, fy = 0, fz = 0;
+    treeforce(ind, i, nterms, s.part[i].x, s.part[i].y, s.part[i].z, acc, &fx, &fy, &fz,
+              s.tree, s.part);
+    ex[t] = fx;
+    ey[t] = fy;
+    ez[t] = fz;
+  }
+  destroy(s, nterms);
+  destroy(sp, nterms + 1);
+}
+
 // shielded_pot_particle_ds (tree3d.cpp:1966): -sum_{j != i} q_j exp(-k r)/r.
 void ref_shielded_pot_ds(int n, const double *x, const double *y, const double *z,
                          const double *q, double kappa, int nt, const int *tidx,

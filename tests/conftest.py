@@ -17,7 +17,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 FIXTURES = ["box16k", "uniform10k", "uniform12k_reg", "blobs12k_screened", "aniso6k", "tiny1",
-            "tiny5", "tiny300"]
+            "tiny5", "tiny300", "subtree_blobs3k"]
 
 
 def pytest_configure(config):
@@ -44,6 +44,26 @@ def fixture_members(fx):
             kids = [mem[c] for c in ch[i] if c >= 0]
             mem[i] = np.sort(np.concatenate(kids))
     return mem
+
+
+def subtree_expected(fx):
+    """The subtree fixture's reference values in this library's conventions,
+    per (inds[t], tidx[t]) pair:
+      inside   particle tidx[t] is a member of node inds[t]
+      phi      +phi of the subtree at the particle (delta 0, MAC shift 0.01):
+               shielded_treepot's raw value with its sign and self term undone
+               (the self term is added when the target's own leaf is summed
+               directly, i.e. when it is a member of the subtree)
+      pot_pt   treepotential(ind, -1, x_i): regularized (delta^2 = 0.01), the
+               target's own charge included as q_i/sqrt(0.01) when inside
+      field    treeforce(ind, i)"""
+    mem = fixture_members(fx)
+    inds, tidx = fx["inds"], fx["tidx"]
+    inside = np.array([np.isin(i, mem[nd]) for nd, i in zip(inds, tidx)])
+    q = fx["q"][tidx]
+    phi = -(fx["sub_shielded_raw"] - np.where(inside, q * float(fx["intG"]), 0.0))
+    return dict(inside=inside, phi=phi, pot_pt=fx["sub_treepotential_point"],
+                field=fx["sub_treeforce"])
 
 
 def rel_l2(a, b):

@@ -20,6 +20,9 @@
 //                                     freespace_particles_ds_noreg/_reg
 //   dropin_treecode primitives IN OUT one-cluster Taylor sums and one-pair gradients
 //                                     (host only)
+//   dropin_treecode subtree IN OUT    calls that start at node ind != 0:
+//                                     shielded_treepot(ind, i), treepotential(ind, -1,
+//                                     x_i), treeforce(ind, i) for the (ind, i) in IN
 // OUT receives raw doubles; "PASSED"/"FAILED" on stdout as the reference tests.
 #include <omp.h>
 
@@ -172,6 +175,55 @@ static int points_mode(const char* in, const char* out) {
   delete[] part;
   printf("Test PASSED\n");
   return 0;
+}
+
+// Subtree calls.  IN: n, maxmemb, nterms, acc, alpha, m, then x, y, z, q (n each)
+// and the pairs' nodes and particles (m each, as doubles).  OUT: shielded raw
+// [m], treepotential at the particle [m], field [m][3].  A call with a node
+// outside the tree adds nothing (it prints a message).
+static int subtree_mode(const char* in, const char* out) {
+  FILE* f = fopen(in, "rb");
+  double hdr[6];
+  if (fread(hdr, sizeof(double), 6, f) != 6) return 2;
+  int n = (int)hdr[0], maxmemb = (int)hdr[1], nterms = (int)hdr[2], m = (int)hdr[5];
+  double acc = hdr[3], alpha = hdr[4];
+  std::vector<double> buf(4 * (size_t)n + 2 * (size_t)m);
+  if (fread(buf.data(), sizeof(double), buf.size(), f) != buf.size()) return 2;
+  fclose(f);
+  PARTICLE* part = new PARTICLE[n];
+  for (int i = 0; i < n; i++) {
+    part[i] = PARTICLE();
+    part[i].x = buf[i];
+    part[i].y = buf[n + i];
+    part[i].z = buf[2 * n + i];
+    part[i].charge = buf[3 * n + i];
+    part[i].weight = 1.0;
+    part[i].tot_charge = buf[3 * n + i];
+  }
+  const double* ind = &buf[4 * (size_t)n];
+  const double* tid = ind + m;
+  TREE* tree = new TREE[4 * n + 64];
+  int numtree = treemake(n, nterms, maxmemb, part, tree);
+  treemoments(numtree, nterms, tree, part);
+  std::vector<double> res(5 * (size_t)m, 0.0);  // shielded, pot, Ex, Ey, Ez
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int t = 0; t < m; t++) {
+    const int nd = (int)ind[t], i = (int)tid[t];
+    shielded_treepot(nd, i, nterms, part[i].x, part[i].y, part[i].z, alpha, acc, &res[t], tree,
+                     part, 100, 100, 100, 1.0, 1.0, 1.0);
+    treepotential(nd, -1, nterms, part[i].x, part[i].y, part[i].z, acc, &res[m + t], tree,
+                  part);
+    treeforce(nd, i, nterms, part[i].x, part[i].y, part[i].z, acc, &res[2 * (size_t)m + 3 * t],
+              &res[2 * (size_t)m + 3 * t + 1], &res[2 * (size_t)m + 3 * t + 2], tree, part);
+  }
+  double bad = 0.0;
+  treepotential(numtree, 0, nterms, part[0].x, part[0].y, part[0].z, acc, &bad, tree, part);
+  treeempty(numtree, tree);
+  write_all(out, res);
+  delete[] tree;
+  delete[] part;
+  printf(bad == 0.0 ? "Test PASSED\n" : "Test FAILED\n");
+  return bad == 0.0 ? 0 : 1;
 }
 
 // tests/directsum_mp.cpp: the same direct sums serially and from OpenMP threads.
@@ -421,6 +473,7 @@ int main(int argc, char** argv) {
   if (argc >= 4 && std::string(argv[1]) == "recharge") return recharge_mode(argv[2], argv[3]);
   if (argc >= 4 && std::string(argv[1]) == "points_ds") return points_ds_mode(argv[2], argv[3]);
   if (argc >= 4 && std::string(argv[1]) == "primitives") return primitives_mode(argv[2], argv[3]);
+  if (argc >= 4 && std::string(argv[1]) == "subtree") return subtree_mode(argv[2], argv[3]);
   fprintf(stderr,
           "usage: dropin_treecode field|conv|directsum|treecode OUT | "
           "screened|points|recharge|points_ds IN OUT\n");

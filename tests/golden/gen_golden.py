@@ -19,6 +19,8 @@ Fixture contents (all arrays):
   moments                 treemoments cube of a few nodes (mom_nodes)
   tidx                    target particle indices
   phi_*                   +phi at tidx from the reference (see keys)
+  inds, sub_*             (subtree fixture) raw values of the calls that start
+                          the traversal at node inds[t] for particle tidx[t]
 """
 from __future__ import annotations
 
@@ -113,6 +115,34 @@ def main():
         phi = R.shielded_phi(P.x, P.y, P.z, P.q, mm, 4, 0.5, 0.3, tidx)
         save(name, x=P.x, y=P.y, z=P.z, q=P.q, maxmemb=mm, nterms=4, acc=0.5, kappa=0.3,
              **tree_arrays(t), tidx=tidx, phi_screened=phi)
+
+    # 7. Subtree calls (ind != 0): 3k blobs, maxmemb 40, nterms 6, acc 0.5,
+    #    kappa 0.7.  The root and up to five nodes per depth (leaves included)
+    #    against 40 sampled particles plus one member of each chosen node:
+    #    shielded_treepot(ind, i), treepotential(ind, -1, x_i), treeforce(ind, i).
+    P = workloads.gaussian_blobs(3000, seed=5)
+    mm, nt, acc, kap = 40, 6, 0.5, 0.7
+    t = R.tree(P.x, P.y, P.z, P.q, mm, nt, with_moments=False)
+    depth = np.zeros(t.numtree, dtype=np.int64)
+    for i in range(t.numtree):
+        depth[t.children[i][t.children[i] >= 0]] = depth[i] + 1
+    nodes = [0]
+    for d in range(1, int(depth.max()) + 1):
+        cand = np.flatnonzero(depth == d)
+        nodes += cand[np.linspace(0, len(cand) - 1, min(5, len(cand))).astype(int)].tolist()
+    nodes = np.array(sorted(set(nodes)), dtype=np.int32)
+    rng = np.random.default_rng(15)
+    targets = set(workloads.sample_targets(P.n, 40, seed=15).tolist())
+    for nd in nodes:
+        mem = t.node_members(int(nd))
+        targets.add(int(mem[rng.integers(len(mem))]))
+    targets = np.array(sorted(targets), dtype=np.int32)
+    inds = np.repeat(nodes, len(targets)).astype(np.int32)
+    tidx = np.tile(targets, len(nodes)).astype(np.int32)
+    sh, pp, E = R.subtree_calls(P.x, P.y, P.z, P.q, mm, nt, acc, kap, inds, tidx)
+    save("subtree_blobs3k", x=P.x, y=P.y, z=P.z, q=P.q, maxmemb=mm, nterms=nt, acc=acc,
+         kappa=kap, **tree_arrays(t), inds=inds, tidx=tidx, sub_shielded_raw=sh,
+         sub_treepotential_point=pp, sub_treeforce=E, intG=R.intG)
 
 
 if __name__ == "__main__":

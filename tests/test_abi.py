@@ -56,9 +56,9 @@ def test_struct_layouts_match_header(tmp_path):
     src.write_text(
         '#include <stddef.h>\n#include <stdio.h>\n#include "bit_amd.h"\n'
         "int main(void) {\n"
-        '  printf("%zu %zu %zu %zu %zu\\n", sizeof(bit_params), offsetof(bit_params, acc),\n'
+        '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(bit_params), offsetof(bit_params, acc),\n'
         "         offsetof(bit_params, precision), offsetof(bit_params, kernel_order),\n"
-        "         sizeof(bit_stats));\n"
+        "         offsetof(bit_params, root), sizeof(bit_stats));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
@@ -67,10 +67,11 @@ def test_struct_layouts_match_header(tmp_path):
                                           check=True).stdout.split()]
     P = bit_amd.Params
     assert got == [ctypes.sizeof(P), P.acc.offset, P.precision.offset, P.kernel_order.offset,
-                   ctypes.sizeof(bit_amd.Stats)]
+                   P.root.offset, ctypes.sizeof(bit_amd.Stats)]
     p = bit_amd.Params()
+    p.root = 7
     bit_amd.lib.bit_params_default(ctypes.byref(p))
-    assert p.kernel_order == 0 and p.mac_sub == 0.0
+    assert p.kernel_order == 0 and p.mac_sub == 0.0 and p.root == 0
 
 
 def test_oracle_params_layout_matches_header(tmp_path):
@@ -79,11 +80,13 @@ def test_oracle_params_layout_matches_header(tmp_path):
     src.write_text(
         '#include <stddef.h>\n#include <stdio.h>\n#include "bit_oracle.h"\n'
         "int main(void) {\n"
-        '  printf("%zu %zu\\n", sizeof(ora_params), offsetof(ora_params, kernel_order));\n'
+        '  printf("%zu %zu %zu\\n", sizeof(ora_params), offsetof(ora_params, kernel_order),\n'
+        "         offsetof(ora_params, root));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "ora"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "oracle"), str(src), "-o", str(exe)],
                    check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
                                           check=True).stdout.split()]
-    assert got == [ctypes.sizeof(OraParams), OraParams.kernel_order.offset]
+    assert got == [ctypes.sizeof(OraParams), OraParams.kernel_order.offset,
+                   OraParams.root.offset]

@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, load_fixture, rel_max
+from conftest import ROOT, load_fixture, rel_max, subtree_expected
 
 pytestmark = pytest.mark.gpu
 BIN = os.path.join(ROOT, "tests", "cpp", "bin", "dropin_treecode")
@@ -168,3 +168,28 @@ def test_direct_sum_functions(tmp_path, oracle):
     scale = CHARGE_CONSTANT * q[:, None]
     assert rel_max(f_noreg, oracle.direct_force(x, y, z, q, 0.0) * scale) < 1e-12
     assert rel_max(f_reg, oracle.direct_force(x, y, z, q, 0.01) * scale) < 1e-12
+
+
+def test_subtree_calls(tmp_path):
+    """Calls with ind != 0 (the traversal from a subtree, the reference's own
+    recursion step) from OpenMP threads: shielded_treepot(ind, i) with its self
+    term only for the subtree's members, treepotential(ind, -1, x_i) and
+    treeforce(ind, i) equal the reference's values; a node outside the tree
+    adds nothing."""
+    fx = load_fixture("subtree_blobs3k")
+    n, m = fx["x"].size, fx["inds"].size
+    inp = tmp_path / "in.bin"
+    hdr = np.array([n, fx["maxmemb"], fx["nterms"], fx["acc"], fx["kappa"], m], dtype=np.float64)
+    np.concatenate([hdr, fx["x"], fx["y"], fx["z"], fx["q"], fx["inds"].astype(np.float64),
+                    fx["tidx"].astype(np.float64)]).tofile(inp)
+    out = tmp_path / "s.bin"
+    _run("subtree", str(inp), str(out))
+    r = np.fromfile(out)
+    sh, pot, E = r[:m], r[m:2 * m], r[2 * m:].reshape(m, 3)
+    want = subtree_expected(fx)
+    assert want["inside"].any() and not want["inside"].all()
+    for nd in np.unique(fx["inds"]):
+        s = fx["inds"] == nd
+        assert rel_max(sh[s], fx["sub_shielded_raw"][s]) < 1e-10, nd
+        assert rel_max(pot[s], want["pot_pt"][s]) < 1e-10, nd
+        assert rel_max(E[s], want["field"][s]) < 1e-10, nd

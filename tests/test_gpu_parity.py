@@ -9,7 +9,7 @@ import pytest
 
 import bit_amd
 from bit_amd import workloads
-from conftest import FIXTURES, fixture_members, load_fixture, rel_l2, rel_max
+from conftest import FIXTURES, fixture_members, load_fixture, rel_l2, rel_max, subtree_expected
 
 pytestmark = pytest.mark.gpu
 
@@ -340,6 +340,52 @@ def test_oom_and_budget_errors_leave_context_usable(ctx):
     assert e.value.code == bit_amd.BIT_E_OOM
     ok = s.potential(6, 0.5, 0.0, 0.0, None, "fp64")
     assert np.isfinite(ok).all()
+
+
+def test_subtree_root_matches_reference(ctx, oracle):
+    """Evaluations that start at node `root` (the reference's ind != 0 calls),
+    for every node of the subtree fixture (each depth, leaves included): FP64
+    potential, point potential and field within 1e-10 of the reference's
+    values, FP32 within FP32_TOL of FP64, the lists of every third node equal
+    to the oracle's traversal from the same node, a mixed launch equal to the
+    single evaluation, and a root outside the tree rejected."""
+    fx = load_fixture("subtree_blobs3k")
+    nt, acc, kap = int(fx["nterms"]), float(fx["acc"]), float(fx["kappa"])
+    s = _system(ctx, fx)
+    want = subtree_expected(fx)
+    inds, tidx = fx["inds"], fx["tidx"]
+    perm = s.export_tree().perm
+    o = oracle.tree(fx["x"], fx["y"], fx["z"], int(fx["maxmemb"]))
+    nodes = np.unique(inds)
+    for k, nd in enumerate(nodes):
+        nd = int(nd)
+        sel = inds == nd
+        ti = tidx[sel]
+        p64 = s.potential(nt, acc, kap, 0.0, 0.01, "fp64", root=nd)
+        assert rel_max(p64[ti], want["phi"][sel]) < 1e-10, nd
+        p32 = s.potential(nt, acc, kap, 0.0, 0.01, "fp32", root=nd)
+        assert rel_l2(p32, p64) < FP32_TOL, nd
+        pp = s.potential_at(fx["x"][ti], fx["y"][ti], fx["z"][ti], nt, acc, 0.0, 0.01, 0.01,
+                            "fp64", root=nd)
+        assert rel_max(pp, want["pot_pt"][sel]) < 1e-10, nd
+        E = s.field(nt, acc, 0.01, 0.01, 0, "fp64", root=nd)
+        assert rel_max(E[ti], want["field"][sel]) < 1e-10, nd
+        E32 = s.field(nt, acc, 0.01, 0.01, 0, "fp32", root=nd)
+        assert rel_l2(E32, E) < FP32_TOL, nd
+        if k % 3 == 0:
+            far, near = _per_target_lists(s.lists(nt, acc, 0.0, 0.01, root=nd), s.n)
+            for slot in range(s.n):
+                f, nr = o.lists(int(perm[slot]), nt, acc, 0.0, 0.01, root=nd)
+                assert far[slot] == list(f) and near[slot] == list(nr), (nd, slot)
+        if k % 5 == 0:
+            prm = bit_amd.make_params(nt, acc, kap, 0.0, 0.01, "fp32", root=nd)
+            outs, _ = bit_amd.eval_multi(ctx, [s], [prm])
+            assert np.array_equal(outs[0], p32), nd
+    for bad in (-1, s.numtree):
+        with pytest.raises(bit_amd.BitError) as e:
+            s.potential(nt, acc, kap, 0.0, 0.01, "fp64", root=bad)
+        assert e.value.code == bit_amd.BIT_E_INVALID
+    s.close()
 
 
 def test_invalid_params_rejected(ctx):

@@ -3,7 +3,7 @@ tests/golden/gen_golden.py from the compiled reference)."""
 import numpy as np
 import pytest
 
-from conftest import FIXTURES, fixture_members, load_fixture, rel_max
+from conftest import FIXTURES, fixture_members, load_fixture, rel_max, subtree_expected
 
 INTG = -0.0000483811229725  # shielded_treepot self term (tree3d.cpp:3575-3596)
 
@@ -77,3 +77,28 @@ def test_field_matches_treeforce(oracle):
     assert rel_max(E, fx["field_tree"]) < 1e-12
     Ed = oracle.direct_force(fx["x"], fx["y"], fx["z"], fx["q"], 0.01, fx["tidx"])
     assert rel_max(Ed, fx["field_ds"]) < 1e-12
+
+
+def test_subtree_calls_match_reference(oracle):
+    """Traversals that start at a node other than the root (the reference's
+    ind != 0) for the potential, the point potential and the field: every
+    depth of the tree, leaves included, targets inside and outside."""
+    fx = load_fixture("subtree_blobs3k")
+    nt, acc, kap = int(fx["nterms"]), float(fx["acc"]), float(fx["kappa"])
+    t = oracle.tree(fx["x"], fx["y"], fx["z"], int(fx["maxmemb"]))
+    m = t.moments(fx["q"], nt)
+    want = subtree_expected(fx)
+    inds, tidx = fx["inds"], fx["tidx"]
+    assert 0 < want["inside"].sum() < len(inds) and len(np.unique(inds)) > 20
+    for nd in np.unique(inds):
+        s = inds == nd
+        ti = tidx[s]
+        phi = t.potential(fx["q"], m, nt, acc, kap, 0.0, 0.01, ti, root=int(nd))
+        assert rel_max(phi, want["phi"][s]) < 1e-12, nd
+        pp = t.potential_points(fx["q"], m, nt, acc, 0.0, 0.01, 0.01, fx["x"][ti], fx["y"][ti],
+                                fx["z"][ti], root=int(nd))
+        assert rel_max(pp, want["pot_pt"][s]) < 1e-12, nd
+        E = t.treeforce(fx["q"], m, nt, acc, 0.01, 0.01, ti, root=int(nd))
+        assert rel_max(E, want["field"][s]) < 1e-12, nd
+    with pytest.raises(ValueError):
+        t.potential(fx["q"], m, nt, acc, kap, 0.0, 0.01, tidx[:1], root=t.numtree)
