@@ -96,6 +96,7 @@ struct bit_ctx {
   // build (fork: the tree is done; join: the moments are)
   std::mutex fork_mu;
   hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t lists_done = nullptr;  // near lists and counts ready (far ordering may follow)
   // persistent launches of mixed evaluations: their work counters (two, used
   // in turn; serve_mu orders the launches on the context's stream)
   std::mutex serve_mu;
@@ -749,8 +750,13 @@ static int stage_cap(unsigned long long mx, int nb, long long total) {
 
 // One traversal when the capacities from the context's last build of this
 // kind hold every batch (count and stage, exclusive scan, compaction); two
-// otherwise (count, exclusive scan, fill).  Both give the same lists.
-int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const bit_params* p) {
+// otherwise (count, exclusive scan, fill).  Both give the same lists.  Each
+// batch's far items are then put in the far kernel's pairing order, with the
+// pair statistics: on far_stream when given (after an event that marks the
+// rest of the build; the caller joins far_stream before the far field), else
+// on the context's stream.
+int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const bit_params* p,
+                hipStream_t far_stream = nullptr) {
   L.valid = false;  // set again only when both passes were issued
   bit_ctx* c = s->ctx;
   const int nb = (tv.nt + bit::dev::kBatch - 1) / bit::dev::kBatch;
@@ -782,14 +788,13 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   s->t_lists.start(c->stream);
   if (staged)
     HIP_TRY(bit::dev::lists_stage(tv, mp, s->stack_cap, L.far_cnt.as<int>(),
-                                  L.near_cnt.as<int>(), s->pstat.as<unsigned long long>(),
-                                  cap_far, cap_near, s->stage_far.as<bit::dev::Item>(),
+                                  L.near_cnt.as<int>(), cap_far, cap_near,
+                                  s->stage_far.as<bit::dev::Item>(),
                                   s->stage_near.as<bit::dev::Item>(), s->err_flag.as<int>(),
                                   c->stream));
   else
     HIP_TRY(bit::dev::lists_count(tv, mp, s->stack_cap, L.far_cnt.as<int>(),
-                                  L.near_cnt.as<int>(), s->pstat.as<unsigned long long>(),
-                                  s->err_flag.as<int>(), c->stream));
+                                  L.near_cnt.as<int>(), s->err_flag.as<int>(), c->stream));
   HIP_TRY(bit::dev::list_stats(nb, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
                                s->lstats.as<bit::dev::ListStats>(), c->stream));
   size_t tmp = 0;
@@ -819,31 +824,35 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   BIT_TRY(L.near_items.ensure(c, (size_t)std::max(1, tot[1]) * sizeof(bit::dev::Item)));
   const bool fits = staged && s->hstats.far_max <= (unsigned long long)cap_far &&
                     s->hstats.near_max <= (unsigned long long)cap_near;
-  if (fits)
-    HIP_TRY(bit::dev::lists_compact(nb, L.far_cnt.as<int>(), L.near_cnt.as<int>(),
-                                    L.far_off.as<int>(), L.near_off.as<int>(), cap_far,
-                                    cap_near, s->stage_far.as<bit::dev::Item>(),
-                                    s->stage_near.as<bit::dev::Item>(),
-                                    L.far_items.as<bit::dev::Item>(),
-                                    L.near_items.as<bit::dev::Item>(), tv.range,
-                                    s->pstat.as<unsigned long long>(), c->stream));
-  else
+  // near items from the staging regions (or in place after the second
+  // traversal) with their pair statistics; far items into the pairing order,
+  // from the staging regions or in place
+  if (!fits)
     HIP_TRY(bit::dev::lists_fill(tv, mp, s->stack_cap, L.far_off.as<int>(),
                                  L.near_off.as<int>(), L.far_items.as<bit::dev::Item>(),
                                  L.near_items.as<bit::dev::Item>(), s->err_flag.as<int>(),
                                  c->stream));
-  s->t_lists.stop(c->stream);
-  // pair statistics: counted by the count traversal, else by the compaction;
-  // a staged build that fell back to the fill takes them from the final lists
-  if (staged && !fits)
-    HIP_TRY(bit::dev::lists_compact(nb, nullptr, nullptr, L.far_off.as<int>(),
-                                    L.near_off.as<int>(), 0, 0, nullptr, nullptr,
-                                    L.far_items.as<bit::dev::Item>(),
-                                    L.near_items.as<bit::dev::Item>(), tv.range,
-                                    s->pstat.as<unsigned long long>(), c->stream));
+  HIP_TRY(bit::dev::near_compact(nb, L.near_off.as<int>(), fits ? cap_near : 0,
+                                 fits ? s->stage_near.as<bit::dev::Item>() : nullptr,
+                                 L.near_items.as<bit::dev::Item>(), tv.range,
+                                 s->pstat.as<unsigned long long>(), c->stream));
   BIT_TRY(s->pair_dev.ensure(c, 4 * sizeof(unsigned long long)));
+  hipStream_t fs = c->stream;
+  if (far_stream) {
+    s->t_lists.stop(c->stream);
+    if (!c->lists_done) HIP_TRY(hipEventCreateWithFlags(&c->lists_done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(c->lists_done, c->stream));
+    HIP_TRY(hipStreamWaitEvent(far_stream, c->lists_done, 0));
+    fs = far_stream;
+  }
+  HIP_TRY(bit::dev::far_order(nb, L.far_off.as<int>(), fits ? cap_far : 0,
+                              fits ? s->stage_far.as<bit::dev::Item>()
+                                   : L.far_items.as<bit::dev::Item>(),
+                              L.far_items.as<bit::dev::Item>(),
+                              s->pstat.as<unsigned long long>(), fs));
+  if (!far_stream) s->t_lists.stop(c->stream);
   HIP_TRY(bit::dev::pair_stats(nb, s->pstat.as<unsigned long long>(),
-                               s->pair_dev.as<unsigned long long>(), c->stream));
+                               s->pair_dev.as<unsigned long long>(), fs));
   s->pair_pending = true;
   L.passes = fits ? 1 : 2;
   const long long total = (long long)tot[0] + tot[1];
@@ -861,11 +870,12 @@ int build_lists(bit_system* s, ListSet& L, const bit::dev::TreeView& tv, const b
   return BIT_OK;
 }
 
-// Lists of the particles as targets, cached per MAC parameters.
-int ensure_lists(bit_system* s, const bit_params* p) {
+// Lists of the particles as targets, cached per MAC parameters and start
+// node; far_stream as build_lists.
+int ensure_lists(bit_system* s, const bit_params* p, hipStream_t far_stream = nullptr) {
   ListSet& L = s->pl;
   if (lists_match(L, p)) return BIT_OK;
-  return build_lists(s, L, s->view(), p);
+  return build_lists(s, L, s->view(), p, far_stream);
 }
 
 }  // namespace
@@ -929,6 +939,7 @@ int bit_ctx_destroy(bit_ctx* c) {
   if (c->built) (void)hipEventDestroy(c->built);
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
+  if (c->lists_done) (void)hipEventDestroy(c->lists_done);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->serve_ctr) (void)hipFree(c->serve_ctr);
   delete c;
@@ -1315,9 +1326,10 @@ static void collect_times(bit_system* s) {
 // The moments for params (and, after them on the same stream, the cluster
 // coefficients `coeffs` launches) and the lists.  When both have to be made,
 // the moments run on the side stream while the lists (bound by scalar issue)
-// build on the context's stream; the context's stream then waits for them, or,
-// with defer_join, leaves that to join_moments (the caller runs work that needs
-// the lists only in between).
+// build on the context's stream, and the lists' far ordering follows the
+// moments on the side stream; the context's stream then waits for the side
+// stream, or, with defer_join, leaves that to join_moments (the caller runs
+// work that needs only the near lists in between).
 static int moments_and_lists(bit_system* s, const bit_params* p, bool defer_join,
                              const std::function<int(hipStream_t)>& coeffs) {
   bit_ctx* c = s->ctx;
@@ -1332,8 +1344,10 @@ static int moments_and_lists(bit_system* s, const bit_params* p, bool defer_join
     HIP_TRY(hipStreamWaitEvent(c->side, c->fork, 0));
     int rc = moments_on(s, p->nterms, c->side);
     if (rc == BIT_OK) rc = coeffs(c->side);
+    // the lists' far ordering follows on the side stream (it overlaps the near
+    // field, which needs only the near lists)
+    if (rc == BIT_OK) rc = ensure_lists(s, p, c->side);
     HIP_TRY(hipEventRecord(c->join, c->side));
-    if (rc == BIT_OK) rc = ensure_lists(s, p);
     if (defer_join && rc == BIT_OK) s->join_pending = true;
     else HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
     return rc;
@@ -1663,7 +1677,10 @@ int bit_eval_field_points(bit_system* s, const bit_params* p, int m, const doubl
 
 int bit_system_stats(const bit_system* s, bit_stats* out) {
   if (!s || !out) return fail(BIT_E_INVALID, "null system or output");
-  if (s->pair_pending) {  // the last list build's pair totals
+  // the last list build's pair totals (made on the side stream when its join
+  // is still pending)
+  BIT_TRY(join_moments(const_cast<bit_system*>(s)));
+  if (s->pair_pending) {
     unsigned long long v[4];
     HIP_TRY(hipSetDevice(s->ctx->device));
     HIP_TRY(hipMemcpyAsync(v, s->pair_dev.p, sizeof(v), hipMemcpyDeviceToHost, s->ctx->stream));

@@ -222,30 +222,37 @@ hipError_t far_coeffs64(int numtree, int nterms, const double* mom_hat64, double
                         hipStream_t s);
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
-                       int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s);
+                       int* near_cnt, int* err_flag, hipStream_t s);
 // Totals and maxima over nb batches of the counts.
 hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt, ListStats* out,
                       hipStream_t s);
 // Totals over nb batches of pstat (4 per batch: far pairs, near pairs,
-// k_near32's pair slots per lane, k_farq's sweeps), written by lists_count or
-// lists_compact, into out[4].
+// k_near32's pair slots per lane, k_farq's sweeps), written by far_order and
+// near_compact, into out[4].
 hipError_t pair_stats(int nb, const unsigned long long* pstat, unsigned long long* out,
                       hipStream_t s);
+// Writes the items at the scanned offsets (far_order then orders the far
+// items in place).
 hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int* far_off,
                       const int* near_off, Item* far_items, Item* near_items, int* err_flag,
                       hipStream_t s);
 // One traversal that counts like lists_count and writes batch b's first
 // cap_far / cap_near entries to far_stage[b*cap_far ..] / near_stage[b*cap_near ..].
 hipError_t lists_stage(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
-                       int* near_cnt, unsigned long long* pstat, int cap_far, int cap_near,
-                       Item* far_stage, Item* near_stage, int* err_flag, hipStream_t s);
-// Staged entries to the scanned offsets (valid when no count exceeded its cap),
-// and each batch's pair statistics into pstat (range: the nodes' slot ranges);
-// with null stages the statistics of the final lists only.
-hipError_t lists_compact(int nb, const int* far_cnt, const int* near_cnt, const int* far_off,
-                         const int* near_off, int cap_far, int cap_near, const Item* far_stage,
-                         const Item* near_stage, Item* far_items, Item* near_items,
-                         const int2* range, unsigned long long* pstat, hipStream_t s);
+                       int* near_cnt, int cap_far, int cap_near, Item* far_stage,
+                       Item* near_stage, int* err_flag, hipStream_t s);
+// Far items of batch b from src + b * cap (cap > 0: staging regions) or
+// src + far_off[b] (cap == 0; src may be far_items itself) to far_items at
+// far_off[b], each batch's in the far kernel's pairing order (k_far_order in
+// kernels.hip), with the batch's far pair statistics (pstat[4b], [4b + 3]).
+hipError_t far_order(int nb, const int* far_off, int cap, const Item* src, Item* far_items,
+                     unsigned long long* pstat, hipStream_t s);
+// Near items copied from stage + b * cap when cap > 0 (else read in place), with
+// the batch's near pair statistics (pstat[4b + 1], [4b + 2]; range: the nodes'
+// slot ranges).
+hipError_t near_compact(int nb, const int* near_off, int cap, const Item* stage,
+                        Item* near_items, const int2* range, unsigned long long* pstat,
+                        hipStream_t s);
 hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t& tmp_bytes,
                           hipStream_t s);
 

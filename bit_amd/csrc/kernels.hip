@@ -1608,7 +1608,6 @@ template <int MODE>
 __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int stack_cap,
                                               int* __restrict__ far_cnt,
                                               int* __restrict__ near_cnt,
-                                              unsigned long long* __restrict__ pstat,
                                               const int* __restrict__ far_off,
                                               const int* __restrict__ near_off, int cap_far,
                                               int cap_near, Item* __restrict__ far_items,
@@ -1628,21 +1627,13 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   // once per visit
   const cptr<double> gcenter = (cptr<double>)(tv.center);
   const cptr<int> gchild = (cptr<int>)(tv.child);
-  const cptr<int> grange = (cptr<int>)(tv.range);
   if (lane == 0) {
     st_node[0] = mp.root;
     st_mask[0] = active;
   }
   int sp = 1;
   int nf = 0, nn = 0;
-  // per-batch counts: far pairs <= 64 numtree, slots <= 4 n, sweeps <= numtree
-  unsigned far_pairs = 0, far_sweeps = 0;
-  unsigned long long near_pairs = 0, near_slots = 0;
-  int fpend = -1;  // targets of an item k_farq may still pair with the next one
   constexpr bool kWrite = MODE != kListCount;
-  // the per-batch pair statistics: counted here in the two-traversal path,
-  // from the items by the compaction after a staged traversal
-  constexpr bool kStats = MODE == kListCount;
   const size_t fbase = MODE == kListFill    ? (size_t)far_off[b]
                        : MODE == kListStage ? (size_t)b * (size_t)cap_far
                                             : 0;
@@ -1680,18 +1671,6 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
           far_items[fbase + (nf - 63 + lane)] = Item{fbm, fbn, 0};
       }
       nf++;
-      if constexpr (kStats) {
-        const int pc = __popcll(am);
-        far_pairs += pc;
-        // k_farq's units: an item joins the previous unpaired one when the
-        // two have at most 64 targets together, else it starts a sweep of its own
-        if (fpend >= 0 && fpend + pc <= kBatch) {
-          fpend = -1;
-        } else {
-          far_sweeps++;
-          fpend = pc;
-        }
-      }
     }
     const unsigned long long rm = m & ~am;
     if (rm) {
@@ -1705,13 +1684,6 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
             near_items[nbase + (nn - 63 + lane)] = Item{nbm, nbn, 0};
         }
         nn++;
-        if constexpr (kStats) {
-          const int2 r = make_int2(grange[2 * node], grange[2 * node + 1]);
-          const int pc = __popcll(rm);
-          const unsigned long long len = (unsigned long long)(r.y - r.x);
-          near_pairs += (unsigned long long)pc * len;
-          near_slots += near_item_slots(pc, len);
-        }
       } else {
         int kid = kids[0];  // child `lane` (lanes 0..7)
 #pragma unroll
@@ -1740,12 +1712,6 @@ __global__ __launch_bounds__(64) void k_lists(TreeView tv, MacParams mp, int sta
   if (MODE != kListFill && lane == 0) {
     far_cnt[b] = nf;
     near_cnt[b] = nn;
-  }
-  if (kStats && lane == 0) {
-    pstat[4 * b] = far_pairs;
-    pstat[4 * b + 1] = near_pairs;
-    pstat[4 * b + 2] = near_slots;
-    pstat[4 * b + 3] = far_sweeps;
   }
 }
 
@@ -1785,23 +1751,23 @@ hipError_t list_stats(int nb, const int* far_cnt, const int* near_cnt, ListStats
 }
 
 hipError_t lists_count(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
-                       int* near_cnt, unsigned long long* pstat, int* err_flag, hipStream_t s) {
+                       int* near_cnt, int* err_flag, hipStream_t s) {
   int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
   hipLaunchKernelGGL(k_lists<kListCount>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
-                     far_cnt, near_cnt, pstat, (const int*)nullptr, (const int*)nullptr, 0, 0,
+                     far_cnt, near_cnt, (const int*)nullptr, (const int*)nullptr, 0, 0,
                      (Item*)nullptr, (Item*)nullptr, err_flag);
   return hipGetLastError();
 }
 
 hipError_t lists_stage(const TreeView& tv, MacParams mp, int stack_cap, int* far_cnt,
-                       int* near_cnt, unsigned long long* pstat, int cap_far, int cap_near,
-                       Item* far_stage, Item* near_stage, int* err_flag, hipStream_t s) {
+                       int* near_cnt, int cap_far, int cap_near, Item* far_stage,
+                       Item* near_stage, int* err_flag, hipStream_t s) {
   if (cap_far <= 0 || cap_near <= 0) return hipErrorInvalidValue;
   int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
   hipLaunchKernelGGL(k_lists<kListStage>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
-                     far_cnt, near_cnt, pstat, (const int*)nullptr, (const int*)nullptr, cap_far,
+                     far_cnt, near_cnt, (const int*)nullptr, (const int*)nullptr, cap_far,
                      cap_near, far_stage, near_stage, err_flag);
   return hipGetLastError();
 }
@@ -1812,85 +1778,222 @@ hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int
   int nb = (tv.nt + kBatch - 1) / kBatch;
   size_t sh = (size_t)stack_cap * (sizeof(unsigned long long) + sizeof(int));
   hipLaunchKernelGGL(k_lists<kListFill>, dim3(nb), dim3(kBatch), sh, s, tv, mp, stack_cap,
-                     (int*)nullptr, (int*)nullptr, (unsigned long long*)nullptr, far_off,
-                     near_off, 0, 0, far_items, near_items, err_flag);
+                     (int*)nullptr, (int*)nullptr, far_off, near_off, 0, 0, far_items,
+                     near_items, err_flag);
   return hipGetLastError();
 }
 
-// Staged items to their scanned offsets: one wavefront per batch and list,
-// 16-byte items copied as uint4 (far lists in blocks [0, nb), near after).  The
-// same pass writes the batch's pair statistics (pstat, 4 per batch): far pairs
-// and k_farq's sweeps, near pairs and k_near32's pair slots.  With no staging
-// (far_stage null) it reads the final lists and writes the statistics only.
-__global__ __launch_bounds__(256) void k_list_compact(int nb, const int* __restrict__ far_cnt,
-                                                      const int* __restrict__ near_cnt,
-                                                      const int* __restrict__ far_off,
-                                                      const int* __restrict__ near_off,
-                                                      int cap_far, int cap_near,
-                                                      const uint4* __restrict__ far_stage,
-                                                      const uint4* __restrict__ near_stage,
-                                                      uint4* __restrict__ far_items,
-                                                      uint4* __restrict__ near_items,
-                                                      const int2* __restrict__ range,
-                                                      unsigned long long* __restrict__ pstat) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (w >= 2 * nb) return;
-  const bool isfar = w < nb;
-  const int b = isfar ? w : w - nb;
-  const bool copy = far_stage != nullptr;
-  const int n = copy ? (isfar ? far_cnt[b] : near_cnt[b])
-                     : (isfar ? far_off[b + 1] - far_off[b] : near_off[b + 1] - near_off[b]);
-  uint4* dst = isfar ? far_items + far_off[b] : near_items + near_off[b];
-  const uint4* src = !copy ? dst
-                     : isfar ? far_stage + (size_t)b * (size_t)cap_far
-                             : near_stage + (size_t)b * (size_t)cap_near;
-  unsigned long long s0 = 0, s1 = 0;  // far: pairs, sweeps; near: pairs, slots
-  // far sweeps: item i starts one unless it is the second of a pair; pairs are
-  // taken greedily from the start of each run of consecutive items whose
-  // neighbours fit together (target counts summing to at most 64), so an item
-  // starts a sweep iff its distance from its run's start is even
-  int prev_pc = 0, run0 = 0;  // carried across 64-item passes
+// Far items a batch may hold for the pairing order below (kept in registers,
+// kFarOrderCap / 64 per lane); a batch with more keeps its traversal order.
+constexpr int kFarOrderCap = 1024;
+constexpr int kFarOrderPer = kFarOrderCap / kBatch;
+
+__device__ __forceinline__ int item_pc(uint4 v) {
+  return __popcll(((unsigned long long)v.y << 32) | v.x);
+}
+__device__ __forceinline__ int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Far items to their scanned offsets, in k_farq's pairing order, and each
+// batch's far pair statistics (pstat[4b]: pairs, pstat[4b + 3]: k_farq's
+// sweeps); one wavefront per batch.  Batch b's items are read from
+// src + b * cap (cap > 0: staging regions) or src + far_off[b] (cap == 0; src
+// may be the output itself: the batch is read whole before it is written).
+//
+// k_farq sweeps two consecutive items together when their target counts sum to
+// at most 64, else one.  The order below makes that rule form as many pairs as
+// any pairing of the batch's items can (the two-pointer matching of items
+// sorted by target count):
+//   bucket c = the items of c targets (1..64), each in traversal order;
+//   repeat, with j the largest non-empty bucket and i the smallest:
+//     i == j: emit bucket j (pairs of two when 2j <= 64) and stop;
+//     i + j <= 64: emit k = min(|i|, |j|) pairs, each the next item of j then
+//                  the next of i;
+//     else: emit bucket j, each item alone (no remaining item fits with it).
+// An item emitted alone is never followed by one it fits with (j + any
+// remaining count > 64), and a pair's second item starts no unit, so the
+// kernel's rule forms exactly these units.  The order depends only on the
+// batch's items in traversal order.  A batch of more than kFarOrderCap items
+// keeps its traversal order.
+//
+// Work: each lane holds up to kFarOrderPer items in registers; an item's rank
+// in its bucket comes from a match of equal counts (seven ballots per 64
+// items); the bucket walk above runs on bucket counts (one lane per bucket)
+// and records, per bucket, segments of consecutive ranks with their output
+// position and stride (1, or 2 for one side of a run of pairs); each item then
+// finds its segment and is stored once.
+__global__ __launch_bounds__(64) void k_far_order(int nb, const int* __restrict__ far_off,
+                                                  int cap, const uint4* src_all,
+                                                  uint4* far_items,
+                                                  unsigned long long* __restrict__ pstat) {
+  // segments: at most two per walk step, and every step but the last empties
+  // a bucket, so at most 2 * 64 + 1
+  __shared__ int fill[kBatch];
+  __shared__ int seg_first[kBatch];
+  __shared__ int4 seg[4 * kBatch];  // (first rank, count, output position, stride)
+  __shared__ int seg_next[4 * kBatch];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (b >= nb) return;
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    const int i = i0 + lane;
-    const bool in = i < n;
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (in) {
-      v = src[i];
-      if (copy) dst[i] = v;
+  const int n = far_off[b + 1] - far_off[b];
+  uint4* dst = far_items + far_off[b];
+  const uint4* src = cap > 0 ? src_all + (size_t)b * (size_t)cap : src_all + far_off[b];
+  unsigned long long pairs = 0, sweeps = 0;  // per lane, summed below
+  if (n <= kFarOrderCap) {
+    fill[lane] = 0;
+    seg_first[lane] = -1;
+    uint4 v[kFarOrderPer];
+    int rk[kFarOrderPer];
+#pragma unroll
+    for (int u = 0; u < kFarOrderPer; u++) {
+      const int i = u * kBatch + lane;
+      v[u] = i < n ? src[i] : make_uint4(0, 0, 0, 0);
     }
-    const int pc = __popcll(((unsigned long long)v.y << 32) | v.x);
-    if (isfar) {
+    __syncthreads();
+    // rank of each item among the earlier items of its count
+#pragma unroll
+    for (int u = 0; u < kFarOrderPer; u++) {
+      if (u * kBatch >= n) break;
+      const bool in = u * kBatch + lane < n;
+      const int pc = in ? item_pc(v[u]) : 0;
+      unsigned long long same = ballot(in);
+#pragma unroll
+      for (int bit = 0; bit < 7; bit++) {
+        const bool on = (pc >> bit) & 1;
+        const unsigned long long m = ballot(on);
+        same &= on ? m : ~m;
+      }
+      const int f = in ? fill[pc - 1] : 0;
+      rk[u] = f + __popcll(same & below);
+      if (in && (same & below) == 0ull) fill[pc - 1] = f + __popcll(same);
+      __syncthreads();
+    }
+    // the bucket walk (lane l: bucket l + 1)
+    int cnt = fill[lane];
+    pairs = (unsigned long long)cnt * (unsigned long long)(lane + 1);
+    int take = 0, last = -1, p = 0, ns = 0;
+    unsigned long long units = 0;
+    auto add_seg = [&](int bl, int r0, int k, int p0, int stride) {
+      const int lb = lane_of(last, bl);
+      if (lane == 0) {
+        seg[ns] = make_int4(r0, k, p0, stride);
+        seg_next[ns] = -1;
+        if (lb < 0) seg_first[bl] = ns;
+        else seg_next[lb] = ns;
+      }
+      last = lane == bl ? ns : last;
+      ns++;
+    };
+    for (;;) {
+      const unsigned long long nz = ballot(cnt > 0);
+      if (!nz) break;
+      const int jl = 63 - __builtin_clzll(nz), il = __builtin_ctzll(nz);  // bucket - 1
+      const int cj = lane_of(cnt, jl), tj = lane_of(take, jl);
+      if (il == jl) {
+        add_seg(jl, tj, cj, p, 1);
+        units += 2 * (jl + 1) <= kBatch ? (unsigned long long)(cj + 1) / 2 : cj;
+        break;
+      }
+      if ((il + 1) + (jl + 1) <= kBatch) {
+        const int k = min(cj, lane_of(cnt, il));
+        add_seg(jl, tj, k, p, 2);
+        add_seg(il, lane_of(take, il), k, p + 1, 2);
+        p += 2 * k;
+        units += k;
+        cnt -= (lane == jl || lane == il) ? k : 0;
+        take += (lane == jl || lane == il) ? k : 0;
+      } else {
+        add_seg(jl, tj, cj, p, 1);
+        p += cj;
+        units += cj;
+        cnt -= lane == jl ? cj : 0;
+        take += lane == jl ? cj : 0;
+      }
+    }
+    sweeps = lane == 0 ? units : 0;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kFarOrderPer; u++) {
+      if (u * kBatch >= n) break;
+      if (u * kBatch + lane < n) {
+        int s = seg_first[item_pc(v[u]) - 1];
+        int4 g = seg[s];
+        while (rk[u] >= g.x + g.y) {
+          s = seg_next[s];
+          g = seg[s];
+        }
+        dst[g.z + (rk[u] - g.x) * g.w] = v[u];
+      }
+    }
+  } else {
+    // traversal order; sweeps by the kernel's rule: pairs are taken greedily
+    // from the start of each run of consecutive items whose neighbours fit
+    // together, so an item starts a sweep iff its distance from its run's
+    // start is even
+    int prev_pc = 0, run0 = 0;  // carried across 64-item passes
+    for (int i0 = 0; i0 < n; i0 += kBatch) {
+      const int i = i0 + lane;
+      const bool in = i < n;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (in) {
+        v = src[i];
+        if (src != dst) dst[i] = v;
+      }
+      const int pc = in ? item_pc(v) : 0;
       const int left = __shfl_up(pc, 1);
       const int lpc = lane == 0 ? prev_pc : left;
       const bool start = in && (i == 0 || lpc + pc > kBatch);  // a run starts here
       const unsigned long long st = ballot(start);
       const unsigned long long sb = st & (below | (1ull << lane));
       const int rs = sb ? i0 + 63 - __builtin_clzll(sb) : run0;  // this item's run start
-      s0 += in ? pc : 0;
-      s1 += (in && ((i - rs) & 1) == 0) ? 1 : 0;
+      pairs += in ? pc : 0;
+      sweeps += (in && ((i - rs) & 1) == 0) ? 1 : 0;
       prev_pc = __shfl(pc, 63);
       run0 = st ? i0 + 63 - __builtin_clzll(st) : run0;
-    } else if (in) {
-      const int2 r = range[(int)v.z];
-      const unsigned long long len = (unsigned long long)(r.y - r.x);
-      s0 += (unsigned long long)pc * len;
-      s1 += near_item_slots(pc, len);
     }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    pairs += __shfl_xor(pairs, o);
+    sweeps += __shfl_xor(sweeps, o);
+  }
+  if (lane == 0) {
+    pstat[4 * b] = pairs;
+    pstat[4 * b + 3] = sweeps;
+  }
+}
+
+// Near items to their scanned offsets (from staging regions of cap entries when
+// stage is non-null, else read in place) and each batch's near pair statistics
+// (pstat[4b + 1]: pairs, pstat[4b + 2]: k_near32's pair slots); one wavefront
+// per batch.
+__global__ __launch_bounds__(256) void k_near_compact(int nb, const int* __restrict__ near_off,
+                                                      int cap, const uint4* __restrict__ stage,
+                                                      uint4* __restrict__ near_items,
+                                                      const int2* __restrict__ range,
+                                                      unsigned long long* __restrict__ pstat) {
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= nb) return;
+  const int n = near_off[b + 1] - near_off[b];
+  uint4* dst = near_items + near_off[b];
+  const uint4* src = stage ? stage + (size_t)b * (size_t)cap : dst;
+  unsigned long long s0 = 0, s1 = 0;
+  for (int i = lane; i < n; i += kBatch) {
+    const uint4 v = src[i];
+    if (src != dst) dst[i] = v;
+    const int2 r = range[(int)v.z];
+    const unsigned long long len = (unsigned long long)(r.y - r.x);
+    const int pc = item_pc(v);
+    s0 += (unsigned long long)pc * len;
+    s1 += near_item_slots(pc, len);
   }
   for (int o = 32; o > 0; o >>= 1) {
     s0 += __shfl_xor(s0, o);
     s1 += __shfl_xor(s1, o);
   }
   if (lane == 0) {
-    if (isfar) {
-      pstat[4 * b] = s0;
-      pstat[4 * b + 3] = s1;
-    } else {
-      pstat[4 * b + 1] = s0;
-      pstat[4 * b + 2] = s1;
-    }
+    pstat[4 * b + 1] = s0;
+    pstat[4 * b + 2] = s1;
   }
 }
 
@@ -1918,18 +2021,24 @@ hipError_t pair_stats(int nb, const unsigned long long* pstat, unsigned long lon
   return hipGetLastError();
 }
 
-hipError_t lists_compact(int nb, const int* far_cnt, const int* near_cnt, const int* far_off,
-                         const int* near_off, int cap_far, int cap_near, const Item* far_stage,
-                         const Item* near_stage, Item* far_items, Item* near_items,
-                         const int2* range, unsigned long long* pstat, hipStream_t s) {
+hipError_t far_order(int nb, const int* far_off, int cap, const Item* src, Item* far_items,
+                     unsigned long long* pstat, hipStream_t s) {
   static_assert(sizeof(Item) == sizeof(uint4), "items are copied as 16-byte words");
   if (nb == 0) return hipSuccess;
-  if ((far_stage == nullptr) != (near_stage == nullptr)) return hipErrorInvalidValue;
-  const int nblk = (2 * nb + 3) / 4;
-  hipLaunchKernelGGL(k_list_compact, dim3(nblk), dim3(256), 0, s, nb, far_cnt, near_cnt, far_off,
-                     near_off, cap_far, cap_near, reinterpret_cast<const uint4*>(far_stage),
-                     reinterpret_cast<const uint4*>(near_stage),
-                     reinterpret_cast<uint4*>(far_items), reinterpret_cast<uint4*>(near_items),
+  if (!src) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_far_order, dim3(nb), dim3(kBatch), 0, s, nb, far_off, cap,
+                     reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(far_items),
+                     pstat);
+  return hipGetLastError();
+}
+
+hipError_t near_compact(int nb, const int* near_off, int cap, const Item* stage,
+                        Item* near_items, const int2* range, unsigned long long* pstat,
+                        hipStream_t s) {
+  if (nb == 0) return hipSuccess;
+  if ((cap > 0) != (stage != nullptr)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_near_compact, dim3((nb + 3) / 4), dim3(256), 0, s, nb, near_off, cap,
+                     reinterpret_cast<const uint4*>(stage), reinterpret_cast<uint4*>(near_items),
                      range, pstat);
   return hipGetLastError();
 }
@@ -2060,8 +2169,9 @@ __device__ __forceinline__ double far32_batch(const EvalDesc& d, int b, float* m
   const float kappa32 = (float)d.kappa, delta2_32 = (float)d.delta2;
   const gptr<float> qc = as_global(d.qc32);
   const gptr<Item> items = as_global(d.far_items);
-  const gptr<double> center = (gptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
-  const gptr<double> scale = as_global(tv.scale);
+  // read-only node data at wave-uniform addresses: scalar loads
+  const cptr<double> center = (cptr<double>)(tv.center);  // xmid, ymid, zmid, sqradius
+  const cptr<double> scale = (cptr<double>)(tv.scale);
   const gptr<int> far_off = as_global(d.far_off);
   const int lane = threadIdx.x;
   const int t = b * kBatch + lane;
@@ -2129,16 +2239,20 @@ __device__ __forceinline__ double far32_batch(const EvalDesc& d, int b, float* m
       const int k0 = __popcll(m0), k1 = __popcll(m1);
       if (in0) tslot[__popcll(m0 & below)] = lane;
       if (in1) tslot[k0 + __popcll(m1 & below)] = lane;
+      // both clusters' centres and scales by uniform (scalar) loads, chosen per
+      // lane below, instead of per-lane loads on the unit's critical path
+      const double c0x = center[4 * n0], c0y = center[4 * n0 + 1], c0z = center[4 * n0 + 2];
+      const double c1x = center[4 * n1], c1y = center[4 * n1 + 1], c1z = center[4 * n1 + 2];
+      const float s0 = (float)scale[n0], s1 = (float)scale[n1];
       __syncthreads();
       const bool act = lane < k0 + k1, second = lane >= k0;
       const int tl = act ? tslot[lane] : lane;
       const double qx = __shfl(px, tl), qy = __shfl(py, tl), qz = __shfl(pz, tl);
       if (act) {
-        const int node = second ? n1 : n0;
-        const double cx = center[4 * node], cy = center[4 * node + 1], cz = center[4 * node + 2];
         QPair<NT, float> v;
-        const float rinv = qsetup<NT, SCR>(v, qx, qy, qz, cx, cy, cz, (float)scale[node],
-                                           kappa32, delta2_32);
+        const float rinv = qsetup<NT, SCR>(v, qx, qy, qz, second ? c1x : c0x,
+                                           second ? c1y : c0y, second ? c1z : c0z,
+                                           second ? s1 : s0, kappa32, delta2_32);
         res[second ? 1 : 0][tl] = sweep32<NT>(v, mbuf + (second ? T2 : 0)) * rinv;
       }
       __syncthreads();

@@ -78,7 +78,8 @@ def _per_target_lists(lst, n):
                                              ("blobs12k_screened", 0.3, 2.5e-5, 2.5e-5),
                                              ("tiny300", 0.5, 0.0, 0.01)])
 def test_lists_bit_exact(ctx, oracle, name, acc, d2, sub):
-    """Every target's far clusters and near leaves, in traversal order, equal
+    """Every target's far clusters (as a set: a batch's far items are ordered
+    for the far kernel's pairing) and near leaves (in traversal order) equal
     the oracle's per-target recursion (whose tree equals the reference's)."""
     fx = load_fixture(name)
     s = _system(ctx, fx)
@@ -88,7 +89,7 @@ def test_lists_bit_exact(ctx, oracle, name, acc, d2, sub):
     o = oracle.tree(fx["x"], fx["y"], fx["z"], int(fx["maxmemb"]))
     for slot in range(s.n):
         f, nr = o.lists(int(perm[slot]), 4, acc, d2, sub)
-        assert far[slot] == list(f), slot
+        assert sorted(far[slot]) == sorted(f), slot
         assert near[slot] == list(nr), slot
 
 
@@ -119,7 +120,7 @@ def test_lists_deep_tree_bit_exact(ctx, oracle):
     o = oracle.tree(x, y, z, 1)
     for slot in range(s.n):
         f, nr = o.lists(int(t.perm[slot]), 4, 0.5, 0.0, 0.0)
-        assert far[slot] == list(f), slot
+        assert sorted(far[slot]) == sorted(f), slot
         assert near[slot] == list(nr), slot
 
 
@@ -376,7 +377,7 @@ def test_subtree_root_matches_reference(ctx, oracle):
             far, near = _per_target_lists(s.lists(nt, acc, 0.0, 0.01, root=nd), s.n)
             for slot in range(s.n):
                 f, nr = o.lists(int(perm[slot]), nt, acc, 0.0, 0.01, root=nd)
-                assert far[slot] == list(f) and near[slot] == list(nr), (nd, slot)
+                assert sorted(far[slot]) == sorted(f) and near[slot] == list(nr), (nd, slot)
         if k % 5 == 0:
             prm = bit_amd.make_params(nt, acc, kap, 0.0, 0.01, "fp32", root=nd)
             outs, _ = bit_amd.eval_multi(ctx, [s], [prm])
@@ -626,6 +627,83 @@ def test_lane_stats_count_the_kernels_slots(ctx):
     assert st["lane_near_pairs"] == 64 * slots
     assert st["near_pairs"] == int(np.sum(npc * nlen))
     assert sparse.any() and (~sparse).any() and sweeps < len(fpc)
+    s.close()
+
+
+def _pairing_order(items):
+    """A batch's far items (traversal order) in the far kernel's pairing order:
+    buckets by target count, the largest remaining bucket j against the
+    smallest i: i == j -> all of j; i + j <= 64 -> min(|i|, |j|) pairs (j, i);
+    else all of j alone."""
+    buckets = {}
+    for it in items:
+        buckets.setdefault(bin(int(it[0])).count("1"), []).append(it)
+    out = []
+    while any(buckets.values()):
+        full = sorted(c for c, v in buckets.items() if v)
+        i, j = full[0], full[-1]
+        if i == j:
+            out += buckets[j]
+            buckets[j] = []
+        elif i + j <= 64:
+            k = min(len(buckets[i]), len(buckets[j]))
+            for a, b in zip(buckets[j][:k], buckets[i][:k]):
+                out += [a, b]
+            buckets[i], buckets[j] = buckets[i][k:], buckets[j][k:]
+        else:
+            out += buckets[j]
+            buckets[j] = []
+    return out
+
+
+def _min_sweeps(pcs):
+    """Fewest sweeps of at most two items of at most 64 targets together
+    (greedy matching of sorted counts, largest with smallest)."""
+    a = sorted(pcs)
+    i, j, n = 0, len(a) - 1, 0
+    while i <= j:
+        if i < j and a[i] + a[j] <= 64:
+            i += 1
+        j -= 1
+        n += 1
+    return n
+
+
+def test_far_items_in_pairing_order(ctx):
+    """Each batch's far items are its traversal-order items (depth-first
+    preorder of their clusters) rearranged as the pairing order states, the
+    far kernel's rule (an item shares a sweep with the next when the two have
+    at most 64 targets together) then forms the fewest sweeps possible, and
+    lane_far counts them."""
+    P = workloads.gaussian_blobs(60000, seed=13)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 200)
+    s.potential(9, 0.4, 0.5, 1e-5, None, "fp32")
+    st = s.stats()
+    lst = s.lists(9, 0.4, 1e-5, None)
+    ch = s.export_tree().children
+    pre = np.zeros(len(ch), dtype=np.int64)
+    stack, k = [0], 0
+    while stack:
+        v = stack.pop()
+        pre[v] = k
+        k += 1
+        stack += [c for c in ch[v][::-1] if c >= 0]
+    fo, fm, fn = lst["far_off"], lst["far_mask"], lst["far_node"]
+    sweeps, total = 0, 0
+    for b in range(len(fo) - 1):
+        got = list(zip(fm[fo[b]:fo[b + 1]].tolist(), fn[fo[b]:fo[b + 1]].tolist()))
+        trav = sorted(got, key=lambda it: pre[it[1]])
+        assert got == _pairing_order(trav), b
+        pcs = [bin(m).count("1") for m, _ in got]
+        pend = -1
+        for pc in pcs:
+            if pend >= 0 and pend + pc <= 64:
+                pend = -1
+            else:
+                sweeps += 1
+                pend = pc
+        total += _min_sweeps(pcs)
+    assert sweeps == total and st["lane_far"] == 64 * sweeps
     s.close()
 
 
