@@ -1783,10 +1783,19 @@ hipError_t lists_fill(const TreeView& tv, MacParams mp, int stack_cap, const int
   return hipGetLastError();
 }
 
-// Far items a batch may hold for the pairing order below (kept in registers,
-// kFarOrderCap / 64 per lane); a batch with more keeps its traversal order.
-constexpr int kFarOrderCap = 1024;
-constexpr int kFarOrderPer = kFarOrderCap / kBatch;
+// Far items of a batch are ordered for the pairing in windows of this many
+// consecutive items (traversal order between windows; each lane holds
+// kFarOrderWindow / 64 of a window's items in registers).  At 4M blobs, windows
+// of 64 / 128 / 256 / a whole batch: 93.6 / 94.7 / 95.6 / 96.2% of the far
+// kernel's lane slots used, 3.7 / 4.9 / 7.4 / 10.3 GB of its L2-to-memory
+// traffic per launch (2.9 GB in traversal order), far kernel time within
+// 0.2 ms of each other, 128 the fastest.
+#ifndef BIT_FAR_ORDER_WINDOW
+#define BIT_FAR_ORDER_WINDOW 128
+#endif
+constexpr int kFarOrderWindow = BIT_FAR_ORDER_WINDOW;
+constexpr int kFarOrderPer = kFarOrderWindow / kBatch;
+static_assert(kFarOrderWindow % kBatch == 0, "a window is whole 64-item chunks");
 
 __device__ __forceinline__ int item_pc(uint4 v) {
   return __popcll(((unsigned long long)v.y << 32) | v.x);
@@ -1797,12 +1806,14 @@ __device__ __forceinline__ int lane_of(int v, int l) { return __builtin_amdgcn_r
 // batch's far pair statistics (pstat[4b]: pairs, pstat[4b + 3]: k_farq's
 // sweeps); one wavefront per batch.  Batch b's items are read from
 // src + b * cap (cap > 0: staging regions) or src + far_off[b] (cap == 0; src
-// may be the output itself: the batch is read whole before it is written).
+// may be the output itself: each window is read whole before it is written).
 //
 // k_farq sweeps two consecutive items together when their target counts sum to
-// at most 64, else one.  The order below makes that rule form as many pairs as
-// any pairing of the batch's items can (the two-pointer matching of items
-// sorted by target count):
+// at most 64, else one.  Within each window of kFarOrderWindow consecutive
+// items (traversal order between windows, so that batches processed together
+// still visit clusters in about the same order and share them in the L2), the
+// order below makes that rule form as many pairs as any pairing of the
+// window's items can (the two-pointer matching of items sorted by target count):
 //   bucket c = the items of c targets (1..64), each in traversal order;
 //   repeat, with j the largest non-empty bucket and i the smallest:
 //     i == j: emit bucket j (pairs of two when 2j <= 64) and stop;
@@ -1810,17 +1821,15 @@ __device__ __forceinline__ int lane_of(int v, int l) { return __builtin_amdgcn_r
 //                  the next of i;
 //     else: emit bucket j, each item alone (no remaining item fits with it).
 // An item emitted alone is never followed by one it fits with (j + any
-// remaining count > 64), and a pair's second item starts no unit, so the
-// kernel's rule forms exactly these units.  The order depends only on the
-// batch's items in traversal order.  A batch of more than kFarOrderCap items
-// keeps its traversal order.
+// remaining count > 64), a pair's second item starts no unit, and a window's
+// last unit ends at the window's end, so the kernel's rule forms exactly these
+// units.  The order depends only on the batch's items in traversal order.
 //
-// Work: each lane holds up to kFarOrderPer items in registers; an item's rank
-// in its bucket comes from a match of equal counts (seven ballots per 64
-// items); the bucket walk above runs on bucket counts (one lane per bucket)
-// and records, per bucket, segments of consecutive ranks with their output
-// position and stride (1, or 2 for one side of a run of pairs); each item then
-// finds its segment and is stored once.
+// Work per window: an item's rank in its bucket comes from a match of equal
+// counts (seven ballots per 64 items); the bucket walk above runs on bucket
+// counts (one lane per bucket) and records, per bucket, segments of
+// consecutive ranks with their output position and stride (1, or 2 for one
+// side of a run of pairs); each item then finds its segment and is stored once.
 __global__ __launch_bounds__(64) void k_far_order(int nb, const int* __restrict__ far_off,
                                                   int cap, const uint4* src_all,
                                                   uint4* far_items,
@@ -1838,8 +1847,9 @@ __global__ __launch_bounds__(64) void k_far_order(int nb, const int* __restrict_
   const int n = far_off[b + 1] - far_off[b];
   uint4* dst = far_items + far_off[b];
   const uint4* src = cap > 0 ? src_all + (size_t)b * (size_t)cap : src_all + far_off[b];
-  unsigned long long pairs = 0, sweeps = 0;  // per lane, summed below
-  if (n <= kFarOrderCap) {
+  unsigned long long pairs = 0, units = 0;  // per lane / wave-uniform
+  for (int w0 = 0; w0 < n; w0 += kFarOrderWindow) {
+    const int wn = min(kFarOrderWindow, n - w0);
     fill[lane] = 0;
     seg_first[lane] = -1;
     uint4 v[kFarOrderPer];
@@ -1847,14 +1857,14 @@ __global__ __launch_bounds__(64) void k_far_order(int nb, const int* __restrict_
 #pragma unroll
     for (int u = 0; u < kFarOrderPer; u++) {
       const int i = u * kBatch + lane;
-      v[u] = i < n ? src[i] : make_uint4(0, 0, 0, 0);
+      v[u] = i < wn ? src[w0 + i] : make_uint4(0, 0, 0, 0);
     }
     __syncthreads();
-    // rank of each item among the earlier items of its count
+    // rank of each item among the window's earlier items of its count
 #pragma unroll
     for (int u = 0; u < kFarOrderPer; u++) {
-      if (u * kBatch >= n) break;
-      const bool in = u * kBatch + lane < n;
+      if (u * kBatch >= wn) break;
+      const bool in = u * kBatch + lane < wn;
       const int pc = in ? item_pc(v[u]) : 0;
       unsigned long long same = ballot(in);
 #pragma unroll
@@ -1870,9 +1880,8 @@ __global__ __launch_bounds__(64) void k_far_order(int nb, const int* __restrict_
     }
     // the bucket walk (lane l: bucket l + 1)
     int cnt = fill[lane];
-    pairs = (unsigned long long)cnt * (unsigned long long)(lane + 1);
+    pairs += (unsigned long long)cnt * (unsigned long long)(lane + 1);
     int take = 0, last = -1, p = 0, ns = 0;
-    unsigned long long units = 0;
     auto add_seg = [&](int bl, int r0, int k, int p0, int stride) {
       const int lb = lane_of(last, bl);
       if (lane == 0) {
@@ -1910,55 +1919,26 @@ __global__ __launch_bounds__(64) void k_far_order(int nb, const int* __restrict_
         take += lane == jl ? cj : 0;
       }
     }
-    sweeps = lane == 0 ? units : 0;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kFarOrderPer; u++) {
-      if (u * kBatch >= n) break;
-      if (u * kBatch + lane < n) {
+      if (u * kBatch >= wn) break;
+      if (u * kBatch + lane < wn) {
         int s = seg_first[item_pc(v[u]) - 1];
         int4 g = seg[s];
         while (rk[u] >= g.x + g.y) {
           s = seg_next[s];
           g = seg[s];
         }
-        dst[g.z + (rk[u] - g.x) * g.w] = v[u];
+        dst[w0 + g.z + (rk[u] - g.x) * g.w] = v[u];
       }
     }
-  } else {
-    // traversal order; sweeps by the kernel's rule: pairs are taken greedily
-    // from the start of each run of consecutive items whose neighbours fit
-    // together, so an item starts a sweep iff its distance from its run's
-    // start is even
-    int prev_pc = 0, run0 = 0;  // carried across 64-item passes
-    for (int i0 = 0; i0 < n; i0 += kBatch) {
-      const int i = i0 + lane;
-      const bool in = i < n;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (in) {
-        v = src[i];
-        if (src != dst) dst[i] = v;
-      }
-      const int pc = in ? item_pc(v) : 0;
-      const int left = __shfl_up(pc, 1);
-      const int lpc = lane == 0 ? prev_pc : left;
-      const bool start = in && (i == 0 || lpc + pc > kBatch);  // a run starts here
-      const unsigned long long st = ballot(start);
-      const unsigned long long sb = st & (below | (1ull << lane));
-      const int rs = sb ? i0 + 63 - __builtin_clzll(sb) : run0;  // this item's run start
-      pairs += in ? pc : 0;
-      sweeps += (in && ((i - rs) & 1) == 0) ? 1 : 0;
-      prev_pc = __shfl(pc, 63);
-      run0 = st ? i0 + 63 - __builtin_clzll(st) : run0;
-    }
+    __syncthreads();  // the window's LDS tables are rewritten by the next
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    pairs += __shfl_xor(pairs, o);
-    sweeps += __shfl_xor(sweeps, o);
-  }
+  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
   if (lane == 0) {
     pstat[4 * b] = pairs;
-    pstat[4 * b + 3] = sweeps;
+    pstat[4 * b + 3] = units;
   }
 }
 
@@ -2183,7 +2163,9 @@ __device__ __forceinline__ double far32_batch(const EvalDesc& d, int b, float* m
   // items [u, u + n): n = 2 when item u+1 exists and the two items have at
   // most 64 targets together (one sweep, lanes remapped to their targets)
   auto unit_len = [&](int u) -> int {
-    if (u + 1 >= i1) return 1;
+    // the list build orders the items for this rule window by window
+    // (k_far_order): a unit never spans two windows
+    if (u + 1 >= i1 || (u + 1 - i0) % kFarOrderWindow == 0) return 1;
     const unsigned long long m0 = uniform64(items[u].mask), m1 = uniform64(items[u + 1].mask);
     return __popcll(m0) + __popcll(m1) <= kBatch ? 2 : 1;
   };

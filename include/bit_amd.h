@@ -224,7 +224,11 @@ int bit_eval_field_points(bit_system *sys, const bit_params *params, int m, cons
  * (tree3d.cpp:3473 accept -> far, childless -> near) grouped by batches of 64
  * consecutive sorted targets.  First call with NULL arrays to get the sizes
  * (nbatch, nfar, nnear), then with arrays of nbatch+1 / nfar / nnear entries.
- * Item t of batch b covers sorted slot 64*b + lane for every set lane bit. */
+ * Item t of batch b covers sorted slot 64*b + lane for every set lane bit.
+ * Near items are in traversal order; a batch's far items are in the FP32 far
+ * kernel's pairing order (sorted by target count, the largest paired with the
+ * smallest that fits), so a target's far clusters are the traversal's as a
+ * set. */
 int bit_system_export_lists(bit_system *sys, const bit_params *params, int *nbatch, int *nfar,
                             int *nnear, int *far_off, uint64_t *far_mask, int *far_node,
                             int *near_off, uint64_t *near_mask, int *near_node);

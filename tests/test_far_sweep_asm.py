@@ -170,3 +170,15 @@ def test_asm_sweep_equals_cpp_order(nt):
         assert got.tobytes() == want.tobytes(), (nt, trial, got, want)
         if nt >= 4:
             assert run_ref(nt, Q, mult, g, z, plant=True).tobytes() != want.tobytes()
+
+
+def test_far_order_window_matches_tests():
+    """The pairing-order window the GPU tests restate is the kernel's."""
+    import re as _re
+    src = open(os.path.join(CSRC, "kernels.hip")).read()
+    m = _re.search(r"#define BIT_FAR_ORDER_WINDOW (\d+)", src)
+    spec = importlib.util.spec_from_file_location(
+        "tgp", os.path.join(ROOT, "tests", "test_gpu_parity.py"))
+    text = open(spec.origin).read()
+    w = int(_re.search(r"FAR_ORDER_WINDOW = (\d+)", text).group(1))
+    assert m and int(m.group(1)) == w

@@ -610,15 +610,7 @@ def test_lane_stats_count_the_kernels_slots(ctx):
     t = s.export_tree()
     size = (t.end - t.begin).astype(np.int64)
     fo, fpc = lst["far_off"], _popcount64(lst["far_mask"])
-    sweeps = 0
-    for b in range(len(fo) - 1):
-        pend = -1
-        for pc in fpc[fo[b]:fo[b + 1]]:
-            if pend >= 0 and pend + pc <= 64:
-                pend = -1
-            else:
-                sweeps += 1
-                pend = pc
+    sweeps = sum(_kernel_sweeps(fpc[fo[b]:fo[b + 1]]) for b in range(len(fo) - 1))
     npc, nlen = _popcount64(lst["near_mask"]), size[lst["near_node"]]
     sparse = npc <= 40
     slots = int(np.sum(nlen[~sparse]))
@@ -630,11 +622,33 @@ def test_lane_stats_count_the_kernels_slots(ctx):
     s.close()
 
 
+# Items per window of the far pairing order (kFarOrderWindow in kernels.hip;
+# tests/test_far_sweep_asm.py checks the two agree).
+FAR_ORDER_WINDOW = 128
+
+
+def _kernel_sweeps(pcs):
+    """k_farq's units over a batch's far items of target counts pcs: an item
+    shares a sweep with the next when the two have at most 64 targets
+    together, never across a window boundary."""
+    sweeps, pend = 0, -1
+    for k, pc in enumerate(pcs):
+        if pend >= 0 and pend + pc <= 64 and k % FAR_ORDER_WINDOW != 0:
+            pend = -1
+        else:
+            sweeps += 1
+            pend = pc
+    return sweeps
+
+
 def _pairing_order(items):
-    """A batch's far items (traversal order) in the far kernel's pairing order:
-    buckets by target count, the largest remaining bucket j against the
-    smallest i: i == j -> all of j; i + j <= 64 -> min(|i|, |j|) pairs (j, i);
-    else all of j alone."""
+    """A batch's far items (traversal order) in the far kernel's pairing order,
+    window by window: buckets by target count, the largest remaining bucket j
+    against the smallest i: i == j -> all of j; i + j <= 64 -> min(|i|, |j|)
+    pairs (j, i); else all of j alone."""
+    if len(items) > FAR_ORDER_WINDOW:
+        return (_pairing_order(items[:FAR_ORDER_WINDOW]) +
+                _pairing_order(items[FAR_ORDER_WINDOW:]))
     buckets = {}
     for it in items:
         buckets.setdefault(bin(int(it[0])).count("1"), []).append(it)
@@ -673,13 +687,14 @@ def test_far_items_in_pairing_order(ctx):
     """Each batch's far items are its traversal-order items (depth-first
     preorder of their clusters) rearranged as the pairing order states, the
     far kernel's rule (an item shares a sweep with the next when the two have
-    at most 64 targets together) then forms the fewest sweeps possible, and
-    lane_far counts them."""
-    P = workloads.gaussian_blobs(60000, seed=13)
-    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 200)
-    s.potential(9, 0.4, 0.5, 1e-5, None, "fp32")
+    at most 64 targets together, within a window) then forms the fewest sweeps
+    possible in each window, and lane_far counts them (200k blobs, acc 0.25:
+    most batches hold more than one window)."""
+    P = workloads.gaussian_blobs(200000, seed=13)
+    s = bit_amd.System(ctx, P.x, P.y, P.z, P.q, 100)
+    s.potential(5, 0.25, 0.5, 1e-5, None, "fp32")
     st = s.stats()
-    lst = s.lists(9, 0.4, 1e-5, None)
+    lst = s.lists(5, 0.25, 1e-5, None)
     ch = s.export_tree().children
     pre = np.zeros(len(ch), dtype=np.int64)
     stack, k = [0], 0
@@ -689,21 +704,18 @@ def test_far_items_in_pairing_order(ctx):
         k += 1
         stack += [c for c in ch[v][::-1] if c >= 0]
     fo, fm, fn = lst["far_off"], lst["far_mask"], lst["far_node"]
-    sweeps, total = 0, 0
+    sweeps, total, windows = 0, 0, 0
     for b in range(len(fo) - 1):
         got = list(zip(fm[fo[b]:fo[b + 1]].tolist(), fn[fo[b]:fo[b + 1]].tolist()))
         trav = sorted(got, key=lambda it: pre[it[1]])
         assert got == _pairing_order(trav), b
         pcs = [bin(m).count("1") for m, _ in got]
-        pend = -1
-        for pc in pcs:
-            if pend >= 0 and pend + pc <= 64:
-                pend = -1
-            else:
-                sweeps += 1
-                pend = pc
-        total += _min_sweeps(pcs)
+        sweeps += _kernel_sweeps(pcs)
+        total += sum(_min_sweeps(pcs[w:w + FAR_ORDER_WINDOW])
+                     for w in range(0, len(pcs), FAR_ORDER_WINDOW))
+        windows = max(windows, (len(pcs) + FAR_ORDER_WINDOW - 1) // FAR_ORDER_WINDOW)
     assert sweeps == total and st["lane_far"] == 64 * sweeps
+    assert windows >= 2  # batches of more than one window occur
     s.close()
 
 
