@@ -42,14 +42,14 @@ PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 MEASURED_TRAFFIC = {
     # (kernel, n, seed, nterms, acc, maxmemb, kappa, delta, precision): bytes per launch
     # (profiles/pmc_traffic_blobs4m_all_kernels.md)
-    ("far", 4_000_000, 3, 12, 0.255, 500, 0.5, 0.005, "fp32"): 2.992660e9,
-    ("near", 4_000_000, 3, 12, 0.255, 500, 0.5, 0.005, "fp32"): 1.352660e9,
+    ("far", 4_000_000, 3, 12, 0.255, 500, 0.5, 0.005, "fp32"): 5.041481e9,
+    ("near", 4_000_000, 3, 12, 0.255, 500, 0.5, 0.005, "fp32"): 1.366242e9,
 }
 # Executed FP32 lane FLOPs per launch: 64 x rocprofv3's SQ_INSTS_VALU_FLOPS_FP32 (a
 # per-wave count, 2 FMA + ADD + MUL + TRANS) on this exact configuration
 # (profiles/pmc_flops_k_farq_k_near32_blobs4m.md); masked lanes are included.
 MEASURED_EXEC_FLOPS = {
-    ("far", 4_000_000, 3, 12, 0.255, 500, 0.5, 0.005, "fp32"): 64 * 3.166841e10,
+    ("far", 4_000_000, 3, 12, 0.255, 500, 0.5, 0.005, "fp32"): 64 * 2.914759e10,
     ("near", 4_000_000, 3, 12, 0.255, 500, 0.5, 0.005, "fp32"): 64 * 1.303061e10,
 }
 
